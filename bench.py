@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the voxel GI path on one MI355X.
+
+Workload: the procedural room, 262,144 triangles, 256^3 grid with 8 mip levels, a
+1920x1080 ray-cast G-buffer, 9 diffuse 60-degree cones + 1 roughness-driven specular
+cone, 4 point lights (synthetic data, seeded). One step = one frame: voxelize + inject and
+mip chain + cone trace, with all inputs resident in HBM. The headline value is the frame
+rate in cone-traced Mpixels/s with FP16 radiance mips (FP32 cone accumulation); the plain
+FP32 form is measured beside it.
+
+Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_l2(x, y):
+    return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30))
+
+
+def run_precision(ctx, vgi, w, mesh, gb, prec, steps, warmup):
+    grid = vgi.make_grid(**w.grid)
+    lights = vgi.make_lights(w.lights)
+    cones = vgi.make_cones(**w.cones)
+    scene = ctx.scene()
+    od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    p = vgi.PREC_NAMES[prec]
+
+    def frame():
+        scene.voxelize([mesh], grid)
+        scene.build_mips(lights, p)
+        scene.trace(gb, cones, od, osp)
+
+    for _ in range(warmup):
+        frame()
+    ctx.sync()
+    # whole frames, device time between two events on the context stream
+    ctx.timer_start()
+    for _ in range(steps):
+        frame()
+    total_ms = ctx.timer_stop()
+    # one extra profiled frame for the per-kernel split (HIP events around each phase)
+    ctx.set_profiling(True)
+    frame()
+    t = ctx.timings()
+    ctx.set_profiling(False)
+    # per-kernel averages over several profiled trace launches
+    trace_ms = []
+    ctx.set_profiling(True)
+    for _ in range(max(3, min(steps, 10))):
+        scene.trace(gb, cones, od, osp)
+        trace_ms.append(ctx.timings().trace_ms)
+    ctx.set_profiling(False)
+    out = dict(prec=prec, frame_ms=total_ms / steps, vox_ms=t.voxelize_total_ms,
+               vox_raster_ms=t.voxelize_raster_ms, mips_ms=t.mips_ms,
+               trace_ms=float(np.mean(trace_ms)), trilinear=int(t.trace_trilinear_samples),
+               stats=scene.stats(), diffuse=od.numpy(), specular=osp.numpy())
+    scene.release()
+    return out
+
+
+def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
+    """Oracle FP32 timed on this host: full voxelize + mips, trace on a pixel sample
+    scaled to the frame. Also the FP64 error of the GPU outputs on a 4096-pixel subset."""
+    from oracle import oracle as orc
+
+    cores = orc.num_threads()
+    t0 = time.perf_counter()
+    acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+    t_vox = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lv = orc.inject_mips(acc, w.grid, w.lights, "fp32")
+    t_mip = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    px = rng.choice(w.num_pixels, sample_px, replace=False)
+    t0 = time.perf_counter()
+    orc.trace(lv, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough, pixels=px)
+    t_tr = (time.perf_counter() - t0) * (w.num_pixels / sample_px)
+    frame_s = t_vox + t_mip + t_tr
+    # error vs FP64 on a seeded 4096-pixel subset
+    sub = rng.choice(w.num_pixels, 4096, replace=False)
+    lv64 = orc.inject_mips(acc, w.grid, w.lights, f64=True)
+    d64, s64, _ = orc.trace(lv64, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough,
+                            pixels=sub, f64=True)
+    ref = np.concatenate([d64[:, :3], s64[:, :3]], 1)
+    errs = {}
+    for r in gpu_results:
+        got = np.concatenate([r["diffuse"][sub, :3], r["specular"][sub, :3]], 1)
+        errs[r["prec"]] = dict(rel_l2=rel_l2(got, ref), max_abs=float(np.abs(got - ref).max()))
+    return dict(value=w.num_pixels / frame_s / 1e6, unit="Mpixels/s", cores=cores,
+                kind="port",
+                sample=(f"oracle FP32 ({cores} threads): full voxelize {t_vox:.2f}s + mips "
+                        f"{t_mip:.2f}s + trace of {sample_px} of {w.num_pixels} pixels scaled "
+                        f"to the frame ({t_tr:.2f}s)")), errs
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=262_144)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=8)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cones", type=int, default=9)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp32,fp16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path shards by scene; see the design notes)", file=sys.stderr)
+        return 2
+
+    from final184_amd import scenegen as sg
+    from final184_amd import vgi
+
+    t0 = time.perf_counter()
+    w = sg.room(ntri=args.tris, n=args.n, levels=args.levels, width=args.width,
+                height=args.height, num_diffuse=args.cones)
+    log(f"workload {w.name} generated in {time.perf_counter() - t0:.1f}s")
+    ctx = vgi.Context(0)
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    results = []
+    for prec in args.precisions.split(","):
+        r = run_precision(ctx, vgi, w, mesh, gb, prec, args.steps, args.warmup)
+        log(f"{prec}: frame {r['frame_ms']:.3f} ms (voxelize {r['vox_ms']:.3f}, mips "
+            f"{r['mips_ms']:.3f}, trace {r['trace_ms']:.3f}); trilinear/px "
+            f"{r['trilinear'] / w.num_pixels:.1f}")
+        results.append(r)
+    by = {r["prec"]: r for r in results}
+    head = by.get("fp16", results[-1])
+    base = by.get("fp32", results[0])
+    texel_b = {"fp32": 16, "fp16": 8, "bf16": 8}
+    # dominant kernel: the cone march
+    px = w.num_pixels
+    alg_bytes = px * (24 + 4 + 32) + head["trilinear"] * 8 * texel_b[head["prec"]]
+    achieved = alg_bytes / (head["trace_ms"] * 1e-3) / 1e9
+    fp32_alg = px * (24 + 4 + 32) + base["trilinear"] * 8 * texel_b[base["prec"]]
+    fp32_achieved = fp32_alg / (base["trace_ms"] * 1e-3) / 1e9
+    mip_levels_bytes = sum(((args.n >> k) ** 3) for k in range(args.levels))
+    line = {
+        "metric": "cone-traced Mpixels/s + voxelized Mtris/s; p99 latency @ QPS, FP16 vs FP32",
+        "value": px / (head["frame_ms"] * 1e-3) / 1e6,
+        "unit": "Mpixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["frame_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "triangles": w.num_triangles, "grid": args.n,
+                   "levels": args.levels, "gbuffer": [w.width, w.height],
+                   "cones": f"{args.cones} diffuse 60deg + 1 specular (roughness)",
+                   "mip_storage": head["prec"], "accumulate": "f32", "parallelism": "single GPU"},
+        "voxelize_mtris_s": w.num_triangles / (head["vox_ms"] * 1e-3) / 1e6,
+        "trace_mpixels_s": px / (head["trace_ms"] * 1e-3) / 1e6,
+        "mips_gbs": (args.n ** 3 * 32 + mip_levels_bytes * texel_b[head["prec"]]) / (head["mips_ms"] * 1e-3) / 1e9,
+        "phases_ms": {r["prec"]: {"voxelize": r["vox_ms"], "voxelize_raster": r["vox_raster_ms"],
+                                  "mips": r["mips_ms"], "trace": r["trace_ms"], "frame": r["frame_ms"]}
+                      for r in results},
+        "fp32_frame_mpixels_s": px / (base["frame_ms"] * 1e-3) / 1e6,
+        "fp16_vs_fp32_frame_speedup": base["frame_ms"] / head["frame_ms"],
+        "trilinear_per_pixel": head["trilinear"] / px,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_trace<{head['prec']}>",
+                     "fp32_achieved": fp32_achieved, "fp32_frac": fp32_achieved / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cb, errs = cpu_baseline(w, results, args.cpu_sample)
+        line["cpu_baseline"] = cb
+        line["error_vs_fp64"] = errs
+    else:
+        line["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,98 @@
+"""Build recipes for the native parts (explicit hipcc / g++ command lines, in-tree).
+
+  final184_amd/libvgi.so      HIP kernels + C ABI (include/vgi.h), gfx950 only
+  final184_amd/libvgi_gen.so  seeded synthetic scene / G-buffer generators (host C++)
+  oracle/_build/libvgi_oracle.so   CPU oracle (test infrastructure, built beside its source)
+
+Every object is rebuilt on each call; objects go to final184_amd/_build/.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "abi.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    # packed-FP32 instructions off for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+GXX = os.environ.get("CXX_HOST", "g++")
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+    # keep real diagnostics visible, drop the known host-pass note about the feature flag
+    lines = [l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    if lines:
+        sys.stderr.write("\n".join(lines) + "\n")
+
+
+def build_vgi() -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
+        list(ex.map(_run, cmds))
+    out = os.path.join(PKG, "libvgi.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    return out
+
+
+def build_scenegen() -> str:
+    out = os.path.join(PKG, "libvgi_gen.so")
+    tmp = out + ".tmp"
+    _run([GXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+          os.path.join(CSRC, "scenegen.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> str:
+    bdir = os.path.join(ORACLE_DIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    out = os.path.join(bdir, "libvgi_oracle.so")
+    tmp = out + ".tmp"
+    _run([GXX, "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+          "-ffp-contract=off", os.path.join(ORACLE_DIR, "vgi_oracle.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    build_scenegen()
+    build_oracle()
+    build_vgi()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built:", os.path.join(PKG, "libvgi.so"))

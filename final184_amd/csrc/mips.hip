@@ -1,0 +1,264 @@
+// mips.hip — radiance injection + fused 2x2x2 mip chain for gfx950.
+//
+// No reference code exists for this stage: the reference keeps a single 128^3 RG16UI
+// volume and lights voxels at lookup time (Shader/Lighting/indirect.frag:157-169); the
+// LOD volumes appear only as a plan (docs/index.md:36).
+//
+// One launch reads a 16^3 tile of its source level per workgroup (512 threads, one
+// 2x2x2 group each) and produces up to four levels above it, staging each level's tile
+// in LDS: 16^3 -> 8^3 -> 4^3 -> 2^3 -> 1. The first launch's source is the integer
+// accumulator: each thread resolves its 8 voxels (rounded mean albedo through the
+// (i/255)^2.2 table, mean quantized normal, direct light at the voxel centre) into level
+// 0 on the way. Further launches start from the last level written until the chain is
+// complete (256^3: levels 0-4, then 5-7). Sums run in a fixed order (x fastest, then y,
+// then z) and every level is rounded to its storage precision before it is reduced, so
+// the result is a pure function of the accumulator.
+//
+// Level k is stored padded: (n_k + 2)^3 texels with a zero border, x fastest.
+#include "vgi_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace vgi {
+
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint16_t b) {
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t x = __builtin_bit_cast(uint32_t, f);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
+  return __builtin_bit_cast(float, (uint32_t)b << 16);
+}
+
+template <int PREC> struct Texel;
+
+template <> struct Texel<VGI_PREC_FP32> {
+  __device__ static float4 load(const void* base, size_t i) { return ((const float4*)base)[i]; }
+  __device__ static void store(void* base, size_t i, float4 v) { ((float4*)base)[i] = v; }
+  __device__ static float4 round(float4 v) { return v; }
+};
+
+template <> struct Texel<VGI_PREC_FP16> {
+  __device__ static float4 load(const void* base, size_t i) {
+    const uint2 u = ((const uint2*)base)[i];
+    return make_float4(f16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.x >> 16)),
+                       f16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
+  }
+  __device__ static uint2 pack(float4 v) {
+    return make_uint2((uint32_t)f32_to_f16_bits(v.x) | ((uint32_t)f32_to_f16_bits(v.y) << 16),
+                      (uint32_t)f32_to_f16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
+  }
+  __device__ static void store(void* base, size_t i, float4 v) { ((uint2*)base)[i] = pack(v); }
+  __device__ static float4 round(float4 v) {
+    return make_float4(f16_bits_to_f32(f32_to_f16_bits(v.x)), f16_bits_to_f32(f32_to_f16_bits(v.y)),
+                       f16_bits_to_f32(f32_to_f16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
+  }
+};
+
+// RGB in bf16, opacity in fp16.
+template <> struct Texel<VGI_PREC_BF16> {
+  __device__ static float4 load(const void* base, size_t i) {
+    const uint2 u = ((const uint2*)base)[i];
+    return make_float4(bf16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), bf16_bits_to_f32((uint16_t)(u.x >> 16)),
+                       bf16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
+  }
+  __device__ static void store(void* base, size_t i, float4 v) {
+    ((uint2*)base)[i] =
+        make_uint2((uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
+                   (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
+  }
+  __device__ static float4 round(float4 v) {
+    return make_float4(bf16_bits_to_f32(f32_to_bf16_bits(v.x)), bf16_bits_to_f32(f32_to_bf16_bits(v.y)),
+                       bf16_bits_to_f32(f32_to_bf16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
+  }
+};
+
+__device__ __forceinline__ size_t padded_index(uint32_t x, uint32_t y, uint32_t z, uint32_t nk) {
+  const size_t p1 = (size_t)nk + 2;
+  return (size_t)(x + 1) + p1 * ((size_t)(y + 1) + p1 * (size_t)(z + 1));
+}
+
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 scale4(float4 a, float s) {
+  return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
+}
+
+// Radiance of level-0 voxel (x, y, z) from the accumulator.
+__device__ float4 inject(const MipParams& P, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
+  const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
+  if (P.bin_count[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t nbr = P.n >> 3;
+  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
+  const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
+  const uint4* a = (const uint4*)(P.acc + ((size_t)brick * 512u + local) * kAccWords);
+  const uint4 s0 = a[0], s1 = a[1];
+  const uint32_t c = s0.w;
+  if (c == 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  const float lr = P.lut[(s0.x + c / 2) / c];
+  const float lg = P.lut[(s0.y + c / 2) / c];
+  const float lb = P.lut[(s0.z + c / 2) / c];
+  const float inv = 1.0f / ((float)c * 1024.0f);
+  const int32_t bias = (int32_t)(1024u * c);
+  const float nx = (float)((int32_t)s1.x - bias) * inv;
+  const float ny = (float)((int32_t)s1.y - bias) * inv;
+  const float nz = (float)((int32_t)s1.z - bias) * inv;
+  const float px = P.origin[0] + ((float)x + 0.5f) * P.h;
+  const float py = P.origin[1] + ((float)y + 0.5f) * P.h;
+  const float pz = P.origin[2] + ((float)z + 0.5f) * P.h;
+  float er = 0.f, eg = 0.f, eb = 0.f;
+  for (uint32_t l = 0; l < P.nlights; l++) {
+    const LightDev& L = P.lights[l];
+    if (L.type == VGI_LIGHT_DIRECTIONAL) {
+      const float cosv = (nx * -L.vec[0] + ny * -L.vec[1]) + nz * -L.vec[2];
+      if (cosv > 0.f) {
+        er = er + L.color[0] * cosv;
+        eg = eg + L.color[1] * cosv;
+        eb = eb + L.color[2] * cosv;
+      }
+    } else {
+      const float dx = L.vec[0] - px, dy = L.vec[1] - py, dz = L.vec[2] - pz;
+      const float d2 = (dx * dx + dy * dy) + dz * dz;
+      if (d2 > 0.f) {
+        const float cosv = ((nx * dx + ny * dy) + nz * dz) / __builtin_sqrtf(d2);
+        if (cosv > 0.f) {
+          const float md = L.min_distance;
+          const float att = fmaxf(d2, md * md);
+          const float f = cosv / att;
+          er = er + L.color[0] * f;
+          eg = eg + L.color[1] * f;
+          eb = eb + L.color[2] * f;
+        }
+      }
+    }
+  }
+  return make_float4(lr * er, lg * eg, lb * eb, 1.0f);
+}
+
+template <int PREC, bool SRC_ACC>
+__global__ __launch_bounds__(512) void k_mips(MipParams P, uint32_t nout) {
+  using TX = Texel<PREC>;
+  __shared__ float4 s_a[512];
+  __shared__ float4 s_b[64];
+  const uint32_t src = P.src_level;
+  const uint32_t ns = P.nk[src];
+  const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge
+  const uint32_t tpa = ns / S;             // tiles per axis
+  const uint32_t tile = blockIdx.x;
+  const uint32_t tx = tile % tpa, ty = (tile / tpa) % tpa, tz = tile / (tpa * tpa);
+  const uint32_t G = S >> 1;  // groups per tile edge (>= 1 when nout >= 1)
+  const uint32_t g = threadIdx.x;
+
+  if (SRC_ACC && nout == 0) {
+    // single-level chain: resolve level 0 only, one voxel per thread
+    for (uint32_t v = g; v < S * S * S; v += 512) {
+      const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
+      TX::store(P.level[0], padded_index(x, y, z, ns), TX::round(inject(P, x, y, z)));
+    }
+    return;
+  }
+
+  if (g < G * G * G) {
+    const uint32_t gx = g % G, gy = (g / G) % G, gz = g / (G * G);
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+      for (uint32_t dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (uint32_t dx = 0; dx < 2; dx++) {
+          const uint32_t x = tx * S + 2 * gx + dx, y = ty * S + 2 * gy + dy, z = tz * S + 2 * gz + dz;
+          float4 c;
+          if (SRC_ACC) {
+            c = TX::round(inject(P, x, y, z));
+            TX::store(P.level[0], padded_index(x, y, z, ns), c);
+          } else {
+            c = TX::load(P.level[src], padded_index(x, y, z, ns));
+          }
+          sum = add4(sum, c);
+        }
+    const float4 v = TX::round(scale4(sum, 0.125f));
+    const uint32_t nd = P.nk[src + 1];
+    TX::store(P.level[src + 1], padded_index(tx * G + gx, ty * G + gy, tz * G + gz, nd), v);
+    s_a[g] = v;
+  }
+  __syncthreads();
+  float4* in = s_a;
+  float4* out = s_b;
+  uint32_t Gk = G;
+  for (uint32_t j = 2; j <= nout; j++) {
+    const uint32_t Gn = Gk >> 1;
+    if (g < Gn * Gn * Gn) {
+      const uint32_t gx = g % Gn, gy = (g / Gn) % Gn, gz = g / (Gn * Gn);
+      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (uint32_t dy = 0; dy < 2; dy++)
+#pragma unroll
+          for (uint32_t dx = 0; dx < 2; dx++)
+            sum = add4(sum, in[(2 * gx + dx) + Gk * ((2 * gy + dy) + Gk * (2 * gz + dz))]);
+      const float4 v = TX::round(scale4(sum, 0.125f));
+      const uint32_t nd = P.nk[src + j];
+      TX::store(P.level[src + j], padded_index(tx * Gn + gx, ty * Gn + gy, tz * Gn + gz, nd), v);
+      out[g] = v;
+    }
+    __syncthreads();
+    float4* t = in;
+    in = out;
+    out = t;
+    Gk = Gn;
+  }
+}
+
+template <int PREC>
+static hipError_t launch_prec(const MipParams& P0, hipStream_t s, uint32_t* produced_top) {
+  MipParams P = P0;
+  P.src_level = 0;
+  uint32_t top = 0;
+  bool first = true;
+  for (;;) {
+    const uint32_t ns = P.nk[P.src_level];
+    const uint32_t S = ns < 16u ? ns : 16u;
+    uint32_t lg = 0;
+    while ((1u << (lg + 1)) <= S) lg++;
+    const uint32_t remaining = P.levels - 1 - P.src_level;
+    const uint32_t nout = lg < remaining ? lg : remaining;
+    const uint32_t tpa = ns / S;
+    const uint32_t blocks = tpa * tpa * tpa;
+    if (first) {
+      hipLaunchKernelGGL((k_mips<PREC, true>), dim3(blocks), dim3(512), 0, s, P, nout);
+      first = false;
+    } else {
+      if (nout == 0) break;
+      hipLaunchKernelGGL((k_mips<PREC, false>), dim3(blocks), dim3(512), 0, s, P, nout);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    top = P.src_level + nout;
+    if (top >= P.levels - 1 || nout == 0) break;
+    P.src_level = top;
+  }
+  if (produced_top) *produced_top = top;
+  return hipSuccess;
+}
+
+hipError_t launch_mips(const MipParams& P, hipStream_t s, uint32_t* produced_top) {
+  switch (P.prec) {
+    case VGI_PREC_FP32: return launch_prec<VGI_PREC_FP32>(P, s, produced_top);
+    case VGI_PREC_FP16: return launch_prec<VGI_PREC_FP16>(P, s, produced_top);
+    case VGI_PREC_BF16: return launch_prec<VGI_PREC_BF16>(P, s, produced_top);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace vgi

@@ -1,0 +1,136 @@
+// vgi_internal.hpp — device-side structures and launch declarations shared by the
+// kernels (voxelize.hip, mips.hip, trace.hip) and the C ABI layer (abi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vgi.h"
+
+namespace vgi {
+
+constexpr int kFixedOne = 256;         // fixed-point units per voxel (1 << VGI_FIXED_BITS)
+constexpr int kAccWords = 8;           // uint32 words per accumulator voxel
+constexpr int kAccBrick = 8;           // accumulator storage brick edge (voxels)
+constexpr int kMaxChunk = 64;          // triangles per voxelize work item (LDS field width)
+constexpr int kNormalBias = 1024;      // quantized normal offset in the accumulator
+
+// Per-mesh descriptor, device resident.
+struct MeshDev {
+  const float* pos;
+  const void* idx;
+  const uint8_t* albedo;
+  uint32_t stride;
+  uint32_t nverts;
+  uint32_t idx_type;
+  uint32_t ntri;
+  uint32_t tri_base;  // first global triangle id of this mesh
+  uint32_t base_rgb;  // r | g << 8 | b << 16
+  float m[12];
+};
+
+// Triangle record written by the setup kernel (64 bytes).
+struct __align__(16) TriRec {
+  int32_t q[9];       // snapped vertices, 1/256-voxel units relative to the grid origin
+  uint32_t rgb;       // r | g << 8 | b << 16 | status << 24 (0 ok, 1 rejected, 2 degenerate)
+  int16_t nq[3];      // unit normal x 1024, rounded
+  uint8_t axis;       // dominant axis index 0=x 1=y 2=z (VoxelGS rule)
+  uint8_t pad0;
+  int16_t lo[3];      // voxel bounding box clipped to the grid (inclusive); lo > hi if empty
+  int16_t hi[3];
+  uint32_t pad1;
+};
+static_assert(sizeof(TriRec) == 64, "TriRec layout");
+
+struct DevStats {
+  unsigned long long rejected;
+  unsigned long long degenerate;
+  uint32_t work_count;
+  uint32_t pad;
+};
+
+struct VoxParams {
+  const MeshDev* meshes;
+  TriRec* tris;
+  uint32_t* bin_count;   // nbins + 1
+  uint32_t* bin_offset;  // nbins + 1 (exclusive scan of bin_count)
+  uint32_t* bin_cursor;  // nbins
+  uint32_t* pairs;       // triangle ids grouped by bin
+  uint2* work;           // (bin, chunk | nchunks << 16)
+  uint32_t* acc;         // n^3 x 8 words, brick-major (8^3 bricks)
+  DevStats* stats;
+  float origin[3];
+  float snap_scale;      // float(n * 256) / extent
+  uint32_t nmesh;
+  uint32_t ntri;
+  uint32_t n;
+  uint32_t bin_shift;    // log2(bin edge in voxels)
+  uint32_t nbins_axis;   // n >> bin_shift
+  uint32_t chunk;        // triangles per work item (<= kMaxChunk)
+  uint32_t pairs_cap;
+};
+
+struct LightDev {
+  uint32_t type;
+  float vec[3];
+  float color[3];
+  float min_distance;
+};
+
+struct MipParams {
+  const uint32_t* acc;
+  const uint32_t* bin_count;
+  const float* lut;  // 256 entries: (i/255)^2.2
+  void* level[VGI_MAX_LEVELS];
+  uint32_t nk[VGI_MAX_LEVELS];
+  uint32_t n;
+  uint32_t levels;
+  uint32_t bin_shift;
+  uint32_t nbins_axis;
+  uint32_t prec;
+  uint32_t src_level;   // level read by this launch (0 with the accumulator as source)
+  uint32_t nlights;
+  float origin[3];
+  float h;
+  LightDev lights[VGI_MAX_LIGHTS];
+};
+
+struct TraceParams {
+  const void* level[VGI_MAX_LEVELS];
+  uint32_t nk[VGI_MAX_LEVELS];
+  uint32_t levels;
+  uint32_t n;
+  uint32_t prec;
+  uint32_t width, height;
+  uint32_t tile_w, tile_h, tiles_x, tiles_y;
+  uint32_t num_diffuse;
+  uint32_t specular;
+  float origin[3];
+  float h, inv_h;
+  float tan_d, tan_s;
+  float step, max_dist, cutoff, normal_off, start_off;
+  float eye[3];
+  float dirs[VGI_MAX_DIFFUSE_CONES][3];
+  float w[VGI_MAX_DIFFUSE_CONES];
+  const float* pos;
+  const float* nrm;
+  const float* rough;
+  float4* out_d;
+  float4* out_s;
+  unsigned long long* counter;  // trilinear gathers (may be null)
+};
+
+// Texel bytes per precision.
+inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u : 8u; }
+
+// ---- launchers (each returns hipError_t) ----
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+hipError_t launch_setup_count(const VoxParams& P, hipStream_t s);
+hipError_t launch_emit(const VoxParams& P, hipStream_t s);
+hipError_t launch_worklist(const VoxParams& P, hipStream_t s);
+hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, hipStream_t s);
+hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const uint32_t* in, uint32_t* out, uint32_t count,
+                     hipStream_t s);
+hipError_t launch_mips(const MipParams& P, hipStream_t s, uint32_t* produced_top);
+hipError_t launch_trace(const TraceParams& P, hipStream_t s);
+
+}  // namespace vgi

@@ -1,0 +1,401 @@
+"""ctypes binding of libvgi.so (the C ABI in include/vgi.h).
+
+This is plumbing for tests, benchmarks and Python callers: device memory comes from
+the library's own vgi_malloc (no torch), host arrays are numpy. Every call checks its
+status and raises VgiError with the library's message. There is no CPU fallback: if
+libvgi.so is missing or cannot be loaded, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libvgi.so")
+
+VGI_OK = 0
+ERRORS = {
+    -1: "VGI_E_INVALID", -2: "VGI_E_NOMEM", -3: "VGI_E_HIP", -4: "VGI_E_BUDGET",
+    -5: "VGI_E_NOT_READY", -6: "VGI_E_RANGE", -7: "VGI_E_NODEVICE", -8: "VGI_E_STATE",
+}
+PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+PREC_NAMES = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
+INDEX_NONE, INDEX_U16, INDEX_U32 = 0, 1, 2
+LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
+MAX_LEVELS, MAX_LIGHTS, MAX_DIFFUSE_CONES = 11, 16, 32
+
+
+class VgiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("extent", C.c_float), ("n", C.c_uint32),
+                ("levels", C.c_uint32)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("position_stride", C.c_uint32),
+                ("num_vertices", C.c_uint32), ("indices", C.c_void_p), ("index_type", C.c_uint32),
+                ("num_triangles", C.c_uint32), ("albedo_rgb8", C.c_void_p),
+                ("base_color", C.c_uint8 * 4), ("model", C.c_float * 12)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("vec", C.c_float * 3), ("color", C.c_float * 3),
+                ("min_distance", C.c_float)]
+
+
+class Lights(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("light", Light * MAX_LIGHTS)]
+
+
+class ConeParams(C.Structure):
+    _fields_ = [("num_diffuse", C.c_uint32), ("specular", C.c_uint32),
+                ("diffuse_aperture_deg", C.c_float), ("diffuse_ring_deg", C.c_float),
+                ("specular_aperture_deg", C.c_float), ("step_factor", C.c_float),
+                ("max_distance", C.c_float), ("alpha_cutoff", C.c_float),
+                ("normal_offset", C.c_float), ("start_offset", C.c_float), ("eye", C.c_float * 3)]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [("position", C.c_void_p), ("normal", C.c_void_p), ("roughness", C.c_void_p),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("bin_size", C.c_uint32), ("chunk_tris", C.c_uint32), ("tile_w", C.c_uint32),
+                ("tile_h", C.c_uint32)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("triangles_in", C.c_uint64), ("triangles_rejected", C.c_uint64),
+                ("triangles_degenerate", C.c_uint64), ("bin_pairs", C.c_uint64),
+                ("work_items", C.c_uint64), ("device_bytes", C.c_uint64), ("n", C.c_uint32),
+                ("levels", C.c_uint32), ("precision", C.c_uint32), ("has_voxels", C.c_uint32),
+                ("has_mips", C.c_uint32), ("bin_size", C.c_uint32)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("voxelize_setup_ms", C.c_float), ("voxelize_sort_ms", C.c_float),
+                ("voxelize_raster_ms", C.c_float), ("voxelize_total_ms", C.c_float),
+                ("mips_ms", C.c_float), ("trace_ms", C.c_float),
+                ("trace_trilinear_samples", C.c_uint64)]
+
+
+# (name, restype, argtypes) for every exported entry point of include/vgi.h
+_P = C.c_void_p
+SIGNATURES = [
+    ("vgi_version", C.c_char_p, []),
+    ("vgi_abi_version", C.c_uint32, []),
+    ("vgi_last_error", C.c_char_p, []),
+    ("vgi_create", C.c_int, [C.POINTER(_P), C.c_int]),
+    ("vgi_destroy", C.c_int, [_P]),
+    ("vgi_set_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
+    ("vgi_get_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
+    ("vgi_set_profiling", C.c_int, [_P, C.c_int]),
+    ("vgi_get_timings", C.c_int, [_P, C.POINTER(Timings)]),
+    ("vgi_sync", C.c_int, [_P, _P]),
+    ("vgi_scene_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("vgi_scene_release", C.c_int, [_P]),
+    ("vgi_scene_get_stats", C.c_int, [_P, C.POINTER(SceneStats)]),
+    ("vgi_scene_bytes_estimate", C.c_uint64, [C.POINTER(Grid), C.c_uint32, C.c_uint64]),
+    ("vgi_voxelize", C.c_int, [_P, _P, C.POINTER(Mesh), C.c_uint32, C.POINTER(Grid), _P]),
+    ("vgi_build_mips", C.c_int, [_P, _P, C.POINTER(Lights), C.c_uint32, _P]),
+    ("vgi_trace", C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(ConeParams), _P, _P, _P]),
+    ("vgi_scene_read_voxels", C.c_int, [_P, _P, _P]),
+    ("vgi_scene_read_level", C.c_int, [_P, _P, C.c_uint32, _P]),
+    ("vgi_malloc", C.c_int, [_P, C.POINTER(_P), C.c_size_t]),
+    ("vgi_free", C.c_int, [_P, _P]),
+    ("vgi_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t]),
+    ("vgi_memcpy_d2h", C.c_int, [_P, _P, _P, C.c_size_t]),
+    ("vgi_memset", C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    ("vgi_stream_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("vgi_stream_destroy", C.c_int, [_P, _P]),
+    ("vgi_timer_start", C.c_int, [_P, _P]),
+    ("vgi_timer_stop", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+]
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m final184_amd.build` "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load_library()
+
+
+def _check(rc: int) -> None:
+    if rc != VGI_OK:
+        raise VgiError(rc, lib.vgi_last_error().decode())
+
+
+def make_grid(origin, extent: float, n: int, levels: int) -> Grid:
+    g = Grid()
+    g.origin[:] = [float(v) for v in origin]
+    g.extent = float(extent)
+    g.n = int(n)
+    g.levels = int(levels)
+    return g
+
+
+def make_lights(lights) -> Lights:
+    """lights: iterable of dicts {type: 'directional'|'point', vec, color, min_distance}."""
+    L = Lights()
+    items = list(lights)
+    if len(items) > MAX_LIGHTS:
+        raise ValueError("too many lights")
+    L.count = len(items)
+    for i, d in enumerate(items):
+        L.light[i].type = LIGHT_DIRECTIONAL if d["type"] == "directional" else LIGHT_POINT
+        L.light[i].vec[:] = [float(v) for v in d["vec"]]
+        L.light[i].color[:] = [float(v) for v in d["color"]]
+        L.light[i].min_distance = float(d.get("min_distance", 0.0))
+    return L
+
+
+def make_cones(num_diffuse=6, specular=True, diffuse_aperture_deg=60.0, diffuse_ring_deg=60.0,
+               specular_aperture_deg=10.0, step_factor=0.5, max_distance=1.0, alpha_cutoff=0.99,
+               normal_offset=1.0, start_offset=1.0, eye=(0.5, 0.5, 2.0)) -> ConeParams:
+    c = ConeParams()
+    c.num_diffuse = num_diffuse
+    c.specular = 1 if specular else 0
+    c.diffuse_aperture_deg = diffuse_aperture_deg
+    c.diffuse_ring_deg = diffuse_ring_deg
+    c.specular_aperture_deg = specular_aperture_deg
+    c.step_factor = step_factor
+    c.max_distance = max_distance
+    c.alpha_cutoff = alpha_cutoff
+    c.normal_offset = normal_offset
+    c.start_offset = start_offset
+    c.eye[:] = [float(v) for v in eye]
+    return c
+
+
+IDENTITY_3X4 = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+class DeviceArray:
+    """A device allocation owned by a Context, with numpy-like shape/dtype metadata."""
+
+    def __init__(self, ctx: "Context", nbytes: int, shape=None, dtype=np.uint8):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.shape = tuple(shape) if shape is not None else (self.nbytes,)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        _check(lib.vgi_malloc(ctx.handle, C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, ctx: "Context", arr: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(arr)
+        d = cls(ctx, a.nbytes, a.shape, a.dtype)
+        if a.nbytes:
+            _check(lib.vgi_memcpy_h2d(ctx.handle, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    @classmethod
+    def empty(cls, ctx: "Context", shape, dtype) -> "DeviceArray":
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        return cls(ctx, n, shape, dt)
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if out.nbytes:
+            _check(lib.vgi_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self) -> None:
+        _check(lib.vgi_memset(self.ctx.handle, self.ptr, 0, self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(lib.vgi_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                lib.vgi_free(self.ctx.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class MeshData:
+    """Device-resident mesh (positions + optional indices/albedo) and its vgi_mesh view."""
+
+    def __init__(self, ctx: "Context", positions: np.ndarray, indices: np.ndarray | None = None,
+                 albedo: np.ndarray | None = None, base_color=(255, 255, 255),
+                 model=IDENTITY_3X4, index_type: int | None = None, position_stride: int = 0):
+        pos = np.ascontiguousarray(positions, dtype=np.float32)
+        self.pos = DeviceArray.from_numpy(ctx, pos)
+        self.num_vertices = pos.size * 4 // (position_stride or 12)
+        if indices is not None:
+            idx = np.ascontiguousarray(indices)
+            if index_type is None:
+                index_type = INDEX_U16 if idx.dtype == np.uint16 else INDEX_U32
+            idx = idx.astype(np.uint16 if index_type == INDEX_U16 else np.uint32, copy=False)
+            self.idx = DeviceArray.from_numpy(ctx, idx)
+            self.num_triangles = idx.size // 3
+        else:
+            self.idx = None
+            index_type = INDEX_NONE
+            self.num_triangles = self.num_vertices // 3
+        self.albedo = DeviceArray.from_numpy(ctx, np.ascontiguousarray(albedo, np.uint8)) \
+            if albedo is not None else None
+        m = Mesh()
+        m.positions = self.pos.ptr
+        m.position_stride = position_stride
+        m.num_vertices = self.num_vertices
+        m.indices = self.idx.ptr if self.idx is not None else None
+        m.index_type = index_type
+        m.num_triangles = self.num_triangles
+        m.albedo_rgb8 = self.albedo.ptr if self.albedo is not None else None
+        m.base_color[:] = [int(base_color[0]), int(base_color[1]), int(base_color[2]), 0]
+        m.model[:] = [float(v) for v in model]
+        self.desc = m
+
+
+class GBufferData:
+    def __init__(self, ctx: "Context", position: np.ndarray, normal: np.ndarray, width: int,
+                 height: int, roughness: np.ndarray | None = None):
+        self.pos = DeviceArray.from_numpy(ctx, np.ascontiguousarray(position, np.float32))
+        self.nrm = DeviceArray.from_numpy(ctx, np.ascontiguousarray(normal, np.float32))
+        self.rough = DeviceArray.from_numpy(ctx, np.ascontiguousarray(roughness, np.float32)) \
+            if roughness is not None else None
+        g = GBuffer()
+        g.position = self.pos.ptr
+        g.normal = self.nrm.ptr
+        g.roughness = self.rough.ptr if self.rough is not None else None
+        g.width = width
+        g.height = height
+        self.desc = g
+        self.num_pixels = width * height
+
+
+class Scene:
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib.vgi_scene_create(ctx.handle, C.byref(h)))
+        self.handle = h.value
+        self.grid = None
+
+    def voxelize(self, meshes, grid: Grid, stream=None) -> None:
+        arr = (Mesh * max(1, len(meshes)))()
+        for i, m in enumerate(meshes):
+            arr[i] = m.desc if isinstance(m, MeshData) else m
+        _check(lib.vgi_voxelize(self.ctx.handle, self.handle, arr, len(meshes), C.byref(grid), stream))
+        self.grid = grid
+
+    def build_mips(self, lights: Lights, precision: int = PREC_FP32, stream=None) -> None:
+        _check(lib.vgi_build_mips(self.ctx.handle, self.handle, C.byref(lights), precision, stream))
+
+    def trace(self, gbuffer: GBufferData, cones: ConeParams, out_diffuse: DeviceArray,
+              out_specular: DeviceArray | None = None, stream=None) -> None:
+        _check(lib.vgi_trace(self.ctx.handle, self.handle, C.byref(gbuffer.desc), C.byref(cones),
+                             out_diffuse.ptr, out_specular.ptr if out_specular is not None else None,
+                             stream))
+
+    def read_voxels(self) -> np.ndarray:
+        n = self.grid.n
+        out = np.empty((n, n, n, 8), np.uint32)  # [z][y][x][word]
+        _check(lib.vgi_scene_read_voxels(self.ctx.handle, self.handle, out.ctypes.data))
+        return out
+
+    def read_level(self, level: int) -> np.ndarray:
+        nk = self.grid.n >> level
+        out = np.empty((nk, nk, nk, 4), np.float32)
+        _check(lib.vgi_scene_read_level(self.ctx.handle, self.handle, level, out.ctypes.data))
+        return out
+
+    def stats(self) -> SceneStats:
+        s = SceneStats()
+        _check(lib.vgi_scene_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def release(self) -> None:
+        if self.handle:
+            _check(lib.vgi_scene_release(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.vgi_scene_release(self.handle)
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.vgi_create(C.byref(h), device))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            _check(lib.vgi_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def scene(self) -> Scene:
+        return Scene(self)
+
+    def sync(self, stream=None) -> None:
+        _check(lib.vgi_sync(self.handle, stream))
+
+    def set_tuning(self, bin_size=8, chunk_tris=64, tile_w=16, tile_h=16) -> None:
+        t = Tuning(bin_size, chunk_tris, tile_w, tile_h)
+        _check(lib.vgi_set_tuning(self.handle, C.byref(t)))
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib.vgi_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib.vgi_set_profiling(self.handle, 1 if on else 0))
+
+    def timings(self) -> Timings:
+        t = Timings()
+        _check(lib.vgi_get_timings(self.handle, C.byref(t)))
+        return t
+
+    def stream_create(self):
+        s = C.c_void_p()
+        _check(lib.vgi_stream_create(self.handle, C.byref(s)))
+        return s.value
+
+    def stream_destroy(self, s) -> None:
+        _check(lib.vgi_stream_destroy(self.handle, s))
+
+    def timer_start(self, stream=None) -> None:
+        _check(lib.vgi_timer_start(self.handle, stream))
+
+    def timer_stop(self, stream=None) -> float:
+        ms = C.c_float()
+        _check(lib.vgi_timer_stop(self.handle, stream, C.byref(ms)))
+        return ms.value
+
+
+def scene_bytes_estimate(grid: Grid, precision: int, num_triangles: int) -> int:
+    return int(lib.vgi_scene_bytes_estimate(C.byref(grid), precision, num_triangles))

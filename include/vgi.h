@@ -1,0 +1,242 @@
+/*
+ * vgi.h — C ABI of the MI355X-native voxel global-illumination path.
+ *
+ * The path: triangles -> conservative voxelization into an integer colour/normal
+ * accumulator -> radiance injection + 2x2x2 mip chain -> per-pixel diffuse and
+ * specular cone tracing -> indirect-light buffer.
+ *
+ * Every entry point returns an int status (VGI_OK or a negative VGI_E_* code) and
+ * never throws. vgi_last_error() returns the message of the calling thread's last
+ * failure. All GPU work is asynchronous on the given stream (a hipStream_t passed as
+ * void*; NULL selects the context's own stream); vgi_sync() blocks.
+ *
+ * Reference interfaces each entry replaces (tobyc11/Final184, path:line):
+ *   - CVoxelizeRenderer::RenderList / Render   Foreground/Renderer/VoxelizeRenderer.cpp:19-30,81-127
+ *     (the voxelize draw per primitive, RG16UI imageStore in Pipelang/Internal/main.lua:242-275)
+ *   - the per-frame clear of the voxel image    Foreground/Renderer/MegaPipeline.cpp:196
+ *   - the lighting_indirect fullscreen pass     Foreground/Renderer/MegaPipeline.cpp:252-268
+ *     (Shader/Lighting/indirect.frag:104-243)
+ *   - the voxel image ownership                 Foreground/Renderer/MegaPipeline.cpp:470-490
+ *
+ * Input/output buffers passed to vgi_voxelize / vgi_trace are DEVICE pointers
+ * (the reference hands GPU vertex/index buffers and images to its passes); the
+ * vgi_malloc/vgi_memcpy helpers exist for callers without their own allocator.
+ */
+#ifndef VGI_H
+#define VGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGI_ABI_VERSION 1u
+
+#define VGI_MAX_LEVELS 11u   /* 1024^3 -> 1^3 */
+#define VGI_MAX_LIGHTS 16u
+#define VGI_MAX_DIFFUSE_CONES 32u
+/* Snapped vertex coordinates must satisfy |v| <= VGI_GUARD_FIXED in 1/256-voxel units. */
+#define VGI_GUARD_FIXED 262144
+#define VGI_FIXED_BITS 8
+
+typedef enum vgi_status {
+  VGI_OK = 0,
+  VGI_E_INVALID = -1,   /* bad argument */
+  VGI_E_NOMEM = -2,     /* device or host allocation failed */
+  VGI_E_HIP = -3,       /* HIP runtime error */
+  VGI_E_BUDGET = -4,    /* refused by the memory budget */
+  VGI_E_NOT_READY = -5, /* scene has no voxels / mips yet */
+  VGI_E_RANGE = -6,     /* size beyond a documented maximum */
+  VGI_E_NODEVICE = -7,  /* no usable gfx950 device */
+  VGI_E_STATE = -8      /* object used in the wrong state */
+} vgi_status;
+
+typedef enum vgi_precision {
+  VGI_PREC_FP32 = 0, /* radiance RGBA fp32, 16 B/texel */
+  VGI_PREC_FP16 = 1, /* radiance RGB fp16 + opacity fp16, 8 B/texel */
+  VGI_PREC_BF16 = 2  /* radiance RGB bf16 + opacity fp16, 8 B/texel */
+} vgi_precision;
+
+typedef enum vgi_index_type {
+  VGI_INDEX_NONE = 0, /* triangle i = vertices 3i, 3i+1, 3i+2 */
+  VGI_INDEX_U16 = 1,
+  VGI_INDEX_U32 = 2
+} vgi_index_type;
+
+typedef enum vgi_light_type { VGI_LIGHT_DIRECTIONAL = 0, VGI_LIGHT_POINT = 1 } vgi_light_type;
+
+typedef struct vgi_context vgi_context;
+typedef struct vgi_scene vgi_scene;
+
+/* Voxel grid: the cube [origin, origin + extent]^3 split into n^3 voxels (n a power
+ * of two, 8 <= n <= 1024), with `levels` mip levels (level k has n>>k voxels per axis,
+ * 1 <= levels, (n >> (levels-1)) >= 1). Replaces the voxelizer camera's VoxelView /
+ * VoxelProj pair (App/MainBehaviour.cpp:54-64, Foreground/SceneGraph/Camera.cpp:104-111). */
+typedef struct vgi_grid {
+  float origin[3];
+  float extent;
+  uint32_t n;
+  uint32_t levels;
+} vgi_grid;
+
+/* One drawable primitive: CTriangleMesh (vertex + index buffer, element count;
+ * Foreground/Shape/TriangleMesh.h:29-89) + CBasicMaterial albedo
+ * (Foreground/Material/BasicMaterial.h) + the model matrix RenderList passes
+ * (Foreground/Renderer/VoxelizeRenderer.cpp:114-120). All pointers are device pointers. */
+typedef struct vgi_mesh {
+  const float* positions;     /* xyz float32 per vertex */
+  uint32_t position_stride;   /* bytes between vertices; 0 means 12 */
+  uint32_t num_vertices;
+  const void* indices;        /* NULL for VGI_INDEX_NONE */
+  uint32_t index_type;        /* vgi_index_type */
+  uint32_t num_triangles;
+  const uint8_t* albedo_rgb8; /* 3 bytes per triangle, or NULL to use base_color */
+  uint8_t base_color[4];      /* RGB8 (+ unused) used when albedo_rgb8 is NULL */
+  float model[12];            /* row-major 3x4: world = M * (x, y, z, 1) */
+} vgi_mesh;
+
+typedef struct vgi_light {
+  uint32_t type;      /* vgi_light_type */
+  float vec[3];       /* directional: unit direction the light travels; point: position */
+  float color[3];     /* RGB radiant intensity (colour x intensity) */
+  float min_distance; /* point: distance clamp for 1/d^2 falloff (world units) */
+} vgi_light;
+
+typedef struct vgi_lights {
+  uint32_t count;
+  vgi_light light[VGI_MAX_LIGHTS];
+} vgi_lights;
+
+/* Cone set and march schedule. Diffuse cone 0 points along the normal, cones
+ * 1..num_diffuse-1 sit on a ring at diffuse_ring_deg from the normal, evenly in
+ * azimuth; weights are proportional to cos(polar angle) and sum to one. */
+typedef struct vgi_cone_params {
+  uint32_t num_diffuse;        /* 0..VGI_MAX_DIFFUSE_CONES */
+  uint32_t specular;           /* 0: no specular cone */
+  float diffuse_aperture_deg;  /* full cone angle */
+  float diffuse_ring_deg;      /* polar angle of the ring cones */
+  float specular_aperture_deg; /* full angle, used when the G-buffer has no roughness */
+  float step_factor;           /* step = step_factor * cone diameter */
+  float max_distance;          /* world units */
+  float alpha_cutoff;          /* stop when accumulated opacity reaches this */
+  float normal_offset;         /* cone origin offset along the normal, in voxels */
+  float start_offset;          /* first sample distance along the cone, in voxels */
+  float eye[3];                /* camera position for the specular reflection */
+} vgi_cone_params;
+
+/* G-buffer: world position and normal float32x3 per pixel, row-major width x height;
+ * optional per-pixel roughness (tan of the specular half-angle). Device pointers. */
+typedef struct vgi_gbuffer {
+  const float* position;
+  const float* normal;
+  const float* roughness; /* may be NULL */
+  uint32_t width;
+  uint32_t height;
+} vgi_gbuffer;
+
+/* Launch-shape knobs (the autotuner's search space). */
+typedef struct vgi_tuning {
+  uint32_t bin_size;    /* voxelize triangle bin edge: 8, 16 or 32 voxels */
+  uint32_t chunk_tris;  /* triangles per voxelize work item: 16..64 */
+  uint32_t tile_w;      /* cone-march screen tile: 8x8, 16x16 or 32x8 */
+  uint32_t tile_h;
+} vgi_tuning;
+
+typedef struct vgi_scene_stats {
+  uint64_t triangles_in;       /* triangles submitted */
+  uint64_t triangles_rejected; /* non-finite, outside the guard band or bad index */
+  uint64_t triangles_degenerate; /* zero area after snapping */
+  uint64_t bin_pairs;          /* (triangle, bin) pairs after the bin-level overlap test */
+  uint64_t work_items;         /* voxelize work items (bin, chunk) */
+  uint64_t device_bytes;       /* device memory held by the scene */
+  uint32_t n;
+  uint32_t levels;
+  uint32_t precision;
+  uint32_t has_voxels;
+  uint32_t has_mips;
+  uint32_t bin_size;
+} vgi_scene_stats;
+
+/* Per-phase GPU durations of the last call on a context with profiling enabled,
+ * measured with HIP events on the stream the kernels ran on. */
+typedef struct vgi_timings {
+  float voxelize_setup_ms; /* snap + triangle records + bin counts */
+  float voxelize_sort_ms;  /* scan + emit + work list */
+  float voxelize_raster_ms;/* the brick voxelize kernel */
+  float voxelize_total_ms;
+  float mips_ms;           /* inject + all mip levels */
+  float trace_ms;          /* the cone-march kernel */
+  uint64_t trace_trilinear_samples; /* trilinear gathers executed (8 texels each) */
+} vgi_timings;
+
+/* ---- library / context ---------------------------------------------------- */
+const char* vgi_version(void);
+uint32_t vgi_abi_version(void);
+/* Message of the calling thread's last failed call ("" if none). */
+const char* vgi_last_error(void);
+
+/* Create a context on a HIP device (replaces CForegroundBootstrapper::Init's device
+ * globals, Foreground/ForegroundBootstrapper.cpp:8-17). */
+int vgi_create(vgi_context** out, int device);
+int vgi_destroy(vgi_context* ctx);
+int vgi_set_tuning(vgi_context* ctx, const vgi_tuning* tuning);
+int vgi_get_tuning(const vgi_context* ctx, vgi_tuning* tuning);
+int vgi_set_profiling(vgi_context* ctx, int enabled);
+int vgi_get_timings(const vgi_context* ctx, vgi_timings* out);
+int vgi_sync(vgi_context* ctx, void* stream);
+
+/* ---- scenes (resident voxel volume + mip chain) ---------------------------- */
+/* Replaces CMegaPipeline::CreateVoxelizePass (MegaPipeline.cpp:470-490): the scene owns
+ * the accumulator and the mip levels; memory is sized on the first voxelize. */
+int vgi_scene_create(vgi_context* ctx, vgi_scene** out);
+int vgi_scene_release(vgi_scene* scene);
+int vgi_scene_get_stats(const vgi_scene* scene, vgi_scene_stats* out);
+/* Device bytes a scene of this grid and precision will hold (admission control). */
+uint64_t vgi_scene_bytes_estimate(const vgi_grid* grid, uint32_t precision, uint64_t num_triangles);
+
+/* Clear + voxelize all meshes into the scene's accumulator (one frame's voxelize pass).
+ * Replaces MegaPipeline.cpp:196 (clear) + :218-223 (VoxelizeRenderer.RenderList), i.e.
+ * VoxelizeRenderer.cpp:19-30,81-127 and the VoxelGS/VoxelPS pipeline
+ * (Pipelang/Internal/main.lua:83-144,242-275). Conservative, order-independent. */
+int vgi_voxelize(vgi_context* ctx, vgi_scene* scene, const vgi_mesh* meshes, uint32_t num_meshes,
+                 const vgi_grid* grid, void* stream);
+
+/* Inject direct light into level 0 and build the 2x2x2 box mip chain in the given
+ * precision. No reference code (planned in docs/index.md:36). */
+int vgi_build_mips(vgi_context* ctx, vgi_scene* scene, const vgi_lights* lights,
+                   uint32_t precision, void* stream);
+
+/* Cone-trace every G-buffer pixel. out_diffuse: float4 per pixel (irradiance RGB,
+ * ambient visibility); out_specular: float4 per pixel (radiance RGB, opacity) or NULL.
+ * Replaces the lighting_indirect pass (MegaPipeline.cpp:252-268,
+ * Shader/Lighting/indirect.frag:104-222). Device pointers. */
+int vgi_trace(vgi_context* ctx, const vgi_scene* scene, const vgi_gbuffer* gbuffer,
+              const vgi_cone_params* cones, float* out_diffuse, float* out_specular,
+              void* stream);
+
+/* ---- parity readback (synchronous) ----------------------------------------- */
+/* Accumulator as n^3 voxels x 8 uint32 (x fastest): R sum, G sum, B sum, count,
+ * sum(nx+1024), sum(ny+1024), sum(nz+1024), 0 (normals quantized to 1/1024). */
+int vgi_scene_read_voxels(vgi_context* ctx, const vgi_scene* scene, uint32_t* host_out);
+/* Mip level as (n>>level)^3 texels x 4 float (x fastest), decoded from storage. */
+int vgi_scene_read_level(vgi_context* ctx, const vgi_scene* scene, uint32_t level, float* host_out);
+
+/* ---- device memory / stream helpers ---------------------------------------- */
+int vgi_malloc(vgi_context* ctx, void** out, size_t bytes);
+int vgi_free(vgi_context* ctx, void* ptr);
+int vgi_memcpy_h2d(vgi_context* ctx, void* dst, const void* src, size_t bytes);
+int vgi_memcpy_d2h(vgi_context* ctx, void* dst, const void* src, size_t bytes);
+int vgi_memset(vgi_context* ctx, void* dst, int value, size_t bytes);
+int vgi_stream_create(vgi_context* ctx, void** out);
+int vgi_stream_destroy(vgi_context* ctx, void* stream);
+/* Device-side elapsed time between two points on a stream (HIP events). */
+int vgi_timer_start(vgi_context* ctx, void* stream);
+int vgi_timer_stop(vgi_context* ctx, void* stream, float* out_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VGI_H */
