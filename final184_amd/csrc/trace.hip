@@ -78,7 +78,9 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
   float t = P.start_off * P.h;
   const float nf = (float)P.n;
   const float top = (float)(P.levels - 1);
-  while (t <= P.max_dist) {
+  // the step cap bounds the loop even when t stops growing in float (zero aperture,
+  // zero direction, huge max distance); no workload comes near it
+  for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= P.max_dist; it++) {
     const float ux = ((ox + dx * t) - P.origin[0]) * P.inv_h;
     const float uy = ((oy + dy * t) - P.origin[1]) * P.inv_h;
     const float uz = ((oz + dz * t) - P.origin[2]) * P.inv_h;

@@ -40,6 +40,8 @@ extern "C" {
 /* Snapped vertex coordinates must satisfy |v| <= VGI_GUARD_FIXED in 1/256-voxel units. */
 #define VGI_GUARD_FIXED 262144
 #define VGI_FIXED_BITS 8
+/* Upper bound on samples per cone (a loop guard; real schedules stay far below). */
+#define VGI_MAX_MARCH_STEPS 4096
 
 typedef enum vgi_status {
   VGI_OK = 0,

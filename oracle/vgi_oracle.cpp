@@ -492,7 +492,7 @@ static void cone_march(const Vol<T>& V, const T o[3], const T d[3], T tan_half, 
   T t = (T)P.start_offset * V.h;
   T maxd = (T)P.max_distance;
   uint64_t cnt = 0;
-  while (t <= maxd) {
+  for (int it = 0; it < 4096 && t <= maxd; it++) {  // step cap, as in the product
     T u[3];
     bool inside = true;
     for (int ax = 0; ax < 3; ax++) {
