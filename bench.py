@@ -125,6 +125,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp32,fp16")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run a k_copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -139,6 +141,18 @@ def main() -> int:
                 height=args.height, num_diffuse=args.cones)
     log(f"workload {w.name} generated in {time.perf_counter() - t0:.1f}s")
     ctx = vgi.Context(0)
+    if args.calibrate_copy_mib > 0:
+        # a plain copy of known size: the reference dispatch for HBM traffic counters
+        nb = args.calibrate_copy_mib << 20
+        src = vgi.DeviceArray(ctx, nb)
+        dst = vgi.DeviceArray(ctx, nb)
+        src.zero()
+        for _ in range(3):
+            ctx.copy(dst, src)
+        ctx.sync()
+        log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB (k_copy)")
+        src.free()
+        dst.free()
     mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
     gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
     results = []

@@ -642,6 +642,7 @@ int vgi_trace(vgi_context* ctx, const vgi_scene* s, const vgi_gbuffer* gb, const
   TraceParams P{};
   for (uint32_t k = 0; k < g.levels; k++) {
     P.level[k] = s->level[k];
+    P.level_bytes[k] = s->level_bytes[k];
     P.nk[k] = g.n >> k;
   }
   P.levels = g.levels;
@@ -796,6 +797,15 @@ int vgi_memset(vgi_context* ctx, void* dst, int value, size_t bytes) {
   if (!ctx || (bytes && !dst)) return fail(VGI_E_INVALID, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
   if (bytes) HIP_TRY(hipMemset(dst, value, bytes));
+  return VGI_OK;
+}
+
+int vgi_memcpy_d2d(vgi_context* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(VGI_E_INVALID, "NULL argument");
+  if (bytes % 16) return fail(VGI_E_INVALID, "bytes (%zu) must be a multiple of 16", bytes);
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(VGI_E_INVALID, "pointers must be 16-byte aligned");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_copy(dst, src, bytes, pick(ctx, stream)));
   return VGI_OK;
 }
 

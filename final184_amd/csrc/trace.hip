@@ -13,64 +13,146 @@
 // 8x8 block), tiles are remapped so that each XCD gets a contiguous band of the image
 // (neighbouring tiles share that XCD's L2), and every lane of a wavefront marches the
 // same cone index at the same time, so directions stay coherent across the wavefront.
+//
+// Gather: a quadrilinear sample is 16 weighted texels, sum_i w_i c_i with the level blend
+// folded into the weights, accumulated in FP32 (FP16 texels feed v_fma_mix directly; no
+// unpacking). For cones whose schedule t_i does not depend on the pixel (all diffuse
+// cones, and the specular cone without a roughness buffer) the level is wave-uniform:
+// the level's buffer descriptor and its row/slice pitches live in SGPRs and the eight
+// texels of a level come from one VGPR byte offset (plus SGPR pitch and immediate +1
+// texel), four 16-byte loads per level for FP16 (texel x and x+1 share a load), eight for
+// FP32. The per-pixel-roughness specular cone uses 64-bit global loads.
 #include "vgi_internal.hpp"
 
 namespace vgi {
 
-__device__ __forceinline__ float h2f(uint32_t b) { return (float)__builtin_bit_cast(_Float16, (uint16_t)b); }
-__device__ __forceinline__ float bf2f(uint32_t b) { return __builtin_bit_cast(float, b << 16); }
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-template <int PREC> struct Fetch;
-template <> struct Fetch<VGI_PREC_FP32> {
-  __device__ static float4 at(const void* base, size_t i) { return ((const float4*)base)[i]; }
-};
-template <> struct Fetch<VGI_PREC_FP16> {
-  __device__ static float4 at(const void* base, size_t i) {
-    const uint2 u = ((const uint2*)base)[i];
-    return make_float4(h2f(u.x & 0xFFFFu), h2f(u.x >> 16), h2f(u.y & 0xFFFFu), h2f(u.y >> 16));
+// Accumulate w * texel into a (FP32) from the raw 32-bit words of one texel.
+template <int PREC> struct Acc;
+template <> struct Acc<VGI_PREC_FP32> {
+  __device__ static void add(float4& a, float w, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+    a.x = fmaf(__builtin_bit_cast(float, d0), w, a.x);
+    a.y = fmaf(__builtin_bit_cast(float, d1), w, a.y);
+    a.z = fmaf(__builtin_bit_cast(float, d2), w, a.z);
+    a.w = fmaf(__builtin_bit_cast(float, d3), w, a.w);
   }
 };
-template <> struct Fetch<VGI_PREC_BF16> {
-  __device__ static float4 at(const void* base, size_t i) {
-    const uint2 u = ((const uint2*)base)[i];
-    return make_float4(bf2f(u.x & 0xFFFFu), bf2f(u.x >> 16), bf2f(u.y & 0xFFFFu), h2f(u.y >> 16));
+template <> struct Acc<VGI_PREC_FP16> {
+  __device__ static void add(float4& a, float w, uint32_t d0, uint32_t d1) {
+    const half2_t rg = __builtin_bit_cast(half2_t, d0);
+    const half2_t ba = __builtin_bit_cast(half2_t, d1);
+    a.x = fmaf((float)rg.x, w, a.x);
+    a.y = fmaf((float)rg.y, w, a.y);
+    a.z = fmaf((float)ba.x, w, a.z);
+    a.w = fmaf((float)ba.y, w, a.w);
+  }
+};
+template <> struct Acc<VGI_PREC_BF16> {  // RGB bf16, opacity fp16
+  __device__ static void add(float4& a, float w, uint32_t d0, uint32_t d1) {
+    const half2_t ba = __builtin_bit_cast(half2_t, d1);
+    a.x = fmaf(__builtin_bit_cast(float, d0 << 16), w, a.x);
+    a.y = fmaf(__builtin_bit_cast(float, d0 & 0xFFFF0000u), w, a.y);
+    a.z = fmaf(__builtin_bit_cast(float, d1 << 16), w, a.z);
+    a.w = fmaf((float)ba.y, w, a.w);
   }
 };
 
-__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float f) {
-  return make_float4(a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, a.z + (b.z - a.z) * f,
-                     a.w + (b.w - a.w) * f);
-}
+template <int PREC> constexpr uint32_t kTexelBytes = (PREC == VGI_PREC_FP32) ? 16u : 8u;
 
-// Trilinear sample of level k at level-0 voxel coordinates u (texel centres at i+0.5).
-template <int PREC>
-__device__ __forceinline__ float4 trilinear(const TraceParams& P, uint32_t k, float ux, float uy,
-                                            float uz) {
+// Trilinear footprint of level k at level-0 voxel coordinates u (texel centres at i+0.5):
+// byte offset of texel (x0, y0, z0) in the padded level, and the 8 weights scaled by wl.
+struct Foot {
+  uint32_t off;
+  float w00, w10, w01, w11;  // xy weights
+  float wz0, wz1;            // z weights x level weight
+};
+
+__device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, uint32_t tb, float ux, float uy,
+                                          float uz, float wl) {
   const float sc = __builtin_ldexpf(1.0f, -(int)k);
   const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
   const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
   const float fx = gx - fx0, fy = gy - fy0, fz = gz - fz0;
-  const uint32_t p1 = P.nk[k] + 2;
-  const size_t p2 = (size_t)p1 * p1;
+  const float ex = 1.0f - fx, ey = 1.0f - fy;
+  Foot f;
   // +1: padded border
-  const size_t base = (size_t)((int)fx0 + 1) + (size_t)p1 * (size_t)((int)fy0 + 1) +
-                      p2 * (size_t)((int)fz0 + 1);
-  const void* L = P.level[k];
-  const float4 c000 = Fetch<PREC>::at(L, base);
-  const float4 c100 = Fetch<PREC>::at(L, base + 1);
-  const float4 c010 = Fetch<PREC>::at(L, base + p1);
-  const float4 c110 = Fetch<PREC>::at(L, base + p1 + 1);
-  const float4 c001 = Fetch<PREC>::at(L, base + p2);
-  const float4 c101 = Fetch<PREC>::at(L, base + p2 + 1);
-  const float4 c011 = Fetch<PREC>::at(L, base + p2 + p1);
-  const float4 c111 = Fetch<PREC>::at(L, base + p2 + p1 + 1);
-  const float4 y00 = lerp4(c000, c100, fx), y10 = lerp4(c010, c110, fx);
-  const float4 y01 = lerp4(c001, c101, fx), y11 = lerp4(c011, c111, fx);
-  const float4 z0 = lerp4(y00, y10, fy), z1 = lerp4(y01, y11, fy);
-  return lerp4(z0, z1, fz);
+  f.off = ((uint32_t)((int)fx0 + 1) + p1 * ((uint32_t)((int)fy0 + 1) + p1 * (uint32_t)((int)fz0 + 1))) * tb;
+  f.w00 = ex * ey;
+  f.w10 = fx * ey;
+  f.w01 = ex * fy;
+  f.w11 = fx * fy;
+  f.wz0 = (1.0f - fz) * wl;
+  f.wz1 = fz * wl;
+  return f;
 }
 
+// ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
 template <int PREC>
+__device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
+                                               float uz, float wl, float4& acc) {
+  constexpr uint32_t TB = kTexelBytes<PREC>;
+  const uint32_t p1 = P.nk[k] + 2;
+  const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)P.level_bytes[k], 0x00020000);
+  const Foot f = footprint(k, p1, TB, ux, uy, uz, wl);
+  const int soff[4] = {0, rowb, slab, rowb + slab};
+  const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float wz = q < 2 ? f.wz0 : f.wz1;
+    if constexpr (PREC == VGI_PREC_FP32) {
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, f.off, soff[q], 0);
+      const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, f.off + 16, soff[q], 0);
+      const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+      const uint32_t b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+      Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
+      Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
+    } else {
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, f.off, soff[q], 0);
+      const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+      Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
+      Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
+    }
+  }
+}
+
+// ---- per-lane level: 64-bit global loads ----
+template <int PREC>
+__device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, float ux, float uy,
+                                            float uz, float wl, float4& acc) {
+  constexpr uint32_t TB = kTexelBytes<PREC>;
+  const uint32_t p1 = P.nk[k] + 2;
+  const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
+  const Foot f = footprint(k, p1, TB, ux, uy, uz, wl);
+  const char* base = (const char*)P.level[k] + f.off;
+  const size_t offs[4] = {0, rowb, slab, rowb + slab};
+  const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float wz = q < 2 ? f.wz0 : f.wz1;
+    if constexpr (PREC == VGI_PREC_FP32) {
+      const uint4 a = *(const uint4*)(base + offs[q]);
+      const uint4 b = *(const uint4*)(base + offs[q] + 16);
+      Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a.x, a.y, a.z, a.w);
+      Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
+    } else {
+      const uint2 a = *(const uint2*)(base + offs[q]);
+      const uint2 b = *(const uint2*)(base + offs[q] + 8);
+      Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
+      Acc<PREC>::add(acc, wxy[q][1] * wz, b.x, b.y);
+    }
+  }
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// One cone. UNIFORM: the schedule (t, diameter, level) is the same for every active lane
+// of the wavefront (it depends only on start, step, aperture and voxel size).
+template <int PREC, bool UNIFORM>
 __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, float oy, float oz,
                                              float dx, float dy, float dz, float tan_half,
                                              uint32_t& count) {
@@ -85,32 +167,43 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
     const float uy = ((oy + dy * t) - P.origin[1]) * P.inv_h;
     const float uz = ((oz + dz * t) - P.origin[2]) * P.inv_h;
     if (!(ux >= 0.f && ux <= nf && uy >= 0.f && uy <= nf && uz >= 0.f && uz <= nf)) break;
-    const float diam = fmaxf(P.h, 2.0f * t * tan_half);
-    const float lvl = fmaxf(0.0f, __log2f(diam * P.inv_h));
-    float4 s;
+    float diam = fmaxf(P.h, 2.0f * t * tan_half);
+    float lvl = fmaxf(0.0f, __log2f(diam * P.inv_h));
+    if (UNIFORM) {
+      diam = uniform_f(diam);
+      lvl = uniform_f(lvl);
+    }
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lvl >= top) {
-      s = trilinear<PREC>(P, P.levels - 1, ux, uy, uz);
+      if (UNIFORM) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s);
+      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s);
       count += 1;
     } else {
       const float fl = floorf(lvl);
-      const uint32_t l0 = (uint32_t)fl;
-      const float4 s0 = trilinear<PREC>(P, l0, ux, uy, uz);
-      const float4 s1 = trilinear<PREC>(P, l0 + 1, ux, uy, uz);
-      s = lerp4(s0, s1, lvl - fl);
+      const float fr = lvl - fl;
+      uint32_t l0 = (uint32_t)fl;
+      if (UNIFORM) {
+        l0 = __builtin_amdgcn_readfirstlane(l0);
+        gather_uniform<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s);
+        gather_uniform<PREC>(P, l0 + 1, ux, uy, uz, fr, s);
+      } else {
+        gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s);
+        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s);
+      }
       count += 2;
     }
     const float w = 1.0f - a;
-    cr = cr + w * s.x;
-    cg = cg + w * s.y;
-    cb = cb + w * s.z;
-    a = a + w * s.w;
+    cr = fmaf(w, s.x, cr);
+    cg = fmaf(w, s.y, cg);
+    cb = fmaf(w, s.z, cb);
+    a = fmaf(w, s.w, a);
     if (a >= P.cutoff) break;
     t = t + P.step * diam;
   }
   return make_float4(cr, cg, cb, a);
 }
 
-template <int PREC>
+template <int PREC, bool BUF>
 __global__ __launch_bounds__(256) void k_trace(TraceParams P) {
   // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a contiguous band.
   const uint32_t ntiles = P.tiles_x * P.tiles_y;
@@ -148,12 +241,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P) {
         const float dx = (t0 * lx + b0 * ly) + nx * lz;
         const float dy = (t1 * lx + b1 * ly) + ny * lz;
         const float dz = (t2 * lx + b2 * ly) + nz * lz;
-        const float4 r = cone_march<PREC>(P, ox, oy, oz, dx, dy, dz, P.tan_d, count);
+        const float4 r = cone_march<PREC, BUF>(P, ox, oy, oz, dx, dy, dz, P.tan_d, count);
         const float wc = P.w[c];
-        ar = ar + wc * r.x;
-        ag = ag + wc * r.y;
-        abl = abl + wc * r.z;
-        aa = aa + wc * r.w;
+        ar = fmaf(wc, r.x, ar);
+        ag = fmaf(wc, r.y, ag);
+        abl = fmaf(wc, r.z, abl);
+        aa = fmaf(wc, r.w, aa);
       }
       od = make_float4(ar, ag, abl, 1.0f - aa);
       if (P.out_s && P.specular) {
@@ -163,9 +256,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P) {
         const float ix = -vx * vi, iy = -vy * vi, iz = -vz * vi;
         const float dn = (nx * ix + ny * iy) + nz * iz;
         const float rx = ix - 2.0f * dn * nx, ry = iy - 2.0f * dn * ny, rz = iz - 2.0f * dn * nz;
-        float th = P.tan_s;
-        if (P.rough) th = fminf(fmaxf(P.rough[p], 0.0087f), 1.0f);
-        os = cone_march<PREC>(P, ox, oy, oz, rx, ry, rz, th, count);
+        if (P.rough) {
+          const float th = fminf(fmaxf(P.rough[p], 0.0087f), 1.0f);
+          os = cone_march<PREC, false>(P, ox, oy, oz, rx, ry, rz, th, count);
+        } else {
+          os = cone_march<PREC, BUF>(P, ox, oy, oz, rx, ry, rz, P.tan_s, count);
+        }
       }
     }
     P.out_d[p] = od;
@@ -179,14 +275,22 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P) {
   }
 }
 
+template <int PREC>
+static void launch_prec(const TraceParams& P, dim3 grid, dim3 block, hipStream_t s) {
+  bool buf = true;  // 32-bit buffer offsets reach every level
+  for (uint32_t k = 0; k < P.levels; k++) buf = buf && P.level_bytes[k] < 0x7FFFFFF0ull;
+  if (buf) hipLaunchKernelGGL((k_trace<PREC, true>), grid, block, 0, s, P);
+  else hipLaunchKernelGGL((k_trace<PREC, false>), grid, block, 0, s, P);
+}
+
 hipError_t launch_trace(const TraceParams& P, hipStream_t s) {
   const uint32_t ntiles = P.tiles_x * P.tiles_y;
   if (ntiles == 0) return hipSuccess;
-  const dim3 block(P.tile_w * P.tile_h);
+  const dim3 block(P.tile_w * P.tile_h), grid(ntiles);
   switch (P.prec) {
-    case VGI_PREC_FP32: hipLaunchKernelGGL(k_trace<VGI_PREC_FP32>, dim3(ntiles), block, 0, s, P); break;
-    case VGI_PREC_FP16: hipLaunchKernelGGL(k_trace<VGI_PREC_FP16>, dim3(ntiles), block, 0, s, P); break;
-    case VGI_PREC_BF16: hipLaunchKernelGGL(k_trace<VGI_PREC_BF16>, dim3(ntiles), block, 0, s, P); break;
+    case VGI_PREC_FP32: launch_prec<VGI_PREC_FP32>(P, grid, block, s); break;
+    case VGI_PREC_FP16: launch_prec<VGI_PREC_FP16>(P, grid, block, s); break;
+    case VGI_PREC_BF16: launch_prec<VGI_PREC_BF16>(P, grid, block, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -96,6 +96,7 @@ struct MipParams {
 
 struct TraceParams {
   const void* level[VGI_MAX_LEVELS];
+  uint64_t level_bytes[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t levels;
   uint32_t n;
@@ -124,6 +125,7 @@ inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u 
 
 // ---- launchers (each returns hipError_t) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_setup_count(const VoxParams& P, hipStream_t s);
 hipError_t launch_emit(const VoxParams& P, hipStream_t s);
 hipError_t launch_worklist(const VoxParams& P, hipStream_t s);

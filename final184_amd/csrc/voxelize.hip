@@ -49,6 +49,22 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
   return hipGetLastError();
 }
 
+__global__ void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  const size_t count = bytes / 16;
+  if (count == 0) return hipSuccess;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src,
+                     count);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
 

@@ -114,6 +114,7 @@ SIGNATURES = [
     ("vgi_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("vgi_memcpy_d2h", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("vgi_memset", C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    ("vgi_memcpy_d2d", C.c_int, [_P, _P, _P, C.c_size_t, _P]),
     ("vgi_stream_create", C.c_int, [_P, C.POINTER(_P)]),
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
     ("vgi_timer_start", C.c_int, [_P, _P]),
@@ -379,6 +380,10 @@ class Context:
         t = Timings()
         _check(lib.vgi_get_timings(self.handle, C.byref(t)))
         return t
+
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None, stream=None) -> None:
+        n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
+        _check(lib.vgi_memcpy_d2d(self.handle, dst.ptr, src.ptr, n, stream))
 
     def stream_create(self):
         s = C.c_void_p()

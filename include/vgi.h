@@ -231,6 +231,10 @@ int vgi_free(vgi_context* ctx, void* ptr);
 int vgi_memcpy_h2d(vgi_context* ctx, void* dst, const void* src, size_t bytes);
 int vgi_memcpy_d2h(vgi_context* ctx, void* dst, const void* src, size_t bytes);
 int vgi_memset(vgi_context* ctx, void* dst, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16 B per lane, grid-stride), async on
+ * the stream; bytes must be a multiple of 16. Also the known-size reference kernel for
+ * calibrating HBM traffic counters. */
+int vgi_memcpy_d2d(vgi_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 int vgi_stream_create(vgi_context* ctx, void** out);
 int vgi_stream_destroy(vgi_context* ctx, void* stream);
 /* Device-side elapsed time between two points on a stream (HIP events). */
