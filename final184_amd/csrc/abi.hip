@@ -81,24 +81,25 @@ struct vgi_scene {
   uint32_t bin_shift = 3;
   uint32_t nbins_axis = 0;
   uint32_t chunk = 64;
-  uint32_t* acc = nullptr;
+  uint32_t* acc = nullptr;   // n^3 x 8 words, brick-major
+  uint32_t* mask = nullptr;  // 16 words per 8^3 brick
   size_t acc_bytes = 0;
   TriRec* tris = nullptr;
+  uint32_t* tri_slot = nullptr;
+  uint32_t* big_list = nullptr;
   size_t tris_cap = 0;
   MeshDev* meshes = nullptr;
   MeshDev* meshes_host = nullptr;
   size_t meshes_cap = 0;
-  uint32_t* bin_count = nullptr;
-  uint32_t* bin_offset = nullptr;
-  uint32_t* bin_cursor = nullptr;
+  uint32_t* bin_small = nullptr;  // bin_small | bin_big | big_cursor | zero_list, nbins each
+  unsigned long long* bin_packed = nullptr;
+  unsigned long long* bin_scan = nullptr;
   size_t bins_cap = 0;
   uint32_t* pairs = nullptr;
   size_t pairs_cap = 0;
-  uint2* work = nullptr;
-  size_t work_cap = 0;
   DevStats* stats = nullptr;
   DevStats* h_stats = nullptr;
-  uint32_t* h_total = nullptr;
+  unsigned long long* h_total = nullptr;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   hipEvent_t staging_done = nullptr;
@@ -113,9 +114,9 @@ struct vgi_scene {
 namespace {
 
 size_t scene_device_bytes(const vgi_scene* s) {
-  size_t b = s->acc_bytes + s->tris_cap * sizeof(TriRec) + s->meshes_cap * sizeof(MeshDev) +
-             s->bins_cap * 3 * sizeof(uint32_t) + s->pairs_cap * sizeof(uint32_t) +
-             s->work_cap * sizeof(uint2) + s->scan_tmp_bytes + sizeof(DevStats);
+  size_t b = s->acc_bytes + s->acc_bytes / 16 + s->tris_cap * (sizeof(TriRec) + 4 * (kSmallBins + 1)) +
+             s->meshes_cap * sizeof(MeshDev) + s->bins_cap * (4 * 4 + 2 * 8) +
+             s->pairs_cap * sizeof(uint32_t) + s->scan_tmp_bytes + sizeof(DevStats);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) b += s->level_bytes[k];
   return b;
 }
@@ -310,7 +311,7 @@ int vgi_scene_create(vgi_context* ctx, vgi_scene** out) {
   s->ctx = ctx;
   if (hipMalloc((void**)&s->stats, sizeof(DevStats)) != hipSuccess ||
       hipHostMalloc((void**)&s->h_stats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&s->h_total, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming) != hipSuccess) {
     vgi_scene_release(s);
     return fail(VGI_E_NOMEM, "scene allocation failed");
@@ -323,8 +324,8 @@ int vgi_scene_release(vgi_scene* s) {
   if (!s) return VGI_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipDeviceSynchronize();
-  void* dev[] = {s->acc, s->tris, s->meshes, s->bin_count, s->bin_offset, s->bin_cursor,
-                 s->pairs, s->work, s->stats, s->scan_tmp};
+  void* dev[] = {s->acc, s->mask, s->tris, s->tri_slot, s->big_list, s->meshes, s->bin_small,
+                 s->bin_packed, s->bin_scan, s->pairs, s->stats, s->scan_tmp};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++)
@@ -368,7 +369,6 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
     HIP_TRY(hipMemcpy(s->h_stats, s->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
     out->triangles_rejected = s->h_stats->rejected;
     out->triangles_degenerate = s->h_stats->degenerate;
-    out->work_items = s->h_stats->work_count;
   }
   return VGI_OK;
 }
@@ -412,39 +412,58 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_
   s->has_voxels = false;
   s->has_mips = false;
 
-  // ---- allocations ----
+  // ---- allocations (grown on demand, kept across calls) ----
   const size_t acc_words = (size_t)n * n * n * kAccWords;
   if (s->acc_bytes != acc_words * 4) {
     if (s->acc) (void)hipFree(s->acc);
+    if (s->mask) (void)hipFree(s->mask);
     s->acc = nullptr;
+    s->mask = nullptr;
     s->acc_bytes = 0;
-    hipError_t e = hipMalloc((void**)&s->acc, acc_words * 4);
-    if (e != hipSuccess) {
+    const size_t bricks = (size_t)(n / 8) * (n / 8) * (n / 8);
+    if (hipMalloc((void**)&s->acc, acc_words * 4) != hipSuccess ||
+        hipMalloc((void**)&s->mask, bricks * 16 * 4) != hipSuccess) {
       (void)hipGetLastError();
       return fail(VGI_E_NOMEM, "accumulator allocation (%zu bytes) failed", acc_words * 4);
     }
     s->acc_bytes = acc_words * 4;
   }
-  TRY(grow(&s->tris, &s->tris_cap, std::max<size_t>(ntri, 1), "triangle records"));
+  const size_t tneed = std::max<size_t>(ntri, 1);
+  if (s->tris_cap < tneed) {
+    void* arrs[3] = {s->tris, s->tri_slot, s->big_list};
+    for (void* a : arrs)
+      if (a) (void)hipFree(a);
+    s->tris = nullptr;
+    s->tri_slot = s->big_list = nullptr;
+    s->tris_cap = 0;
+    const size_t cap = std::max(tneed, (size_t)1024);
+    if (hipMalloc((void**)&s->tris, cap * sizeof(TriRec)) != hipSuccess ||
+        hipMalloc((void**)&s->tri_slot, cap * kSmallBins * 4) != hipSuccess ||
+        hipMalloc((void**)&s->big_list, cap * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "triangle record allocation failed");
+    }
+    s->tris_cap = cap;
+  }
   if (s->bins_cap < (size_t)nbins + 1) {
-    // bin_count, bin_offset and bin_cursor share one capacity
-    uint32_t** arrs[3] = {&s->bin_count, &s->bin_offset, &s->bin_cursor};
-    for (uint32_t** a : arrs) {
-      if (*a) (void)hipFree(*a);
-      *a = nullptr;
-    }
+    if (s->bin_small) (void)hipFree(s->bin_small);
+    if (s->bin_packed) (void)hipFree(s->bin_packed);
+    if (s->bin_scan) (void)hipFree(s->bin_scan);
+    s->bin_small = nullptr;
+    s->bin_packed = s->bin_scan = nullptr;
     s->bins_cap = 0;
-    for (uint32_t** a : arrs) {
-      if (hipMalloc((void**)a, ((size_t)nbins + 1) * 4) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(VGI_E_NOMEM, "bin array allocation failed");
-      }
+    const size_t cap = (size_t)nbins + 1;
+    if (hipMalloc((void**)&s->bin_small, cap * 4 * 4) != hipSuccess ||
+        hipMalloc((void**)&s->bin_packed, cap * 8) != hipSuccess ||
+        hipMalloc((void**)&s->bin_scan, cap * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "bin array allocation failed");
     }
-    s->bins_cap = (size_t)nbins + 1;
+    s->bins_cap = cap;
   }
   {
     size_t need = 0;
-    HIP_TRY(scan_bins(nullptr, &need, s->bin_count, s->bin_offset, nbins + 1, st));
+    HIP_TRY(scan_bins(nullptr, &need, s->bin_packed, s->bin_scan, nbins + 1, st));
     if (need > s->scan_tmp_bytes) {
       if (s->scan_tmp) (void)hipFree(s->scan_tmp);
       s->scan_tmp = nullptr;
@@ -497,44 +516,54 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_
   VoxParams P{};
   P.meshes = s->meshes;
   P.tris = s->tris;
-  P.bin_count = s->bin_count;
-  P.bin_offset = s->bin_offset;
-  P.bin_cursor = s->bin_cursor;
+  P.tri_slot = s->tri_slot;
+  P.big_list = s->big_list;
+  P.bin_small = s->bin_small;
+  P.bin_big = s->bin_small + s->bins_cap;
+  P.big_cursor = s->bin_small + 2 * s->bins_cap;
+  P.zero_list = s->bin_small + 3 * s->bins_cap;
+  P.bin_packed = s->bin_packed;
+  P.bin_scan = s->bin_scan;
   P.acc = s->acc;
+  P.mask = s->mask;
   P.stats = s->stats;
   for (int a = 0; a < 3; a++) P.origin[a] = grid->origin[a];
   P.snap_scale = (float)(n * 256u) / grid->extent;
   P.nmesh = num_meshes;
-  P.ntri = (uint32_t)ntri;
+  P.ntri = num_meshes ? (uint32_t)ntri : 0u;
   P.n = n;
   P.bin_shift = s->bin_shift;
   P.nbins_axis = s->nbins_axis;
   P.chunk = s->chunk;
 
+  const uint32_t persistent = (uint32_t)ctx->num_cus * 4;
   const bool prof = ctx->profiling != 0;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  HIP_TRY(launch_fill_u32(s->bin_count, 0u, nbins + 1, st));
+  // bin_small, bin_big, big_cursor: 3 x bins_cap words; stats
+  HIP_TRY(launch_fill_u32(s->bin_small, 0u, 3 * s->bins_cap, st));
   HIP_TRY(launch_fill_u32((uint32_t*)s->stats, 0u, sizeof(DevStats) / 4, st));
-  if (num_meshes > 0) HIP_TRY(launch_setup_count(P, st));
+  HIP_TRY(launch_setup(P, st));
+  HIP_TRY(launch_big(P, false, persistent, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
-  HIP_TRY(scan_bins(s->scan_tmp, &s->scan_tmp_bytes, s->bin_count, s->bin_offset, nbins + 1, st));
-  HIP_TRY(hipMemcpyAsync(s->h_total, s->bin_offset + nbins, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(launch_bins(P, st));
+  HIP_TRY(scan_bins(s->scan_tmp, &s->scan_tmp_bytes, s->bin_packed, s->bin_scan, nbins + 1, st));
+  HIP_TRY(hipMemcpyAsync(s->h_total, s->bin_scan + nbins, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  const uint32_t total_pairs = s->h_total[0];
+  const unsigned long long tot = s->h_total[0];
+  const uint32_t total_pairs = (uint32_t)tot, total_work = (uint32_t)(tot >> 32);
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
   TRY(grow(&s->pairs, &s->pairs_cap, std::max<size_t>(total_pairs, 1), "bin pairs"));
-  const size_t work_bound = (size_t)std::min<uint64_t>(nbins, total_pairs) + total_pairs / s->chunk + 1;
-  TRY(grow(&s->work, &s->work_cap, work_bound, "work items"));
   P.pairs = s->pairs;
-  P.pairs_cap = (uint32_t)s->pairs_cap;
-  P.work = s->work;
-  HIP_TRY(hipMemcpyAsync(s->bin_cursor, s->bin_offset, (size_t)nbins * 4, hipMemcpyDeviceToDevice, st));
-  if (total_pairs > 0) HIP_TRY(launch_emit(P, st));
-  HIP_TRY(launch_worklist(P, st));
+  P.pairs_cap = (uint32_t)std::min<size_t>(s->pairs_cap, 0xFFFFFFFFu);
+  if (total_pairs > 0) {
+    HIP_TRY(launch_emit_small(P, st));
+    HIP_TRY(launch_big(P, true, persistent, st));
+    HIP_TRY(launch_zero(P, persistent, st));
+  }
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
   const uint32_t tile = std::min(1u << s->bin_shift, 16u);
-  const uint32_t grid_blocks = (uint32_t)std::min<size_t>(work_bound, (size_t)ctx->num_cus * 8);
-  if (total_pairs > 0) HIP_TRY(launch_voxelize(P, tile, grid_blocks, st));
+  const uint32_t grid_blocks = std::min<uint32_t>(total_work, (uint32_t)ctx->num_cus * 8);
+  HIP_TRY(launch_voxelize(P, tile, grid_blocks, total_work, st));
   if (prof) {
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
     ctx->ev_vox = true;
@@ -543,6 +572,7 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_
   s->grid_set = true;
   s->tris_in = ntri;
   s->num_pairs = total_pairs;
+  s->work_items = total_work;
   s->has_voxels = true;
   return VGI_OK;
 }
@@ -583,7 +613,8 @@ int vgi_build_mips(vgi_context* ctx, vgi_scene* s, const vgi_lights* lights, uin
   }
   MipParams P{};
   P.acc = s->acc;
-  P.bin_count = s->bin_count;
+  P.mask = s->mask;
+  P.bin_packed = s->bin_packed;
   P.lut = ctx->d_lut;
   for (uint32_t k = 0; k < g.levels; k++) {
     P.level[k] = s->level[k];
@@ -712,20 +743,23 @@ int vgi_scene_read_voxels(vgi_context* ctx, const vgi_scene* s, uint32_t* host_o
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipDeviceSynchronize());
   const uint32_t n = s->grid.n, nb = s->nbins_axis, bs = s->bin_shift, nbr = n >> 3;
-  std::vector<uint32_t> acc((size_t)n * n * n * kAccWords), cnt((size_t)nb * nb * nb);
+  std::vector<uint32_t> acc((size_t)n * n * n * kAccWords), mask((size_t)nbr * nbr * nbr * 16);
+  std::vector<unsigned long long> packed((size_t)nb * nb * nb);
   HIP_TRY(hipMemcpy(acc.data(), s->acc, acc.size() * 4, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(cnt.data(), s->bin_count, cnt.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mask.data(), s->mask, mask.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(packed.data(), s->bin_packed, packed.size() * 8, hipMemcpyDeviceToHost));
   for (uint32_t z = 0; z < n; z++)
     for (uint32_t y = 0; y < n; y++)
       for (uint32_t x = 0; x < n; x++) {
         uint32_t* o = host_out + ((size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)) * kAccWords;
         const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-        if (cnt[bin] == 0) {
+        const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
+        const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
+        const bool occ = (uint32_t)packed[bin] != 0 && ((mask[(size_t)brick * 16 + (local >> 5)] >> (local & 31u)) & 1u);
+        if (!occ) {
           std::memset(o, 0, kAccWords * 4);
           continue;
         }
-        const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
-        const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
         std::memcpy(o, acc.data() + ((size_t)brick * 512 + local) * kAccWords, kAccWords * 4);
       }
   return VGI_OK;

@@ -96,14 +96,15 @@ __device__ __forceinline__ float4 scale4(float4 a, float s) {
 __device__ float4 inject(const MipParams& P, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-  if (P.bin_count[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((uint32_t)P.bin_packed[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
   const uint32_t nbr = P.n >> 3;
   const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
   const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
+  if (!((P.mask[(size_t)brick * 16u + (local >> 5)] >> (local & 31u)) & 1u))
+    return make_float4(0.f, 0.f, 0.f, 0.f);  // unoccupied voxel: its record is not written
   const uint4* a = (const uint4*)(P.acc + ((size_t)brick * 512u + local) * kAccWords);
   const uint4 s0 = a[0], s1 = a[1];
   const uint32_t c = s0.w;
-  if (c == 0) return make_float4(0.f, 0.f, 0.f, 0.f);
   const float lr = P.lut[(s0.x + c / 2) / c];
   const float lg = P.lut[(s0.y + c / 2) / c];
   const float lb = P.lut[(s0.z + c / 2) / c];

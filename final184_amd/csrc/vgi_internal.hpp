@@ -44,19 +44,26 @@ static_assert(sizeof(TriRec) == 64, "TriRec layout");
 struct DevStats {
   unsigned long long rejected;
   unsigned long long degenerate;
-  uint32_t work_count;
-  uint32_t pad;
+  uint32_t big_count;   // triangles spanning more than kSmallBins bins
+  uint32_t zero_count;  // bins split over several work items
 };
+
+constexpr int kSmallBins = 8;  // a triangle whose bin box has <= 8 bins keeps its slots
 
 struct VoxParams {
   const MeshDev* meshes;
   TriRec* tris;
-  uint32_t* bin_count;   // nbins + 1
-  uint32_t* bin_offset;  // nbins + 1 (exclusive scan of bin_count)
-  uint32_t* bin_cursor;  // nbins
-  uint32_t* pairs;       // triangle ids grouped by bin
-  uint2* work;           // (bin, chunk | nchunks << 16)
-  uint32_t* acc;         // n^3 x 8 words, brick-major (8^3 bricks)
+  uint32_t* tri_slot;              // kSmallBins slots per triangle (small triangles)
+  uint32_t* big_list;              // ids of large triangles
+  uint32_t* bin_small;             // nbins: pairs from small triangles (slot counters)
+  uint32_t* bin_big;               // nbins: pairs from large triangles
+  uint32_t* big_cursor;            // nbins
+  unsigned long long* bin_packed;  // nbins + 1: work items << 32 | pairs
+  unsigned long long* bin_scan;    // nbins + 1: exclusive scan of bin_packed
+  uint32_t* zero_list;             // bins with more than one work item
+  uint32_t* pairs;                 // triangle ids grouped by bin
+  uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
+  uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
   DevStats* stats;
   float origin[3];
   float snap_scale;      // float(n * 256) / extent
@@ -78,7 +85,8 @@ struct LightDev {
 
 struct MipParams {
   const uint32_t* acc;
-  const uint32_t* bin_count;
+  const uint32_t* mask;
+  const unsigned long long* bin_packed;
   const float* lut;  // 256 entries: (i/255)^2.2
   void* level[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
@@ -126,12 +134,15 @@ inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u 
 // ---- launchers (each returns hipError_t) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
-hipError_t launch_setup_count(const VoxParams& P, hipStream_t s);
-hipError_t launch_emit(const VoxParams& P, hipStream_t s);
-hipError_t launch_worklist(const VoxParams& P, hipStream_t s);
-hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, hipStream_t s);
-hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const uint32_t* in, uint32_t* out, uint32_t count,
-                     hipStream_t s);
+hipError_t launch_setup(const VoxParams& P, hipStream_t s);
+hipError_t launch_big(const VoxParams& P, bool emit, uint32_t blocks, hipStream_t s);
+hipError_t launch_bins(const VoxParams& P, hipStream_t s);
+hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
+                     uint32_t count, hipStream_t s);
+hipError_t launch_emit_small(const VoxParams& P, hipStream_t s);
+hipError_t launch_zero(const VoxParams& P, uint32_t blocks, hipStream_t s);
+hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
+                           hipStream_t s);
 hipError_t launch_mips(const MipParams& P, hipStream_t s, uint32_t* produced_top);
 hipError_t launch_trace(const TraceParams& P, hipStream_t s);
 

@@ -5,21 +5,29 @@
 // imageStore with last-writer-wins (main.lua:242-275), issued one draw per primitive by
 // CVoxelizeRenderer::Render (Foreground/Renderer/VoxelizeRenderer.cpp:81-127).
 //
-// Pipeline (all on one stream):
-//   k_setup_count  one thread per triangle: fetch + model transform + snap to 1/256 voxel,
-//                  exact integer normal, dominant axis (VoxelGS rule), quantized normal,
-//                  voxel bounding box; exact box test against every bin (bin = B^3
-//                  voxels) in the box; count (triangle, bin) pairs per bin.
-//   scan_bins      exclusive scan of the per-bin counts (rocPRIM via hipCUB).
-//   k_emit         the same loop again, scattering triangle ids into per-bin lists.
-//   k_worklist     one wave per non-empty bin: split its list into chunks of <= 64
-//                  triangles (work items); zero the bin's accumulator when it has >1 chunk.
-//   k_voxelize     persistent work loop; per item: the chunk's triangle edge/plane
-//                  coefficients go to LDS, columns of the dominant-axis projection are
-//                  spread over the workgroup, each column tests its 1-4 candidate voxels
-//                  with the exact separating-axis test and adds packed 64-bit increments
-//                  into a per-tile LDS accumulator; the tile is then flushed: plain
-//                  vector stores when the bin has one chunk, integer atomics otherwise.
+// Pipeline (one stream; the host reads one 8-byte total between steps 4 and 5):
+//   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
+//                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
+//                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
+//                   (bin = B^3 voxels) tests each bin exactly and takes a slot in every
+//                   overlapped bin with a wave-aggregated atomic (the slot is kept);
+//                   larger triangles go to a list.
+//   2 k_big_count   one workgroup per listed triangle: its bins, tested cooperatively.
+//   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
+//                   packed as (items << 32 | pairs); bins with > 1 item are listed.
+//   4 scan          one 64-bit exclusive scan gives every bin its pair offset and its
+//                   first work item at once.
+//   5 k_emit_small  scatter small triangles to their kept slots (no atomics);
+//     k_emit_big    large triangles append after the small ones of each bin.
+//   6 k_zero        clear the accumulator and occupancy mask of multi-item bins.
+//   7 k_voxelize    persistent loop over work items (bin found by binary search over the
+//                   scanned item offsets); per item: chunk triangles' edge/plane
+//                   coefficients to LDS, columns of the dominant-axis projection spread
+//                   over the workgroup, each column tests its 1-4 candidate voxels with the
+//                   exact separating-axis test and adds packed 64-bit increments into a
+//                   per-tile LDS accumulator; flush: occupancy mask + the records of
+//                   occupied voxels, by plain stores for single-item bins and by integer
+//                   atomics otherwise.
 //
 // All coverage decisions are exact integer arithmetic on snapped vertices, so the
 // voxel set and the integer sums do not depend on order, rounding or thread mapping.
@@ -126,32 +134,61 @@ __device__ __forceinline__ void load_q(const TriRec& R, int64_t (&q)[3][3], int6
   N[2] = e0[0] * e1[1] - e0[1] * e1[0];
 }
 
-// Visit every bin the triangle overlaps (exact test at bin granularity).
-template <typename F>
-__device__ __forceinline__ void for_each_bin(const VoxParams& P, const TriRec& R, F&& f) {
-  int64_t q[3][3], N[3];
-  load_q(R, q, N);
-  const int bs = (int)P.bin_shift;
-  const int64_t d = (int64_t)kFixedOne << bs;
-  const int bx0 = R.lo[0] >> bs, bx1 = R.hi[0] >> bs;
-  const int by0 = R.lo[1] >> bs, by1 = R.hi[1] >> bs;
-  const int bz0 = R.lo[2] >> bs, bz1 = R.hi[2] >> bs;
-  const uint32_t nb = P.nbins_axis;
-  for (int bz = bz0; bz <= bz1; bz++)
-    for (int by = by0; by <= by1; by++)
-      for (int bx = bx0; bx <= bx1; bx++) {
-        const int64_t b[3] = {(int64_t)bx * d, (int64_t)by * d, (int64_t)bz * d};
-        if (sat_box(q, N, b, d)) f((uint32_t)bx + nb * ((uint32_t)by + nb * (uint32_t)bz));
-      }
+__device__ __forceinline__ bool tri_ok(const TriRec& R) {
+  return (R.rgb >> 24) == 0 && R.lo[0] <= R.hi[0] && R.lo[1] <= R.hi[1] && R.lo[2] <= R.hi[2];
+}
+
+// Bin-space bounding box of a triangle record.
+struct BinBox {
+  int x0, y0, z0, nx, ny, nz;
+  __device__ int count() const { return nx * ny * nz; }
+};
+__device__ __forceinline__ BinBox bin_box(const TriRec& R, uint32_t bs) {
+  BinBox b;
+  b.x0 = R.lo[0] >> bs;
+  b.y0 = R.lo[1] >> bs;
+  b.z0 = R.lo[2] >> bs;
+  b.nx = (R.hi[0] >> bs) - b.x0 + 1;
+  b.ny = (R.hi[1] >> bs) - b.y0 + 1;
+  b.nz = (R.hi[2] >> bs) - b.z0 + 1;
+  return b;
+}
+
+// j-th bin of a bin box (j in [0, count)), and whether the triangle overlaps it.
+__device__ __forceinline__ bool bin_candidate(const VoxParams& P, const int64_t (&q)[3][3],
+                                              const int64_t (&N)[3], const BinBox& bb, int j,
+                                              uint32_t& bin) {
+  const int bx = bb.x0 + j % bb.nx, by = bb.y0 + (j / bb.nx) % bb.ny, bz = bb.z0 + j / (bb.nx * bb.ny);
+  const int64_t d = (int64_t)kFixedOne << P.bin_shift;
+  const int64_t b[3] = {(int64_t)bx * d, (int64_t)by * d, (int64_t)bz * d};
+  bin = (uint32_t)bx + P.nbins_axis * ((uint32_t)by + P.nbins_axis * (uint32_t)bz);
+  return sat_box(q, N, b, d);
+}
+
+// Wave-aggregated atomic increment with per-lane keys: every active lane gets a distinct
+// slot in counter[key] (old value + its rank among the lanes sharing the key).
+__device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bool active) {
+  const uint32_t lane = __lane_id();
+  uint64_t pending = __ballot(active);
+  uint32_t slot = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const uint64_t same = __ballot(active && key == lkey) & pending;
+    uint32_t base = 0;
+    if (lane == (uint32_t)leader) base = atomicAdd(&counter[lkey], (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return slot;
 }
 
 // ---------------------------------------------------------------------------
-// setup + per-bin counts
+// 1 setup
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_setup_count(VoxParams P) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.ntri) return;
+__device__ void make_record(const VoxParams& P, uint32_t t, TriRec& R) {
   const MeshDev& M = P.meshes[find_mesh(P.meshes, P.nmesh, t)];
   const uint32_t lt = t - M.tri_base;
   uint32_t vid[3];
@@ -164,7 +201,6 @@ __global__ __launch_bounds__(256) void k_setup_count(VoxParams P) {
   } else {
     vid[0] = lt * 3; vid[1] = lt * 3 + 1; vid[2] = lt * 3 + 2;
   }
-  TriRec R;
   uint32_t status = 0;
   int32_t qi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
@@ -202,7 +238,8 @@ __global__ __launch_bounds__(256) void k_setup_count(VoxParams P) {
   } else {
     rgb = M.base_rgb & 0xFFFFFFu;
   }
-  bool empty = true;
+#pragma unroll
+  for (int a = 0; a < 3; a++) { R.lo[a] = 1; R.hi[a] = 0; }
   if (status == 0) {
     const double nx = (double)N[0], ny = (double)N[1], nz = (double)N[2];
     const double len = __builtin_sqrt((nx * nx + ny * ny) + nz * nz);
@@ -215,68 +252,154 @@ __global__ __launch_bounds__(256) void k_setup_count(VoxParams P) {
     // VoxelGS (main.lua:102-115): X if x>y && x>z; else Y if y>z; Z otherwise.
     if (ax > ay) R.axis = (ax > az) ? 0 : 2;
     else R.axis = (ay > az) ? 1 : 2;
-    empty = false;
+    bool empty = false;
+    int16_t lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      int32_t mn = min(qi[0][a], min(qi[1][a], qi[2][a]));
-      int32_t mx = max(qi[0][a], max(qi[1][a], qi[2][a]));
-      int32_t lo = max((mn - 1) >> 8, 0);               // floor((mn-1)/256)
-      int32_t hi = min(mx >> 8, (int32_t)P.n - 1);      // floor(mx/256)
-      R.lo[a] = (int16_t)lo;
-      R.hi[a] = (int16_t)hi;
-      if (lo > hi) empty = true;
+      const int32_t mn = min(qi[0][a], min(qi[1][a], qi[2][a]));
+      const int32_t mx = max(qi[0][a], max(qi[1][a], qi[2][a]));
+      const int32_t l = max((mn - 1) >> 8, 0);               // floor((mn-1)/256)
+      const int32_t h = min(mx >> 8, (int32_t)P.n - 1);      // floor(mx/256)
+      lo[a] = (int16_t)l;
+      hi[a] = (int16_t)h;
+      empty = empty || l > h;
     }
-  } else {
+    if (!empty)
 #pragma unroll
-    for (int a = 0; a < 3; a++) { R.lo[a] = 1; R.hi[a] = 0; }
-  }
-  if (empty && status == 0) {
-#pragma unroll
-    for (int a = 0; a < 3; a++) { R.lo[a] = 1; R.hi[a] = 0; }
+      for (int a = 0; a < 3; a++) { R.lo[a] = lo[a]; R.hi[a] = hi[a]; }
   }
   R.rgb = rgb | (status << 24);
-  P.tris[t] = R;
-  if (status == 1) { atomicAdd(&P.stats->rejected, 1ull); return; }
-  if (status == 2) { atomicAdd(&P.stats->degenerate, 1ull); return; }
-  if (empty) return;
-  for_each_bin(P, R, [&](uint32_t bin) { atomicAdd(&P.bin_count[bin], 1u); });
 }
 
-hipError_t launch_setup_count(const VoxParams& P, hipStream_t s) {
+__global__ __launch_bounds__(256) void k_setup(VoxParams P) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < P.ntri;
+  TriRec R;
+  bool small = false;
+  BinBox bb{0, 0, 0, 0, 0, 0};
+  int64_t q[3][3], N[3];
+  if (in) {
+    make_record(P, t, R);
+    P.tris[t] = R;
+    const uint32_t status = R.rgb >> 24;
+    if (status == 1) atomicAdd(&P.stats->rejected, 1ull);
+    if (status == 2) atomicAdd(&P.stats->degenerate, 1ull);
+    if (tri_ok(R)) {
+      bb = bin_box(R, P.bin_shift);
+      small = bb.count() <= kSmallBins;
+      if (!small) P.big_list[atomicAdd(&P.stats->big_count, 1u)] = t;
+      else load_q(R, q, N);
+    }
+  }
+  // every lane runs the same 8 slot rounds so the aggregated atomics stay convergent
+  uint32_t nslot = 0;
+#pragma unroll
+  for (int j = 0; j < kSmallBins; j++) {
+    uint32_t bin = 0;
+    const bool hit = small && j < bb.count() && bin_candidate(P, q, N, bb, j, bin);
+    const uint32_t slot = agg_slot(P.bin_small, bin, hit);
+    if (hit) P.tri_slot[(size_t)t * kSmallBins + nslot++] = slot;
+  }
+}
+
+hipError_t launch_setup(const VoxParams& P, hipStream_t s) {
   if (P.ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_count, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_setup, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 
-hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const uint32_t* in, uint32_t* out, uint32_t count,
-                     hipStream_t s) {
+// ---------------------------------------------------------------------------
+// 2 large triangles: count their bins cooperatively
+// ---------------------------------------------------------------------------
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_big(VoxParams P) {
+  const uint32_t nbig = P.stats->big_count;
+  for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+    const uint32_t t = P.big_list[i];
+    const TriRec R = P.tris[t];
+    int64_t q[3][3], N[3];
+    load_q(R, q, N);
+    const BinBox bb = bin_box(R, P.bin_shift);
+    const int nbb = bb.count();
+    for (int j = threadIdx.x; j < nbb; j += blockDim.x) {
+      uint32_t bin;
+      if (!bin_candidate(P, q, N, bb, j, bin)) continue;
+      if (!EMIT) {
+        atomicAdd(&P.bin_big[bin], 1u);
+      } else {
+        const uint32_t pos = (uint32_t)P.bin_scan[bin] + P.bin_small[bin] + atomicAdd(&P.big_cursor[bin], 1u);
+        if (pos < P.pairs_cap) P.pairs[pos] = t;
+      }
+    }
+  }
+}
+
+hipError_t launch_big(const VoxParams& P, bool emit, uint32_t blocks, hipStream_t s) {
+  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 3 per-bin totals, 4 scan
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_bins(VoxParams P) {
+  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  if (b == nbins) {
+    P.bin_packed[b] = 0ull;
+    return;
+  }
+  const uint32_t pairs = P.bin_small[b] + P.bin_big[b];
+  const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
+  P.bin_packed[b] = ((unsigned long long)items << 32) | pairs;
+  if (items > 1) P.zero_list[atomicAdd(&P.stats->zero_count, 1u)] = b;
+}
+
+hipError_t launch_bins(const VoxParams& P, hipStream_t s) {
+  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+  hipLaunchKernelGGL(k_bins, dim3((nbins + 1 + 255) / 256), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
+                     uint32_t count, hipStream_t s) {
   return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
 }
 
 // ---------------------------------------------------------------------------
-// emit (triangle, bin) pairs
+// 5 scatter small triangles into their slots
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_emit(VoxParams P) {
+__global__ __launch_bounds__(256) void k_emit_small(VoxParams P) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.ntri) return;
   const TriRec R = P.tris[t];
-  if ((R.rgb >> 24) != 0) return;
-  if (R.lo[0] > R.hi[0] || R.lo[1] > R.hi[1] || R.lo[2] > R.hi[2]) return;
-  for_each_bin(P, R, [&](uint32_t bin) {
-    uint32_t pos = atomicAdd(&P.bin_cursor[bin], 1u);
+  if (!tri_ok(R)) return;
+  const BinBox bb = bin_box(R, P.bin_shift);
+  if (bb.count() > kSmallBins) return;
+  int64_t q[3][3], N[3];
+  load_q(R, q, N);
+  uint32_t nslot = 0;
+  for (int j = 0; j < bb.count(); j++) {
+    uint32_t bin;
+    if (!bin_candidate(P, q, N, bb, j, bin)) continue;
+    const uint32_t pos = (uint32_t)P.bin_scan[bin] + P.tri_slot[(size_t)t * kSmallBins + nslot++];
     if (pos < P.pairs_cap) P.pairs[pos] = t;
-  });
+  }
 }
 
-hipError_t launch_emit(const VoxParams& P, hipStream_t s) {
+hipError_t launch_emit_small(const VoxParams& P, hipStream_t s) {
   if (P.ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_emit_small, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// work list: one wave per bin
+// 6 clear the bins that several work items add into
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ size_t acc_index(uint32_t x, uint32_t y, uint32_t z, uint32_t nbr) {
@@ -285,37 +408,29 @@ __device__ __forceinline__ size_t acc_index(uint32_t x, uint32_t y, uint32_t z, 
   return ((size_t)brick * 512u + local) * kAccWords;
 }
 
-__global__ __launch_bounds__(64) void k_worklist(VoxParams P) {
-  const uint32_t bin = blockIdx.x;
-  const uint32_t cnt = P.bin_count[bin];
-  if (cnt == 0) return;
-  const uint32_t nch = (cnt + P.chunk - 1) / P.chunk;
-  uint32_t base = 0;
-  if (threadIdx.x == 0) base = atomicAdd(&P.stats->work_count, nch);
-  base = __shfl(base, 0);
-  for (uint32_t i = threadIdx.x; i < nch; i += 64) P.work[base + i] = make_uint2(bin, i);
-  if (nch > 1) {
-    // zero the bin's accumulator bricks (one 8^3 brick = 16 KiB contiguous)
-    const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
+__global__ __launch_bounds__(256) void k_zero(VoxParams P) {
+  const uint32_t nz = P.stats->zero_count;
+  const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
+  for (uint32_t i = blockIdx.x; i < nz; i += gridDim.x) {
+    const uint32_t bin = P.zero_list[i];
     const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
-    const uint32_t bpb = B >> 3;  // bricks per bin edge
     for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
-      const uint32_t x = bx * B + 8 * (k % bpb), y = by * B + 8 * ((k / bpb) % bpb),
-                     z = bz * B + 8 * (k / (bpb * bpb));
-      uint4* dst = (uint4*)(P.acc + acc_index(x, y, z, nbr));
-      for (uint32_t i = threadIdx.x; i < 1024; i += 64) dst[i] = make_uint4(0, 0, 0, 0);
+      const uint32_t bxx = bx * bpb + k % bpb, byy = by * bpb + (k / bpb) % bpb, bzz = bz * bpb + k / (bpb * bpb);
+      const uint32_t brick = bxx + nbr * (byy + nbr * bzz);
+      uint4* dst = (uint4*)(P.acc + (size_t)brick * 512u * kAccWords);
+      for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) dst[j] = make_uint4(0, 0, 0, 0);
+      if (threadIdx.x < 4) ((uint4*)(P.mask + (size_t)brick * 16u))[threadIdx.x] = make_uint4(0, 0, 0, 0);
     }
   }
 }
 
-hipError_t launch_worklist(const VoxParams& P, hipStream_t s) {
-  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  hipLaunchKernelGGL(k_worklist, dim3(nbins), dim3(64), 0, s, P);
+hipError_t launch_zero(const VoxParams& P, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// brick voxelize
+// 7 brick voxelize
 // ---------------------------------------------------------------------------
 
 // Per-triangle coefficients in LDS. Edge function of projection u (up axis), edge i:
@@ -400,23 +515,37 @@ __device__ __forceinline__ bool edge_ok(const TriCoef& C, int u, int va, int vb)
 }
 
 template <int TS>
-__global__ __launch_bounds__(256) void k_voxelize(VoxParams P) {
+__global__ __launch_bounds__(256) void k_voxelize(VoxParams P, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
+  constexpr int BT = (TS / 8) * (TS / 8) * (TS / 8);  // 8^3 accumulator bricks per tile
   __shared__ unsigned long long s_acc[2 * TV];
   __shared__ TriCoef s_tri[kMaxChunk];
   __shared__ uint32_t s_pref[kMaxChunk + 1];
+  __shared__ uint32_t s_mask[BT * 16];
+  __shared__ uint32_t s_bin;
 
   const uint32_t tid = threadIdx.x;
-  const uint32_t nwork = P.stats->work_count;
   const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
+  const uint32_t nbins = nb * nb * nb;
   const uint32_t sub = B / TS;  // sub-tiles per bin edge
 
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = P.work[w];
-    const uint32_t bin = item.x, chunk = item.y;
-    const uint32_t cnt_bin = P.bin_count[bin];
-    const uint32_t nch = (cnt_bin + P.chunk - 1) / P.chunk;
-    const uint32_t first = P.bin_offset[bin] + chunk * P.chunk;
+    if (tid == 0) {
+      // largest bin whose first work item is <= w
+      uint32_t lo = 0, hi = nbins - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)(P.bin_scan[mid] >> 32) <= w) lo = mid;
+        else hi = mid - 1;
+      }
+      s_bin = lo;
+    }
+    __syncthreads();
+    const uint32_t bin = s_bin;
+    const unsigned long long packed = P.bin_packed[bin];
+    const uint32_t cnt_bin = (uint32_t)packed, nch = (uint32_t)(packed >> 32);
+    const uint32_t chunk = w - (uint32_t)(P.bin_scan[bin] >> 32);
+    const uint32_t first = (uint32_t)P.bin_scan[bin] + chunk * P.chunk;
     const uint32_t cnt = min(P.chunk, cnt_bin - chunk * P.chunk);
     const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
 
@@ -431,6 +560,7 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P) {
       const int ty0 = (int)(by * B + TS * ((st / sub) % sub));
       const int tz0 = (int)(bz * B + TS * (st / (sub * sub)));
       for (int i = tid; i < 2 * TV; i += 256) s_acc[i] = 0ull;
+      for (int i = tid; i < BT * 16; i += 256) s_mask[i] = 0u;
       // clip to the tile, count columns of the dominant projection, scan (wave 0)
       if (tid < 64) {
         uint32_t cols = 0;
@@ -497,19 +627,25 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P) {
         }
       }
       __syncthreads();
-      // flush the tile
+      // flush the tile: occupancy mask (per 8^3 brick) + records of occupied voxels
       for (int li = tid; li < TV; li += 256) {
-        const unsigned long long w0 = s_acc[2 * li], w1 = s_acc[2 * li + 1];
+        const unsigned long long w0 = s_acc[2 * li];
+        const uint32_t c = (uint32_t)(w0 >> 48);
+        if (c == 0) continue;
+        const unsigned long long w1 = s_acc[2 * li + 1];
         const uint32_t lx = li % TS, ly = (li / TS) % TS, lz = li / (TS * TS);
+        const uint32_t lb = (lx >> 3) + (TS / 8) * ((ly >> 3) + (TS / 8) * (lz >> 3));
+        const uint32_t bit = (lx & 7u) + 8u * ((ly & 7u) + 8u * (lz & 7u));
+        atomicOr(&s_mask[lb * 16 + (bit >> 5)], 1u << (bit & 31u));
         uint32_t* dst = P.acc + acc_index(tx0 + lx, ty0 + ly, tz0 + lz, nbr);
         const uint32_t r = (uint32_t)(w0 & 0xFFFFu), g = (uint32_t)((w0 >> 16) & 0xFFFFu),
-                       b = (uint32_t)((w0 >> 32) & 0xFFFFu), c = (uint32_t)(w0 >> 48);
+                       b = (uint32_t)((w0 >> 32) & 0xFFFFu);
         const uint32_t nx = (uint32_t)(w1 & 0xFFFFFu), ny = (uint32_t)((w1 >> 20) & 0xFFFFFu),
                        nz = (uint32_t)((w1 >> 40) & 0xFFFFFu);
         if (nch == 1) {
           ((uint4*)dst)[0] = make_uint4(r, g, b, c);
           ((uint4*)dst)[1] = make_uint4(nx, ny, nz, 0u);
-        } else if (c != 0) {
+        } else {
           atomicAdd(dst + 0, r);
           atomicAdd(dst + 1, g);
           atomicAdd(dst + 2, b);
@@ -520,14 +656,25 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P) {
         }
       }
       __syncthreads();
+      for (int i = tid; i < BT * 16; i += 256) {
+        const uint32_t lb = i >> 4;
+        const uint32_t bxx = (tx0 >> 3) + lb % (TS / 8), byy = (ty0 >> 3) + (lb / (TS / 8)) % (TS / 8),
+                       bzz = (tz0 >> 3) + lb / ((TS / 8) * (TS / 8));
+        uint32_t* m = P.mask + (size_t)(bxx + nbr * (byy + nbr * bzz)) * 16u + (i & 15);
+        const uint32_t v = s_mask[i];
+        if (nch == 1) *m = v;
+        else if (v) atomicOr(m, v);
+      }
+      __syncthreads();
     }
   }
 }
 
-hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, hipStream_t s) {
-  if (grid_blocks == 0) return hipSuccess;
-  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, P);
-  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, P);
+hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
+                           hipStream_t s) {
+  if (grid_blocks == 0 || nwork == 0) return hipSuccess;
+  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, P, nwork);
+  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, P, nwork);
   return hipGetLastError();
 }
 
