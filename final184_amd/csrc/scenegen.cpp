@@ -331,7 +331,7 @@ int64_t sg_room(uint64_t seed, uint32_t ntri_total, float dx, float dy, float dz
   // debris
   Rng gd(sub_seed(seed, 3));
   while (s.n < ntri_total) {
-    V3 c{gd.range(0.02 * X, 0.98 * X), gd.range(0.005, 0.12), gd.range(0.02 * Z, 0.98 * Z)};
+    V3 c{gd.range(0.02 * X, 0.98 * X), gd.range(0.008, 0.12), gd.range(0.02 * Z, 0.98 * Z)};
     V3 v0 = c + ball(gd, 0.008), v1 = c + ball(gd, 0.008), v2 = c + ball(gd, 0.008);
     uint8_t col[3] = {gd.byte(), gd.byte(), gd.byte()};
     s.tri(v0, v1, v2, col, (float)gd.range(0.05, 0.5));

@@ -90,7 +90,7 @@ __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, uint32_t tb, 
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
 template <int PREC>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
-                                               float uz, float wl, float4& acc) {
+                                               float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
@@ -109,11 +109,13 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
       const uint32_t b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
+      abits |= a3 | b3;
     } else {
       const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, f.off, soff[q], 0);
       const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
       Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
       Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
+      abits |= (a1 | a3) & 0xFFFF0000u;  // opacity is the high half
     }
   }
 }
@@ -121,7 +123,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 // ---- per-lane level: 64-bit global loads ----
 template <int PREC>
 __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, float ux, float uy,
-                                            float uz, float wl, float4& acc) {
+                                            float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
@@ -137,11 +139,13 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
       const uint4 b = *(const uint4*)(base + offs[q] + 16);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a.x, a.y, a.z, a.w);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
+      abits |= a.w | b.w;
     } else {
       const uint2 a = *(const uint2*)(base + offs[q]);
       const uint2 b = *(const uint2*)(base + offs[q] + 8);
       Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
       Acc<PREC>::add(acc, wxy[q][1] * wz, b.x, b.y);
+      abits |= (a.y | b.y) & 0xFFFF0000u;
     }
   }
 }
@@ -174,21 +178,26 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
       lvl = uniform_f(lvl);
     }
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t abits = 0;
     if (lvl >= top) {
-      if (UNIFORM) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s);
-      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s);
+      if (UNIFORM) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
+      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
       count += 1;
     } else {
       const float fl = floorf(lvl);
       const float fr = lvl - fl;
       uint32_t l0 = (uint32_t)fl;
+      // The coarse footprint (level l0+1) holds the parents of every fine (level l0)
+      // footprint texel; a parent with zero opacity has an all-zero subtree (texels are
+      // box averages of non-negative values and opacity cannot underflow to zero), so
+      // when the coarse footprint's opacity is zero the fine gather is exactly zero.
       if (UNIFORM) {
         l0 = __builtin_amdgcn_readfirstlane(l0);
-        gather_uniform<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s);
-        gather_uniform<PREC>(P, l0 + 1, ux, uy, uz, fr, s);
+        gather_uniform<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
+        if (abits) gather_uniform<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       } else {
-        gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s);
-        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s);
+        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
+        if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       }
       count += 2;
     }
