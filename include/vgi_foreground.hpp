@@ -1,0 +1,219 @@
+// vgi_foreground.hpp — header-only C++ adapter that keeps the reference's pass interface
+// over the vgi C ABI, so a renderer written against tobyc11/Final184's Foreground layer
+// can swap its voxel GI passes for the MI355X path.
+//
+//   Foreground::CVoxelizeRenderer   <- Foreground/Renderer/VoxelizeRenderer.h:14-27
+//       SetRenderPass, RenderList(context, modelMats, primitives),
+//       PreparePrimitiveResources, ClearResourceCache, GarbageCollectResourceCache
+//   Foreground::CIndirectPass       <- the lighting_indirect CMaterial pass,
+//       Foreground/Renderer/MegaPipeline.cpp:252-268 (Shader/Lighting/indirect.frag)
+//   Foreground::CTriangleMesh / CBasicMaterial / CPrimitive
+//                                   <- Foreground/Shape/TriangleMesh.h:26-103,
+//                                      Foreground/Material/BasicMaterial.h,
+//                                      Foreground/SceneGraph/Primitive.h:12-25
+//   tc::Matrix3x4                   <- Math/Matrix3x4.h (row-major 3x4, m00..m23)
+//
+// RHI is an empty submodule in the reference, so buffers here are plain device pointers
+// and the render context is a HIP stream. Error behaviour follows the reference: a pass
+// that cannot run returns without drawing (VoxelizeRenderer.cpp:49-50,87-88); the status
+// of the last call is kept in LastStatus() / LastError() instead of being lost.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "vgi.h"
+
+namespace tc {
+// Row-major 3x4 affine matrix, the layout of the reference's tc::Matrix3x4
+// (m00 m01 m02 m03 / m10 ... / m20 ... m23).
+struct Matrix3x4 {
+  float m[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  static Matrix3x4 Identity() { return Matrix3x4{}; }
+};
+}  // namespace tc
+
+namespace Foreground {
+
+// Stand-in for RHI::IRenderContext: the stream every pass records onto.
+struct CRenderContext {
+  void* Stream = nullptr;  // hipStream_t; NULL = the vgi context's stream
+};
+
+// Stand-in for RHI::CRenderPass::Ref (the voxelize pass owns no render target here).
+struct CRenderPass {};
+
+class CTriangleMesh {
+public:
+  // Vertex positions (float32 x3, device memory) with the byte stride of the binding.
+  void SetPositions(const float* device_positions, uint32_t vertex_count, uint32_t stride = 12) {
+    Positions = device_positions;
+    VertexCount = vertex_count;
+    Stride = stride;
+  }
+  // Index buffer in device memory: VGI_INDEX_U16 or VGI_INDEX_U32 (the reference's
+  // SetIndexBuffer(buffer, format, offset), TriangleMesh.h:57-62); NULL for non-indexed.
+  void SetIndexBuffer(const void* device_indices, vgi_index_type format) {
+    Indices = device_indices;
+    IndexFormat = device_indices ? format : VGI_INDEX_NONE;
+  }
+  // Number of indices (or vertices when non-indexed) drawn, as DrawIndexed(ElementCount).
+  void SetElementCount(uint32_t count) { ElementCount = count; }
+  uint32_t GetElementCount() const { return ElementCount; }
+
+  const float* Positions = nullptr;
+  uint32_t VertexCount = 0;
+  uint32_t Stride = 12;
+  const void* Indices = nullptr;
+  vgi_index_type IndexFormat = VGI_INDEX_NONE;
+  uint32_t ElementCount = 0;
+};
+
+class CBasicMaterial {
+public:
+  // BaseColorFactor (main.lua:169-177); RGB in [0, 1], stored as RGB8.
+  void SetAlbedo(float r, float g, float b, float /*a*/ = 1.0f) {
+    auto q = [](float v) { return (uint8_t)(v <= 0.f ? 0 : v >= 1.f ? 255 : (int)(v * 255.f + 0.5f)); };
+    Albedo[0] = q(r);
+    Albedo[1] = q(g);
+    Albedo[2] = q(b);
+  }
+  // Optional per-triangle RGB8 albedo in device memory (replaces the texture lookup).
+  void SetAlbedoPerTriangle(const uint8_t* device_rgb8) { PerTriangle = device_rgb8; }
+
+  uint8_t Albedo[3] = {255, 255, 255};
+  const uint8_t* PerTriangle = nullptr;
+};
+
+class CPrimitive : public std::enable_shared_from_this<CPrimitive> {
+public:
+  std::shared_ptr<CBasicMaterial> GetMaterial() const { return Material; }
+  void SetMaterial(std::shared_ptr<CBasicMaterial> material) { Material = std::move(material); }
+  std::shared_ptr<CTriangleMesh> GetShape() const { return Shape; }
+  void SetShape(std::shared_ptr<CTriangleMesh> shape) { Shape = std::move(shape); }
+
+private:
+  std::shared_ptr<CBasicMaterial> Material;
+  std::shared_ptr<CTriangleMesh> Shape;
+};
+
+class CVoxelizeRenderer {
+public:
+  // The reference takes the owning CMegaPipeline; here the owner is a vgi context and
+  // the scene that holds the voxel volume, plus the voxel grid (the voxelizer view).
+  CVoxelizeRenderer(vgi_context* ctx, vgi_scene* scene, const vgi_grid& grid)
+      : Ctx(ctx), Scene(scene), Grid(grid) {}
+
+  void SetGrid(const vgi_grid& grid) { Grid = grid; }
+
+  // Kept for interface parity; voxelization needs no render pass here.
+  void SetRenderPass(const CRenderPass& /*renderPass*/, uint32_t /*subpass*/ = 0) {}
+
+  // One voxelize pass over the visible primitives with their model matrices
+  // (VoxelizeRenderer.cpp:19-30): clears the volume and voxelizes every primitive in one
+  // batched launch on the context's stream.
+  void RenderList(CRenderContext& context, const std::vector<tc::Matrix3x4>& modelMats,
+                  const std::vector<CPrimitive*>& primitives) {
+    GarbageCollectResourceCache();
+    Meshes.clear();
+    const size_t count = std::min(modelMats.size(), primitives.size());
+    for (size_t i = 0; i < count; i++) {
+      CPrimitive* prim = primitives[i];
+      if (!prim) continue;
+      auto it = Cache.find(prim->weak_from_this());
+      if (it == Cache.end()) {
+        PreparePrimitiveResources(prim->shared_from_this());
+        it = Cache.find(prim->weak_from_this());
+        if (it == Cache.end()) continue;  // not drawable: skipped, as the reference does
+      }
+      vgi_mesh m = it->second;
+      for (int k = 0; k < 12; k++) m.model[k] = modelMats[i].m[k];
+      Meshes.push_back(m);
+    }
+    Status = vgi_voxelize(Ctx, Scene, Meshes.data(), (uint32_t)Meshes.size(), &Grid, context.Stream);
+    if (Status != VGI_OK) Error = vgi_last_error();
+  }
+
+  // Builds the cached draw description of a primitive (VoxelizeRenderer.cpp:32-67).
+  void PreparePrimitiveResources(std::shared_ptr<CPrimitive> primitive) {
+    if (!primitive) return;
+    auto shape = primitive->GetShape();
+    auto mat = primitive->GetMaterial();
+    if (!shape || !shape->Positions) return;
+    vgi_mesh m{};
+    m.positions = shape->Positions;
+    m.position_stride = shape->Stride;
+    m.num_vertices = shape->VertexCount;
+    m.indices = shape->Indices;
+    m.index_type = shape->IndexFormat;
+    m.num_triangles = shape->ElementCount / 3;
+    if (mat) {
+      m.albedo_rgb8 = mat->PerTriangle;
+      m.base_color[0] = mat->Albedo[0];
+      m.base_color[1] = mat->Albedo[1];
+      m.base_color[2] = mat->Albedo[2];
+    } else {
+      m.base_color[0] = m.base_color[1] = m.base_color[2] = 255;
+    }
+    Cache[primitive] = m;
+  }
+
+  void ClearResourceCache() { Cache.clear(); }
+
+  // Drops entries of primitives that no longer exist (erase returns the next iterator).
+  void GarbageCollectResourceCache() {
+    for (auto it = Cache.begin(); it != Cache.end();) {
+      if (it->first.expired()) it = Cache.erase(it);
+      else ++it;
+    }
+  }
+
+  size_t CachedPrimitives() const { return Cache.size(); }
+  int LastStatus() const { return Status; }
+  const std::string& LastError() const { return Error; }
+
+private:
+  vgi_context* Ctx;
+  vgi_scene* Scene;
+  vgi_grid Grid;
+  std::map<std::weak_ptr<CPrimitive>, vgi_mesh, std::owner_less<std::weak_ptr<CPrimitive>>> Cache;
+  std::vector<vgi_mesh> Meshes;
+  int Status = VGI_OK;
+  std::string Error;
+};
+
+// The lighting_indirect pass: direct light injected into the volume, the mip chain, and
+// the cone trace of every G-buffer pixel into the indirect buffer.
+class CIndirectPass {
+public:
+  CIndirectPass(vgi_context* ctx, vgi_scene* scene) : Ctx(ctx), Scene(scene) {}
+
+  void SetLights(const vgi_lights& lights) { Lights = lights; }
+  void SetCones(const vgi_cone_params& cones) { Cones = cones; }
+  void SetPrecision(vgi_precision p) { Precision = p; }
+
+  // out_diffuse / out_specular: float4 per pixel, device memory (specular may be NULL).
+  void Render(CRenderContext& context, const vgi_gbuffer& gbuffer, float* out_diffuse,
+              float* out_specular) {
+    Status = vgi_build_mips(Ctx, Scene, &Lights, (uint32_t)Precision, context.Stream);
+    if (Status == VGI_OK)
+      Status = vgi_trace(Ctx, Scene, &gbuffer, &Cones, out_diffuse, out_specular, context.Stream);
+    if (Status != VGI_OK) Error = vgi_last_error();
+  }
+
+  int LastStatus() const { return Status; }
+  const std::string& LastError() const { return Error; }
+
+private:
+  vgi_context* Ctx;
+  vgi_scene* Scene;
+  vgi_lights Lights{};
+  vgi_cone_params Cones{6, 1, 60.f, 60.f, 10.f, 0.5f, 1.f, 0.99f, 1.f, 1.f, {0.f, 0.f, 0.f}};
+  vgi_precision Precision = VGI_PREC_FP16;
+  int Status = VGI_OK;
+  std::string Error;
+};
+
+}  // namespace Foreground

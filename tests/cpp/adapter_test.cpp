@@ -1,0 +1,161 @@
+// Drives include/vgi_foreground.hpp the way Foreground/Renderer/MegaPipeline.cpp drives its
+// passes (RenderList with per-primitive model matrices, then the indirect pass).
+//   adapter_test nodevice : cache and error behaviour without a GPU
+//   adapter_test gpu      : two primitives voxelized through the adapter == the oracle
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "vgi_foreground.hpp"
+
+extern "C" {
+struct orc_vox_stats { uint64_t a, b, c, d; };
+int orc_voxelize(const float*, uint32_t, const uint32_t*, uint32_t, const uint8_t*, const uint8_t*,
+                 const float*, const float*, float, uint32_t, uint32_t*, orc_vox_stats*);
+}
+
+#define CHECK(c)                                                      \
+  do {                                                                \
+    if (!(c)) {                                                       \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c);         \
+      return 1;                                                       \
+    }                                                                 \
+  } while (0)
+
+using namespace Foreground;
+
+static int nodevice() {
+  vgi_grid g{{0, 0, 0}, 1.0f, 32, 5};
+  CVoxelizeRenderer vr(nullptr, nullptr, g);
+  vr.SetRenderPass(CRenderPass{});
+  auto mesh = std::make_shared<CTriangleMesh>();
+  static float fake[9] = {0};
+  mesh->SetPositions(fake, 3);
+  mesh->SetElementCount(3);
+  auto prim = std::make_shared<CPrimitive>();
+  prim->SetShape(mesh);
+  prim->SetMaterial(std::make_shared<CBasicMaterial>());
+  {
+    auto dead = std::make_shared<CPrimitive>();
+    dead->SetShape(mesh);
+    vr.PreparePrimitiveResources(dead);
+    vr.PreparePrimitiveResources(prim);
+    CHECK(vr.CachedPrimitives() == 2);
+  }
+  vr.GarbageCollectResourceCache();  // the expired primitive goes, the live one stays
+  CHECK(vr.CachedPrimitives() == 1);
+  CRenderContext ctx;
+  vr.RenderList(ctx, {tc::Matrix3x4::Identity()}, {prim.get()});
+  CHECK(vr.LastStatus() == VGI_E_INVALID);
+  CHECK(!vr.LastError().empty());
+  vr.ClearResourceCache();
+  CHECK(vr.CachedPrimitives() == 0);
+  CIndirectPass ip(nullptr, nullptr);
+  vgi_gbuffer gb{};
+  ip.Render(ctx, gb, nullptr, nullptr);
+  CHECK(ip.LastStatus() == VGI_E_INVALID);
+  std::printf("ADAPTER NODEVICE OK\n");
+  return 0;
+}
+
+static void* dev_copy(vgi_context* c, const void* src, size_t bytes) {
+  void* p = nullptr;
+  if (vgi_malloc(c, &p, bytes) != VGI_OK) return nullptr;
+  vgi_memcpy_h2d(c, p, src, bytes);
+  return p;
+}
+
+static int gpu() {
+  vgi_context* c = nullptr;
+  CHECK(vgi_create(&c, 0) == VGI_OK);
+  vgi_scene* s = nullptr;
+  CHECK(vgi_scene_create(c, &s) == VGI_OK);
+  const uint32_t n = 32;
+  vgi_grid g{{-1.f, -1.f, -1.f}, 2.0f, n, 5};
+  std::mt19937 rng(7);
+  std::uniform_real_distribution<float> U(-0.6f, 0.6f);
+  // primitive A: 200 shared vertices, 300 u16-indexed triangles, constant albedo
+  std::vector<float> va(600);
+  for (auto& v : va) v = U(rng);
+  std::vector<uint16_t> ia(900);
+  for (auto& i : ia) i = (uint16_t)(rng() % 200);
+  // primitive B: 150 non-indexed triangles with per-triangle albedo
+  std::vector<float> vb(1350);
+  for (size_t i = 0; i < vb.size(); i++) vb[i] = U(rng) * 0.5f + (i % 3 == 0 ? 0.2f : 0.f);
+  std::vector<uint8_t> ab(450);
+  for (auto& a : ab) a = (uint8_t)(rng() & 255);
+  auto meshA = std::make_shared<CTriangleMesh>();
+  meshA->SetPositions((const float*)dev_copy(c, va.data(), va.size() * 4), 200);
+  meshA->SetIndexBuffer(dev_copy(c, ia.data(), ia.size() * 2), VGI_INDEX_U16);
+  meshA->SetElementCount(900);
+  auto matA = std::make_shared<CBasicMaterial>();
+  matA->SetAlbedo(0.2f, 0.4f, 0.8f);
+  auto meshB = std::make_shared<CTriangleMesh>();
+  meshB->SetPositions((const float*)dev_copy(c, vb.data(), vb.size() * 4), 450);
+  meshB->SetElementCount(450);
+  auto matB = std::make_shared<CBasicMaterial>();
+  matB->SetAlbedoPerTriangle((const uint8_t*)dev_copy(c, ab.data(), ab.size()));
+  auto pA = std::make_shared<CPrimitive>(), pB = std::make_shared<CPrimitive>();
+  pA->SetShape(meshA);
+  pA->SetMaterial(matA);
+  pB->SetShape(meshB);
+  pB->SetMaterial(matB);
+  tc::Matrix3x4 mA, mB;
+  const float cs = std::cos(0.4f), sn = std::sin(0.4f);
+  float ma[12] = {cs, -sn, 0, 0.1f, sn, cs, 0, 0, 0, 0, 1, -0.05f};
+  std::memcpy(mA.m, ma, sizeof(ma));
+  float mb[12] = {1.1f, 0, 0, 0, 0, 0.9f, 0, 0.1f, 0, 0, 1, 0};
+  std::memcpy(mB.m, mb, sizeof(mb));
+
+  CVoxelizeRenderer vr(c, s, g);
+  CRenderContext ctx;
+  vr.RenderList(ctx, {mA, mB}, {pA.get(), pB.get()});
+  CHECK(vr.LastStatus() == VGI_OK);
+  std::vector<uint32_t> got((size_t)n * n * n * 8), ra(got.size()), rb(got.size());
+  CHECK(vgi_scene_read_voxels(c, s, got.data()) == VGI_OK);
+  std::vector<uint32_t> ia32(ia.begin(), ia.end());
+  const uint8_t baseA[3] = {matA->Albedo[0], matA->Albedo[1], matA->Albedo[2]};
+  orc_vox_stats st;
+  orc_voxelize(va.data(), 200, ia32.data(), 300, nullptr, baseA, ma, g.origin, g.extent, n, ra.data(), &st);
+  orc_voxelize(vb.data(), 450, nullptr, 150, ab.data(), nullptr, mb, g.origin, g.extent, n, rb.data(), &st);
+  size_t bad = 0, occ = 0;
+  for (size_t i = 0; i < got.size(); i++) {
+    bad += got[i] != ra[i] + rb[i];
+    if (i % 8 == 3 && got[i]) occ++;
+  }
+  std::printf("occupied voxels %zu, mismatched words %zu\n", occ, bad);
+  CHECK(bad == 0 && occ > 100);
+
+  // indirect pass on a tiny G-buffer
+  const uint32_t W = 16, H = 16;
+  std::vector<float> gp(W * H * 3), gn(W * H * 3);
+  for (uint32_t i = 0; i < W * H; i++) {
+    gp[i * 3] = U(rng) * 0.5f; gp[i * 3 + 1] = U(rng) * 0.5f; gp[i * 3 + 2] = U(rng) * 0.5f;
+    gn[i * 3] = 0; gn[i * 3 + 1] = 0; gn[i * 3 + 2] = 1;
+  }
+  vgi_gbuffer gb{(const float*)dev_copy(c, gp.data(), gp.size() * 4),
+                 (const float*)dev_copy(c, gn.data(), gn.size() * 4), nullptr, W, H};
+  float* od = nullptr;
+  CHECK(vgi_malloc(c, (void**)&od, W * H * 16) == VGI_OK);
+  vgi_lights L{};
+  L.count = 1;
+  L.light[0] = vgi_light{VGI_LIGHT_DIRECTIONAL, {0.f, 0.f, -1.f}, {1.f, 1.f, 1.f}, 0.f};
+  CIndirectPass ip(c, s);
+  ip.SetLights(L);
+  ip.Render(ctx, gb, od, nullptr);
+  CHECK(ip.LastStatus() == VGI_OK);
+  std::vector<float> out(W * H * 4);
+  CHECK(vgi_memcpy_d2h(c, out.data(), od, out.size() * 4) == VGI_OK);
+  for (uint32_t i = 0; i < W * H; i++) CHECK(out[i * 4 + 3] >= 0.f && out[i * 4 + 3] <= 1.f);
+  vgi_scene_release(s);
+  vgi_destroy(c);
+  std::printf("ADAPTER GPU OK\n");
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) return gpu();
+  return nodevice();
+}
