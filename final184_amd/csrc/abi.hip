@@ -97,6 +97,8 @@ struct vgi_scene {
   size_t bins_cap = 0;
   uint32_t* pairs = nullptr;
   size_t pairs_cap = 0;
+  uint32_t* work_bin = nullptr;
+  size_t work_cap = 0;
   DevStats* stats = nullptr;
   DevStats* h_stats = nullptr;
   unsigned long long* h_total = nullptr;
@@ -116,7 +118,7 @@ namespace {
 size_t scene_device_bytes(const vgi_scene* s) {
   size_t b = s->acc_bytes + s->acc_bytes / 16 + s->tris_cap * (sizeof(TriRec) + 4 * (kSmallBins + 1)) +
              s->meshes_cap * sizeof(MeshDev) + s->bins_cap * (4 * 4 + 2 * 8) +
-             s->pairs_cap * sizeof(uint32_t) + s->scan_tmp_bytes + sizeof(DevStats);
+             (s->pairs_cap + s->work_cap) * sizeof(uint32_t) + s->scan_tmp_bytes + sizeof(DevStats);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) b += s->level_bytes[k];
   return b;
 }
@@ -325,7 +327,7 @@ int vgi_scene_release(vgi_scene* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipDeviceSynchronize();
   void* dev[] = {s->acc, s->mask, s->tris, s->tri_slot, s->big_list, s->meshes, s->bin_small,
-                 s->bin_packed, s->bin_scan, s->pairs, s->stats, s->scan_tmp};
+                 s->bin_packed, s->bin_scan, s->pairs, s->work_bin, s->stats, s->scan_tmp};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++)
@@ -553,9 +555,12 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_
   const uint32_t total_pairs = (uint32_t)tot, total_work = (uint32_t)(tot >> 32);
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
   TRY(grow(&s->pairs, &s->pairs_cap, std::max<size_t>(total_pairs, 1), "bin pairs"));
+  TRY(grow(&s->work_bin, &s->work_cap, std::max<size_t>(total_work, 1), "work items"));
   P.pairs = s->pairs;
+  P.work_bin = s->work_bin;
   P.pairs_cap = (uint32_t)std::min<size_t>(s->pairs_cap, 0xFFFFFFFFu);
   if (total_pairs > 0) {
+    HIP_TRY(launch_workmap(P, st));
     HIP_TRY(launch_emit_small(P, st));
     HIP_TRY(launch_big(P, true, persistent, st));
     HIP_TRY(launch_zero(P, persistent, st));

@@ -93,7 +93,7 @@ __device__ __forceinline__ float4 scale4(float4 a, float s) {
 }
 
 // Radiance of level-0 voxel (x, y, z) from the accumulator.
-__device__ float4 inject(const MipParams& P, uint32_t x, uint32_t y, uint32_t z) {
+__device__ float4 inject(const MipParams& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
   if ((uint32_t)P.bin_packed[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
@@ -105,9 +105,9 @@ __device__ float4 inject(const MipParams& P, uint32_t x, uint32_t y, uint32_t z)
   const uint4* a = (const uint4*)(P.acc + ((size_t)brick * 512u + local) * kAccWords);
   const uint4 s0 = a[0], s1 = a[1];
   const uint32_t c = s0.w;
-  const float lr = P.lut[(s0.x + c / 2) / c];
-  const float lg = P.lut[(s0.y + c / 2) / c];
-  const float lb = P.lut[(s0.z + c / 2) / c];
+  const float lr = lut[(s0.x + c / 2) / c];
+  const float lg = lut[(s0.y + c / 2) / c];
+  const float lb = lut[(s0.z + c / 2) / c];
   const float inv = 1.0f / ((float)c * 1024.0f);
   const int32_t bias = (int32_t)(1024u * c);
   const float nx = (float)((int32_t)s1.x - bias) * inv;
@@ -158,12 +158,17 @@ __global__ __launch_bounds__(512) void k_mips(MipParams P, uint32_t nout) {
   const uint32_t tx = tile % tpa, ty = (tile / tpa) % tpa, tz = tile / (tpa * tpa);
   const uint32_t G = S >> 1;  // groups per tile edge (>= 1 when nout >= 1)
   const uint32_t g = threadIdx.x;
+  __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
+  if (SRC_ACC) {
+    if (g < 256) s_lut[g] = P.lut[g];
+    __syncthreads();
+  }
 
   if (SRC_ACC && nout == 0) {
     // single-level chain: resolve level 0 only, one voxel per thread
     for (uint32_t v = g; v < S * S * S; v += 512) {
       const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
-      TX::store(P.level[0], padded_index(x, y, z, ns), TX::round(inject(P, x, y, z)));
+      TX::store(P.level[0], padded_index(x, y, z, ns), TX::round(inject(P, s_lut, x, y, z)));
     }
     return;
   }
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(512) void k_mips(MipParams P, uint32_t nout) {
           const uint32_t x = tx * S + 2 * gx + dx, y = ty * S + 2 * gy + dy, z = tz * S + 2 * gz + dz;
           float4 c;
           if (SRC_ACC) {
-            c = TX::round(inject(P, x, y, z));
+            c = TX::round(inject(P, s_lut, x, y, z));
             TX::store(P.level[0], padded_index(x, y, z, ns), c);
           } else {
             c = TX::load(P.level[src], padded_index(x, y, z, ns));

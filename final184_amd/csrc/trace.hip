@@ -61,23 +61,26 @@ template <> struct Acc<VGI_PREC_BF16> {  // RGB bf16, opacity fp16
 template <int PREC> constexpr uint32_t kTexelBytes = (PREC == VGI_PREC_FP32) ? 16u : 8u;
 
 // Trilinear footprint of level k at level-0 voxel coordinates u (texel centres at i+0.5):
-// byte offset of texel (x0, y0, z0) in the padded level, and the 8 weights scaled by wl.
+// index of texel (x0, y0, z0) in the padded level, and the 8 weights scaled by wl.
 struct Foot {
-  uint32_t off;
+  uint32_t idx;
   float w00, w10, w01, w11;  // xy weights
   float wz0, wz1;            // z weights x level weight
 };
 
-__device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, uint32_t tb, float ux, float uy,
-                                          float uz, float wl) {
+__device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, float uy, float uz,
+                                          float wl) {
   const float sc = __builtin_ldexpf(1.0f, -(int)k);
   const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
-  const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
-  const float fx = gx - fx0, fy = gy - fy0, fz = gz - fz0;
+  // integer part by a flooring convert, fraction by v_fract (x - floor(x))
+  const int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+  const float fx = __builtin_amdgcn_fractf(gx), fy = __builtin_amdgcn_fractf(gy),
+              fz = __builtin_amdgcn_fractf(gz);
   const float ex = 1.0f - fx, ey = 1.0f - fy;
   Foot f;
-  // +1: padded border
-  f.off = ((uint32_t)((int)fx0 + 1) + p1 * ((uint32_t)((int)fy0 + 1) + p1 * (uint32_t)((int)fz0 + 1))) * tb;
+  // +1: padded border; every factor is < 2^11, so 24-bit multiplies are exact
+  const uint32_t row = __umul24(p1, (uint32_t)(iz + 1)) + (uint32_t)(iy + 1);
+  f.idx = __umul24(p1, row) + (uint32_t)(ix + 1);  // < 1026^3 < 2^31
   f.w00 = ex * ey;
   f.w10 = fx * ey;
   f.w01 = ex * fy;
@@ -96,22 +99,23 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)P.level_bytes[k], 0x00020000);
-  const Foot f = footprint(k, p1, TB, ux, uy, uz, wl);
+  const Foot f = footprint(k, p1, ux, uy, uz, wl);
+  const uint32_t off = f.idx * TB;  // levels on this path are < 2^31 bytes
   const int soff[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const float wz = q < 2 ? f.wz0 : f.wz1;
     if constexpr (PREC == VGI_PREC_FP32) {
-      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, f.off, soff[q], 0);
-      const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, f.off + 16, soff[q], 0);
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
+      const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, soff[q], 0);
       const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
       const uint32_t b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
       abits |= a3 | b3;
     } else {
-      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, f.off, soff[q], 0);
+      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
       const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
       Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
       Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
@@ -127,8 +131,8 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
-  const Foot f = footprint(k, p1, TB, ux, uy, uz, wl);
-  const char* base = (const char*)P.level[k] + f.off;
+  const Foot f = footprint(k, p1, ux, uy, uz, wl);
+  const char* base = (const char*)P.level[k] + (size_t)f.idx * TB;
   const size_t offs[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
 #pragma unroll
@@ -166,10 +170,14 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
   const float top = (float)(P.levels - 1);
   // the step cap bounds the loop even when t stops growing in float (zero aperture,
   // zero direction, huge max distance); no workload comes near it
+  // cone origin and direction in level-0 voxel units: u(t) = uo + ud t
+  const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
+  const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
+  const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
   for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= P.max_dist; it++) {
-    const float ux = ((ox + dx * t) - P.origin[0]) * P.inv_h;
-    const float uy = ((oy + dy * t) - P.origin[1]) * P.inv_h;
-    const float uz = ((oz + dz * t) - P.origin[2]) * P.inv_h;
+    const float ux = fmaf(udx, t, uox);
+    const float uy = fmaf(udy, t, uoy);
+    const float uz = fmaf(udz, t, uoz);
     if (!(ux >= 0.f && ux <= nf && uy >= 0.f && uy <= nf && uz >= 0.f && uz <= nf)) break;
     float diam = fmaxf(P.h, 2.0f * t * tan_half);
     float lvl = fmaxf(0.0f, __log2f(diam * P.inv_h));

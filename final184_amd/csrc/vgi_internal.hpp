@@ -61,6 +61,7 @@ struct VoxParams {
   unsigned long long* bin_packed;  // nbins + 1: work items << 32 | pairs
   unsigned long long* bin_scan;    // nbins + 1: exclusive scan of bin_packed
   uint32_t* zero_list;             // bins with more than one work item
+  uint32_t* work_bin;              // bin of every work item
   uint32_t* pairs;                 // triangle ids grouped by bin
   uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
   uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
@@ -139,6 +140,7 @@ hipError_t launch_big(const VoxParams& P, bool emit, uint32_t blocks, hipStream_
 hipError_t launch_bins(const VoxParams& P, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
+hipError_t launch_workmap(const VoxParams& P, hipStream_t s);
 hipError_t launch_emit_small(const VoxParams& P, hipStream_t s);
 hipError_t launch_zero(const VoxParams& P, uint32_t blocks, hipStream_t s);
 hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,

@@ -166,12 +166,15 @@ __device__ __forceinline__ bool bin_candidate(const VoxParams& P, const int64_t 
 }
 
 // Wave-aggregated atomic increment with per-lane keys: every active lane gets a distinct
-// slot in counter[key] (old value + its rank among the lanes sharing the key).
+// slot in counter[key] (old value + its rank among the lanes sharing the key). Up to two
+// shared keys are served with one atomic each (neighbouring triangles of a mesh usually
+// share bins); lanes left over take one atomic each, so the number of dependent atomic
+// round trips per call stays at three when keys are all distinct.
 __device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bool active) {
   const uint32_t lane = __lane_id();
   uint64_t pending = __ballot(active);
   uint32_t slot = 0;
-  while (pending) {
+  for (int round = 0; round < 2 && pending; round++) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const uint32_t lkey = __shfl(key, leader);
     const uint64_t same = __ballot(active && key == lkey) & pending;
@@ -181,6 +184,7 @@ __device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bo
     if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
     pending &= ~same;
   }
+  if ((pending >> lane) & 1ull) slot = atomicAdd(&counter[key], 1u);
   return slot;
 }
 
@@ -291,10 +295,10 @@ __global__ __launch_bounds__(256) void k_setup(VoxParams P) {
       else load_q(R, q, N);
     }
   }
-  // every lane runs the same 8 slot rounds so the aggregated atomics stay convergent
+  // every lane runs the same slot rounds so the aggregated atomics stay convergent
   uint32_t nslot = 0;
-#pragma unroll
   for (int j = 0; j < kSmallBins; j++) {
+    if (!__ballot(small && j < bb.count())) break;  // wave-uniform exit
     uint32_t bin = 0;
     const bool hit = small && j < bb.count() && bin_candidate(P, q, N, bb, j, bin);
     const uint32_t slot = agg_slot(P.bin_small, bin, hit);
@@ -368,6 +372,22 @@ hipError_t launch_bins(const VoxParams& P, hipStream_t s) {
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s) {
   return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
+}
+
+// work item -> bin, from the scanned item offsets
+__global__ __launch_bounds__(256) void k_workmap(VoxParams P) {
+  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const uint32_t items = (uint32_t)(P.bin_packed[b] >> 32);
+  const uint32_t first = (uint32_t)(P.bin_scan[b] >> 32);
+  for (uint32_t i = 0; i < items; i++) P.work_bin[first + i] = b;
+}
+
+hipError_t launch_workmap(const VoxParams& P, hipStream_t s) {
+  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+  hipLaunchKernelGGL(k_workmap, dim3((nbins + 255) / 256), dim3(256), 0, s, P);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -522,26 +542,13 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P, uint32_t nwork) {
   __shared__ TriCoef s_tri[kMaxChunk];
   __shared__ uint32_t s_pref[kMaxChunk + 1];
   __shared__ uint32_t s_mask[BT * 16];
-  __shared__ uint32_t s_bin;
 
   const uint32_t tid = threadIdx.x;
   const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
-  const uint32_t nbins = nb * nb * nb;
   const uint32_t sub = B / TS;  // sub-tiles per bin edge
 
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    if (tid == 0) {
-      // largest bin whose first work item is <= w
-      uint32_t lo = 0, hi = nbins - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if ((uint32_t)(P.bin_scan[mid] >> 32) <= w) lo = mid;
-        else hi = mid - 1;
-      }
-      s_bin = lo;
-    }
-    __syncthreads();
-    const uint32_t bin = s_bin;
+    const uint32_t bin = P.work_bin[w];
     const unsigned long long packed = P.bin_packed[bin];
     const uint32_t cnt_bin = (uint32_t)packed, nch = (uint32_t)(packed >> 32);
     const uint32_t chunk = w - (uint32_t)(P.bin_scan[bin] >> 32);
