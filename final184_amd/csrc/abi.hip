@@ -1,18 +1,29 @@
 // abi.hip — the C ABI of include/vgi.h: contexts, scenes (device memory owners),
-// the voxelize / mips / trace passes and parity readback.
+// per-stream workspaces, the voxelize / mips / trace passes (single scenes and batches)
+// and parity readback.
 //
 // Ownership follows the reference: CMegaPipeline owns the voxel image
 // (Foreground/Renderer/MegaPipeline.cpp:470-490, MegaPipeline.h:60,72) and the passes
-// borrow the caller's GPU buffers for the duration of a draw. Here a vgi_scene owns the
-// accumulator, the bin/sort scratch and the mip levels; meshes, G-buffers and outputs are
-// caller device pointers borrowed on the given stream. Errors never throw across the
-// ABI: every entry returns a vgi_status and sets a thread-local message, replacing the
-// reference's silent early returns (VoxelizeRenderer.cpp:49-50,87-88).
+// borrow the caller's GPU buffers for the duration of a draw. Here a vgi_scene owns what
+// stays resident between passes (accumulator, occupancy mask, bin table, mip levels);
+// everything a voxelize call needs only while it runs (triangle records, bin counters,
+// pair lists, the work map, scan scratch) and the staging of per-call descriptor tables
+// belong to a workspace of the stream the call runs on, so calls on different streams
+// never share scratch. Meshes, G-buffers and outputs are caller device pointers borrowed
+// on the given stream. Errors never throw across the ABI: every entry returns a
+// vgi_status and sets a thread-local message, replacing the reference's silent early
+// returns (VoxelizeRenderer.cpp:49-50,87-88).
+//
+// A single-scene call is a batch of one: every pass builds its descriptor table(s) on the
+// host, uploads them with one copy from a pinned staging slot, and launches the
+// batch-native kernels.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -43,9 +54,9 @@ int fail(int code, const char* fmt, ...) {
                   #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
   } while (0)
 
-#define TRY(expr)          \
-  do {                     \
-    int r_ = (expr);       \
+#define TRY(expr)                \
+  do {                           \
+    int r_ = (expr);             \
     if (r_ != VGI_OK) return r_; \
   } while (0)
 
@@ -55,8 +66,88 @@ uint32_t ilog2(uint32_t v) {
   while ((1u << (r + 1)) <= v) r++;
   return r;
 }
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Device allocation grown on demand (content is not preserved). The caller has made sure
+// no queued work still reads the old block.
+int grow_bytes(void** p, size_t* cap, size_t need, const char* what) {
+  if (*p && *cap >= need) return VGI_OK;
+  const size_t n = std::max(need, *cap + *cap / 2);
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc(p, std::max<size_t>(n, 256));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(VGI_E_NOMEM, "device allocation of %zu bytes for %s failed: %s", n, what, hipGetErrorString(e));
+  }
+  *cap = n;
+  return VGI_OK;
+}
+
+// Pinned staging for descriptor tables: a ring of slots per stream. A slot's host half is
+// reused once the copy that read it has completed (its event); its device half is reused
+// four calls later on the same stream, when the kernels that read it have run.
+constexpr int kSlots = 4;
+struct Slot {
+  uint8_t* h = nullptr;
+  uint8_t* d = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+};
+
+// Carves aligned sub-ranges out of one staging slot.
+struct Carve {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  }
+};
 
 }  // namespace
+
+// Per-stream workspace: voxelize scratch + staging ring. Calls on one stream are
+// serialized by its mutex.
+struct vgi_workspace {
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  Slot slot[kSlots];
+  uint32_t next = 0;
+  void* tris = nullptr;       // TriRec per triangle
+  size_t tris_bytes = 0;
+  void* tri_slot = nullptr;   // kSmallBins words per triangle
+  size_t tri_slot_bytes = 0;
+  void* big_list = nullptr;   // one word per triangle
+  size_t big_list_bytes = 0;
+  void* zeroed = nullptr;     // bin_small | bin_big | big_cursor | DevStats[] | totals[]
+  size_t zeroed_bytes = 0;
+  void* zero_list = nullptr;
+  size_t zero_list_bytes = 0;
+  void* packed_all = nullptr;
+  size_t packed_bytes = 0;
+  void* scan_all = nullptr;
+  size_t scan_bytes = 0;
+  void* work_map = nullptr;
+  size_t work_bytes = 0;
+  void* pairs = nullptr;
+  size_t pairs_bytes = 0;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  uint8_t* h_read = nullptr;  // pinned readback of DevStats[] + totals[]
+  size_t h_read_cap = 0;
+
+  size_t device_bytes() const {
+    size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
+               scan_bytes + work_bytes + pairs_bytes + scan_tmp_bytes;
+    for (const Slot& s : slot) b += s.cap;
+    return b;
+  }
+};
 
 struct vgi_context {
   int device = 0;
@@ -71,72 +162,107 @@ struct vgi_context {
   bool ev_vox = false, ev_mips = false, ev_trace = false;
   hipEvent_t tstart = nullptr, tstop = nullptr;
   vgi_timings timings{};
-  std::mutex mu;
+  std::mutex mu;  // tuning, the workspace map
+  std::map<hipStream_t, std::unique_ptr<vgi_workspace>> ws;
 };
 
 struct vgi_scene {
   vgi_context* ctx = nullptr;
   vgi_grid grid{};
-  bool grid_set = false;
   uint32_t bin_shift = 3;
   uint32_t nbins_axis = 0;
   uint32_t chunk = 64;
-  uint32_t* acc = nullptr;   // n^3 x 8 words, brick-major
-  uint32_t* mask = nullptr;  // 16 words per 8^3 brick
+  uint32_t* acc = nullptr;  // n^3 x 8 words, brick-major
+  uint32_t* mask = nullptr; // 16 words per 8^3 brick
   size_t acc_bytes = 0;
-  TriRec* tris = nullptr;
-  uint32_t* tri_slot = nullptr;
-  uint32_t* big_list = nullptr;
-  size_t tris_cap = 0;
-  MeshDev* meshes = nullptr;
-  MeshDev* meshes_host = nullptr;
-  size_t meshes_cap = 0;
-  uint32_t* bin_small = nullptr;  // bin_small | bin_big | big_cursor | zero_list, nbins each
-  unsigned long long* bin_packed = nullptr;
-  unsigned long long* bin_scan = nullptr;
-  size_t bins_cap = 0;
-  uint32_t* pairs = nullptr;
-  size_t pairs_cap = 0;
-  uint32_t* work_bin = nullptr;
-  size_t work_cap = 0;
-  DevStats* stats = nullptr;
-  DevStats* h_stats = nullptr;
-  unsigned long long* h_total = nullptr;
-  void* scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
-  hipEvent_t staging_done = nullptr;
+  size_t mask_bytes = 0;
+  void* bin_packed = nullptr;  // items << 32 | pairs per bin
+  size_t bin_packed_bytes = 0;
   uint32_t prec = 0;
   void* level[VGI_MAX_LEVELS] = {};
   size_t level_bytes[VGI_MAX_LEVELS] = {};
   uint32_t alloc_levels = 0, alloc_n = 0, alloc_prec = 0;
   bool has_voxels = false, has_mips = false;
-  uint64_t tris_in = 0, num_pairs = 0, work_items = 0;
+  uint64_t tris_in = 0, num_pairs = 0, work_items = 0, rejected = 0, degenerate = 0;
 };
 
 namespace {
 
 size_t scene_device_bytes(const vgi_scene* s) {
-  size_t b = s->acc_bytes + s->acc_bytes / 16 + s->tris_cap * (sizeof(TriRec) + 4 * (kSmallBins + 1)) +
-             s->meshes_cap * sizeof(MeshDev) + s->bins_cap * (4 * 4 + 2 * 8) +
-             (s->pairs_cap + s->work_cap) * sizeof(uint32_t) + s->scan_tmp_bytes + sizeof(DevStats);
+  size_t b = s->acc_bytes + s->mask_bytes + s->bin_packed_bytes;
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) b += s->level_bytes[k];
   return b;
 }
 
-template <typename T>
-int grow(T** p, size_t* cap, size_t need, const char* what) {
-  if (*cap >= need && *p) return VGI_OK;
-  size_t n = std::max(need, *cap + *cap / 2);
-  if (*p) (void)hipFree(*p);
-  *p = nullptr;
-  *cap = 0;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(VGI_E_NOMEM, "device allocation of %zu bytes for %s failed: %s", n * sizeof(T), what,
-                hipGetErrorString(e));
+void free_workspace(vgi_workspace* w) {
+  void* dev[] = {w->tris, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all,
+                 w->scan_all, w->work_map, w->pairs, w->scan_tmp};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  for (Slot& s : w->slot) {
+    if (s.d) (void)hipFree(s.d);
+    if (s.h) (void)hipHostFree(s.h);
+    if (s.done) (void)hipEventDestroy(s.done);
   }
-  *cap = n;
+  if (w->h_read) (void)hipHostFree(w->h_read);
+}
+
+hipStream_t pick(vgi_context* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
+
+// The workspace of a stream, created on first use.
+vgi_workspace* workspace(vgi_context* ctx, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->ws.find(st);
+  if (it != ctx->ws.end()) return it->second.get();
+  auto w = std::make_unique<vgi_workspace>();
+  w->stream = st;
+  for (Slot& s : w->slot)
+    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      free_workspace(w.get());
+      return nullptr;
+    }
+  vgi_workspace* raw = w.get();
+  ctx->ws.emplace(st, std::move(w));
+  return raw;
+}
+
+vgi_tuning tuning_of(vgi_context* ctx) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return ctx->tuning;
+}
+
+// Takes the next staging slot with room for `bytes`; waits for the copy that last used it.
+int acquire_slot(vgi_workspace* w, size_t bytes, Slot** out) {
+  Slot& s = w->slot[w->next];
+  w->next = (w->next + 1) % kSlots;
+  if (s.pending) {
+    HIP_TRY(hipEventSynchronize(s.done));
+    s.pending = false;
+  }
+  if (s.cap < bytes) {
+    HIP_TRY(hipStreamSynchronize(w->stream));  // the device half may still be read
+    if (s.d) (void)hipFree(s.d);
+    if (s.h) (void)hipHostFree(s.h);
+    s.d = nullptr;
+    s.h = nullptr;
+    s.cap = 0;
+    const size_t cap = std::max<size_t>(align_up(bytes + bytes / 2, 4096), 65536);
+    if (hipMalloc((void**)&s.d, cap) != hipSuccess ||
+        hipHostMalloc((void**)&s.h, cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "staging allocation of %zu bytes failed", cap);
+    }
+    s.cap = cap;
+  }
+  *out = &s;
+  return VGI_OK;
+}
+
+int upload(vgi_workspace* w, Slot* s, size_t offset, size_t bytes) {
+  if (bytes) HIP_TRY(hipMemcpyAsync(s->d + offset, s->h + offset, bytes, hipMemcpyHostToDevice, w->stream));
+  HIP_TRY(hipEventRecord(s->done, w->stream));
+  s->pending = true;
   return VGI_OK;
 }
 
@@ -153,8 +279,61 @@ int check_grid(const vgi_grid* g) {
   return VGI_OK;
 }
 
-hipStream_t pick(vgi_context* ctx, void* stream) {
-  return stream ? (hipStream_t)stream : ctx->stream;
+int check_meshes(const vgi_mesh* meshes, uint32_t num_meshes, uint64_t* ntri_out) {
+  if (num_meshes > 0 && !meshes) return fail(VGI_E_INVALID, "meshes is NULL");
+  uint64_t ntri = 0;
+  for (uint32_t i = 0; i < num_meshes; i++) {
+    const vgi_mesh& m = meshes[i];
+    if (m.num_triangles == 0) continue;
+    if (!m.positions) return fail(VGI_E_INVALID, "mesh %u: positions is NULL", i);
+    const uint32_t stride = m.position_stride ? m.position_stride : 12u;
+    if (stride < 12 || (stride & 3)) return fail(VGI_E_INVALID, "mesh %u: position_stride %u", i, stride);
+    if (m.index_type > VGI_INDEX_U32) return fail(VGI_E_INVALID, "mesh %u: index_type %u", i, m.index_type);
+    if (m.index_type != VGI_INDEX_NONE && !m.indices) return fail(VGI_E_INVALID, "mesh %u: indices is NULL", i);
+    if (m.index_type == VGI_INDEX_NONE && (uint64_t)m.num_triangles * 3 > m.num_vertices)
+      return fail(VGI_E_INVALID, "mesh %u: non-indexed mesh needs 3 vertices per triangle", i);
+    for (int k = 0; k < 12; k++)
+      if (!std::isfinite(m.model[k])) return fail(VGI_E_INVALID, "mesh %u: model matrix not finite", i);
+    ntri += m.num_triangles;
+  }
+  if (ntri >= (1ull << 31)) return fail(VGI_E_RANGE, "%llu triangles exceed 2^31", (unsigned long long)ntri);
+  *ntri_out = ntri;
+  return VGI_OK;
+}
+
+// Resident allocations of a voxelized scene (accumulator, mask, bin table).
+int scene_reserve(vgi_scene* s, uint32_t n, uint32_t nbins) {
+  const size_t acc_bytes = (size_t)n * n * n * kAccWords * 4;
+  const size_t bricks = (size_t)(n / 8) * (n / 8) * (n / 8);
+  if (s->acc_bytes != acc_bytes) {
+    if (s->acc || s->mask) HIP_TRY(hipDeviceSynchronize());
+    if (s->acc) (void)hipFree(s->acc);
+    if (s->mask) (void)hipFree(s->mask);
+    s->acc = nullptr;
+    s->mask = nullptr;
+    s->acc_bytes = s->mask_bytes = 0;
+    if (hipMalloc((void**)&s->acc, acc_bytes) != hipSuccess ||
+        hipMalloc((void**)&s->mask, bricks * 16 * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "accumulator allocation (%zu bytes) failed", acc_bytes);
+    }
+    s->acc_bytes = acc_bytes;
+    s->mask_bytes = bricks * 16 * 4;
+  }
+  if (s->bin_packed_bytes < (size_t)nbins * 8) {
+    if (s->bin_packed) {
+      HIP_TRY(hipDeviceSynchronize());
+      (void)hipFree(s->bin_packed);
+    }
+    s->bin_packed = nullptr;
+    s->bin_packed_bytes = 0;
+    if (hipMalloc(&s->bin_packed, (size_t)nbins * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "bin table allocation failed");
+    }
+    s->bin_packed_bytes = (size_t)nbins * 8;
+  }
+  return VGI_OK;
 }
 
 // Host-side storage decode (readback only).
@@ -178,11 +357,551 @@ float host_bf16(uint16_t b) {
   return f;
 }
 
+bool distinct_scenes(std::vector<const vgi_scene*> v) {
+  std::sort(v.begin(), v.end());
+  return std::adjacent_find(v.begin(), v.end()) == v.end();
+}
+
+// ---------------------------------------------------------------------------
+// voxelize batch
+// ---------------------------------------------------------------------------
+
+struct VoxPlan {
+  vgi_scene* s;
+  const vgi_voxelize_item* it;
+  uint64_t ntri;
+  uint32_t n, bin_shift, nbins_axis, nbins;
+  size_t bin_base, tri_off, mesh_off;
+  uint32_t tri_blocks, bin_blocks;
+};
+
+int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t count, void* stream) {
+  if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
+  if (count == 0) return VGI_OK;
+  if (!items) return fail(VGI_E_INVALID, "items is NULL");
+  const vgi_tuning tn = tuning_of(ctx);
+  std::vector<VoxPlan> plan(count);
+  std::vector<const vgi_scene*> seen(count);
+  size_t total_bins = 0, total_tri = 0, total_mesh = 0, nblk_t = 0, nblk_b = 0;
+  uint32_t ts = 16;
+  for (uint32_t i = 0; i < count; i++) {
+    const vgi_voxelize_item& it = items[i];
+    if (!it.scene) return fail(VGI_E_INVALID, "item %u: scene is NULL", i);
+    if (it.scene->ctx != ctx) return fail(VGI_E_INVALID, "item %u: scene belongs to another context", i);
+    TRY(check_grid(&it.grid));
+    VoxPlan& p = plan[i];
+    p.s = it.scene;
+    p.it = &it;
+    TRY(check_meshes(it.meshes, it.num_meshes, &p.ntri));
+    p.n = it.grid.n;
+    p.bin_shift = ilog2(std::min(tn.bin_size, p.n));
+    p.nbins_axis = p.n >> p.bin_shift;
+    p.nbins = p.nbins_axis * p.nbins_axis * p.nbins_axis;
+    p.bin_base = total_bins;
+    p.tri_off = total_tri;
+    p.mesh_off = total_mesh;
+    p.tri_blocks = ceil_div(p.ntri, kBlockTris);
+    p.bin_blocks = ceil_div(p.nbins, kBlockBins);
+    total_bins += p.nbins;
+    total_tri += p.ntri;
+    total_mesh += it.num_meshes;
+    nblk_t += p.tri_blocks;
+    nblk_b += p.bin_blocks;
+    ts = std::min(ts, 1u << p.bin_shift);
+    seen[i] = it.scene;
+  }
+  if (!distinct_scenes(seen)) return fail(VGI_E_INVALID, "a scene appears twice in one batch");
+  if (total_bins + 1 >= (1ull << 32) || nblk_t >= (1ull << 31) || nblk_b >= (1ull << 31))
+    return fail(VGI_E_RANGE, "batch too large (%zu bins, %zu triangles)", total_bins, total_tri);
+
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipStream_t st = pick(ctx, stream);
+  vgi_workspace* w = workspace(ctx, st);
+  if (!w) return fail(VGI_E_HIP, "workspace creation failed");
+  std::lock_guard<std::mutex> wl(w->mu);
+
+  for (VoxPlan& p : plan) {
+    p.s->has_voxels = false;
+    p.s->has_mips = false;
+    TRY(scene_reserve(p.s, p.n, p.nbins));
+  }
+
+  // ---- scratch sized for this batch (grown; queued work on this stream is drained
+  // first so nothing still reads a block being replaced) ----
+  const size_t stats_off = align_up(3 * total_bins * 4, 256);
+  const size_t totals_off = stats_off + (size_t)count * sizeof(DevStats);
+  const size_t zeroed_need = totals_off + ((size_t)count + 1) * 8;
+  const size_t read_bytes = zeroed_need - stats_off;
+  size_t scan_need = 0;
+  HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(total_bins + 1), st));
+  const size_t tneed = std::max<size_t>(total_tri, 1);
+  const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
+                    w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need;
+  if (grow) HIP_TRY(hipStreamSynchronize(st));
+  TRY(grow_bytes(&w->tris, &w->tris_bytes, tneed * sizeof(TriRec), "triangle records"));
+  TRY(grow_bytes(&w->tri_slot, &w->tri_slot_bytes, tneed * kSmallBins * 4, "triangle slots"));
+  TRY(grow_bytes(&w->big_list, &w->big_list_bytes, tneed * 4, "large-triangle list"));
+  TRY(grow_bytes(&w->zeroed, &w->zeroed_bytes, zeroed_need, "bin counters"));
+  TRY(grow_bytes(&w->zero_list, &w->zero_list_bytes, std::max<size_t>(total_bins, 1) * 4, "bin list"));
+  TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
+  TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
+  TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
+  if (w->h_read_cap < read_bytes) {
+    if (w->h_read) (void)hipHostFree(w->h_read);
+    w->h_read = nullptr;
+    w->h_read_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&w->h_read, read_bytes * 2, hipHostMallocDefault));
+    w->h_read_cap = read_bytes * 2;
+  }
+
+  // ---- descriptor tables: scenes | meshes | triangle blocks | bin blocks ----
+  const bool multi = count > 1;
+  Carve cv;
+  const size_t o_scenes = cv.take((size_t)count * sizeof(VoxScene));
+  const size_t o_meshes = cv.take(std::max<size_t>(total_mesh, 1) * sizeof(MeshDev));
+  const size_t o_tblk = cv.take(multi ? nblk_t * sizeof(uint2) : 0);
+  const size_t o_bblk = cv.take(multi ? nblk_b * sizeof(uint2) : 0);
+  Slot* sl = nullptr;
+  TRY(acquire_slot(w, cv.off, &sl));
+  VoxScene* hs = (VoxScene*)(sl->h + o_scenes);
+  MeshDev* hm = (MeshDev*)(sl->h + o_meshes);
+  uint2* htb = (uint2*)(sl->h + o_tblk);
+  uint2* hbb = (uint2*)(sl->h + o_bblk);
+  uint32_t* zb = (uint32_t*)w->zeroed;
+  DevStats* dstats = (DevStats*)((uint8_t*)w->zeroed + stats_off);
+  unsigned long long* dtotals = (unsigned long long*)((uint8_t*)w->zeroed + totals_off);
+  size_t tb = 0, bb = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    const VoxPlan& p = plan[i];
+    const vgi_voxelize_item& it = *p.it;
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < it.num_meshes; k++) {
+      const vgi_mesh& m = it.meshes[k];
+      MeshDev& d = hm[p.mesh_off + k];
+      std::memset(&d, 0, sizeof(d));
+      d.pos = m.positions;
+      d.idx = m.indices;
+      d.albedo = m.albedo_rgb8;
+      d.stride = m.position_stride ? m.position_stride : 12u;
+      d.nverts = m.num_vertices;
+      d.idx_type = m.index_type;
+      d.ntri = m.num_triangles;
+      d.tri_base = base;
+      d.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) | ((uint32_t)m.base_color[2] << 16);
+      std::memcpy(d.m, m.model, sizeof(d.m));
+      base += m.num_triangles;
+    }
+    VoxScene& S = hs[i];
+    std::memset(&S, 0, sizeof(S));
+    S.meshes = (const MeshDev*)(sl->d + o_meshes) + p.mesh_off;
+    S.tris = (TriRec*)w->tris + p.tri_off;
+    S.tri_slot = (uint32_t*)w->tri_slot + p.tri_off * kSmallBins;
+    S.big_list = (uint32_t*)w->big_list + p.tri_off;
+    S.bin_small = zb + p.bin_base;
+    S.bin_big = zb + total_bins + p.bin_base;
+    S.big_cursor = zb + 2 * total_bins + p.bin_base;
+    S.zero_list = (uint32_t*)w->zero_list + p.bin_base;
+    S.bin_packed = (unsigned long long*)p.s->bin_packed;
+    S.pairs = nullptr;
+    S.acc = p.s->acc;
+    S.mask = p.s->mask;
+    S.stats = dstats + i;
+    for (int a = 0; a < 3; a++) S.origin[a] = it.grid.origin[a];
+    S.snap_scale = (float)(p.n * 256u) / it.grid.extent;
+    S.nmesh = it.num_meshes;
+    S.ntri = (uint32_t)p.ntri;
+    S.n = p.n;
+    S.bin_shift = p.bin_shift;
+    S.nbins_axis = p.nbins_axis;
+    S.chunk = tn.chunk_tris;
+    S.pairs_cap = 0;
+    S.bin_base = (uint32_t)p.bin_base;
+    if (multi) {
+      for (uint32_t b = 0; b < p.tri_blocks; b++) htb[tb++] = make_uint2(i, b * kBlockTris);
+      for (uint32_t b = 0; b < p.bin_blocks; b++) hbb[bb++] = make_uint2(i, b * kBlockBins);
+    }
+  }
+  TRY(upload(w, sl, 0, cv.off));
+
+  VoxBatch B{};
+  B.scenes = (const VoxScene*)(sl->d + o_scenes);
+  B.tri_blocks = multi ? (const uint2*)(sl->d + o_tblk) : nullptr;
+  B.bin_blocks = multi ? (const uint2*)(sl->d + o_bblk) : nullptr;
+  B.packed_all = (unsigned long long*)w->packed_all;
+  B.scan_all = (unsigned long long*)w->scan_all;
+  B.totals = dtotals;
+  B.work_map = nullptr;
+  B.nscenes = count;
+  B.total_bins = (uint32_t)total_bins;
+
+  const uint32_t persistent = (uint32_t)ctx->num_cus * 4;
+  const bool prof = ctx->profiling != 0;
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
+  // bin_small, bin_big, big_cursor and the per-scene counters start at zero
+  HIP_TRY(launch_fill_u32(zb, 0u, totals_off / 4, st));
+  HIP_TRY(launch_setup(B, (uint32_t)nblk_t, st));
+  HIP_TRY(launch_big(B, false, persistent, st));
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
+  HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
+  HIP_TRY(launch_totals(B, st));
+  HIP_TRY(hipMemcpyAsync(w->h_read, dstats, read_bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
+
+  // ---- size the pair lists and the work map from the per-scene totals ----
+  const DevStats* hst = (const DevStats*)w->h_read;
+  const unsigned long long* htot = (const unsigned long long*)(w->h_read + (totals_off - stats_off));
+  uint64_t pairs_sum = 0;
+  std::vector<size_t> pair_off(count);
+  for (uint32_t i = 0; i < count; i++) {
+    pair_off[i] = (size_t)pairs_sum;
+    pairs_sum += hst[i].pairs;
+  }
+  if (pairs_sum >= (1ull << 32))
+    return fail(VGI_E_RANGE, "batch produces %llu (triangle, bin) pairs; the limit is 2^32 - split the batch",
+                (unsigned long long)pairs_sum);
+  const uint32_t total_work = (uint32_t)(htot[count] >> 32);
+  TRY(grow_bytes(&w->pairs, &w->pairs_bytes, std::max<uint64_t>(pairs_sum, 1) * 4, "bin pairs"));
+  TRY(grow_bytes(&w->work_map, &w->work_bytes, std::max<size_t>(total_work, 1) * sizeof(uint2), "work map"));
+  for (uint32_t i = 0; i < count; i++) {
+    hs[i].pairs = (uint32_t*)w->pairs + pair_off[i];
+    hs[i].pairs_cap = (uint32_t)hst[i].pairs;
+  }
+  TRY(upload(w, sl, o_scenes, (size_t)count * sizeof(VoxScene)));
+  B.work_map = (uint2*)w->work_map;
+  if (pairs_sum > 0) {
+    HIP_TRY(launch_workmap(B, (uint32_t)nblk_b, st));
+    HIP_TRY(launch_emit_small(B, (uint32_t)nblk_t, st));
+    HIP_TRY(launch_big(B, true, persistent, st));
+    HIP_TRY(launch_zero(B, persistent, st));
+  }
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
+  const uint32_t grid_blocks = std::min<uint32_t>(total_work, (uint32_t)ctx->num_cus * 8);
+  HIP_TRY(launch_voxelize(B, ts, grid_blocks, total_work, st));
+  if (prof) {
+    HIP_TRY(hipEventRecord(ctx->ev[4], st));
+    ctx->ev_vox = true;
+  }
+  for (uint32_t i = 0; i < count; i++) {
+    vgi_scene* s = plan[i].s;
+    s->grid = items[i].grid;
+    s->bin_shift = plan[i].bin_shift;
+    s->nbins_axis = plan[i].nbins_axis;
+    s->chunk = tn.chunk_tris;
+    s->tris_in = plan[i].ntri;
+    s->num_pairs = hst[i].pairs;
+    s->work_items = (uint32_t)(htot[i] >> 32);
+    s->rejected = hst[i].rejected;
+    s->degenerate = hst[i].degenerate;
+    s->has_voxels = true;
+  }
+  return VGI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mips batch
+// ---------------------------------------------------------------------------
+
+int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
+  const vgi_grid& g = s->grid;
+  if (s->alloc_n == g.n && s->alloc_levels == g.levels && s->alloc_prec == precision) return VGI_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) {
+    if (s->level[k]) (void)hipFree(s->level[k]);
+    s->level[k] = nullptr;
+    s->level_bytes[k] = 0;
+  }
+  s->alloc_n = s->alloc_levels = 0;
+  for (uint32_t k = 0; k < g.levels; k++) {
+    const size_t p = (size_t)(g.n >> k) + 2;
+    const size_t bytes = p * p * p * texel_bytes(precision);
+    hipError_t e = hipMalloc(&s->level[k], bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(VGI_E_NOMEM, "mip level %u allocation (%zu bytes) failed", k, bytes);
+    }
+    s->level_bytes[k] = bytes;
+    HIP_TRY(launch_fill_u32((uint32_t*)s->level[k], 0u, bytes / 4, st));  // zero border, written once
+  }
+  s->alloc_n = g.n;
+  s->alloc_levels = g.levels;
+  s->alloc_prec = precision;
+  return VGI_OK;
+}
+
+int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uint32_t precision, void* stream) {
+  if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
+  if (count == 0) return VGI_OK;
+  if (!items) return fail(VGI_E_INVALID, "items is NULL");
+  if (precision > VGI_PREC_BF16) return fail(VGI_E_INVALID, "precision %u", precision);
+  std::vector<const vgi_scene*> seen(count);
+  for (uint32_t i = 0; i < count; i++) {
+    const vgi_mips_item& it = items[i];
+    if (!it.scene) return fail(VGI_E_INVALID, "item %u: scene is NULL", i);
+    if (it.scene->ctx != ctx) return fail(VGI_E_INVALID, "item %u: scene belongs to another context", i);
+    if (!it.scene->has_voxels) return fail(VGI_E_NOT_READY, "item %u: scene has not been voxelized", i);
+    if (it.lights) {
+      if (it.lights->count > VGI_MAX_LIGHTS)
+        return fail(VGI_E_RANGE, "item %u: %u lights exceed %u", i, it.lights->count, VGI_MAX_LIGHTS);
+      for (uint32_t l = 0; l < it.lights->count; l++)
+        if (it.lights->light[l].type > VGI_LIGHT_POINT)
+          return fail(VGI_E_INVALID, "item %u: light %u: type %u", i, l, it.lights->light[l].type);
+    }
+    seen[i] = it.scene;
+  }
+  if (!distinct_scenes(seen)) return fail(VGI_E_INVALID, "a scene appears twice in one batch");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipStream_t st = pick(ctx, stream);
+  vgi_workspace* w = workspace(ctx, st);
+  if (!w) return fail(VGI_E_HIP, "workspace creation failed");
+  std::lock_guard<std::mutex> wl(w->mu);
+  for (uint32_t i = 0; i < count; i++) TRY(scene_levels(items[i].scene, precision, st));
+
+  // Stage plan: stage 0 resolves the accumulator and builds up to four levels per 16^3
+  // tile; each later stage continues every unfinished chain from its last level.
+  std::vector<std::vector<MipBlock>> stages;
+  for (uint32_t i = 0; i < count; i++) {
+    const vgi_grid& g = items[i].scene->grid;
+    uint32_t src = 0;
+    for (uint32_t j = 0;; j++) {
+      const uint32_t ns = g.n >> src;
+      const uint32_t S = std::min(ns, 16u);
+      const uint32_t nout = std::min(ilog2(S), g.levels - 1 - src);
+      if (j > 0 && nout == 0) break;
+      if (stages.size() <= j) stages.emplace_back();
+      const uint32_t tpa = ns / S;
+      for (uint32_t t = 0; t < tpa * tpa * tpa; t++) stages[j].push_back(MipBlock{i, t, src, nout});
+      src += nout;
+      if (src >= g.levels - 1 || nout == 0) break;
+    }
+  }
+  size_t nblocks = 0;
+  for (const auto& v : stages) nblocks += v.size();
+  Carve cv;
+  const size_t o_scenes = cv.take((size_t)count * sizeof(MipScene));
+  const size_t o_blocks = cv.take(nblocks * sizeof(MipBlock));
+  Slot* sl = nullptr;
+  TRY(acquire_slot(w, cv.off, &sl));
+  MipScene* hs = (MipScene*)(sl->h + o_scenes);
+  for (uint32_t i = 0; i < count; i++) {
+    const vgi_scene* s = items[i].scene;
+    const vgi_grid& g = s->grid;
+    MipScene& P = hs[i];
+    std::memset(&P, 0, sizeof(P));
+    P.acc = s->acc;
+    P.mask = s->mask;
+    P.bin_packed = (const unsigned long long*)s->bin_packed;
+    for (uint32_t k = 0; k < g.levels; k++) {
+      P.level[k] = s->level[k];
+      P.nk[k] = g.n >> k;
+    }
+    P.n = g.n;
+    P.levels = g.levels;
+    P.bin_shift = s->bin_shift;
+    P.nbins_axis = s->nbins_axis;
+    for (int a = 0; a < 3; a++) P.origin[a] = g.origin[a];
+    P.h = g.extent / (float)g.n;
+    const vgi_lights* L = items[i].lights;
+    P.nlights = L ? L->count : 0;
+    for (uint32_t l = 0; l < P.nlights; l++) {
+      P.lights[l].type = L->light[l].type;
+      for (int a = 0; a < 3; a++) {
+        P.lights[l].vec[a] = L->light[l].vec[a];
+        P.lights[l].color[a] = L->light[l].color[a];
+      }
+      P.lights[l].min_distance = L->light[l].min_distance;
+    }
+  }
+  MipBlock* hb = (MipBlock*)(sl->h + o_blocks);
+  size_t off = 0;
+  for (const auto& v : stages) {
+    std::memcpy(hb + off, v.data(), v.size() * sizeof(MipBlock));
+    off += v.size();
+  }
+  TRY(upload(w, sl, 0, cv.off));
+  const bool prof = ctx->profiling != 0;
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[5], st));
+  off = 0;
+  for (size_t j = 0; j < stages.size(); j++) {
+    MipBatch B{};
+    B.scenes = (const MipScene*)(sl->d + o_scenes);
+    B.blocks = (const MipBlock*)(sl->d + o_blocks) + off;
+    B.lut = ctx->d_lut;
+    HIP_TRY(launch_mips(B, precision, j == 0, (uint32_t)stages[j].size(), st));
+    off += stages[j].size();
+  }
+  if (prof) {
+    HIP_TRY(hipEventRecord(ctx->ev[6], st));
+    ctx->ev_mips = true;
+  }
+  for (uint32_t i = 0; i < count; i++) {
+    items[i].scene->prec = precision;
+    items[i].scene->has_mips = true;
+  }
+  return VGI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// trace batch
+// ---------------------------------------------------------------------------
+
+int check_trace_item(vgi_context* ctx, const vgi_trace_item& it, uint32_t i) {
+  const vgi_scene* s = it.scene;
+  if (!s) return fail(VGI_E_INVALID, "item %u: scene is NULL", i);
+  if (s->ctx != ctx) return fail(VGI_E_INVALID, "item %u: scene belongs to another context", i);
+  if (!s->has_mips) return fail(VGI_E_NOT_READY, "item %u: scene has no mip chain (call vgi_build_mips)", i);
+  const vgi_gbuffer& gb = it.gbuffer;
+  const vgi_cone_params& c = it.cones;
+  if (!gb.position || !gb.normal || !it.out_diffuse)
+    return fail(VGI_E_INVALID, "item %u: G-buffer/output pointer is NULL", i);
+  if (c.num_diffuse > VGI_MAX_DIFFUSE_CONES)
+    return fail(VGI_E_RANGE, "item %u: num_diffuse %u exceeds %u", i, c.num_diffuse, VGI_MAX_DIFFUSE_CONES);
+  if (!(c.step_factor > 0.0f) || !std::isfinite(c.max_distance))
+    return fail(VGI_E_INVALID, "item %u: step_factor must be > 0 and max_distance finite", i);
+  if ((uint64_t)gb.width * gb.height >= (1ull << 32)) return fail(VGI_E_RANGE, "item %u: G-buffer too large", i);
+  return VGI_OK;
+}
+
+void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tuning& tn) {
+  const vgi_scene* s = it.scene;
+  const vgi_grid& g = s->grid;
+  const vgi_gbuffer& gb = it.gbuffer;
+  const vgi_cone_params& c = it.cones;
+  std::memset(&P, 0, sizeof(P));
+  for (uint32_t k = 0; k < g.levels; k++) {
+    P.level[k] = s->level[k];
+    P.level_bytes[k] = s->level_bytes[k];
+    P.nk[k] = g.n >> k;
+  }
+  P.levels = g.levels;
+  P.n = g.n;
+  P.width = gb.width;
+  P.height = gb.height;
+  P.tiles_x = (gb.width + tn.tile_w - 1) / tn.tile_w;
+  P.tiles_y = (gb.height + tn.tile_h - 1) / tn.tile_h;
+  P.num_diffuse = c.num_diffuse;
+  P.specular = c.specular;
+  for (int a = 0; a < 3; a++) {
+    P.origin[a] = g.origin[a];
+    P.eye[a] = c.eye[a];
+  }
+  P.h = g.extent / (float)g.n;
+  P.inv_h = (float)g.n / g.extent;
+  const double PI = 3.14159265358979323846;
+  P.tan_d = (float)std::tan((double)c.diffuse_aperture_deg * PI / 360.0);
+  P.tan_s = (float)std::tan((double)c.specular_aperture_deg * PI / 360.0);
+  P.step = c.step_factor;
+  P.max_dist = c.max_distance;
+  P.cutoff = c.alpha_cutoff;
+  P.normal_off = c.normal_offset;
+  P.start_off = c.start_offset;
+  if (c.num_diffuse > 0) {
+    const uint32_t K = c.num_diffuse;
+    const double ring = (double)c.diffuse_ring_deg * PI / 180.0;
+    const double wsum = 1.0 + (double)(K - 1) * std::cos(ring);
+    for (uint32_t k = 0; k < K; k++) {
+      if (k == 0) {
+        P.dirs[0][0] = 0.f;
+        P.dirs[0][1] = 0.f;
+        P.dirs[0][2] = 1.f;
+        P.w[0] = (float)(1.0 / wsum);
+      } else {
+        const double phi = 2.0 * PI * (double)(k - 1) / (double)(K - 1);
+        P.dirs[k][0] = (float)(std::sin(ring) * std::cos(phi));
+        P.dirs[k][1] = (float)(std::sin(ring) * std::sin(phi));
+        P.dirs[k][2] = (float)std::cos(ring);
+        P.w[k] = (float)(std::cos(ring) / wsum);
+      }
+    }
+  }
+  P.pos = gb.position;
+  P.nrm = gb.normal;
+  P.rough = gb.roughness;
+  P.out_d = (float4*)it.out_diffuse;
+  P.out_s = (float4*)it.out_specular;
+}
+
+int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream) {
+  if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
+  if (count == 0) return VGI_OK;
+  if (!items) return fail(VGI_E_INVALID, "items is NULL");
+  for (uint32_t i = 0; i < count; i++) TRY(check_trace_item(ctx, items[i], i));
+  const vgi_tuning tn = tuning_of(ctx);
+  // one launch per storage precision
+  std::vector<uint32_t> group[3];
+  for (uint32_t i = 0; i < count; i++) group[items[i].scene->prec].push_back(i);
+  struct Launch {
+    uint32_t prec, nscenes;
+    size_t o_params, o_blocks, nblocks;
+    bool buffer_path;
+  };
+  std::vector<Launch> launches;
+  Carve cv;
+  for (uint32_t p = 0; p < 3; p++) {
+    if (group[p].empty()) continue;
+    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, true};
+    for (uint32_t i : group[p]) {
+      const vgi_gbuffer& gb = items[i].gbuffer;
+      L.nblocks += (size_t)ceil_div(gb.width, tn.tile_w) * ceil_div(gb.height, tn.tile_h);
+      for (uint32_t k = 0; k < items[i].scene->grid.levels; k++)
+        L.buffer_path = L.buffer_path && items[i].scene->level_bytes[k] < 0x7FFFFFF0ull;  // 31-bit offsets
+    }
+    if (L.nblocks >= (1ull << 31)) return fail(VGI_E_RANGE, "trace batch too large (%zu tiles)", L.nblocks);
+    L.o_params = cv.take((size_t)L.nscenes * sizeof(TraceParams));
+    L.o_blocks = cv.take(L.nscenes > 1 ? L.nblocks * sizeof(uint2) : 0);
+    launches.push_back(L);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipStream_t st = pick(ctx, stream);
+  vgi_workspace* w = workspace(ctx, st);
+  if (!w) return fail(VGI_E_HIP, "workspace creation failed");
+  std::lock_guard<std::mutex> wl(w->mu);
+  Slot* sl = nullptr;
+  TRY(acquire_slot(w, cv.off, &sl));
+  for (const Launch& L : launches) {
+    TraceParams* hp = (TraceParams*)(sl->h + L.o_params);
+    uint2* hb = (uint2*)(sl->h + L.o_blocks);
+    size_t b = 0;
+    for (uint32_t j = 0; j < L.nscenes; j++) {
+      const vgi_trace_item& it = items[group[L.prec][j]];
+      fill_trace_params(hp[j], it, tn);
+      if (L.nscenes > 1) {
+        const uint32_t nt = hp[j].tiles_x * hp[j].tiles_y;
+        for (uint32_t t = 0; t < nt; t++) hb[b++] = make_uint2(j, t);
+      }
+    }
+  }
+  TRY(upload(w, sl, 0, cv.off));
+  const bool prof = ctx->profiling != 0;
+  if (prof) {
+    HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_counter, 0u, 2, st));
+    HIP_TRY(hipEventRecord(ctx->ev[7], st));
+  }
+  for (const Launch& L : launches) {
+    TraceBatch B{};
+    B.scenes = (const TraceParams*)(sl->d + L.o_params);
+    B.blocks = L.nscenes > 1 ? (const uint2*)(sl->d + L.o_blocks) : nullptr;
+    B.counter = prof ? ctx->d_counter : nullptr;
+    B.nblocks = (uint32_t)L.nblocks;
+    B.tile_w = tn.tile_w;
+    B.tile_h = tn.tile_h;
+    B.prec = L.prec;
+    B.nscenes = L.nscenes;
+    HIP_TRY(launch_trace(B, L.buffer_path, st));
+  }
+  if (prof) {
+    HIP_TRY(hipEventRecord(ctx->ev[8], st));
+    ctx->ev_trace = true;
+  }
+  return VGI_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-const char* vgi_version(void) { return "vgi 0.1.0 (gfx950)"; }
+const char* vgi_version(void) { return "vgi 0.2.0 (gfx950)"; }
 uint32_t vgi_abi_version(void) { return VGI_ABI_VERSION; }
 const char* vgi_last_error(void) { return g_err.c_str(); }
 
@@ -206,13 +925,17 @@ int vgi_create(vgi_context** out, int device) {
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete ctx; return fail(VGI_E_HIP, "stream create: %s", hipGetErrorString(e)); }
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(VGI_E_HIP, "stream create: %s", hipGetErrorString(e));
+  }
   float lut[256];
   for (int i = 0; i < 256; i++) lut[i] = (float)std::pow((double)i / 255.0, 2.2);
   if (hipMalloc((void**)&ctx->d_lut, sizeof(lut)) != hipSuccess ||
       hipMemcpy(ctx->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess ||
       hipMalloc((void**)&ctx->d_counter, sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_counter, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
     vgi_destroy(ctx);
     return fail(VGI_E_NOMEM, "context allocation failed");
   }
@@ -226,7 +949,9 @@ int vgi_create(vgi_context** out, int device) {
 int vgi_destroy(vgi_context* ctx) {
   if (!ctx) return VGI_OK;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : ctx->ws) free_workspace(kv.second.get());
+  ctx->ws.clear();
   if (ctx->d_lut) (void)hipFree(ctx->d_lut);
   if (ctx->d_counter) (void)hipFree(ctx->d_counter);
   if (ctx->h_counter) (void)hipHostFree(ctx->h_counter);
@@ -256,7 +981,7 @@ int vgi_set_tuning(vgi_context* ctx, const vgi_tuning* t) {
 
 int vgi_get_tuning(const vgi_context* ctx, vgi_tuning* t) {
   if (!ctx || !t) return fail(VGI_E_INVALID, "NULL argument");
-  *t = ctx->tuning;
+  *t = tuning_of(const_cast<vgi_context*>(ctx));
   return VGI_OK;
 }
 
@@ -277,7 +1002,7 @@ int vgi_get_timings(const vgi_context* cctx, vgi_timings* out) {
     HIP_TRY(hipEventElapsedTime(&t.voxelize_sort_ms, ctx->ev[2], ctx->ev[3]));
     float a = 0.f;
     HIP_TRY(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]));
-    t.voxelize_sort_ms += a;  // scan + host read of the pair total
+    t.voxelize_sort_ms += a;  // bins + scan + host read of the pair totals
     HIP_TRY(hipEventElapsedTime(&t.voxelize_raster_ms, ctx->ev[3], ctx->ev[4]));
     HIP_TRY(hipEventElapsedTime(&t.voxelize_total_ms, ctx->ev[0], ctx->ev[4]));
     ctx->ev_vox = false;
@@ -308,16 +1033,8 @@ int vgi_sync(vgi_context* ctx, void* stream) {
 
 int vgi_scene_create(vgi_context* ctx, vgi_scene** out) {
   if (!ctx || !out) return fail(VGI_E_INVALID, "NULL argument");
-  HIP_TRY(hipSetDevice(ctx->device));
   vgi_scene* s = new vgi_scene();
   s->ctx = ctx;
-  if (hipMalloc((void**)&s->stats, sizeof(DevStats)) != hipSuccess ||
-      hipHostMalloc((void**)&s->h_stats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&s->h_total, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming) != hipSuccess) {
-    vgi_scene_release(s);
-    return fail(VGI_E_NOMEM, "scene allocation failed");
-  }
   *out = s;
   return VGI_OK;
 }
@@ -326,30 +1043,24 @@ int vgi_scene_release(vgi_scene* s) {
   if (!s) return VGI_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipDeviceSynchronize();
-  void* dev[] = {s->acc, s->mask, s->tris, s->tri_slot, s->big_list, s->meshes, s->bin_small,
-                 s->bin_packed, s->bin_scan, s->pairs, s->work_bin, s->stats, s->scan_tmp};
+  void* dev[] = {s->acc, s->mask, s->bin_packed};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++)
     if (s->level[k]) (void)hipFree(s->level[k]);
-  if (s->meshes_host) (void)hipHostFree(s->meshes_host);
-  if (s->h_stats) (void)hipHostFree(s->h_stats);
-  if (s->h_total) (void)hipHostFree(s->h_total);
-  if (s->staging_done) (void)hipEventDestroy(s->staging_done);
   delete s;
   return VGI_OK;
 }
 
 uint64_t vgi_scene_bytes_estimate(const vgi_grid* g, uint32_t precision, uint64_t ntri) {
-  if (!g || !is_pow2(g->n) || g->levels == 0 || g->levels > VGI_MAX_LEVELS) return 0;
+  (void)ntri;
+  if (!g || !is_pow2(g->n) || g->n < 8 || g->levels == 0 || g->levels > VGI_MAX_LEVELS) return 0;
   const uint64_t n = g->n;
-  uint64_t b = n * n * n * kAccWords * 4;
+  uint64_t b = n * n * n * kAccWords * 4 + (n / 8) * (n / 8) * (n / 8) * 64 + (n / 8) * (n / 8) * (n / 8) * 8;
   for (uint32_t k = 0; k < g->levels; k++) {
     const uint64_t p = (n >> k) + 2;
     b += p * p * p * texel_bytes(precision);
   }
-  const uint64_t nb = n / 8;
-  b += nb * nb * nb * 12 + ntri * (sizeof(TriRec) + 16) + (1u << 20);
   return b;
 }
 
@@ -357,6 +1068,8 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   if (!s || !out) return fail(VGI_E_INVALID, "NULL argument");
   std::memset(out, 0, sizeof(*out));
   out->triangles_in = s->tris_in;
+  out->triangles_rejected = s->rejected;
+  out->triangles_degenerate = s->degenerate;
   out->bin_pairs = s->num_pairs;
   out->work_items = s->work_items;
   out->device_bytes = scene_device_bytes(s);
@@ -366,380 +1079,41 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   out->has_voxels = s->has_voxels;
   out->has_mips = s->has_mips;
   out->bin_size = 1u << s->bin_shift;
-  if (s->has_voxels) {
-    (void)hipSetDevice(s->ctx->device);
-    HIP_TRY(hipMemcpy(s->h_stats, s->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
-    out->triangles_rejected = s->h_stats->rejected;
-    out->triangles_degenerate = s->h_stats->degenerate;
-  }
   return VGI_OK;
 }
 
 int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_t num_meshes,
                  const vgi_grid* grid, void* stream) {
   if (!ctx || !s) return fail(VGI_E_INVALID, "NULL context or scene");
-  if (s->ctx != ctx) return fail(VGI_E_INVALID, "scene belongs to another context");
   TRY(check_grid(grid));
-  if (num_meshes > 0 && !meshes) return fail(VGI_E_INVALID, "meshes is NULL");
-  uint64_t ntri = 0;
-  for (uint32_t i = 0; i < num_meshes; i++) {
-    const vgi_mesh& m = meshes[i];
-    if (m.num_triangles == 0) continue;
-    if (!m.positions) return fail(VGI_E_INVALID, "mesh %u: positions is NULL", i);
-    const uint32_t stride = m.position_stride ? m.position_stride : 12u;
-    if (stride < 12 || (stride & 3)) return fail(VGI_E_INVALID, "mesh %u: position_stride %u", i, stride);
-    if (m.index_type > VGI_INDEX_U32) return fail(VGI_E_INVALID, "mesh %u: index_type %u", i, m.index_type);
-    if (m.index_type != VGI_INDEX_NONE && !m.indices)
-      return fail(VGI_E_INVALID, "mesh %u: indices is NULL", i);
-    if (m.index_type == VGI_INDEX_NONE && (uint64_t)m.num_triangles * 3 > m.num_vertices)
-      return fail(VGI_E_INVALID, "mesh %u: non-indexed mesh needs 3 vertices per triangle", i);
-    for (int k = 0; k < 12; k++)
-      if (!std::isfinite(m.model[k])) return fail(VGI_E_INVALID, "mesh %u: model matrix not finite", i);
-    ntri += m.num_triangles;
-  }
-  if (ntri >= (1ull << 31)) return fail(VGI_E_RANGE, "%llu triangles exceed 2^31", (unsigned long long)ntri);
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t st = pick(ctx, stream);
-  vgi_tuning tn;
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    tn = ctx->tuning;
-  }
-  const uint32_t n = grid->n;
-  const uint32_t bin = std::min(tn.bin_size, n);
-  s->bin_shift = ilog2(bin);
-  s->nbins_axis = n >> s->bin_shift;
-  s->chunk = tn.chunk_tris;
-  const uint32_t nbins = s->nbins_axis * s->nbins_axis * s->nbins_axis;
-  s->has_voxels = false;
-  s->has_mips = false;
-
-  // ---- allocations (grown on demand, kept across calls) ----
-  const size_t acc_words = (size_t)n * n * n * kAccWords;
-  if (s->acc_bytes != acc_words * 4) {
-    if (s->acc) (void)hipFree(s->acc);
-    if (s->mask) (void)hipFree(s->mask);
-    s->acc = nullptr;
-    s->mask = nullptr;
-    s->acc_bytes = 0;
-    const size_t bricks = (size_t)(n / 8) * (n / 8) * (n / 8);
-    if (hipMalloc((void**)&s->acc, acc_words * 4) != hipSuccess ||
-        hipMalloc((void**)&s->mask, bricks * 16 * 4) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(VGI_E_NOMEM, "accumulator allocation (%zu bytes) failed", acc_words * 4);
-    }
-    s->acc_bytes = acc_words * 4;
-  }
-  const size_t tneed = std::max<size_t>(ntri, 1);
-  if (s->tris_cap < tneed) {
-    void* arrs[3] = {s->tris, s->tri_slot, s->big_list};
-    for (void* a : arrs)
-      if (a) (void)hipFree(a);
-    s->tris = nullptr;
-    s->tri_slot = s->big_list = nullptr;
-    s->tris_cap = 0;
-    const size_t cap = std::max(tneed, (size_t)1024);
-    if (hipMalloc((void**)&s->tris, cap * sizeof(TriRec)) != hipSuccess ||
-        hipMalloc((void**)&s->tri_slot, cap * kSmallBins * 4) != hipSuccess ||
-        hipMalloc((void**)&s->big_list, cap * 4) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(VGI_E_NOMEM, "triangle record allocation failed");
-    }
-    s->tris_cap = cap;
-  }
-  if (s->bins_cap < (size_t)nbins + 1) {
-    if (s->bin_small) (void)hipFree(s->bin_small);
-    if (s->bin_packed) (void)hipFree(s->bin_packed);
-    if (s->bin_scan) (void)hipFree(s->bin_scan);
-    s->bin_small = nullptr;
-    s->bin_packed = s->bin_scan = nullptr;
-    s->bins_cap = 0;
-    const size_t cap = (size_t)nbins + 1;
-    if (hipMalloc((void**)&s->bin_small, cap * 4 * 4) != hipSuccess ||
-        hipMalloc((void**)&s->bin_packed, cap * 8) != hipSuccess ||
-        hipMalloc((void**)&s->bin_scan, cap * 8) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(VGI_E_NOMEM, "bin array allocation failed");
-    }
-    s->bins_cap = cap;
-  }
-  {
-    size_t need = 0;
-    HIP_TRY(scan_bins(nullptr, &need, s->bin_packed, s->bin_scan, nbins + 1, st));
-    if (need > s->scan_tmp_bytes) {
-      if (s->scan_tmp) (void)hipFree(s->scan_tmp);
-      s->scan_tmp = nullptr;
-      s->scan_tmp_bytes = 0;
-      HIP_TRY(hipMalloc(&s->scan_tmp, need));
-      s->scan_tmp_bytes = need;
-    }
-  }
-  const uint32_t nm = std::max<uint32_t>(num_meshes, 1);
-  if (s->meshes_cap < nm) {
-    if (s->meshes) (void)hipFree(s->meshes);
-    if (s->meshes_host) {
-      (void)hipEventSynchronize(s->staging_done);
-      (void)hipHostFree(s->meshes_host);
-    }
-    s->meshes = nullptr;
-    s->meshes_host = nullptr;
-    s->meshes_cap = 0;
-    HIP_TRY(hipMalloc((void**)&s->meshes, nm * sizeof(MeshDev)));
-    HIP_TRY(hipHostMalloc((void**)&s->meshes_host, nm * sizeof(MeshDev), hipHostMallocDefault));
-    s->meshes_cap = nm;
-  }
-
-  // ---- mesh table ----
-  HIP_TRY(hipEventSynchronize(s->staging_done));
-  {
-    uint32_t base = 0;
-    for (uint32_t i = 0; i < nm; i++) {
-      MeshDev& d = s->meshes_host[i];
-      std::memset(&d, 0, sizeof(d));
-      if (i >= num_meshes) continue;
-      const vgi_mesh& m = meshes[i];
-      d.pos = m.positions;
-      d.idx = m.indices;
-      d.albedo = m.albedo_rgb8;
-      d.stride = m.position_stride ? m.position_stride : 12u;
-      d.nverts = m.num_vertices;
-      d.idx_type = m.index_type;
-      d.ntri = m.num_triangles;
-      d.tri_base = base;
-      d.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) |
-                   ((uint32_t)m.base_color[2] << 16);
-      std::memcpy(d.m, m.model, sizeof(d.m));
-      base += m.num_triangles;
-    }
-  }
-  HIP_TRY(hipMemcpyAsync(s->meshes, s->meshes_host, nm * sizeof(MeshDev), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipEventRecord(s->staging_done, st));
-
-  VoxParams P{};
-  P.meshes = s->meshes;
-  P.tris = s->tris;
-  P.tri_slot = s->tri_slot;
-  P.big_list = s->big_list;
-  P.bin_small = s->bin_small;
-  P.bin_big = s->bin_small + s->bins_cap;
-  P.big_cursor = s->bin_small + 2 * s->bins_cap;
-  P.zero_list = s->bin_small + 3 * s->bins_cap;
-  P.bin_packed = s->bin_packed;
-  P.bin_scan = s->bin_scan;
-  P.acc = s->acc;
-  P.mask = s->mask;
-  P.stats = s->stats;
-  for (int a = 0; a < 3; a++) P.origin[a] = grid->origin[a];
-  P.snap_scale = (float)(n * 256u) / grid->extent;
-  P.nmesh = num_meshes;
-  P.ntri = num_meshes ? (uint32_t)ntri : 0u;
-  P.n = n;
-  P.bin_shift = s->bin_shift;
-  P.nbins_axis = s->nbins_axis;
-  P.chunk = s->chunk;
-
-  const uint32_t persistent = (uint32_t)ctx->num_cus * 4;
-  const bool prof = ctx->profiling != 0;
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  // bin_small, bin_big, big_cursor: 3 x bins_cap words; stats
-  HIP_TRY(launch_fill_u32(s->bin_small, 0u, 3 * s->bins_cap, st));
-  HIP_TRY(launch_fill_u32((uint32_t*)s->stats, 0u, sizeof(DevStats) / 4, st));
-  HIP_TRY(launch_setup(P, st));
-  HIP_TRY(launch_big(P, false, persistent, st));
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
-  HIP_TRY(launch_bins(P, st));
-  HIP_TRY(scan_bins(s->scan_tmp, &s->scan_tmp_bytes, s->bin_packed, s->bin_scan, nbins + 1, st));
-  HIP_TRY(hipMemcpyAsync(s->h_total, s->bin_scan + nbins, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  const unsigned long long tot = s->h_total[0];
-  const uint32_t total_pairs = (uint32_t)tot, total_work = (uint32_t)(tot >> 32);
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-  TRY(grow(&s->pairs, &s->pairs_cap, std::max<size_t>(total_pairs, 1), "bin pairs"));
-  TRY(grow(&s->work_bin, &s->work_cap, std::max<size_t>(total_work, 1), "work items"));
-  P.pairs = s->pairs;
-  P.work_bin = s->work_bin;
-  P.pairs_cap = (uint32_t)std::min<size_t>(s->pairs_cap, 0xFFFFFFFFu);
-  if (total_pairs > 0) {
-    HIP_TRY(launch_workmap(P, st));
-    HIP_TRY(launch_emit_small(P, st));
-    HIP_TRY(launch_big(P, true, persistent, st));
-    HIP_TRY(launch_zero(P, persistent, st));
-  }
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
-  const uint32_t tile = std::min(1u << s->bin_shift, 16u);
-  const uint32_t grid_blocks = std::min<uint32_t>(total_work, (uint32_t)ctx->num_cus * 8);
-  HIP_TRY(launch_voxelize(P, tile, grid_blocks, total_work, st));
-  if (prof) {
-    HIP_TRY(hipEventRecord(ctx->ev[4], st));
-    ctx->ev_vox = true;
-  }
-  s->grid = *grid;
-  s->grid_set = true;
-  s->tris_in = ntri;
-  s->num_pairs = total_pairs;
-  s->work_items = total_work;
-  s->has_voxels = true;
-  return VGI_OK;
+  vgi_voxelize_item it{s, meshes, num_meshes, *grid};
+  return voxelize_batch(ctx, &it, 1, stream);
 }
 
-int vgi_build_mips(vgi_context* ctx, vgi_scene* s, const vgi_lights* lights, uint32_t precision,
-                   void* stream) {
+int vgi_voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t count, void* stream) {
+  return voxelize_batch(ctx, items, count, stream);
+}
+
+int vgi_build_mips(vgi_context* ctx, vgi_scene* s, const vgi_lights* lights, uint32_t precision, void* stream) {
   if (!ctx || !s) return fail(VGI_E_INVALID, "NULL context or scene");
-  if (s->ctx != ctx) return fail(VGI_E_INVALID, "scene belongs to another context");
-  if (!s->has_voxels) return fail(VGI_E_NOT_READY, "scene has not been voxelized");
-  if (precision > VGI_PREC_BF16) return fail(VGI_E_INVALID, "precision %u", precision);
-  if (lights && lights->count > VGI_MAX_LIGHTS)
-    return fail(VGI_E_RANGE, "%u lights exceed %u", lights->count, VGI_MAX_LIGHTS);
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t st = pick(ctx, stream);
-  const vgi_grid& g = s->grid;
-  if (s->alloc_n != g.n || s->alloc_levels != g.levels || s->alloc_prec != precision) {
-    HIP_TRY(hipStreamSynchronize(st));
-    for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) {
-      if (s->level[k]) (void)hipFree(s->level[k]);
-      s->level[k] = nullptr;
-      s->level_bytes[k] = 0;
-    }
-    s->alloc_n = s->alloc_levels = 0;
-    for (uint32_t k = 0; k < g.levels; k++) {
-      const size_t p = (size_t)(g.n >> k) + 2;
-      const size_t bytes = p * p * p * texel_bytes(precision);
-      hipError_t e = hipMalloc(&s->level[k], bytes);
-      if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(VGI_E_NOMEM, "mip level %u allocation (%zu bytes) failed", k, bytes);
-      }
-      s->level_bytes[k] = bytes;
-      HIP_TRY(hipMemset(s->level[k], 0, bytes));  // zero border, written once
-    }
-    s->alloc_n = g.n;
-    s->alloc_levels = g.levels;
-    s->alloc_prec = precision;
-  }
-  MipParams P{};
-  P.acc = s->acc;
-  P.mask = s->mask;
-  P.bin_packed = s->bin_packed;
-  P.lut = ctx->d_lut;
-  for (uint32_t k = 0; k < g.levels; k++) {
-    P.level[k] = s->level[k];
-    P.nk[k] = g.n >> k;
-  }
-  P.n = g.n;
-  P.levels = g.levels;
-  P.bin_shift = s->bin_shift;
-  P.nbins_axis = s->nbins_axis;
-  P.prec = precision;
-  for (int a = 0; a < 3; a++) P.origin[a] = g.origin[a];
-  P.h = g.extent / (float)g.n;
-  P.nlights = lights ? lights->count : 0;
-  for (uint32_t l = 0; l < P.nlights; l++) {
-    const vgi_light& L = lights->light[l];
-    if (L.type > VGI_LIGHT_POINT) return fail(VGI_E_INVALID, "light %u: type %u", l, L.type);
-    P.lights[l].type = L.type;
-    for (int a = 0; a < 3; a++) {
-      P.lights[l].vec[a] = L.vec[a];
-      P.lights[l].color[a] = L.color[a];
-    }
-    P.lights[l].min_distance = L.min_distance;
-  }
-  const bool prof = ctx->profiling != 0;
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[5], st));
-  uint32_t top = 0;
-  HIP_TRY(launch_mips(P, st, &top));
-  if (prof) {
-    HIP_TRY(hipEventRecord(ctx->ev[6], st));
-    ctx->ev_mips = true;
-  }
-  s->prec = precision;
-  s->has_mips = true;
-  return VGI_OK;
+  vgi_mips_item it{s, lights};
+  return mips_batch(ctx, &it, 1, precision, stream);
+}
+
+int vgi_build_mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uint32_t precision,
+                         void* stream) {
+  return mips_batch(ctx, items, count, precision, stream);
 }
 
 int vgi_trace(vgi_context* ctx, const vgi_scene* s, const vgi_gbuffer* gb, const vgi_cone_params* c,
               float* out_diffuse, float* out_specular, void* stream) {
   if (!ctx || !s || !gb || !c) return fail(VGI_E_INVALID, "NULL argument");
-  if (s->ctx != ctx) return fail(VGI_E_INVALID, "scene belongs to another context");
-  if (!s->has_mips) return fail(VGI_E_NOT_READY, "scene has no mip chain (call vgi_build_mips)");
-  if (!gb->position || !gb->normal || !out_diffuse) return fail(VGI_E_INVALID, "G-buffer/output pointer is NULL");
-  if (c->num_diffuse > VGI_MAX_DIFFUSE_CONES)
-    return fail(VGI_E_RANGE, "num_diffuse %u exceeds %u", c->num_diffuse, VGI_MAX_DIFFUSE_CONES);
-  if (!(c->step_factor > 0.0f) || !std::isfinite(c->max_distance))
-    return fail(VGI_E_INVALID, "step_factor must be > 0 and max_distance finite");
-  if ((uint64_t)gb->width * gb->height >= (1ull << 32)) return fail(VGI_E_RANGE, "G-buffer too large");
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t st = pick(ctx, stream);
-  vgi_tuning tn;
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    tn = ctx->tuning;
-  }
-  const vgi_grid& g = s->grid;
-  TraceParams P{};
-  for (uint32_t k = 0; k < g.levels; k++) {
-    P.level[k] = s->level[k];
-    P.level_bytes[k] = s->level_bytes[k];
-    P.nk[k] = g.n >> k;
-  }
-  P.levels = g.levels;
-  P.n = g.n;
-  P.prec = s->prec;
-  P.width = gb->width;
-  P.height = gb->height;
-  P.tile_w = tn.tile_w;
-  P.tile_h = tn.tile_h;
-  P.tiles_x = (gb->width + tn.tile_w - 1) / tn.tile_w;
-  P.tiles_y = (gb->height + tn.tile_h - 1) / tn.tile_h;
-  P.num_diffuse = c->num_diffuse;
-  P.specular = c->specular;
-  for (int a = 0; a < 3; a++) {
-    P.origin[a] = g.origin[a];
-    P.eye[a] = c->eye[a];
-  }
-  P.h = g.extent / (float)g.n;
-  P.inv_h = (float)g.n / g.extent;
-  const double PI = 3.14159265358979323846;
-  P.tan_d = (float)std::tan((double)c->diffuse_aperture_deg * PI / 360.0);
-  P.tan_s = (float)std::tan((double)c->specular_aperture_deg * PI / 360.0);
-  P.step = c->step_factor;
-  P.max_dist = c->max_distance;
-  P.cutoff = c->alpha_cutoff;
-  P.normal_off = c->normal_offset;
-  P.start_off = c->start_offset;
-  if (c->num_diffuse > 0) {
-    const uint32_t K = c->num_diffuse;
-    const double ring = (double)c->diffuse_ring_deg * PI / 180.0;
-    const double wsum = 1.0 + (double)(K - 1) * std::cos(ring);
-    for (uint32_t k = 0; k < K; k++) {
-      if (k == 0) {
-        P.dirs[0][0] = 0.f; P.dirs[0][1] = 0.f; P.dirs[0][2] = 1.f;
-        P.w[0] = (float)(1.0 / wsum);
-      } else {
-        const double phi = 2.0 * PI * (double)(k - 1) / (double)(K - 1);
-        P.dirs[k][0] = (float)(std::sin(ring) * std::cos(phi));
-        P.dirs[k][1] = (float)(std::sin(ring) * std::sin(phi));
-        P.dirs[k][2] = (float)std::cos(ring);
-        P.w[k] = (float)(std::cos(ring) / wsum);
-      }
-    }
-  }
-  P.pos = gb->position;
-  P.nrm = gb->normal;
-  P.rough = gb->roughness;
-  P.out_d = (float4*)out_diffuse;
-  P.out_s = (float4*)out_specular;
-  const bool prof = ctx->profiling != 0;
-  P.counter = prof ? ctx->d_counter : nullptr;
-  if (prof) {
-    HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_counter, 0u, 2, st));
-    HIP_TRY(hipEventRecord(ctx->ev[7], st));
-  }
-  HIP_TRY(launch_trace(P, st));
-  if (prof) {
-    HIP_TRY(hipEventRecord(ctx->ev[8], st));
-    ctx->ev_trace = true;
-  }
-  return VGI_OK;
+  vgi_trace_item it{s, *gb, *c, out_diffuse, out_specular};
+  return trace_batch(ctx, &it, 1, stream);
+}
+
+int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream) {
+  return trace_batch(ctx, items, count, stream);
 }
 
 int vgi_scene_read_voxels(vgi_context* ctx, const vgi_scene* s, uint32_t* host_out) {
@@ -760,7 +1134,8 @@ int vgi_scene_read_voxels(vgi_context* ctx, const vgi_scene* s, uint32_t* host_o
         const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
         const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
         const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
-        const bool occ = (uint32_t)packed[bin] != 0 && ((mask[(size_t)brick * 16 + (local >> 5)] >> (local & 31u)) & 1u);
+        const bool occ =
+            (uint32_t)packed[bin] != 0 && ((mask[(size_t)brick * 16 + (local >> 5)] >> (local & 31u)) & 1u);
         if (!occ) {
           std::memset(o, 0, kAccWords * 4);
           continue;
@@ -836,6 +1211,7 @@ int vgi_memset(vgi_context* ctx, void* dst, int value, size_t bytes) {
   if (!ctx || (bytes && !dst)) return fail(VGI_E_INVALID, "NULL argument");
   HIP_TRY(hipSetDevice(ctx->device));
   if (bytes) HIP_TRY(hipMemset(dst, value, bytes));
+  HIP_TRY(hipDeviceSynchronize());
   return VGI_OK;
 }
 
@@ -862,7 +1238,24 @@ int vgi_stream_destroy(vgi_context* ctx, void* stream) {
   if (!stream) return VGI_OK;
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->ws.find((hipStream_t)stream);
+    if (it != ctx->ws.end()) {
+      free_workspace(it->second.get());
+      ctx->ws.erase(it);
+    }
+  }
   HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+  return VGI_OK;
+}
+
+int vgi_workspace_bytes(vgi_context* ctx, uint64_t* out) {
+  if (!ctx || !out) return fail(VGI_E_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  uint64_t b = 0;
+  for (auto& kv : ctx->ws) b += kv.second->device_bytes();
+  *out = b;
   return VGI_OK;
 }
 

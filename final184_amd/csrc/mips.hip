@@ -4,13 +4,14 @@
 // volume and lights voxels at lookup time (Shader/Lighting/indirect.frag:157-169); the
 // LOD volumes appear only as a plan (docs/index.md:36).
 //
-// One launch reads a 16^3 tile of its source level per workgroup (512 threads, one
-// 2x2x2 group each) and produces up to four levels above it, staging each level's tile
-// in LDS: 16^3 -> 8^3 -> 4^3 -> 2^3 -> 1. The first launch's source is the integer
-// accumulator: each thread resolves its 8 voxels (rounded mean albedo through the
-// (i/255)^2.2 table, mean quantized normal, direct light at the voxel centre) into level
-// 0 on the way. Further launches start from the last level written until the chain is
-// complete (256^3: levels 0-4, then 5-7). Sums run in a fixed order (x fastest, then y,
+// Each workgroup reads a 16^3 tile of a source level (512 threads, one 2x2x2 group each)
+// and produces up to four levels above it, staging each level's tile in LDS:
+// 16^3 -> 8^3 -> 4^3 -> 2^3 -> 1. A launch covers every scene of a batch through its
+// block table (scene, tile, source level, levels produced). The first launch's source is
+// the integer accumulator: each thread resolves its 8 voxels (rounded mean albedo through
+// the (i/255)^2.2 table, mean quantized normal, direct light at the voxel centre) into
+// level 0 on the way. Further launches start from the last level written until every
+// chain is complete (256^3: levels 0-4, then 5-7). Sums run in a fixed order (x fastest, then y,
 // then z) and every level is rounded to its storage precision before it is reduced, so
 // the result is a pure function of the accumulator.
 //
@@ -93,7 +94,7 @@ __device__ __forceinline__ float4 scale4(float4 a, float s) {
 }
 
 // Radiance of level-0 voxel (x, y, z) from the accumulator.
-__device__ float4 inject(const MipParams& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
+__device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
   if ((uint32_t)P.bin_packed[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
@@ -146,21 +147,23 @@ __device__ float4 inject(const MipParams& P, const float* lut, uint32_t x, uint3
 }
 
 template <int PREC, bool SRC_ACC>
-__global__ __launch_bounds__(512) void k_mips(MipParams P, uint32_t nout) {
+__global__ __launch_bounds__(512) void k_mips(MipBatch Bt) {
   using TX = Texel<PREC>;
   __shared__ float4 s_a[512];
   __shared__ float4 s_b[64];
-  const uint32_t src = P.src_level;
+  const MipBlock blk = Bt.blocks[blockIdx.x];
+  const MipScene& P = Bt.scenes[blk.scene];
+  const uint32_t src = blk.src, nout = blk.nout;
   const uint32_t ns = P.nk[src];
   const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge
   const uint32_t tpa = ns / S;             // tiles per axis
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = blk.tile;
   const uint32_t tx = tile % tpa, ty = (tile / tpa) % tpa, tz = tile / (tpa * tpa);
   const uint32_t G = S >> 1;  // groups per tile edge (>= 1 when nout >= 1)
   const uint32_t g = threadIdx.x;
   __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
   if (SRC_ACC) {
-    if (g < 256) s_lut[g] = P.lut[g];
+    if (g < 256) s_lut[g] = Bt.lut[g];
     __syncthreads();
   }
 
@@ -226,45 +229,26 @@ __global__ __launch_bounds__(512) void k_mips(MipParams P, uint32_t nout) {
   }
 }
 
-template <int PREC>
-static hipError_t launch_prec(const MipParams& P0, hipStream_t s, uint32_t* produced_top) {
-  MipParams P = P0;
-  P.src_level = 0;
-  uint32_t top = 0;
-  bool first = true;
-  for (;;) {
-    const uint32_t ns = P.nk[P.src_level];
-    const uint32_t S = ns < 16u ? ns : 16u;
-    uint32_t lg = 0;
-    while ((1u << (lg + 1)) <= S) lg++;
-    const uint32_t remaining = P.levels - 1 - P.src_level;
-    const uint32_t nout = lg < remaining ? lg : remaining;
-    const uint32_t tpa = ns / S;
-    const uint32_t blocks = tpa * tpa * tpa;
-    if (first) {
-      hipLaunchKernelGGL((k_mips<PREC, true>), dim3(blocks), dim3(512), 0, s, P, nout);
-      first = false;
-    } else {
-      if (nout == 0) break;
-      hipLaunchKernelGGL((k_mips<PREC, false>), dim3(blocks), dim3(512), 0, s, P, nout);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    top = P.src_level + nout;
-    if (top >= P.levels - 1 || nout == 0) break;
-    P.src_level = top;
+hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  const dim3 g(blocks), b(512);
+  switch (prec) {
+    case VGI_PREC_FP32:
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B);
+      break;
+    case VGI_PREC_FP16:
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B);
+      break;
+    case VGI_PREC_BF16:
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
-  if (produced_top) *produced_top = top;
-  return hipSuccess;
-}
-
-hipError_t launch_mips(const MipParams& P, hipStream_t s, uint32_t* produced_top) {
-  switch (P.prec) {
-    case VGI_PREC_FP32: return launch_prec<VGI_PREC_FP32>(P, s, produced_top);
-    case VGI_PREC_FP16: return launch_prec<VGI_PREC_FP16>(P, s, produced_top);
-    case VGI_PREC_BF16: return launch_prec<VGI_PREC_BF16>(P, s, produced_top);
-    default: return hipErrorInvalidValue;
-  }
+  return hipGetLastError();
 }
 
 }  // namespace vgi

@@ -10,9 +10,10 @@
 // step t += step_factor * d, stop at max_distance, alpha >= cutoff or the grid boundary.
 //
 // Schedule: a workgroup is one screen tile (8x8, 16x16 or 32x8 pixels; each wavefront an
-// 8x8 block), tiles are remapped so that each XCD gets a contiguous band of the image
-// (neighbouring tiles share that XCD's L2), and every lane of a wavefront marches the
-// same cone index at the same time, so directions stay coherent across the wavefront.
+// 8x8 block) of one scene of the batch; tiles are remapped so that each XCD gets a
+// contiguous band (neighbouring tiles share that XCD's L2), and every lane of a wavefront
+// marches the same cone index at the same time, so directions stay coherent across the
+// wavefront.
 //
 // Gather: a quadrilinear sample is 16 weighted texels, sum_i w_i c_i with the level blend
 // folded into the weights, accumulated in FP32 (FP16 texels feed v_fma_mix directly; no
@@ -221,18 +222,21 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
 }
 
 template <int PREC, bool BUF>
-__global__ __launch_bounds__(256) void k_trace(TraceParams P) {
-  // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a contiguous band.
-  const uint32_t ntiles = P.tiles_x * P.tiles_y;
+__global__ __launch_bounds__(256) void k_trace(TraceBatch Bt) {
+  // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous band of the
+  // batch's tile list (tiles of one scene are consecutive, row-major).
   const uint32_t b = blockIdx.x;
   const uint32_t xcd = b & 7u, slot = b >> 3;
-  const uint32_t full = ntiles >> 3, rem = ntiles & 7u;
-  const uint32_t tile = xcd * full + (xcd < rem ? xcd : rem) + slot;
+  const uint32_t full = Bt.nblocks >> 3, rem = Bt.nblocks & 7u;
+  const uint32_t idx = xcd * full + (xcd < rem ? xcd : rem) + slot;
+  const uint2 st = Bt.nscenes == 1 ? make_uint2(0u, idx) : Bt.blocks[idx];
+  const TraceParams& P = Bt.scenes[st.x];
+  const uint32_t tile = st.y;
   const uint32_t tx = tile % P.tiles_x, ty = tile / P.tiles_x;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t wpr = P.tile_w >> 3;  // 8x8 wave blocks per tile row
-  const uint32_t px = tx * P.tile_w + (wave % wpr) * 8 + (lane & 7u);
-  const uint32_t py = ty * P.tile_h + (wave / wpr) * 8 + (lane >> 3);
+  const uint32_t wpr = Bt.tile_w >> 3;  // 8x8 wave blocks per tile row
+  const uint32_t px = tx * Bt.tile_w + (wave % wpr) * 8 + (lane & 7u);
+  const uint32_t py = ty * Bt.tile_h + (wave / wpr) * 8 + (lane >> 3);
   const bool active = px < P.width && py < P.height;
   uint32_t count = 0;
   if (active) {
@@ -284,31 +288,32 @@ __global__ __launch_bounds__(256) void k_trace(TraceParams P) {
     P.out_d[p] = od;
     if (P.out_s) P.out_s[p] = os;
   }
-  if (P.counter) {
+  if (Bt.counter) {
     unsigned long long c = count;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if (lane == 0) atomicAdd(P.counter, c);
+    if (lane == 0) atomicAdd(Bt.counter, c);
   }
 }
 
-template <int PREC>
-static void launch_prec(const TraceParams& P, dim3 grid, dim3 block, hipStream_t s) {
-  bool buf = true;  // 32-bit buffer offsets reach every level
-  for (uint32_t k = 0; k < P.levels; k++) buf = buf && P.level_bytes[k] < 0x7FFFFFF0ull;
-  if (buf) hipLaunchKernelGGL((k_trace<PREC, true>), grid, block, 0, s, P);
-  else hipLaunchKernelGGL((k_trace<PREC, false>), grid, block, 0, s, P);
-}
-
-hipError_t launch_trace(const TraceParams& P, hipStream_t s) {
-  const uint32_t ntiles = P.tiles_x * P.tiles_y;
-  if (ntiles == 0) return hipSuccess;
-  const dim3 block(P.tile_w * P.tile_h), grid(ntiles);
-  switch (P.prec) {
-    case VGI_PREC_FP32: launch_prec<VGI_PREC_FP32>(P, grid, block, s); break;
-    case VGI_PREC_FP16: launch_prec<VGI_PREC_FP16>(P, grid, block, s); break;
-    case VGI_PREC_BF16: launch_prec<VGI_PREC_BF16>(P, grid, block, s); break;
-    default: return hipErrorInvalidValue;
+hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s) {
+  if (B.nblocks == 0) return hipSuccess;
+  const dim3 block(B.tile_w * B.tile_h), grid(B.nblocks);
+  switch (B.prec) {
+    case VGI_PREC_FP32:
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, true>), grid, block, 0, s, B);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, false>), grid, block, 0, s, B);
+      break;
+    case VGI_PREC_FP16:
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, true>), grid, block, 0, s, B);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, false>), grid, block, 0, s, B);
+      break;
+    case VGI_PREC_BF16:
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, true>), grid, block, 0, s, B);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, false>), grid, block, 0, s, B);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

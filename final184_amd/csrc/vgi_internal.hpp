@@ -1,5 +1,9 @@
 // vgi_internal.hpp — device-side structures and launch declarations shared by the
 // kernels (voxelize.hip, mips.hip, trace.hip) and the C ABI layer (abi.hip).
+//
+// Every kernel is batch-native: it reads per-scene descriptors from device memory and
+// maps its workgroups to (scene, first item) through a block table built on the host. A
+// single scene is a batch of one (its kernels index directly, without the table).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -13,6 +17,9 @@ constexpr int kAccWords = 8;           // uint32 words per accumulator voxel
 constexpr int kAccBrick = 8;           // accumulator storage brick edge (voxels)
 constexpr int kMaxChunk = 64;          // triangles per voxelize work item (LDS field width)
 constexpr int kNormalBias = 1024;      // quantized normal offset in the accumulator
+constexpr int kSmallBins = 8;          // a triangle whose bin box has <= 8 bins keeps its slots
+constexpr uint32_t kBlockTris = 256;   // triangles per setup / emit workgroup
+constexpr uint32_t kBlockBins = 256;   // bins per bins / workmap workgroup
 
 // Per-mesh descriptor, device resident.
 struct MeshDev {
@@ -23,7 +30,7 @@ struct MeshDev {
   uint32_t nverts;
   uint32_t idx_type;
   uint32_t ntri;
-  uint32_t tri_base;  // first global triangle id of this mesh
+  uint32_t tri_base;  // first triangle of this mesh within its scene
   uint32_t base_rgb;  // r | g << 8 | b << 16
   float m[12];
 };
@@ -44,13 +51,16 @@ static_assert(sizeof(TriRec) == 64, "TriRec layout");
 struct DevStats {
   unsigned long long rejected;
   unsigned long long degenerate;
-  uint32_t big_count;   // triangles spanning more than kSmallBins bins
-  uint32_t zero_count;  // bins split over several work items
+  unsigned long long pairs;  // exact (triangle, bin) pair count of the scene
+  uint32_t big_count;        // triangles spanning more than kSmallBins bins
+  uint32_t zero_count;       // bins split over several work items
 };
+static_assert(sizeof(DevStats) == 32, "DevStats layout");
 
-constexpr int kSmallBins = 8;  // a triangle whose bin box has <= 8 bins keeps its slots
-
-struct VoxParams {
+// One scene of a voxelize batch (device memory). acc, mask and bin_packed belong to the
+// scene and stay resident; every other array is this scene's slice of the calling
+// stream's voxelize scratch, valid for the duration of one call.
+struct VoxScene {
   const MeshDev* meshes;
   TriRec* tris;
   uint32_t* tri_slot;              // kSmallBins slots per triangle (small triangles)
@@ -58,23 +68,34 @@ struct VoxParams {
   uint32_t* bin_small;             // nbins: pairs from small triangles (slot counters)
   uint32_t* bin_big;               // nbins: pairs from large triangles
   uint32_t* big_cursor;            // nbins
-  unsigned long long* bin_packed;  // nbins + 1: work items << 32 | pairs
-  unsigned long long* bin_scan;    // nbins + 1: exclusive scan of bin_packed
   uint32_t* zero_list;             // bins with more than one work item
-  uint32_t* work_bin;              // bin of every work item
+  unsigned long long* bin_packed;  // nbins: work items << 32 | pairs (kept with the scene)
   uint32_t* pairs;                 // triangle ids grouped by bin
   uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
   uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
   DevStats* stats;
   float origin[3];
-  float snap_scale;      // float(n * 256) / extent
+  float snap_scale;                // float(n * 256) / extent
   uint32_t nmesh;
   uint32_t ntri;
   uint32_t n;
-  uint32_t bin_shift;    // log2(bin edge in voxels)
-  uint32_t nbins_axis;   // n >> bin_shift
-  uint32_t chunk;        // triangles per work item (<= kMaxChunk)
+  uint32_t bin_shift;              // log2(bin edge in voxels)
+  uint32_t nbins_axis;             // n >> bin_shift
+  uint32_t chunk;                  // triangles per work item (<= kMaxChunk)
   uint32_t pairs_cap;
+  uint32_t bin_base;               // first bin of this scene in the batch-wide arrays
+};
+
+struct VoxBatch {
+  const VoxScene* scenes;
+  const uint2* tri_blocks;          // (scene, first triangle) per setup/emit workgroup
+  const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
+  unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
+  unsigned long long* scan_all;     // exclusive scan of packed_all
+  unsigned long long* totals;       // per scene: items << 32 | pairs; [nscenes] = batch total
+  uint2* work_map;                  // (scene, bin) of every work item
+  uint32_t nscenes;
+  uint32_t total_bins;
 };
 
 struct LightDev {
@@ -84,34 +105,43 @@ struct LightDev {
   float min_distance;
 };
 
-struct MipParams {
+// One scene of a mip batch (device memory).
+struct MipScene {
   const uint32_t* acc;
   const uint32_t* mask;
   const unsigned long long* bin_packed;
-  const float* lut;  // 256 entries: (i/255)^2.2
   void* level[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t n;
   uint32_t levels;
   uint32_t bin_shift;
   uint32_t nbins_axis;
-  uint32_t prec;
-  uint32_t src_level;   // level read by this launch (0 with the accumulator as source)
   uint32_t nlights;
   float origin[3];
   float h;
   LightDev lights[VGI_MAX_LIGHTS];
 };
 
+// Mip workgroup: scene, tile of its source level, source level, levels produced.
+struct MipBlock {
+  uint32_t scene, tile, src, nout;
+};
+
+struct MipBatch {
+  const MipScene* scenes;
+  const MipBlock* blocks;
+  const float* lut;  // 256 entries: (i/255)^2.2
+};
+
+// One scene of a trace batch (device memory).
 struct TraceParams {
   const void* level[VGI_MAX_LEVELS];
   uint64_t level_bytes[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t levels;
   uint32_t n;
-  uint32_t prec;
   uint32_t width, height;
-  uint32_t tile_w, tile_h, tiles_x, tiles_y;
+  uint32_t tiles_x, tiles_y;
   uint32_t num_diffuse;
   uint32_t specular;
   float origin[3];
@@ -126,7 +156,16 @@ struct TraceParams {
   const float* rough;
   float4* out_d;
   float4* out_s;
-  unsigned long long* counter;  // trilinear gathers (may be null)
+};
+
+struct TraceBatch {
+  const TraceParams* scenes;
+  const uint2* blocks;            // (scene, tile) per workgroup, in XCD-band order
+  unsigned long long* counter;    // trilinear gathers (may be null)
+  uint32_t nblocks;
+  uint32_t tile_w, tile_h;
+  uint32_t prec;
+  uint32_t nscenes;
 };
 
 // Texel bytes per precision.
@@ -135,17 +174,18 @@ inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u 
 // ---- launchers (each returns hipError_t) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
-hipError_t launch_setup(const VoxParams& P, hipStream_t s);
-hipError_t launch_big(const VoxParams& P, bool emit, uint32_t blocks, hipStream_t s);
-hipError_t launch_bins(const VoxParams& P, hipStream_t s);
+hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t s);
+hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
-hipError_t launch_workmap(const VoxParams& P, hipStream_t s);
-hipError_t launch_emit_small(const VoxParams& P, hipStream_t s);
-hipError_t launch_zero(const VoxParams& P, uint32_t blocks, hipStream_t s);
-hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
+hipError_t launch_totals(const VoxBatch& B, hipStream_t s);
+hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s);
-hipError_t launch_mips(const MipParams& P, hipStream_t s, uint32_t* produced_top);
-hipError_t launch_trace(const TraceParams& P, hipStream_t s);
+hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s);
+hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s);
 
 }  // namespace vgi

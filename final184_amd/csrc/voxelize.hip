@@ -5,29 +5,33 @@
 // imageStore with last-writer-wins (main.lua:242-275), issued one draw per primitive by
 // CVoxelizeRenderer::Render (Foreground/Renderer/VoxelizeRenderer.cpp:81-127).
 //
-// Pipeline (one stream; the host reads one 8-byte total between steps 4 and 5):
+// Every kernel serves a batch of scenes (one scene is a batch of one): per-triangle and
+// per-bin kernels map workgroups to (scene, first item) through a block table, and the
+// scenes' bins are concatenated so that one scan and one persistent voxelize launch
+// cover the whole batch. Pipeline (one stream; the host reads the per-scene totals
+// between steps 4 and 5 to size the pair lists):
 //   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
 //                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
 //                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
 //                   (bin = B^3 voxels) tests each bin exactly and takes a slot in every
 //                   overlapped bin with a wave-aggregated atomic (the slot is kept);
 //                   larger triangles go to a list.
-//   2 k_big_count   one workgroup per listed triangle: its bins, tested cooperatively.
+//   2 k_big<false>  one workgroup per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
 //                   packed as (items << 32 | pairs); bins with > 1 item are listed.
-//   4 scan          one 64-bit exclusive scan gives every bin its pair offset and its
-//                   first work item at once.
-//   5 k_emit_small  scatter small triangles to their kept slots (no atomics);
-//     k_emit_big    large triangles append after the small ones of each bin.
+//   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
+//                   offset and its first work item at once; k_totals per scene.
+//   5 k_workmap     work item -> (scene, bin);
+//     k_emit_small  scatter small triangles to their kept slots (no atomics);
+//     k_big<true>   large triangles append after the small ones of each bin.
 //   6 k_zero        clear the accumulator and occupancy mask of multi-item bins.
-//   7 k_voxelize    persistent loop over work items (bin found by binary search over the
-//                   scanned item offsets); per item: chunk triangles' edge/plane
-//                   coefficients to LDS, columns of the dominant-axis projection spread
-//                   over the workgroup, each column tests its 1-4 candidate voxels with the
-//                   exact separating-axis test and adds packed 64-bit increments into a
-//                   per-tile LDS accumulator; flush: occupancy mask + the records of
-//                   occupied voxels, by plain stores for single-item bins and by integer
-//                   atomics otherwise.
+//   7 k_voxelize    persistent loop over the batch's work items; per item: chunk
+//                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
+//                   projection spread over the workgroup, each column tests its 1-4
+//                   candidate voxels with the exact separating-axis test and adds packed
+//                   64-bit increments into a per-tile LDS accumulator; flush: occupancy
+//                   mask + the records of occupied voxels, by plain stores for single-item
+//                   bins and by integer atomics otherwise.
 //
 // All coverage decisions are exact integer arithmetic on snapped vertices, so the
 // voxel set and the integer sums do not depend on order, rounding or thread mapping.
@@ -155,14 +159,31 @@ __device__ __forceinline__ BinBox bin_box(const TriRec& R, uint32_t bs) {
 }
 
 // j-th bin of a bin box (j in [0, count)), and whether the triangle overlaps it.
-__device__ __forceinline__ bool bin_candidate(const VoxParams& P, const int64_t (&q)[3][3],
+__device__ __forceinline__ bool bin_candidate(const VoxScene& S, const int64_t (&q)[3][3],
                                               const int64_t (&N)[3], const BinBox& bb, int j,
                                               uint32_t& bin) {
   const int bx = bb.x0 + j % bb.nx, by = bb.y0 + (j / bb.nx) % bb.ny, bz = bb.z0 + j / (bb.nx * bb.ny);
-  const int64_t d = (int64_t)kFixedOne << P.bin_shift;
+  const int64_t d = (int64_t)kFixedOne << S.bin_shift;
   const int64_t b[3] = {(int64_t)bx * d, (int64_t)by * d, (int64_t)bz * d};
-  bin = (uint32_t)bx + P.nbins_axis * ((uint32_t)by + P.nbins_axis * (uint32_t)bz);
+  bin = (uint32_t)bx + S.nbins_axis * ((uint32_t)by + S.nbins_axis * (uint32_t)bz);
   return sat_box(q, N, b, d);
+}
+
+// Workgroup -> (scene, first item) for per-triangle and per-bin kernels.
+__device__ __forceinline__ uint2 tri_block(const VoxBatch& B) {
+  return B.nscenes == 1 ? make_uint2(0u, blockIdx.x * kBlockTris) : B.tri_blocks[blockIdx.x];
+}
+__device__ __forceinline__ uint2 bin_block(const VoxBatch& B) {
+  return B.nscenes == 1 ? make_uint2(0u, blockIdx.x * kBlockBins) : B.bin_blocks[blockIdx.x];
+}
+
+// Offset of bin b's first pair within its scene's pair list, and its first work item
+// within the batch.
+__device__ __forceinline__ uint32_t pair_base(const VoxBatch& B, const VoxScene& S, uint32_t b) {
+  return (uint32_t)B.scan_all[S.bin_base + b] - (uint32_t)B.scan_all[S.bin_base];
+}
+__device__ __forceinline__ uint32_t first_item(const VoxBatch& B, const VoxScene& S, uint32_t b) {
+  return (uint32_t)(B.scan_all[S.bin_base + b] >> 32);
 }
 
 // Wave-aggregated atomic increment with per-lane keys: every active lane gets a distinct
@@ -192,7 +213,7 @@ __device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bo
 // 1 setup
 // ---------------------------------------------------------------------------
 
-__device__ void make_record(const VoxParams& P, uint32_t t, TriRec& R) {
+__device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
   const MeshDev& M = P.meshes[find_mesh(P.meshes, P.nmesh, t)];
   const uint32_t lt = t - M.tri_base;
   uint32_t vid[3];
@@ -275,8 +296,10 @@ __device__ void make_record(const VoxParams& P, uint32_t t, TriRec& R) {
   R.rgb = rgb | (status << 24);
 }
 
-__global__ __launch_bounds__(256) void k_setup(VoxParams P) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_setup(VoxBatch B) {
+  const uint2 blk = tri_block(B);  // a workgroup never spans two scenes
+  const VoxScene& P = B.scenes[blk.x];
+  const uint32_t t = blk.y + threadIdx.x;
   const bool in = t < P.ntri;
   TriRec R;
   bool small = false;
@@ -306,66 +329,76 @@ __global__ __launch_bounds__(256) void k_setup(VoxParams P) {
   }
 }
 
-hipError_t launch_setup(const VoxParams& P, hipStream_t s) {
-  if (P.ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
+hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(kBlockTris), 0, s, B);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// 2 large triangles: count their bins cooperatively
+// 2 large triangles: count (then scatter) their bins, one workgroup per triangle
 // ---------------------------------------------------------------------------
 
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_big(VoxParams P) {
-  const uint32_t nbig = P.stats->big_count;
-  for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
-    const uint32_t t = P.big_list[i];
-    const TriRec R = P.tris[t];
-    int64_t q[3][3], N[3];
-    load_q(R, q, N);
-    const BinBox bb = bin_box(R, P.bin_shift);
-    const int nbb = bb.count();
-    for (int j = threadIdx.x; j < nbb; j += blockDim.x) {
-      uint32_t bin;
-      if (!bin_candidate(P, q, N, bb, j, bin)) continue;
-      if (!EMIT) {
-        atomicAdd(&P.bin_big[bin], 1u);
-      } else {
-        const uint32_t pos = (uint32_t)P.bin_scan[bin] + P.bin_small[bin] + atomicAdd(&P.big_cursor[bin], 1u);
-        if (pos < P.pairs_cap) P.pairs[pos] = t;
+__global__ __launch_bounds__(256) void k_big(VoxBatch B) {
+  for (uint32_t sc = 0; sc < B.nscenes; sc++) {
+    const VoxScene& P = B.scenes[sc];
+    const uint32_t nbig = P.stats->big_count;
+    for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+      const uint32_t t = P.big_list[i];
+      const TriRec R = P.tris[t];
+      int64_t q[3][3], N[3];
+      load_q(R, q, N);
+      const BinBox bb = bin_box(R, P.bin_shift);
+      const int nbb = bb.count();
+      for (int j = threadIdx.x; j < nbb; j += blockDim.x) {
+        uint32_t bin;
+        if (!bin_candidate(P, q, N, bb, j, bin)) continue;
+        if (!EMIT) {
+          atomicAdd(&P.bin_big[bin], 1u);
+        } else {
+          const uint32_t pos = pair_base(B, P, bin) + P.bin_small[bin] + atomicAdd(&P.big_cursor[bin], 1u);
+          if (pos < P.pairs_cap) P.pairs[pos] = t;
+        }
       }
     }
   }
 }
 
-hipError_t launch_big(const VoxParams& P, bool emit, uint32_t blocks, hipStream_t s) {
-  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, P);
-  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, P);
+hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t s) {
+  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, B);
+  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// 3 per-bin totals, 4 scan
+// 3 per-bin totals, 4 scan over every bin of the batch
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_bins(VoxParams P) {
+__global__ __launch_bounds__(256) void k_bins(VoxBatch B) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.packed_all[B.total_bins] = 0ull;
+  const uint2 blk = bin_block(B);
+  const VoxScene& P = B.scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > nbins) return;
-  if (b == nbins) {
-    P.bin_packed[b] = 0ull;
-    return;
+  const uint32_t b = blk.y + threadIdx.x;
+  unsigned long long total = 0;
+  if (b < nbins) {
+    const uint32_t pairs = P.bin_small[b] + P.bin_big[b];
+    const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
+    const unsigned long long packed = ((unsigned long long)items << 32) | pairs;
+    P.bin_packed[b] = packed;
+    B.packed_all[P.bin_base + b] = packed;
+    if (items > 1) P.zero_list[atomicAdd(&P.stats->zero_count, 1u)] = b;
+    total = pairs;
   }
-  const uint32_t pairs = P.bin_small[b] + P.bin_big[b];
-  const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
-  P.bin_packed[b] = ((unsigned long long)items << 32) | pairs;
-  if (items > 1) P.zero_list[atomicAdd(&P.stats->zero_count, 1u)] = b;
+  // exact 64-bit pair count per scene (the packed scan keeps only 32 bits of it)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
+  if (__lane_id() == 0 && total) atomicAdd(&P.stats->pairs, total);
 }
 
-hipError_t launch_bins(const VoxParams& P, hipStream_t s) {
-  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  hipLaunchKernelGGL(k_bins, dim3((nbins + 1 + 255) / 256), dim3(256), 0, s, P);
+hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_bins, dim3(blocks), dim3(kBlockBins), 0, s, B);
   return hipGetLastError();
 }
 
@@ -374,19 +407,38 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
   return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
 }
 
-// work item -> bin, from the scanned item offsets
-__global__ __launch_bounds__(256) void k_workmap(VoxParams P) {
+// per scene: (items << 32 | pairs) of its bins, and the batch total in [nscenes]
+__global__ void k_totals(VoxBatch B) {
+  const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sc > B.nscenes) return;
+  if (sc == B.nscenes) {
+    B.totals[sc] = B.scan_all[B.total_bins];
+    return;
+  }
+  const VoxScene& P = B.scenes[sc];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  const uint32_t items = (uint32_t)(P.bin_packed[b] >> 32);
-  const uint32_t first = (uint32_t)(P.bin_scan[b] >> 32);
-  for (uint32_t i = 0; i < items; i++) P.work_bin[first + i] = b;
+  B.totals[sc] = B.scan_all[P.bin_base + nbins] - B.scan_all[P.bin_base];
 }
 
-hipError_t launch_workmap(const VoxParams& P, hipStream_t s) {
+hipError_t launch_totals(const VoxBatch& B, hipStream_t s) {
+  hipLaunchKernelGGL(k_totals, dim3((B.nscenes + 1 + 255) / 256), dim3(256), 0, s, B);
+  return hipGetLastError();
+}
+
+// work item -> (scene, bin), from the scanned item offsets
+__global__ __launch_bounds__(256) void k_workmap(VoxBatch B) {
+  const uint2 blk = bin_block(B);
+  const VoxScene& P = B.scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  hipLaunchKernelGGL(k_workmap, dim3((nbins + 255) / 256), dim3(256), 0, s, P);
+  const uint32_t b = blk.y + threadIdx.x;
+  if (b >= nbins) return;
+  const uint32_t items = (uint32_t)(P.bin_packed[b] >> 32);
+  const uint32_t first = first_item(B, P, b);
+  for (uint32_t i = 0; i < items; i++) B.work_map[first + i] = make_uint2(blk.x, b);
+}
+
+hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_workmap, dim3(blocks), dim3(kBlockBins), 0, s, B);
   return hipGetLastError();
 }
 
@@ -394,8 +446,10 @@ hipError_t launch_workmap(const VoxParams& P, hipStream_t s) {
 // 5 scatter small triangles into their slots
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_emit_small(VoxParams P) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_emit_small(VoxBatch B) {
+  const uint2 blk = tri_block(B);
+  const VoxScene& P = B.scenes[blk.x];
+  const uint32_t t = blk.y + threadIdx.x;
   if (t >= P.ntri) return;
   const TriRec R = P.tris[t];
   if (!tri_ok(R)) return;
@@ -407,14 +461,14 @@ __global__ __launch_bounds__(256) void k_emit_small(VoxParams P) {
   for (int j = 0; j < bb.count(); j++) {
     uint32_t bin;
     if (!bin_candidate(P, q, N, bb, j, bin)) continue;
-    const uint32_t pos = (uint32_t)P.bin_scan[bin] + P.tri_slot[(size_t)t * kSmallBins + nslot++];
+    const uint32_t pos = pair_base(B, P, bin) + P.tri_slot[(size_t)t * kSmallBins + nslot++];
     if (pos < P.pairs_cap) P.pairs[pos] = t;
   }
 }
 
-hipError_t launch_emit_small(const VoxParams& P, hipStream_t s) {
-  if (P.ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit_small, dim3((P.ntri + 255) / 256), dim3(256), 0, s, P);
+hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit_small, dim3(blocks), dim3(kBlockTris), 0, s, B);
   return hipGetLastError();
 }
 
@@ -428,24 +482,27 @@ __device__ __forceinline__ size_t acc_index(uint32_t x, uint32_t y, uint32_t z, 
   return ((size_t)brick * 512u + local) * kAccWords;
 }
 
-__global__ __launch_bounds__(256) void k_zero(VoxParams P) {
-  const uint32_t nz = P.stats->zero_count;
-  const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
-  for (uint32_t i = blockIdx.x; i < nz; i += gridDim.x) {
-    const uint32_t bin = P.zero_list[i];
-    const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
-    for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
-      const uint32_t bxx = bx * bpb + k % bpb, byy = by * bpb + (k / bpb) % bpb, bzz = bz * bpb + k / (bpb * bpb);
-      const uint32_t brick = bxx + nbr * (byy + nbr * bzz);
-      uint4* dst = (uint4*)(P.acc + (size_t)brick * 512u * kAccWords);
-      for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) dst[j] = make_uint4(0, 0, 0, 0);
-      if (threadIdx.x < 4) ((uint4*)(P.mask + (size_t)brick * 16u))[threadIdx.x] = make_uint4(0, 0, 0, 0);
+__global__ __launch_bounds__(256) void k_zero(VoxBatch Bt) {
+  for (uint32_t sc = 0; sc < Bt.nscenes; sc++) {
+    const VoxScene& P = Bt.scenes[sc];
+    const uint32_t nz = P.stats->zero_count;
+    const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
+    for (uint32_t i = blockIdx.x; i < nz; i += gridDim.x) {
+      const uint32_t bin = P.zero_list[i];
+      const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
+      for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
+        const uint32_t bxx = bx * bpb + k % bpb, byy = by * bpb + (k / bpb) % bpb, bzz = bz * bpb + k / (bpb * bpb);
+        const uint32_t brick = bxx + nbr * (byy + nbr * bzz);
+        uint4* dst = (uint4*)(P.acc + (size_t)brick * 512u * kAccWords);
+        for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) dst[j] = make_uint4(0, 0, 0, 0);
+        if (threadIdx.x < 4) ((uint4*)(P.mask + (size_t)brick * 16u))[threadIdx.x] = make_uint4(0, 0, 0, 0);
+      }
     }
   }
 }
 
-hipError_t launch_zero(const VoxParams& P, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, P);
+hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 
@@ -535,7 +592,7 @@ __device__ __forceinline__ bool edge_ok(const TriCoef& C, int u, int va, int vb)
 }
 
 template <int TS>
-__global__ __launch_bounds__(256) void k_voxelize(VoxParams P, uint32_t nwork) {
+__global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int BT = (TS / 8) * (TS / 8) * (TS / 8);  // 8^3 accumulator bricks per tile
   __shared__ unsigned long long s_acc[2 * TV];
@@ -544,15 +601,18 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P, uint32_t nwork) {
   __shared__ uint32_t s_mask[BT * 16];
 
   const uint32_t tid = threadIdx.x;
-  const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
-  const uint32_t sub = B / TS;  // sub-tiles per bin edge
 
+  // persistent loop over the work items of every scene of the batch
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint32_t bin = P.work_bin[w];
+    const uint2 item = Bt.work_map[w];
+    const VoxScene& P = Bt.scenes[item.x];
+    const uint32_t bin = item.y;
+    const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
+    const uint32_t sub = B / TS;  // sub-tiles per bin edge
     const unsigned long long packed = P.bin_packed[bin];
     const uint32_t cnt_bin = (uint32_t)packed, nch = (uint32_t)(packed >> 32);
-    const uint32_t chunk = w - (uint32_t)(P.bin_scan[bin] >> 32);
-    const uint32_t first = (uint32_t)P.bin_scan[bin] + chunk * P.chunk;
+    const uint32_t chunk = w - first_item(Bt, P, bin);
+    const uint32_t first = pair_base(Bt, P, bin) + chunk * P.chunk;
     const uint32_t cnt = min(P.chunk, cnt_bin - chunk * P.chunk);
     const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
 
@@ -677,11 +737,11 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxParams P, uint32_t nwork) {
   }
 }
 
-hipError_t launch_voxelize(const VoxParams& P, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
+hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s) {
   if (grid_blocks == 0 || nwork == 0) return hipSuccess;
-  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, P, nwork);
-  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, P, nwork);
+  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, nwork);
+  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, nwork);
   return hipGetLastError();
 }
 

@@ -189,6 +189,42 @@ def room(seed: int = SEED_ROOM, ntri: int = 262_144, n: int = 256, levels: int =
         meta=dict(seed=seed, camera=cam.tolist()))
 
 
+SEED_BATCH = 0xF1840004
+
+
+def batch_scenes(seed: int = SEED_BATCH, count: int = 64, median: float = 20_000.0,
+                 sigma: float = 1.0, tri_range=(1_000, 200_000), grids=(32, 64, 128, 256),
+                 sides=(256, 512, 1024), num_diffuse: int = 6) -> list[Workload]:
+    """Independent scenes of uneven size for one batch: triangle counts lognormal (median,
+    sigma) clipped to tri_range, grid edge and square G-buffer side drawn uniformly from
+    the given sets (mip chain down to 2^3), seeded random triangle soups lit by one
+    directional light, G-buffer points sampled on the triangles."""
+    rng = np.random.default_rng(seed)
+    counts = lognormal_ints(seed, count, median, sigma, tri_range[0], tri_range[1])
+    out = []
+    for i in range(count):
+        n = int(rng.choice(grids))
+        side = int(rng.choice(sides))
+        s = seed + 0x9E37 * (i + 1)
+        pos, alb = random_triangles(s, int(counts[i]))
+        gp, gn = gbuffer_on_triangles(s, pos, side * side)
+        d = rng.normal(size=3)
+        d[1] = -abs(d[1]) - 0.5
+        d /= np.linalg.norm(d)
+        out.append(Workload(
+            name=f"batch{i}_{int(counts[i])}tri_{n}^3_{side}^2",
+            positions=pos, albedo=alb,
+            grid=dict(origin=(0.0, 0.0, 0.0), extent=1.0, n=n, levels=int(math.log2(n))),
+            lights=[dict(type="directional", vec=tuple(float(v) for v in d), color=(1.0, 1.0, 1.0))],
+            gb_pos=gp, gb_nrm=gn, width=side, height=side,
+            cones=dict(num_diffuse=num_diffuse, specular=True, diffuse_aperture_deg=60.0,
+                       diffuse_ring_deg=60.0, specular_aperture_deg=10.0, step_factor=0.5,
+                       max_distance=1.0, alpha_cutoff=0.99, normal_offset=1.0, start_offset=1.0,
+                       eye=(0.5, 0.5, 2.0)),
+            meta=dict(seed=s)))
+    return out
+
+
 def cone_samples_estimate(h: float, tan_half: float, step: float, start: float, max_d: float) -> int:
     """Number of samples of one cone with no early exit (closed-form schedule)."""
     t, m = start * h, 0

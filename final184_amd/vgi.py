@@ -87,6 +87,20 @@ class Timings(C.Structure):
                 ("trace_trilinear_samples", C.c_uint64)]
 
 
+class VoxelizeItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("meshes", C.c_void_p), ("num_meshes", C.c_uint32),
+                ("grid", Grid)]
+
+
+class MipsItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("lights", C.c_void_p)]
+
+
+class TraceItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("gbuffer", GBuffer), ("cones", ConeParams),
+                ("out_diffuse", C.c_void_p), ("out_specular", C.c_void_p)]
+
+
 # (name, restype, argtypes) for every exported entry point of include/vgi.h
 _P = C.c_void_p
 SIGNATURES = [
@@ -107,6 +121,9 @@ SIGNATURES = [
     ("vgi_voxelize", C.c_int, [_P, _P, C.POINTER(Mesh), C.c_uint32, C.POINTER(Grid), _P]),
     ("vgi_build_mips", C.c_int, [_P, _P, C.POINTER(Lights), C.c_uint32, _P]),
     ("vgi_trace", C.c_int, [_P, _P, C.POINTER(GBuffer), C.POINTER(ConeParams), _P, _P, _P]),
+    ("vgi_voxelize_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.c_uint32, _P]),
+    ("vgi_build_mips_batch", C.c_int, [_P, C.POINTER(MipsItem), C.c_uint32, C.c_uint32, _P]),
+    ("vgi_trace_batch", C.c_int, [_P, C.POINTER(TraceItem), C.c_uint32, _P]),
     ("vgi_scene_read_voxels", C.c_int, [_P, _P, _P]),
     ("vgi_scene_read_level", C.c_int, [_P, _P, C.c_uint32, _P]),
     ("vgi_malloc", C.c_int, [_P, C.POINTER(_P), C.c_size_t]),
@@ -117,6 +134,7 @@ SIGNATURES = [
     ("vgi_memcpy_d2d", C.c_int, [_P, _P, _P, C.c_size_t, _P]),
     ("vgi_stream_create", C.c_int, [_P, C.POINTER(_P)]),
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
+    ("vgi_workspace_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("vgi_timer_start", C.c_int, [_P, _P]),
     ("vgi_timer_stop", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
 ]
@@ -392,6 +410,50 @@ class Context:
 
     def stream_destroy(self, s) -> None:
         _check(lib.vgi_stream_destroy(self.handle, s))
+
+    def workspace_bytes(self) -> int:
+        b = C.c_uint64()
+        _check(lib.vgi_workspace_bytes(self.handle, C.byref(b)))
+        return int(b.value)
+
+    def voxelize_batch(self, jobs, stream=None) -> None:
+        """jobs: sequence of (Scene, [MeshData | Mesh], Grid); one batched voxelize pass."""
+        items = (VoxelizeItem * max(1, len(jobs)))()
+        keep = []
+        for i, (scene, meshes, grid) in enumerate(jobs):
+            arr = (Mesh * max(1, len(meshes)))()
+            for k, m in enumerate(meshes):
+                arr[k] = m.desc if isinstance(m, MeshData) else m
+            keep.append(arr)
+            items[i].scene = scene.handle
+            items[i].meshes = C.cast(arr, C.c_void_p)
+            items[i].num_meshes = len(meshes)
+            items[i].grid = grid
+        _check(lib.vgi_voxelize_batch(self.handle, items, len(jobs), stream))
+        for scene, _, grid in jobs:
+            scene.grid = grid
+
+    def build_mips_batch(self, jobs, precision: int = PREC_FP32, stream=None) -> None:
+        """jobs: sequence of (Scene, Lights | None)."""
+        items = (MipsItem * max(1, len(jobs)))()
+        keep = []
+        for i, (scene, lights) in enumerate(jobs):
+            items[i].scene = scene.handle
+            if lights is not None:
+                keep.append(lights)
+                items[i].lights = C.addressof(lights)
+        _check(lib.vgi_build_mips_batch(self.handle, items, len(jobs), precision, stream))
+
+    def trace_batch(self, jobs, stream=None) -> None:
+        """jobs: sequence of (Scene, GBufferData, ConeParams, out_diffuse, out_specular | None)."""
+        items = (TraceItem * max(1, len(jobs)))()
+        for i, (scene, gb, cones, out_d, out_s) in enumerate(jobs):
+            items[i].scene = scene.handle
+            items[i].gbuffer = gb.desc
+            items[i].cones = cones
+            items[i].out_diffuse = out_d.ptr
+            items[i].out_specular = out_s.ptr if out_s is not None else None
+        _check(lib.vgi_trace_batch(self.handle, items, len(jobs), stream))
 
     def timer_start(self, stream=None) -> None:
         _check(lib.vgi_timer_start(self.handle, stream))

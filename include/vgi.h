@@ -195,7 +195,10 @@ int vgi_sync(vgi_context* ctx, void* stream);
 int vgi_scene_create(vgi_context* ctx, vgi_scene** out);
 int vgi_scene_release(vgi_scene* scene);
 int vgi_scene_get_stats(const vgi_scene* scene, vgi_scene_stats* out);
-/* Device bytes a scene of this grid and precision will hold (admission control). */
+/* Device bytes a scene of this grid and precision holds while resident: accumulator,
+ * occupancy mask, bin table and mip levels (admission control). The voxelize scratch
+ * (about 112 bytes per triangle, 4 per bin pair and 40 per bin) belongs to the stream
+ * the call runs on and is reported by vgi_workspace_bytes. num_triangles is unused. */
 uint64_t vgi_scene_bytes_estimate(const vgi_grid* grid, uint32_t precision, uint64_t num_triangles);
 
 /* Clear + voxelize all meshes into the scene's accumulator (one frame's voxelize pass).
@@ -218,6 +221,37 @@ int vgi_trace(vgi_context* ctx, const vgi_scene* scene, const vgi_gbuffer* gbuff
               const vgi_cone_params* cones, float* out_diffuse, float* out_specular,
               void* stream);
 
+/* ---- batches: many scenes through one launch per stage ---------------------- */
+/* Scenes of uneven triangle count and grid size are voxelized together: one setup
+ * launch, one scan over all their bins and one persistent voxelize launch over all
+ * their work items (a scene may appear once per batch). No reference counterpart: the
+ * reference voxelizes one scene with one draw per primitive (VoxelizeRenderer.cpp:19-30). */
+typedef struct vgi_voxelize_item {
+  vgi_scene* scene;
+  const vgi_mesh* meshes;
+  uint32_t num_meshes;
+  vgi_grid grid;
+} vgi_voxelize_item;
+
+typedef struct vgi_mips_item {
+  vgi_scene* scene;
+  const vgi_lights* lights; /* may be NULL: no direct light */
+} vgi_mips_item;
+
+typedef struct vgi_trace_item {
+  const vgi_scene* scene;
+  vgi_gbuffer gbuffer;
+  vgi_cone_params cones;
+  float* out_diffuse;
+  float* out_specular; /* may be NULL */
+} vgi_trace_item;
+
+int vgi_voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t count, void* stream);
+int vgi_build_mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count,
+                         uint32_t precision, void* stream);
+/* Scenes whose mips have different precisions are traced in one launch per precision. */
+int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream);
+
 /* ---- parity readback (synchronous) ----------------------------------------- */
 /* Accumulator as n^3 voxels x 8 uint32 (x fastest): R sum, G sum, B sum, count,
  * sum(nx+1024), sum(ny+1024), sum(nz+1024), 0 (normals quantized to 1/1024). */
@@ -236,7 +270,11 @@ int vgi_memset(vgi_context* ctx, void* dst, int value, size_t bytes);
  * calibrating HBM traffic counters. */
 int vgi_memcpy_d2d(vgi_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 int vgi_stream_create(vgi_context* ctx, void** out);
+/* Synchronizes and destroys the stream, and frees the scratch the context held for it. */
 int vgi_stream_destroy(vgi_context* ctx, void* stream);
+/* Device bytes of voxelize scratch and descriptor staging the context holds, summed over
+ * the streams it has run on (each stream keeps its own, grown to its largest batch). */
+int vgi_workspace_bytes(vgi_context* ctx, uint64_t* out);
 /* Device-side elapsed time between two points on a stream (HIP events). */
 int vgi_timer_start(vgi_context* ctx, void* stream);
 int vgi_timer_stop(vgi_context* ctx, void* stream, float* out_ms);

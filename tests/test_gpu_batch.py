@@ -1,0 +1,146 @@
+"""Batched passes: every scene of a batch must come out exactly as it does alone.
+
+One voxelize / mips / trace call over many scenes of uneven size (different triangle
+counts, grids, level counts, G-buffer sizes, precisions, an empty scene) is compared
+with the same scenes run one call each: accumulator records, every mip level and the
+traced outputs must be identical bit for bit (the kernels do the same integer and float
+operations per scene; only the workgroup-to-scene mapping differs). The single-scene
+path is itself held to the oracle by the other GPU tests.
+"""
+import numpy as np
+import pytest
+
+from final184_amd import scenegen as sg
+
+pytestmark = pytest.mark.gpu
+
+
+def _vgi():
+    from final184_amd import vgi
+    return vgi
+
+
+def _workloads():
+    w = sg.batch_scenes(count=5, median=3000.0, tri_range=(200, 12_000), grids=(16, 32, 64),
+                        sides=(24, 40, 64))
+    # an empty scene and a scene whose grid is smaller than one bin of 16
+    empty = sg.batch_scenes(seed=7, count=1, median=10, tri_range=(1, 1), grids=(8,), sides=(16,))[0]
+    empty.positions = empty.positions[:0]
+    empty.albedo = empty.albedo[:0]
+    w.append(empty)
+    # different level counts than the default chain
+    w[0].grid["levels"] = 1
+    w[1].grid["levels"] = 3
+    return w
+
+
+class Job:
+    def __init__(self, ctx, w):
+        vgi = _vgi()
+        self.w = w
+        self.meshes = [vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)] \
+            if w.num_triangles else []
+        self.grid = vgi.make_grid(**w.grid)
+        self.lights = vgi.make_lights(w.lights)
+        self.gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height)
+        self.cones = vgi.make_cones(**w.cones)
+        self.od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        self.os = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        self.scene = ctx.scene()
+
+    def levels(self):
+        return [self.scene.read_level(k) for k in range(self.w.grid["levels"])]
+
+
+def _run_alone(ctx, jobs, precs):
+    for j, p in zip(jobs, precs):
+        j.scene.voxelize(j.meshes, j.grid)
+        j.scene.build_mips(j.lights, p)
+        j.scene.trace(j.gb, j.cones, j.od, j.os)
+    ctx.sync()
+
+
+def _run_batch(ctx, jobs, precs, stream=None):
+    ctx.voxelize_batch([(j.scene, j.meshes, j.grid) for j in jobs], stream)
+    for p in sorted(set(precs)):
+        ctx.build_mips_batch([(j.scene, j.lights) for j, q in zip(jobs, precs) if q == p], p, stream)
+    ctx.trace_batch([(j.scene, j.gb, j.cones, j.od, j.os) for j in jobs], stream)
+    ctx.sync(stream)
+
+
+def _compare(a, b):
+    for ja, jb in zip(a, b):
+        name = ja.w.name
+        assert ja.scene.stats().bin_pairs == jb.scene.stats().bin_pairs, name
+        assert ja.scene.stats().work_items == jb.scene.stats().work_items, name
+        assert np.array_equal(ja.scene.read_voxels(), jb.scene.read_voxels()), f"{name}: voxels differ"
+        for k, (la, lb) in enumerate(zip(ja.levels(), jb.levels())):
+            assert np.array_equal(la, lb), f"{name}: level {k} differs"
+        assert np.array_equal(ja.od.numpy(), jb.od.numpy()), f"{name}: diffuse differs"
+        assert np.array_equal(ja.os.numpy(), jb.os.numpy()), f"{name}: specular differs"
+
+
+@pytest.mark.parametrize("bin_size,tile", [(8, (16, 16)), (16, (32, 8)), (32, (8, 8))])
+def test_batch_equals_alone(ctx, bin_size, tile):
+    vgi = _vgi()
+    print(f"case batch_equals_alone bin={bin_size} tile={tile}", flush=True)
+    ctx.set_tuning(bin_size=bin_size, chunk_tris=64, tile_w=tile[0], tile_h=tile[1])
+    try:
+        ws = _workloads()
+        precs = [vgi.PREC_FP16, vgi.PREC_FP32, vgi.PREC_FP16, vgi.PREC_BF16, vgi.PREC_FP16, vgi.PREC_FP32]
+        alone = [Job(ctx, w) for w in ws]
+        batch = [Job(ctx, w) for w in ws]
+        _run_alone(ctx, alone, precs)
+        _run_batch(ctx, batch, precs)
+        _compare(alone, batch)
+        # the empty scene holds nothing
+        assert not batch[-1].scene.read_voxels().any()
+        # a second batch call reuses the workspace and the scenes' allocations
+        _run_batch(ctx, batch, precs)
+        _compare(alone, batch)
+    finally:
+        ctx.set_tuning()
+
+
+def test_batch_two_streams(ctx):
+    """Two batches on two streams at once give the same results as one after the other."""
+    vgi = _vgi()
+    print("case batch_two_streams", flush=True)
+    ws = _workloads()
+    precs = [vgi.PREC_FP16] * len(ws)
+    alone = [Job(ctx, w) for w in ws]
+    _run_alone(ctx, alone, precs)
+    s1, s2 = ctx.stream_create(), ctx.stream_create()
+    try:
+        a = [Job(ctx, w) for w in ws[:3]]
+        b = [Job(ctx, w) for w in ws[3:]]
+        ctx.voxelize_batch([(j.scene, j.meshes, j.grid) for j in a], s1)
+        ctx.voxelize_batch([(j.scene, j.meshes, j.grid) for j in b], s2)
+        ctx.build_mips_batch([(j.scene, j.lights) for j in a], vgi.PREC_FP16, s1)
+        ctx.build_mips_batch([(j.scene, j.lights) for j in b], vgi.PREC_FP16, s2)
+        ctx.trace_batch([(j.scene, j.gb, j.cones, j.od, j.os) for j in a], s1)
+        ctx.trace_batch([(j.scene, j.gb, j.cones, j.od, j.os) for j in b], s2)
+        ctx.sync(s1)
+        ctx.sync(s2)
+        assert ctx.workspace_bytes() > 0
+        _compare(alone, a + b)
+    finally:
+        ctx.stream_destroy(s1)
+        ctx.stream_destroy(s2)
+
+
+def test_batch_rejects_duplicates(ctx):
+    vgi = _vgi()
+    w = _workloads()[2]
+    j = Job(ctx, w)
+    with pytest.raises(vgi.VgiError) as e:
+        ctx.voxelize_batch([(j.scene, j.meshes, j.grid), (j.scene, j.meshes, j.grid)])
+    assert e.value.code == -1
+    j.scene.voxelize(j.meshes, j.grid)
+    with pytest.raises(vgi.VgiError):
+        ctx.build_mips_batch([(j.scene, j.lights), (j.scene, j.lights)], vgi.PREC_FP16)
+    # a scene that is not voxelized cannot join a mips batch
+    other = ctx.scene()
+    with pytest.raises(vgi.VgiError) as e:
+        ctx.build_mips_batch([(j.scene, j.lights), (other, j.lights)], vgi.PREC_FP16)
+    assert e.value.code == -5

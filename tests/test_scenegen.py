@@ -60,3 +60,17 @@ def test_lognormal_draws_are_clipped():
     x = sg.lognormal_ints(3, 10_000, 20_000, 1.0, 1_000, 200_000)
     assert x.min() >= 1_000 and x.max() <= 200_000
     assert 15_000 < np.median(x) < 25_000
+
+
+def test_batch_scenes_shapes_and_determinism():
+    a = sg.batch_scenes(count=12, median=2000.0, tri_range=(100, 5000))
+    b = sg.batch_scenes(count=12, median=2000.0, tri_range=(100, 5000))
+    for x, y in zip(a, b):
+        assert x.name == y.name and np.array_equal(x.positions, y.positions)
+        assert 100 <= x.num_triangles <= 5000
+        assert x.grid["n"] in (32, 64, 128, 256) and 2 ** x.grid["levels"] == x.grid["n"]
+        assert x.width == x.height and x.width in (256, 512, 1024)
+        assert x.gb_pos.shape == (x.num_pixels, 3)
+        d = np.array(x.lights[0]["vec"])
+        assert abs(np.linalg.norm(d) - 1.0) < 1e-6 and d[1] < 0
+    assert len({x.num_triangles for x in a}) > 6  # uneven sizes
