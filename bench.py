@@ -125,6 +125,11 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp32,fp16")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched run")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the serving run (0 skips it)")
+    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run a k_copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -204,6 +209,29 @@ def main() -> int:
                      "kernel": f"k_trace<{head['prec']}>",
                      "fp32_achieved": fp32_achieved, "fp32_frac": fp32_achieved / HBM_PEAK_GBS},
     }
+    if not args.no_batched:
+        from final184_amd import loadgen
+        try:
+            b = loadgen.run_batched(ctx, vgi, log=log)
+            log(f"batched: {b['scenes']} scenes, frame {b['batch_frame_ms']:.2f} ms vs one call per "
+                f"scene {b['one_call_per_scene_ms']:.2f} ms")
+            line["batched"] = b
+        except Exception as e:  # the headline line is still printed
+            log(f"batched run failed: {e!r}")
+            line["batched"] = {"error": repr(e)}
+    if args.serve_seconds > 0:
+        from final184_amd import loadgen
+        try:
+            sv = loadgen.run_serving(ctx, vgi, seconds=args.serve_seconds, qps=args.serve_qps,
+                                     budget_gb=args.serve_budget_gb, log=log)
+            log(f"serving: p50 {sv['p50_ms']:.2f} ms p99 {sv['p99_ms']:.2f} ms at "
+                f"{sv['qps_achieved']:.1f} req/s, {sv['frames_per_s']:.0f} frames/s")
+            line["serving"] = sv
+            line["p50_ms"] = sv["p50_ms"]
+            line["p99_ms"] = sv["p99_ms"]
+        except Exception as e:
+            log(f"serving run failed: {e!r}")
+            line["serving"] = {"error": repr(e)}
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         cb, errs = cpu_baseline(w, results, args.cpu_sample)

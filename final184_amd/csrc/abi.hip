@@ -111,6 +111,16 @@ struct Carve {
 
 }  // namespace
 
+int vgi::set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
 // Per-stream workspace: voxelize scratch + staging ring. Calls on one stream are
 // serialized by its mutex.
 struct vgi_workspace {
@@ -606,7 +616,9 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
 int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
   const vgi_grid& g = s->grid;
   if (s->alloc_n == g.n && s->alloc_levels == g.levels && s->alloc_prec == precision) return VGI_OK;
-  HIP_TRY(hipDeviceSynchronize());
+  bool held = false;
+  for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) held = held || s->level[k];
+  if (held) HIP_TRY(hipDeviceSynchronize());  // queued work may still read the old levels
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) {
     if (s->level[k]) (void)hipFree(s->level[k]);
     s->level[k] = nullptr;
@@ -746,14 +758,8 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
 // trace batch
 // ---------------------------------------------------------------------------
 
-int check_trace_item(vgi_context* ctx, const vgi_trace_item& it, uint32_t i) {
-  const vgi_scene* s = it.scene;
-  if (!s) return fail(VGI_E_INVALID, "item %u: scene is NULL", i);
-  if (s->ctx != ctx) return fail(VGI_E_INVALID, "item %u: scene belongs to another context", i);
-  if (!s->has_mips) return fail(VGI_E_NOT_READY, "item %u: scene has no mip chain (call vgi_build_mips)", i);
-  const vgi_gbuffer& gb = it.gbuffer;
-  const vgi_cone_params& c = it.cones;
-  if (!gb.position || !gb.normal || !it.out_diffuse)
+int check_frame(const vgi_gbuffer& gb, const vgi_cone_params& c, const float* out_diffuse, uint32_t i) {
+  if (!gb.position || !gb.normal || !out_diffuse)
     return fail(VGI_E_INVALID, "item %u: G-buffer/output pointer is NULL", i);
   if (c.num_diffuse > VGI_MAX_DIFFUSE_CONES)
     return fail(VGI_E_RANGE, "item %u: num_diffuse %u exceeds %u", i, c.num_diffuse, VGI_MAX_DIFFUSE_CONES);
@@ -761,6 +767,14 @@ int check_trace_item(vgi_context* ctx, const vgi_trace_item& it, uint32_t i) {
     return fail(VGI_E_INVALID, "item %u: step_factor must be > 0 and max_distance finite", i);
   if ((uint64_t)gb.width * gb.height >= (1ull << 32)) return fail(VGI_E_RANGE, "item %u: G-buffer too large", i);
   return VGI_OK;
+}
+
+int check_trace_item(vgi_context* ctx, const vgi_trace_item& it, uint32_t i) {
+  const vgi_scene* s = it.scene;
+  if (!s) return fail(VGI_E_INVALID, "item %u: scene is NULL", i);
+  if (s->ctx != ctx) return fail(VGI_E_INVALID, "item %u: scene belongs to another context", i);
+  if (!s->has_mips) return fail(VGI_E_NOT_READY, "item %u: scene has no mip chain (call vgi_build_mips)", i);
+  return check_frame(it.gbuffer, it.cones, it.out_diffuse, i);
 }
 
 void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tuning& tn) {
@@ -898,6 +912,21 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
 }
 
 }  // namespace
+
+int vgi::check_request_input(const vgi_mesh* meshes, uint32_t num_meshes, const vgi_grid* grid,
+                             const vgi_lights* lights, const vgi_frame* frames, uint32_t num_frames) {
+  TRY(check_grid(grid));
+  uint64_t ntri = 0;
+  TRY(check_meshes(meshes, num_meshes, &ntri));
+  if (lights) {
+    if (lights->count > VGI_MAX_LIGHTS) return fail(VGI_E_RANGE, "%u lights exceed %u", lights->count, VGI_MAX_LIGHTS);
+    for (uint32_t l = 0; l < lights->count; l++)
+      if (lights->light[l].type > VGI_LIGHT_POINT) return fail(VGI_E_INVALID, "light %u: type %u", l, lights->light[l].type);
+  }
+  if (num_frames > 0 && !frames) return fail(VGI_E_INVALID, "frames is NULL");
+  for (uint32_t f = 0; f < num_frames; f++) TRY(check_frame(frames[f].gbuffer, frames[f].cones, frames[f].out_diffuse, f));
+  return VGI_OK;
+}
 
 extern "C" {
 

@@ -41,14 +41,21 @@ __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
 template <int PREC> struct Texel;
 
 template <> struct Texel<VGI_PREC_FP32> {
-  __device__ static float4 load(const void* base, size_t i) { return ((const float4*)base)[i]; }
-  __device__ static void store(void* base, size_t i, float4 v) { ((float4*)base)[i] = v; }
+  __device__ static float4 load(const void* base, size_t i) {
+    const uint4 u = gld4((const float4*)base + i);
+    return make_float4(__builtin_bit_cast(float, u.x), __builtin_bit_cast(float, u.y),
+                       __builtin_bit_cast(float, u.z), __builtin_bit_cast(float, u.w));
+  }
+  __device__ static void store(void* base, size_t i, float4 v) {
+    gst4((float4*)base + i, make_uint4(__builtin_bit_cast(uint32_t, v.x), __builtin_bit_cast(uint32_t, v.y),
+                                       __builtin_bit_cast(uint32_t, v.z), __builtin_bit_cast(uint32_t, v.w)));
+  }
   __device__ static float4 round(float4 v) { return v; }
 };
 
 template <> struct Texel<VGI_PREC_FP16> {
   __device__ static float4 load(const void* base, size_t i) {
-    const uint2 u = ((const uint2*)base)[i];
+    const uint2 u = gld2((const uint2*)base + i);
     return make_float4(f16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.x >> 16)),
                        f16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
   }
@@ -56,7 +63,7 @@ template <> struct Texel<VGI_PREC_FP16> {
     return make_uint2((uint32_t)f32_to_f16_bits(v.x) | ((uint32_t)f32_to_f16_bits(v.y) << 16),
                       (uint32_t)f32_to_f16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
   }
-  __device__ static void store(void* base, size_t i, float4 v) { ((uint2*)base)[i] = pack(v); }
+  __device__ static void store(void* base, size_t i, float4 v) { gst2((uint2*)base + i, pack(v)); }
   __device__ static float4 round(float4 v) {
     return make_float4(f16_bits_to_f32(f32_to_f16_bits(v.x)), f16_bits_to_f32(f32_to_f16_bits(v.y)),
                        f16_bits_to_f32(f32_to_f16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
@@ -66,14 +73,14 @@ template <> struct Texel<VGI_PREC_FP16> {
 // RGB in bf16, opacity in fp16.
 template <> struct Texel<VGI_PREC_BF16> {
   __device__ static float4 load(const void* base, size_t i) {
-    const uint2 u = ((const uint2*)base)[i];
+    const uint2 u = gld2((const uint2*)base + i);
     return make_float4(bf16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), bf16_bits_to_f32((uint16_t)(u.x >> 16)),
                        bf16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
   }
   __device__ static void store(void* base, size_t i, float4 v) {
-    ((uint2*)base)[i] =
-        make_uint2((uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
-                   (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
+    gst2((uint2*)base + i,
+         make_uint2((uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
+                    (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16)));
   }
   __device__ static float4 round(float4 v) {
     return make_float4(bf16_bits_to_f32(f32_to_bf16_bits(v.x)), bf16_bits_to_f32(f32_to_bf16_bits(v.y)),
@@ -97,14 +104,14 @@ __device__ __forceinline__ float4 scale4(float4 a, float s) {
 __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-  if ((uint32_t)P.bin_packed[bin] == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
+  if ((uint32_t)gld(P.bin_packed + bin) == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
   const uint32_t nbr = P.n >> 3;
   const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
   const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
-  if (!((P.mask[(size_t)brick * 16u + (local >> 5)] >> (local & 31u)) & 1u))
+  if (!((gld(P.mask + (size_t)brick * 16u + (local >> 5)) >> (local & 31u)) & 1u))
     return make_float4(0.f, 0.f, 0.f, 0.f);  // unoccupied voxel: its record is not written
-  const uint4* a = (const uint4*)(P.acc + ((size_t)brick * 512u + local) * kAccWords);
-  const uint4 s0 = a[0], s1 = a[1];
+  const uint32_t* a = P.acc + ((size_t)brick * 512u + local) * kAccWords;
+  const uint4 s0 = gld4(a), s1 = gld4(a + 4);
   const uint32_t c = s0.w;
   const float lr = lut[(s0.x + c / 2) / c];
   const float lg = lut[(s0.y + c / 2) / c];
@@ -147,12 +154,14 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
 }
 
 template <int PREC, bool SRC_ACC>
-__global__ __launch_bounds__(512) void k_mips(MipBatch Bt) {
+__global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scenes,
+                                               const MipBlock* __restrict__ blocks,
+                                               const float* __restrict__ lut) {
   using TX = Texel<PREC>;
   __shared__ float4 s_a[512];
   __shared__ float4 s_b[64];
-  const MipBlock blk = Bt.blocks[blockIdx.x];
-  const MipScene& P = Bt.scenes[blk.scene];
+  const MipBlock blk = blocks[blockIdx.x];
+  const MipScene& P = scenes[blk.scene];
   const uint32_t src = blk.src, nout = blk.nout;
   const uint32_t ns = P.nk[src];
   const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge
@@ -163,7 +172,7 @@ __global__ __launch_bounds__(512) void k_mips(MipBatch Bt) {
   const uint32_t g = threadIdx.x;
   __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
   if (SRC_ACC) {
-    if (g < 256) s_lut[g] = Bt.lut[g];
+    if (g < 256) s_lut[g] = lut[g];
     __syncthreads();
   }
 
@@ -234,16 +243,16 @@ hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t
   const dim3 g(blocks), b(512);
   switch (prec) {
     case VGI_PREC_FP32:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
       break;
     case VGI_PREC_FP16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
       break;
     case VGI_PREC_BF16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
       break;
     default:
       return hipErrorInvalidValue;

@@ -140,14 +140,14 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
   for (int q = 0; q < 4; q++) {
     const float wz = q < 2 ? f.wz0 : f.wz1;
     if constexpr (PREC == VGI_PREC_FP32) {
-      const uint4 a = *(const uint4*)(base + offs[q]);
-      const uint4 b = *(const uint4*)(base + offs[q] + 16);
+      const uint4 a = gld4(base + offs[q]);
+      const uint4 b = gld4(base + offs[q] + 16);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a.x, a.y, a.z, a.w);
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
       abits |= a.w | b.w;
     } else {
-      const uint2 a = *(const uint2*)(base + offs[q]);
-      const uint2 b = *(const uint2*)(base + offs[q] + 8);
+      const uint2 a = gld2(base + offs[q]);
+      const uint2 b = gld2(base + offs[q] + 8);
       Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
       Acc<PREC>::add(acc, wxy[q][1] * wz, b.x, b.y);
       abits |= (a.y | b.y) & 0xFFFF0000u;
@@ -210,6 +210,9 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
       }
       count += 2;
     }
+    // the sample's opacity is a convex combination of values in [0, 1]; its rounded
+    // weights can sum to 1 + ulp, and the clamp keeps the front-to-back a within [0, 1]
+    s.w = fminf(s.w, 1.0f);
     const float w = 1.0f - a;
     cr = fmaf(w, s.x, cr);
     cg = fmaf(w, s.y, cg);
@@ -222,15 +225,16 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
 }
 
 template <int PREC, bool BUF>
-__global__ __launch_bounds__(256) void k_trace(TraceBatch Bt) {
+__global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams* __restrict__ scenes,
+                                               const uint2* __restrict__ blocks) {
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous band of the
   // batch's tile list (tiles of one scene are consecutive, row-major).
   const uint32_t b = blockIdx.x;
   const uint32_t xcd = b & 7u, slot = b >> 3;
   const uint32_t full = Bt.nblocks >> 3, rem = Bt.nblocks & 7u;
   const uint32_t idx = xcd * full + (xcd < rem ? xcd : rem) + slot;
-  const uint2 st = Bt.nscenes == 1 ? make_uint2(0u, idx) : Bt.blocks[idx];
-  const TraceParams& P = Bt.scenes[st.x];
+  const uint2 st = Bt.nscenes == 1 ? make_uint2(0u, idx) : blocks[idx];
+  const TraceParams& P = scenes[st.x];
   const uint32_t tile = st.y;
   const uint32_t tx = tile % P.tiles_x, ty = tile / P.tiles_x;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -241,8 +245,8 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt) {
   uint32_t count = 0;
   if (active) {
     const size_t p = (size_t)py * P.width + px;
-    const float Px = P.pos[p * 3], Py = P.pos[p * 3 + 1], Pz = P.pos[p * 3 + 2];
-    const float Nx = P.nrm[p * 3], Ny = P.nrm[p * 3 + 1], Nz = P.nrm[p * 3 + 2];
+    const float Px = gld(P.pos + p * 3), Py = gld(P.pos + p * 3 + 1), Pz = gld(P.pos + p * 3 + 2);
+    const float Nx = gld(P.nrm + p * 3), Ny = gld(P.nrm + p * 3 + 1), Nz = gld(P.nrm + p * 3 + 2);
     const float len2 = (Nx * Nx + Ny * Ny) + Nz * Nz;
     float4 od = make_float4(0.f, 0.f, 0.f, 0.f), os = make_float4(0.f, 0.f, 0.f, 0.f);
     if (len2 > 0.f) {
@@ -278,15 +282,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt) {
         const float dn = (nx * ix + ny * iy) + nz * iz;
         const float rx = ix - 2.0f * dn * nx, ry = iy - 2.0f * dn * ny, rz = iz - 2.0f * dn * nz;
         if (P.rough) {
-          const float th = fminf(fmaxf(P.rough[p], 0.0087f), 1.0f);
+          const float th = fminf(fmaxf(gld(P.rough + p), 0.0087f), 1.0f);
           os = cone_march<PREC, false>(P, ox, oy, oz, rx, ry, rz, th, count);
         } else {
           os = cone_march<PREC, BUF>(P, ox, oy, oz, rx, ry, rz, P.tan_s, count);
         }
       }
     }
-    P.out_d[p] = od;
-    if (P.out_s) P.out_s[p] = os;
+    gst4(P.out_d + p, make_uint4(__builtin_bit_cast(uint32_t, od.x), __builtin_bit_cast(uint32_t, od.y),
+                                 __builtin_bit_cast(uint32_t, od.z), __builtin_bit_cast(uint32_t, od.w)));
+    if (P.out_s)
+      gst4(P.out_s + p, make_uint4(__builtin_bit_cast(uint32_t, os.x), __builtin_bit_cast(uint32_t, os.y),
+                                   __builtin_bit_cast(uint32_t, os.z), __builtin_bit_cast(uint32_t, os.w)));
   }
   if (Bt.counter) {
     unsigned long long c = count;
@@ -301,16 +308,16 @@ hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s) {
   const dim3 block(B.tile_w * B.tile_h), grid(B.nblocks);
   switch (B.prec) {
     case VGI_PREC_FP32:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, true>), grid, block, 0, s, B);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, false>), grid, block, 0, s, B);
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, true>), grid, block, 0, s, B, B.scenes, B.blocks);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, false>), grid, block, 0, s, B, B.scenes, B.blocks);
       break;
     case VGI_PREC_FP16:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, true>), grid, block, 0, s, B);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, false>), grid, block, 0, s, B);
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, true>), grid, block, 0, s, B, B.scenes, B.blocks);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, false>), grid, block, 0, s, B, B.scenes, B.blocks);
       break;
     case VGI_PREC_BF16:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, true>), grid, block, 0, s, B);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, false>), grid, block, 0, s, B);
+      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, true>), grid, block, 0, s, B, B.scenes, B.blocks);
+      else hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, false>), grid, block, 0, s, B, B.scenes, B.blocks);
       break;
     default:
       return hipErrorInvalidValue;

@@ -171,6 +171,77 @@ struct TraceBatch {
 // Texel bytes per precision.
 inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u : 8u; }
 
+// ---- global-memory access ----
+// Kernels take their descriptor tables as __restrict__ arguments, so the compiler may
+// read descriptors with scalar loads. The buffer pointers inside a descriptor are generic
+// pointers, and accesses through generic pointers compile to flat instructions (which
+// also wait on the LDS counter); every buffer of this library is in global memory, so
+// the hot accesses go through these helpers, which access it as such.
+#define VGI_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t vgi_u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t vgi_u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {  // scalar types
+  return *(const VGI_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {  // scalar types
+  *(VGI_GLOBAL T*)p = v;
+}
+__device__ __forceinline__ uint2 gld2(const void* p) {
+  const vgi_u32x2 v = *(const VGI_GLOBAL vgi_u32x2*)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gld4(const void* p) {
+  const vgi_u32x4 v = *(const VGI_GLOBAL vgi_u32x4*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst2(void* p, uint2 v) {
+  *(VGI_GLOBAL vgi_u32x2*)p = vgi_u32x2{v.x, v.y};
+}
+__device__ __forceinline__ void gst4(void* p, uint4 v) {
+  *(VGI_GLOBAL vgi_u32x4*)p = vgi_u32x4{v.x, v.y, v.z, v.w};
+}
+// whole records (size a multiple of 16 bytes, 16-byte aligned) as 16-byte accesses
+template <class T>
+__device__ __forceinline__ T gld_rec(const T* p) {
+  static_assert(sizeof(T) % 16 == 0, "record size");
+  struct Raw {
+    uint4 w[sizeof(T) / 16];
+  } r;
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 16; i++) r.w[i] = gld4((const uint4*)p + i);
+  return __builtin_bit_cast(T, r);
+}
+template <class T>
+__device__ __forceinline__ void gst_rec(T* p, const T& v) {
+  static_assert(sizeof(T) % 16 == 0, "record size");
+  struct Raw {
+    uint4 w[sizeof(T) / 16];
+  };
+  const Raw r = __builtin_bit_cast(Raw, v);
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 16; i++) gst4((uint4*)p + i, r.w[i]);
+}
+// relaxed, device scope: the semantics of atomicAdd / atomicOr
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add((VGI_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long gatomic_add(unsigned long long* p, unsigned long long v) {
+  return __hip_atomic_fetch_add((VGI_GLOBAL unsigned long long*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gatomic_or(uint32_t* p, uint32_t v) {
+  __hip_atomic_fetch_or((VGI_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sets the calling thread's vgi_last_error() message and returns code (defined in abi.hip).
+int set_error(int code, const char* fmt, ...);
+
+// Validates one serving request's geometry, lights and frames (defined in abi.hip).
+int check_request_input(const vgi_mesh* meshes, uint32_t num_meshes, const vgi_grid* grid,
+                        const vgi_lights* lights, const vgi_frame* frames, uint32_t num_frames);
+
 // ---- launchers (each returns hipError_t) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -80,12 +80,14 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
 
+static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read as 16-byte words");
+
 __device__ __forceinline__ uint32_t find_mesh(const MeshDev* __restrict__ m, uint32_t nmesh,
                                               uint32_t t) {
   uint32_t lo = 0, hi = nmesh - 1;
   while (lo < hi) {
     uint32_t mid = (lo + hi + 1) >> 1;
-    if (m[mid].tri_base <= t) lo = mid;
+    if (gld(&m[mid].tri_base) <= t) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -180,10 +182,10 @@ __device__ __forceinline__ uint2 bin_block(const VoxBatch& B) {
 // Offset of bin b's first pair within its scene's pair list, and its first work item
 // within the batch.
 __device__ __forceinline__ uint32_t pair_base(const VoxBatch& B, const VoxScene& S, uint32_t b) {
-  return (uint32_t)B.scan_all[S.bin_base + b] - (uint32_t)B.scan_all[S.bin_base];
+  return (uint32_t)gld(B.scan_all + S.bin_base + b) - (uint32_t)gld(B.scan_all + S.bin_base);
 }
 __device__ __forceinline__ uint32_t first_item(const VoxBatch& B, const VoxScene& S, uint32_t b) {
-  return (uint32_t)(B.scan_all[S.bin_base + b] >> 32);
+  return (uint32_t)(gld(B.scan_all + S.bin_base + b) >> 32);
 }
 
 // Wave-aggregated atomic increment with per-lane keys: every active lane gets a distinct
@@ -200,12 +202,12 @@ __device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bo
     const uint32_t lkey = __shfl(key, leader);
     const uint64_t same = __ballot(active && key == lkey) & pending;
     uint32_t base = 0;
-    if (lane == (uint32_t)leader) base = atomicAdd(&counter[lkey], (uint32_t)__popcll(same));
+    if (lane == (uint32_t)leader) base = gatomic_add(&counter[lkey], (uint32_t)__popcll(same));
     base = __shfl(base, leader);
     if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
     pending &= ~same;
   }
-  if ((pending >> lane) & 1ull) slot = atomicAdd(&counter[key], 1u);
+  if ((pending >> lane) & 1ull) slot = gatomic_add(&counter[key], 1u);
   return slot;
 }
 
@@ -214,15 +216,15 @@ __device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bo
 // ---------------------------------------------------------------------------
 
 __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
-  const MeshDev& M = P.meshes[find_mesh(P.meshes, P.nmesh, t)];
+  const MeshDev M = gld_rec(P.meshes + find_mesh(P.meshes, P.nmesh, t));
   const uint32_t lt = t - M.tri_base;
   uint32_t vid[3];
   if (M.idx_type == VGI_INDEX_U32) {
     const uint32_t* ix = (const uint32_t*)M.idx + (size_t)lt * 3;
-    vid[0] = ix[0]; vid[1] = ix[1]; vid[2] = ix[2];
+    vid[0] = gld(ix); vid[1] = gld(ix + 1); vid[2] = gld(ix + 2);
   } else if (M.idx_type == VGI_INDEX_U16) {
     const uint16_t* ix = (const uint16_t*)M.idx + (size_t)lt * 3;
-    vid[0] = ix[0]; vid[1] = ix[1]; vid[2] = ix[2];
+    vid[0] = gld(ix); vid[1] = gld(ix + 1); vid[2] = gld(ix + 2);
   } else {
     vid[0] = lt * 3; vid[1] = lt * 3 + 1; vid[2] = lt * 3 + 2;
   }
@@ -232,7 +234,7 @@ __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
   for (int i = 0; i < 3; i++) {
     if (vid[i] >= M.nverts) { status = 1; continue; }
     const float* p = (const float*)((const char*)M.pos + (size_t)vid[i] * M.stride);
-    const float x = p[0], y = p[1], z = p[2];
+    const float x = gld(p), y = gld(p + 1), z = gld(p + 2);
 #pragma unroll
     for (int r = 0; r < 3; r++) {
       float w = M.m[r * 4 + 0] * x;
@@ -259,7 +261,7 @@ __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
   uint32_t rgb;
   if (M.albedo) {
     const uint8_t* c = M.albedo + (size_t)lt * 3;
-    rgb = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16);
+    rgb = (uint32_t)gld(c) | ((uint32_t)gld(c + 1) << 8) | ((uint32_t)gld(c + 2) << 16);
   } else {
     rgb = M.base_rgb & 0xFFFFFFu;
   }
@@ -296,9 +298,9 @@ __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
   R.rgb = rgb | (status << 24);
 }
 
-__global__ __launch_bounds__(256) void k_setup(VoxBatch B) {
+__global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint2 blk = tri_block(B);  // a workgroup never spans two scenes
-  const VoxScene& P = B.scenes[blk.x];
+  const VoxScene& P = scenes[blk.x];
   const uint32_t t = blk.y + threadIdx.x;
   const bool in = t < P.ntri;
   TriRec R;
@@ -307,14 +309,14 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B) {
   int64_t q[3][3], N[3];
   if (in) {
     make_record(P, t, R);
-    P.tris[t] = R;
+    gst_rec(P.tris + t, R);
     const uint32_t status = R.rgb >> 24;
-    if (status == 1) atomicAdd(&P.stats->rejected, 1ull);
-    if (status == 2) atomicAdd(&P.stats->degenerate, 1ull);
+    if (status == 1) gatomic_add(&P.stats->rejected, 1ull);
+    if (status == 2) gatomic_add(&P.stats->degenerate, 1ull);
     if (tri_ok(R)) {
       bb = bin_box(R, P.bin_shift);
       small = bb.count() <= kSmallBins;
-      if (!small) P.big_list[atomicAdd(&P.stats->big_count, 1u)] = t;
+      if (!small) gst(P.big_list + gatomic_add(&P.stats->big_count, 1u), t);
       else load_q(R, q, N);
     }
   }
@@ -325,13 +327,13 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B) {
     uint32_t bin = 0;
     const bool hit = small && j < bb.count() && bin_candidate(P, q, N, bb, j, bin);
     const uint32_t slot = agg_slot(P.bin_small, bin, hit);
-    if (hit) P.tri_slot[(size_t)t * kSmallBins + nslot++] = slot;
+    if (hit) gst(P.tri_slot + (size_t)t * kSmallBins + nslot++, slot);
   }
 }
 
 hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(kBlockTris), 0, s, B);
+  hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(kBlockTris), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -340,13 +342,13 @@ hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_big(VoxBatch B) {
+__global__ __launch_bounds__(256) void k_big(VoxBatch B, const VoxScene* __restrict__ scenes) {
   for (uint32_t sc = 0; sc < B.nscenes; sc++) {
-    const VoxScene& P = B.scenes[sc];
-    const uint32_t nbig = P.stats->big_count;
+    const VoxScene& P = scenes[sc];
+    const uint32_t nbig = gld(&P.stats->big_count);
     for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
-      const uint32_t t = P.big_list[i];
-      const TriRec R = P.tris[t];
+      const uint32_t t = gld(P.big_list + i);
+      const TriRec R = gld_rec(P.tris + t);
       int64_t q[3][3], N[3];
       load_q(R, q, N);
       const BinBox bb = bin_box(R, P.bin_shift);
@@ -355,10 +357,10 @@ __global__ __launch_bounds__(256) void k_big(VoxBatch B) {
         uint32_t bin;
         if (!bin_candidate(P, q, N, bb, j, bin)) continue;
         if (!EMIT) {
-          atomicAdd(&P.bin_big[bin], 1u);
+          gatomic_add(&P.bin_big[bin], 1u);
         } else {
-          const uint32_t pos = pair_base(B, P, bin) + P.bin_small[bin] + atomicAdd(&P.big_cursor[bin], 1u);
-          if (pos < P.pairs_cap) P.pairs[pos] = t;
+          const uint32_t pos = pair_base(B, P, bin) + gld(P.bin_small + bin) + gatomic_add(&P.big_cursor[bin], 1u);
+          if (pos < P.pairs_cap) gst(P.pairs + pos, t);
         }
       }
     }
@@ -366,8 +368,8 @@ __global__ __launch_bounds__(256) void k_big(VoxBatch B) {
 }
 
 hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t s) {
-  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, B);
-  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, B);
+  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, B, B.scenes);
+  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -375,30 +377,30 @@ hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t
 // 3 per-bin totals, 4 scan over every bin of the batch
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_bins(VoxBatch B) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.packed_all[B.total_bins] = 0ull;
+__global__ __launch_bounds__(256) void k_bins(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) gst(B.packed_all + B.total_bins, 0ull);
   const uint2 blk = bin_block(B);
-  const VoxScene& P = B.scenes[blk.x];
+  const VoxScene& P = scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   const uint32_t b = blk.y + threadIdx.x;
   unsigned long long total = 0;
   if (b < nbins) {
-    const uint32_t pairs = P.bin_small[b] + P.bin_big[b];
+    const uint32_t pairs = gld(P.bin_small + b) + gld(P.bin_big + b);
     const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
     const unsigned long long packed = ((unsigned long long)items << 32) | pairs;
-    P.bin_packed[b] = packed;
-    B.packed_all[P.bin_base + b] = packed;
-    if (items > 1) P.zero_list[atomicAdd(&P.stats->zero_count, 1u)] = b;
+    gst(P.bin_packed + b, packed);
+    gst(B.packed_all + P.bin_base + b, packed);
+    if (items > 1) gst(P.zero_list + gatomic_add(&P.stats->zero_count, 1u), b);
     total = pairs;
   }
   // exact 64-bit pair count per scene (the packed scan keeps only 32 bits of it)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
-  if (__lane_id() == 0 && total) atomicAdd(&P.stats->pairs, total);
+  if (__lane_id() == 0 && total) gatomic_add(&P.stats->pairs, total);
 }
 
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_bins, dim3(blocks), dim3(kBlockBins), 0, s, B);
+  hipLaunchKernelGGL(k_bins, dim3(blocks), dim3(kBlockBins), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -408,37 +410,37 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
 }
 
 // per scene: (items << 32 | pairs) of its bins, and the batch total in [nscenes]
-__global__ void k_totals(VoxBatch B) {
+__global__ void k_totals(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc > B.nscenes) return;
   if (sc == B.nscenes) {
     B.totals[sc] = B.scan_all[B.total_bins];
     return;
   }
-  const VoxScene& P = B.scenes[sc];
+  const VoxScene& P = scenes[sc];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   B.totals[sc] = B.scan_all[P.bin_base + nbins] - B.scan_all[P.bin_base];
 }
 
 hipError_t launch_totals(const VoxBatch& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_totals, dim3((B.nscenes + 1 + 255) / 256), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_totals, dim3((B.nscenes + 1 + 255) / 256), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
 // work item -> (scene, bin), from the scanned item offsets
-__global__ __launch_bounds__(256) void k_workmap(VoxBatch B) {
+__global__ __launch_bounds__(256) void k_workmap(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint2 blk = bin_block(B);
-  const VoxScene& P = B.scenes[blk.x];
+  const VoxScene& P = scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   const uint32_t b = blk.y + threadIdx.x;
   if (b >= nbins) return;
-  const uint32_t items = (uint32_t)(P.bin_packed[b] >> 32);
+  const uint32_t items = (uint32_t)(gld(P.bin_packed + b) >> 32);
   const uint32_t first = first_item(B, P, b);
-  for (uint32_t i = 0; i < items; i++) B.work_map[first + i] = make_uint2(blk.x, b);
+  for (uint32_t i = 0; i < items; i++) gst2(B.work_map + first + i, make_uint2(blk.x, b));
 }
 
 hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_workmap, dim3(blocks), dim3(kBlockBins), 0, s, B);
+  hipLaunchKernelGGL(k_workmap, dim3(blocks), dim3(kBlockBins), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -446,12 +448,12 @@ hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 // 5 scatter small triangles into their slots
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_emit_small(VoxBatch B) {
+__global__ __launch_bounds__(256) void k_emit_small(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint2 blk = tri_block(B);
-  const VoxScene& P = B.scenes[blk.x];
+  const VoxScene& P = scenes[blk.x];
   const uint32_t t = blk.y + threadIdx.x;
   if (t >= P.ntri) return;
-  const TriRec R = P.tris[t];
+  const TriRec R = gld_rec(P.tris + t);
   if (!tri_ok(R)) return;
   const BinBox bb = bin_box(R, P.bin_shift);
   if (bb.count() > kSmallBins) return;
@@ -461,14 +463,14 @@ __global__ __launch_bounds__(256) void k_emit_small(VoxBatch B) {
   for (int j = 0; j < bb.count(); j++) {
     uint32_t bin;
     if (!bin_candidate(P, q, N, bb, j, bin)) continue;
-    const uint32_t pos = pair_base(B, P, bin) + P.tri_slot[(size_t)t * kSmallBins + nslot++];
-    if (pos < P.pairs_cap) P.pairs[pos] = t;
+    const uint32_t pos = pair_base(B, P, bin) + gld(P.tri_slot + (size_t)t * kSmallBins + nslot++);
+    if (pos < P.pairs_cap) gst(P.pairs + pos, t);
   }
 }
 
 hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit_small, dim3(blocks), dim3(kBlockTris), 0, s, B);
+  hipLaunchKernelGGL(k_emit_small, dim3(blocks), dim3(kBlockTris), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -482,27 +484,27 @@ __device__ __forceinline__ size_t acc_index(uint32_t x, uint32_t y, uint32_t z, 
   return ((size_t)brick * 512u + local) * kAccWords;
 }
 
-__global__ __launch_bounds__(256) void k_zero(VoxBatch Bt) {
+__global__ __launch_bounds__(256) void k_zero(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   for (uint32_t sc = 0; sc < Bt.nscenes; sc++) {
-    const VoxScene& P = Bt.scenes[sc];
-    const uint32_t nz = P.stats->zero_count;
+    const VoxScene& P = scenes[sc];
+    const uint32_t nz = gld(&P.stats->zero_count);
     const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
     for (uint32_t i = blockIdx.x; i < nz; i += gridDim.x) {
-      const uint32_t bin = P.zero_list[i];
+      const uint32_t bin = gld(P.zero_list + i);
       const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
       for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
         const uint32_t bxx = bx * bpb + k % bpb, byy = by * bpb + (k / bpb) % bpb, bzz = bz * bpb + k / (bpb * bpb);
         const uint32_t brick = bxx + nbr * (byy + nbr * bzz);
         uint4* dst = (uint4*)(P.acc + (size_t)brick * 512u * kAccWords);
-        for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) dst[j] = make_uint4(0, 0, 0, 0);
-        if (threadIdx.x < 4) ((uint4*)(P.mask + (size_t)brick * 16u))[threadIdx.x] = make_uint4(0, 0, 0, 0);
+        for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) gst4(dst + j, make_uint4(0, 0, 0, 0));
+        if (threadIdx.x < 4) gst4((uint4*)(P.mask + (size_t)brick * 16u) + threadIdx.x, make_uint4(0, 0, 0, 0));
       }
     }
   }
 }
 
 hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -592,7 +594,8 @@ __device__ __forceinline__ bool edge_ok(const TriCoef& C, int u, int va, int vb)
 }
 
 template <int TS>
-__global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
+__global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes,
+                                                  uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int BT = (TS / 8) * (TS / 8) * (TS / 8);  // 8^3 accumulator bricks per tile
   __shared__ unsigned long long s_acc[2 * TV];
@@ -604,12 +607,12 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
 
   // persistent loop over the work items of every scene of the batch
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = Bt.work_map[w];
-    const VoxScene& P = Bt.scenes[item.x];
+    const uint2 item = gld2(Bt.work_map + w);
+    const VoxScene& P = scenes[item.x];
     const uint32_t bin = item.y;
     const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
     const uint32_t sub = B / TS;  // sub-tiles per bin edge
-    const unsigned long long packed = P.bin_packed[bin];
+    const unsigned long long packed = gld(P.bin_packed + bin);
     const uint32_t cnt_bin = (uint32_t)packed, nch = (uint32_t)(packed >> 32);
     const uint32_t chunk = w - first_item(Bt, P, bin);
     const uint32_t first = pair_base(Bt, P, bin) + chunk * P.chunk;
@@ -617,8 +620,8 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
     const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
 
     if (tid < cnt) {
-      const uint32_t t = P.pairs[first + tid];
-      const TriRec R = P.tris[t];
+      const uint32_t t = gld(P.pairs + first + tid);
+      const TriRec R = gld_rec(P.tris + t);
       make_coef(R, s_tri[tid]);
     }
 
@@ -710,16 +713,16 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
         const uint32_t nx = (uint32_t)(w1 & 0xFFFFFu), ny = (uint32_t)((w1 >> 20) & 0xFFFFFu),
                        nz = (uint32_t)((w1 >> 40) & 0xFFFFFu);
         if (nch == 1) {
-          ((uint4*)dst)[0] = make_uint4(r, g, b, c);
-          ((uint4*)dst)[1] = make_uint4(nx, ny, nz, 0u);
+          gst4(dst, make_uint4(r, g, b, c));
+          gst4(dst + 4, make_uint4(nx, ny, nz, 0u));
         } else {
-          atomicAdd(dst + 0, r);
-          atomicAdd(dst + 1, g);
-          atomicAdd(dst + 2, b);
-          atomicAdd(dst + 3, c);
-          atomicAdd(dst + 4, nx);
-          atomicAdd(dst + 5, ny);
-          atomicAdd(dst + 6, nz);
+          gatomic_add(dst + 0, r);
+          gatomic_add(dst + 1, g);
+          gatomic_add(dst + 2, b);
+          gatomic_add(dst + 3, c);
+          gatomic_add(dst + 4, nx);
+          gatomic_add(dst + 5, ny);
+          gatomic_add(dst + 6, nz);
         }
       }
       __syncthreads();
@@ -729,8 +732,8 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
                        bzz = (tz0 >> 3) + lb / ((TS / 8) * (TS / 8));
         uint32_t* m = P.mask + (size_t)(bxx + nbr * (byy + nbr * bzz)) * 16u + (i & 15);
         const uint32_t v = s_mask[i];
-        if (nch == 1) *m = v;
-        else if (v) atomicOr(m, v);
+        if (nch == 1) gst(m, v);
+        else if (v) gatomic_or(m, v);
       }
       __syncthreads();
     }
@@ -740,8 +743,8 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, uint32_t nwork) {
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s) {
   if (grid_blocks == 0 || nwork == 0) return hipSuccess;
-  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, nwork);
-  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, nwork);
+  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
+  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
   return hipGetLastError();
 }
 

@@ -101,6 +101,50 @@ class TraceItem(C.Structure):
                 ("out_diffuse", C.c_void_p), ("out_specular", C.c_void_p)]
 
 
+class AutotuneReport(C.Structure):
+    _fields_ = [("best", Tuning), ("bin_sizes", C.c_uint32 * 3), ("tiles", (C.c_uint32 * 2) * 3),
+                ("build_ms", C.c_float * 3), ("trace_ms", C.c_float * 3),
+                ("frame_ms", (C.c_float * 3) * 3)]
+
+    def as_dict(self) -> dict:
+        return dict(best=dict(bin_size=self.best.bin_size, tile=[self.best.tile_w, self.best.tile_h]),
+                    build_ms={str(self.bin_sizes[b]): round(self.build_ms[b], 4) for b in range(3)},
+                    trace_ms={f"{self.tiles[j][0]}x{self.tiles[j][1]}": round(self.trace_ms[j], 4)
+                              for j in range(3)})
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("precision", C.c_uint32), ("slots", C.c_uint32),
+                ("max_batch_requests", C.c_uint32), ("max_batch_pixels", C.c_uint64)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("gbuffer", GBuffer), ("cones", ConeParams), ("out_diffuse", C.c_void_p),
+                ("out_specular", C.c_void_p)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene_key", C.c_uint64), ("meshes", C.c_void_p), ("num_meshes", C.c_uint32),
+                ("revoxelize", C.c_uint32), ("grid", Grid), ("lights", Lights),
+                ("frames", C.c_void_p), ("num_frames", C.c_uint32), ("pad", C.c_uint32),
+                ("arrival_s", C.c_double), ("tag", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("voxelized", C.c_uint32),
+                ("arrival_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "failed", "builds", "evictions",
+                                          "batches", "frames", "pixels", "resident_scenes",
+                                          "resident_bytes", "peak_resident_bytes")] + \
+               [("max_concurrent_batches", C.c_uint32), ("max_batch_requests_seen", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 # (name, restype, argtypes) for every exported entry point of include/vgi.h
 _P = C.c_void_p
 SIGNATURES = [
@@ -124,6 +168,17 @@ SIGNATURES = [
     ("vgi_voxelize_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.c_uint32, _P]),
     ("vgi_build_mips_batch", C.c_int, [_P, C.POINTER(MipsItem), C.c_uint32, C.c_uint32, _P]),
     ("vgi_trace_batch", C.c_int, [_P, C.POINTER(TraceItem), C.c_uint32, _P]),
+    ("vgi_autotune_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.POINTER(MipsItem),
+                                     C.POINTER(TraceItem), C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(AutotuneReport), _P]),
+    ("vgi_now", C.c_double, []),
+    ("vgi_server_create", C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("vgi_server_submit", C.c_int, [_P, C.POINTER(Request)]),
+    ("vgi_server_poll", C.c_int, [_P, C.POINTER(Completion), C.c_uint32, C.POINTER(C.c_uint32),
+                                  C.c_uint32]),
+    ("vgi_server_drain", C.c_int, [_P]),
+    ("vgi_server_get_stats", C.c_int, [_P, C.POINTER(ServerStats)]),
+    ("vgi_server_destroy", C.c_int, [_P]),
     ("vgi_scene_read_voxels", C.c_int, [_P, _P, _P]),
     ("vgi_scene_read_level", C.c_int, [_P, _P, C.c_uint32, _P]),
     ("vgi_malloc", C.c_int, [_P, C.POINTER(_P), C.c_size_t]),
@@ -359,6 +414,43 @@ class Scene:
             pass
 
 
+def _vox_items(jobs):
+    items = (VoxelizeItem * max(1, len(jobs)))()
+    keep = []
+    for i, (scene, meshes, grid) in enumerate(jobs):
+        arr = (Mesh * max(1, len(meshes)))()
+        for k, m in enumerate(meshes):
+            arr[k] = m.desc if isinstance(m, MeshData) else m
+        keep.append(arr)
+        items[i].scene = scene.handle
+        items[i].meshes = C.cast(arr, C.c_void_p)
+        items[i].num_meshes = len(meshes)
+        items[i].grid = grid
+    return items, keep
+
+
+def _mips_items(jobs):
+    items = (MipsItem * max(1, len(jobs)))()
+    keep = []
+    for i, (scene, lights) in enumerate(jobs):
+        items[i].scene = scene.handle
+        if lights is not None:
+            keep.append(lights)
+            items[i].lights = C.addressof(lights)
+    return items, keep
+
+
+def _trace_items(jobs):
+    items = (TraceItem * max(1, len(jobs)))()
+    for i, (scene, gb, cones, out_d, out_s) in enumerate(jobs):
+        items[i].scene = scene.handle
+        items[i].gbuffer = gb.desc
+        items[i].cones = cones
+        items[i].out_diffuse = out_d.ptr
+        items[i].out_specular = out_s.ptr if out_s is not None else None
+    return items
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = C.c_void_p()
@@ -418,42 +510,33 @@ class Context:
 
     def voxelize_batch(self, jobs, stream=None) -> None:
         """jobs: sequence of (Scene, [MeshData | Mesh], Grid); one batched voxelize pass."""
-        items = (VoxelizeItem * max(1, len(jobs)))()
-        keep = []
-        for i, (scene, meshes, grid) in enumerate(jobs):
-            arr = (Mesh * max(1, len(meshes)))()
-            for k, m in enumerate(meshes):
-                arr[k] = m.desc if isinstance(m, MeshData) else m
-            keep.append(arr)
-            items[i].scene = scene.handle
-            items[i].meshes = C.cast(arr, C.c_void_p)
-            items[i].num_meshes = len(meshes)
-            items[i].grid = grid
+        items, keep = _vox_items(jobs)
         _check(lib.vgi_voxelize_batch(self.handle, items, len(jobs), stream))
         for scene, _, grid in jobs:
             scene.grid = grid
 
     def build_mips_batch(self, jobs, precision: int = PREC_FP32, stream=None) -> None:
         """jobs: sequence of (Scene, Lights | None)."""
-        items = (MipsItem * max(1, len(jobs)))()
-        keep = []
-        for i, (scene, lights) in enumerate(jobs):
-            items[i].scene = scene.handle
-            if lights is not None:
-                keep.append(lights)
-                items[i].lights = C.addressof(lights)
+        items, keep = _mips_items(jobs)
         _check(lib.vgi_build_mips_batch(self.handle, items, len(jobs), precision, stream))
 
     def trace_batch(self, jobs, stream=None) -> None:
         """jobs: sequence of (Scene, GBufferData, ConeParams, out_diffuse, out_specular | None)."""
-        items = (TraceItem * max(1, len(jobs)))()
-        for i, (scene, gb, cones, out_d, out_s) in enumerate(jobs):
-            items[i].scene = scene.handle
-            items[i].gbuffer = gb.desc
-            items[i].cones = cones
-            items[i].out_diffuse = out_d.ptr
-            items[i].out_specular = out_s.ptr if out_s is not None else None
-        _check(lib.vgi_trace_batch(self.handle, items, len(jobs), stream))
+        _check(lib.vgi_trace_batch(self.handle, _trace_items(jobs), len(jobs), stream))
+
+    def autotune_batch(self, vox_jobs, mips_jobs, trace_jobs, precision: int, reps: int = 3,
+                       stream=None) -> AutotuneReport:
+        """Times the batch under every bin size and screen tile; keeps the best in the
+        context's tuning. The three job lists name the same scenes in the same order."""
+        vi, k1 = _vox_items(vox_jobs)
+        mi, k2 = _mips_items(mips_jobs)
+        ti = _trace_items(trace_jobs)
+        r = AutotuneReport()
+        _check(lib.vgi_autotune_batch(self.handle, vi, mi, ti, len(vox_jobs), precision, reps,
+                                      C.byref(r), stream))
+        for scene, _, grid in vox_jobs:
+            scene.grid = grid
+        return r
 
     def timer_start(self, stream=None) -> None:
         _check(lib.vgi_timer_start(self.handle, stream))
@@ -462,6 +545,73 @@ class Context:
         ms = C.c_float()
         _check(lib.vgi_timer_stop(self.handle, stream, C.byref(ms)))
         return ms.value
+
+
+class Server:
+    """Continuous-batching request loop over resident scenes (vgi_server_* in include/vgi.h)."""
+
+    def __init__(self, ctx: "Context", memory_budget: int, precision: int = PREC_FP16, slots: int = 4,
+                 max_batch_requests: int = 16, max_batch_pixels: int = 0):
+        self.ctx = ctx
+        cfg = ServerConfig(memory_budget, precision, slots, max_batch_requests, max_batch_pixels)
+        h = C.c_void_p()
+        _check(lib.vgi_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self._buf = (Completion * 256)()
+
+    def submit(self, scene_key: int, meshes, grid: Grid, lights: Lights, frames, tag: int,
+               revoxelize: bool = False, arrival_s: float = 0.0) -> None:
+        """frames: sequence of (GBufferData | GBuffer, ConeParams, out_diffuse, out_specular)."""
+        marr = (Mesh * max(1, len(meshes)))()
+        for k, m in enumerate(meshes):
+            marr[k] = m.desc if isinstance(m, MeshData) else m
+        farr = (Frame * max(1, len(frames)))()
+        for k, (gb, cones, od, os_) in enumerate(frames):
+            farr[k].gbuffer = gb.desc if isinstance(gb, GBufferData) else gb
+            farr[k].cones = cones
+            farr[k].out_diffuse = od.ptr if isinstance(od, DeviceArray) else od
+            farr[k].out_specular = (os_.ptr if isinstance(os_, DeviceArray) else os_) if os_ is not None \
+                else None
+        r = Request()
+        r.scene_key = scene_key
+        r.meshes = C.cast(marr, C.c_void_p)
+        r.num_meshes = len(meshes)
+        r.revoxelize = 1 if revoxelize else 0
+        r.grid = grid
+        r.lights = lights
+        r.frames = C.cast(farr, C.c_void_p)
+        r.num_frames = len(frames)
+        r.arrival_s = arrival_s
+        r.tag = tag
+        _check(lib.vgi_server_submit(self.handle, C.byref(r)))  # the server copies both tables
+
+    def poll(self, wait_us: int = 0) -> list:
+        n = C.c_uint32()
+        _check(lib.vgi_server_poll(self.handle, self._buf, len(self._buf), C.byref(n), wait_us))
+        return [Completion.from_buffer_copy(self._buf[i]) for i in range(n.value)]
+
+    def drain(self) -> None:
+        _check(lib.vgi_server_drain(self.handle))
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib.vgi_server_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def close(self) -> None:
+        if self.handle:
+            _check(lib.vgi_server_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def now() -> float:
+    return float(lib.vgi_now())
 
 
 def scene_bytes_estimate(grid: Grid, precision: int, num_triangles: int) -> int:

@@ -252,6 +252,101 @@ int vgi_build_mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t 
 /* Scenes whose mips have different precisions are traced in one launch per precision. */
 int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream);
 
+/* ---- autotuner -------------------------------------------------------------- */
+/* Times one batch (the same scenes in vox, mips and trace, count each) under every bin
+ * size {8,16,32} and every screen tile {8x8,16x16,32x8}, keeps the fastest pair in the
+ * context's tuning (chunk_tris unchanged) and reports all timings. The sweep is
+ * separable: the bin size changes only voxelize + mips and the tile only the trace (the
+ * trace reads mip levels, which are bit-identical for every bin size), so a frame time
+ * is build_ms[bin] + trace_ms[tile]. Each candidate runs once untimed, then `reps` timed
+ * runs of which the fastest counts. Synchronous; leaves the scenes built and traced.
+ * No reference counterpart (the reference has fixed 128^3 draws). */
+typedef struct vgi_autotune_report {
+  vgi_tuning best;
+  uint32_t bin_sizes[3];
+  uint32_t tiles[3][2];
+  float build_ms[3];   /* voxelize + mips of the batch, per bin size */
+  float trace_ms[3];   /* trace of the batch, per tile */
+  float frame_ms[3][3];/* [bin][tile] = build_ms[bin] + trace_ms[tile] */
+} vgi_autotune_report;
+
+int vgi_autotune_batch(vgi_context* ctx, const vgi_voxelize_item* vox, const vgi_mips_item* mips,
+                       const vgi_trace_item* trace, uint32_t count, uint32_t precision, uint32_t reps,
+                       vgi_autotune_report* report, void* stream);
+
+/* ---- serving: continuous batching over resident scenes ---------------------- */
+/* A server owns `slots` HIP streams, each driven by its own host thread, and a scheduler
+ * thread. Submitted requests wait in arrival order; whenever a slot is idle the scheduler
+ * fills it with every waiting request it can admit now (up to the batch limits): one
+ * vgi_voxelize_batch + vgi_build_mips_batch for the scenes that must be (re)built and one
+ * vgi_trace_batch for all their frames. Slots run concurrently, so requests that arrive
+ * while others are on the GPU overlap with them. Built scenes stay resident, keyed by
+ * scene_key, until evicted least-recently-used to admit another within memory_budget
+ * (bytes per vgi_scene_bytes_estimate); an evicted scene of the same grid shape is
+ * recycled without freeing. A scene is rebuilt when its request sets revoxelize, when its
+ * grid changed, or after eviction; lights take effect when a scene is built. Requests
+ * whose scene is being built or rebuilt by an in-flight batch wait for it. A result does
+ * not depend on what else is in its batch or in flight. No reference counterpart (the
+ * reference renders one scene per frame, Game.cpp:248-345). */
+typedef struct vgi_server vgi_server;
+
+typedef struct vgi_server_config {
+  uint64_t memory_budget;      /* bytes of resident scenes */
+  uint32_t precision;          /* mip storage of every scene (VGI_PREC_*) */
+  uint32_t slots;              /* concurrent batches, 1..16 */
+  uint32_t max_batch_requests; /* requests per batch, >= 1 */
+  uint64_t max_batch_pixels;   /* stop filling a batch past this many pixels (0: no limit) */
+} vgi_server_config;
+
+typedef struct vgi_frame {
+  vgi_gbuffer gbuffer;
+  vgi_cone_params cones;
+  float* out_diffuse;
+  float* out_specular; /* may be NULL */
+} vgi_frame;
+
+typedef struct vgi_request {
+  uint64_t scene_key;
+  const vgi_mesh* meshes; /* device buffers stay valid until the request completes */
+  uint32_t num_meshes;
+  uint32_t revoxelize;    /* rebuild even if resident (dynamic geometry) */
+  vgi_grid grid;
+  vgi_lights lights;
+  const vgi_frame* frames;
+  uint32_t num_frames;
+  uint32_t pad;
+  double arrival_s;       /* vgi_now() time the request arrived (0: at submit) */
+  uint64_t tag;           /* returned in its completion */
+} vgi_request;
+
+typedef struct vgi_completion {
+  uint64_t tag;
+  int32_t status;         /* VGI_OK, or the error that failed the request */
+  uint32_t voxelized;     /* 1 if this request built its scene */
+  double arrival_s, start_s, done_s; /* vgi_now() times */
+} vgi_completion;
+
+typedef struct vgi_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t builds, evictions, batches;
+  uint64_t frames, pixels;
+  uint64_t resident_scenes, resident_bytes, peak_resident_bytes;
+  uint32_t max_concurrent_batches;
+  uint32_t max_batch_requests_seen;
+} vgi_server_stats;
+
+double vgi_now(void); /* monotonic seconds */
+int vgi_server_create(vgi_context* ctx, const vgi_server_config* cfg, vgi_server** out);
+/* Copies the request (mesh and frame tables); returns at once. */
+int vgi_server_submit(vgi_server* server, const vgi_request* req);
+/* Moves up to max completions into out, waiting up to wait_us for the first one. */
+int vgi_server_poll(vgi_server* server, vgi_completion* out, uint32_t max, uint32_t* count, uint32_t wait_us);
+/* Blocks until every submitted request has completed (completions stay to be polled). */
+int vgi_server_drain(vgi_server* server);
+int vgi_server_get_stats(vgi_server* server, vgi_server_stats* out);
+/* Drains, stops the threads and releases the resident scenes and streams. */
+int vgi_server_destroy(vgi_server* server);
+
 /* ---- parity readback (synchronous) ----------------------------------------- */
 /* Accumulator as n^3 voxels x 8 uint32 (x fastest): R sum, G sum, B sum, count,
  * sum(nx+1024), sum(ny+1024), sum(nz+1024), 0 (normals quantized to 1/1024). */

@@ -517,6 +517,9 @@ static void cone_march(const Vol<T>& V, const T o[3], const T d[3], T tan_half, 
       cnt += 2;
       for (int ch = 0; ch < 4; ch++) s[ch] = s0[ch] + (s1[ch] - s0[ch]) * fr;
     }
+    // a sample's opacity is a convex combination of values in [0, 1]; rounding of its
+    // weights can push it an ulp above 1, and the clamp keeps 0 <= a <= 1 (as the product)
+    s[3] = std::min(s[3], (T)1);
     T w = (T)1 - a;
     for (int ch = 0; ch < 3; ch++) C[ch] = C[ch] + w * s[ch];
     a = a + w * s[3];

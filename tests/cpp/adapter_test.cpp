@@ -2,6 +2,7 @@
 // passes (RenderList with per-primitive model matrices, then the indirect pass).
 //   adapter_test nodevice : cache and error behaviour without a GPU
 //   adapter_test gpu      : two primitives voxelized through the adapter == the oracle
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -148,7 +149,19 @@ static int gpu() {
   CHECK(ip.LastStatus() == VGI_OK);
   std::vector<float> out(W * H * 4);
   CHECK(vgi_memcpy_d2h(c, out.data(), od, out.size() * 4) == VGI_OK);
-  for (uint32_t i = 0; i < W * H; i++) CHECK(out[i * 4 + 3] >= 0.f && out[i * 4 + 3] <= 1.f);
+  float amin = 1e30f, amax = -1e30f;
+  size_t nonfinite = 0;
+  for (uint32_t i = 0; i < W * H; i++) {
+    const float v = out[i * 4 + 3];
+    if (!std::isfinite(v)) nonfinite++;
+    else {
+      amin = std::min(amin, v);
+      amax = std::max(amax, v);
+    }
+  }
+  std::printf("indirect alpha: min %.9g max %.9g non-finite %zu\n", amin, amax, nonfinite);
+  // the diffuse alpha is a weighted mean of per-cone opacities in [0, 1] (weights rounded)
+  CHECK(nonfinite == 0 && amin >= 0.f && amax <= 1.f + 1e-6f);
   vgi_scene_release(s);
   vgi_destroy(c);
   std::printf("ADAPTER GPU OK\n");

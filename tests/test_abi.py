@@ -18,7 +18,7 @@ HAS_GPU = os.path.exists("/dev/kfd")
 
 def declared_functions():
     text = open(HEADER).read()
-    return re.findall(r"^\s*(?:const\s+char\s*\*|int|uint32_t|uint64_t)\s+(vgi_\w+)\s*\(", text, re.M)
+    return re.findall(r"^\s*(?:const\s+char\s*\*|int|uint32_t|uint64_t|double)\s+(vgi_\w+)\s*\(", text, re.M)
 
 
 def test_every_declared_symbol_is_exported_and_bound():
@@ -42,6 +42,15 @@ STRUCTS = {
     "vgi_tuning": (vgi.Tuning, [f for f, _ in vgi.Tuning._fields_]),
     "vgi_scene_stats": (vgi.SceneStats, [f for f, _ in vgi.SceneStats._fields_]),
     "vgi_timings": (vgi.Timings, [f for f, _ in vgi.Timings._fields_]),
+    "vgi_voxelize_item": (vgi.VoxelizeItem, [f for f, _ in vgi.VoxelizeItem._fields_]),
+    "vgi_mips_item": (vgi.MipsItem, [f for f, _ in vgi.MipsItem._fields_]),
+    "vgi_trace_item": (vgi.TraceItem, [f for f, _ in vgi.TraceItem._fields_]),
+    "vgi_autotune_report": (vgi.AutotuneReport, [f for f, _ in vgi.AutotuneReport._fields_]),
+    "vgi_server_config": (vgi.ServerConfig, [f for f, _ in vgi.ServerConfig._fields_]),
+    "vgi_frame": (vgi.Frame, [f for f, _ in vgi.Frame._fields_]),
+    "vgi_request": (vgi.Request, [f for f, _ in vgi.Request._fields_]),
+    "vgi_completion": (vgi.Completion, [f for f, _ in vgi.Completion._fields_]),
+    "vgi_server_stats": (vgi.ServerStats, [f for f, _ in vgi.ServerStats._fields_]),
 }
 
 
@@ -89,6 +98,14 @@ def test_null_arguments_are_rejected_with_a_message():
     assert vgi.lib.vgi_build_mips(None, None, None, 0, None) == -1
     assert vgi.lib.vgi_trace(None, None, None, None, None, None, None) == -1
     assert vgi.lib.vgi_scene_get_stats(None, None) == -1
+    assert vgi.lib.vgi_voxelize_batch(None, None, 0, None) == -1
+    assert vgi.lib.vgi_build_mips_batch(None, None, 0, 0, None) == -1
+    assert vgi.lib.vgi_trace_batch(None, None, 0, None) == -1
+    assert vgi.lib.vgi_autotune_batch(None, None, None, None, 0, 0, 1, None, None) == -1
+    assert vgi.lib.vgi_server_create(None, None, None) == -1
+    assert vgi.lib.vgi_server_submit(None, None) == -1
+    assert vgi.lib.vgi_server_destroy(None) == 0
+    assert vgi.lib.vgi_now() > 0
 
 
 def test_bytes_estimate_scales_with_grid_and_precision():

@@ -144,3 +144,31 @@ def test_batch_rejects_duplicates(ctx):
     with pytest.raises(vgi.VgiError) as e:
         ctx.build_mips_batch([(j.scene, j.lights), (other, j.lights)], vgi.PREC_FP16)
     assert e.value.code == -5
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(ctx):
+    vgi = _vgi()
+    print("case autotune", flush=True)
+    ws = _workloads()
+    precs = [vgi.PREC_FP16] * len(ws)
+    alone = [Job(ctx, w) for w in ws]
+    _run_alone(ctx, alone, precs)
+    jobs = [Job(ctx, w) for w in ws]
+    try:
+        r = ctx.autotune_batch([(j.scene, j.meshes, j.grid) for j in jobs],
+                               [(j.scene, j.lights) for j in jobs],
+                               [(j.scene, j.gb, j.cones, j.od, j.os) for j in jobs],
+                               vgi.PREC_FP16, reps=2)
+        assert list(r.bin_sizes) == [8, 16, 32]
+        assert [tuple(t) for t in r.tiles] == [(8, 8), (16, 16), (32, 8)]
+        assert all(v > 0 for v in r.build_ms) and all(v > 0 for v in r.trace_ms)
+        t = ctx.get_tuning()
+        assert (t.bin_size, t.tile_w, t.tile_h) == (r.best.bin_size, r.best.tile_w, r.best.tile_h)
+        best = min(r.frame_ms[b][j] for b in range(3) for j in range(3))
+        bb = [8, 16, 32].index(r.best.bin_size)
+        bt = [(8, 8), (16, 16), (32, 8)].index((r.best.tile_w, r.best.tile_h))
+        assert r.frame_ms[bb][bt] == best
+        print("autotune", r.as_dict(), flush=True)
+        _compare(alone, jobs)
+    finally:
+        ctx.set_tuning()
