@@ -126,6 +126,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp32,fp16")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched run")
+    ap.add_argument("--no-autotune", action="store_true", help="default tuning for the headline run")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
@@ -160,6 +161,18 @@ def main() -> int:
         dst.free()
     mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
     gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    tuned = None
+    if not args.no_autotune:  # bin size and pixel schedule for this workload (FP16 mips)
+        sc = ctx.scene()
+        grid, lights, cones = vgi.make_grid(**w.grid), vgi.make_lights(w.lights), vgi.make_cones(**w.cones)
+        od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        tuned = ctx.autotune_batch([(sc, [mesh], grid)], [(sc, lights)], [(sc, gb, cones, od, osp)],
+                                   vgi.PREC_FP16, reps=3)
+        log(f"autotune: {tuned.as_dict()}")
+        sc.release()
+        od.free()
+        osp.free()
     results = []
     for prec in args.precisions.split(","):
         r = run_precision(ctx, vgi, w, mesh, gb, prec, args.steps, args.warmup)
@@ -204,6 +217,7 @@ def main() -> int:
         "fp32_frame_mpixels_s": px / (base["frame_ms"] * 1e-3) / 1e6,
         "fp16_vs_fp32_frame_speedup": base["frame_ms"] / head["frame_ms"],
         "trilinear_per_pixel": head["trilinear"] / px,
+        "tuning": tuned.as_dict() if tuned is not None else "default",
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_trace<{head['prec']}>",

@@ -148,12 +148,19 @@ struct vgi_workspace {
   size_t pairs_bytes = 0;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  void* sort_keys = nullptr;  // Morton order: keys in | keys out (u64)
+  size_t sort_keys_bytes = 0;
+  void* sort_pix = nullptr;   // pixel indices in | out (u32)
+  size_t sort_pix_bytes = 0;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
   uint8_t* h_read = nullptr;  // pinned readback of DevStats[] + totals[]
   size_t h_read_cap = 0;
 
   size_t device_bytes() const {
     size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
-               scan_bytes + work_bytes + pairs_bytes + scan_tmp_bytes;
+               scan_bytes + work_bytes + pairs_bytes + scan_tmp_bytes + sort_keys_bytes + sort_pix_bytes +
+               sort_tmp_bytes;
     for (const Slot& s : slot) b += s.cap;
     return b;
   }
@@ -163,7 +170,7 @@ struct vgi_context {
   int device = 0;
   int num_cus = 256;
   hipStream_t stream = nullptr;
-  vgi_tuning tuning{8, 64, 16, 16};
+  vgi_tuning tuning{8, 64, 16, 16, VGI_ORDER_TILES};
   int profiling = 0;
   float* d_lut = nullptr;
   unsigned long long* d_counter = nullptr;
@@ -205,8 +212,8 @@ size_t scene_device_bytes(const vgi_scene* s) {
 }
 
 void free_workspace(vgi_workspace* w) {
-  void* dev[] = {w->tris, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all,
-                 w->scan_all, w->work_map, w->pairs, w->scan_tmp};
+  void* dev[] = {w->tris, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all, w->scan_all,
+                 w->work_map, w->pairs, w->scan_tmp, w->sort_keys, w->sort_pix, w->sort_tmp};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (Slot& s : w->slot) {
@@ -842,28 +849,35 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   if (!items) return fail(VGI_E_INVALID, "items is NULL");
   for (uint32_t i = 0; i < count; i++) TRY(check_trace_item(ctx, items[i], i));
   const vgi_tuning tn = tuning_of(ctx);
+  const bool morton = tn.pixel_order == VGI_ORDER_MORTON;
   // one launch per storage precision
   std::vector<uint32_t> group[3];
   for (uint32_t i = 0; i < count; i++) group[items[i].scene->prec].push_back(i);
   struct Launch {
     uint32_t prec, nscenes;
-    size_t o_params, o_blocks, nblocks;
+    size_t o_params, o_blocks, o_seg, nblocks, entries;
     bool buffer_path;
   };
   std::vector<Launch> launches;
   Carve cv;
+  size_t max_entries = 0;
   for (uint32_t p = 0; p < 3; p++) {
     if (group[p].empty()) continue;
-    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, true};
+    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, true};
     for (uint32_t i : group[p]) {
       const vgi_gbuffer& gb = items[i].gbuffer;
-      L.nblocks += (size_t)ceil_div(gb.width, tn.tile_w) * ceil_div(gb.height, tn.tile_h);
+      if (morton) L.entries += align_up((size_t)gb.width * gb.height, kSortBlock);
+      else L.nblocks += (size_t)ceil_div(gb.width, tn.tile_w) * ceil_div(gb.height, tn.tile_h);
       for (uint32_t k = 0; k < items[i].scene->grid.levels; k++)
         L.buffer_path = L.buffer_path && items[i].scene->level_bytes[k] < 0x7FFFFFF0ull;  // 31-bit offsets
     }
-    if (L.nblocks >= (1ull << 31)) return fail(VGI_E_RANGE, "trace batch too large (%zu tiles)", L.nblocks);
+    if (morton) L.nblocks = L.entries / kSortBlock;
+    if (L.nblocks >= (1ull << 31) || L.entries >= (1ull << 32))
+      return fail(VGI_E_RANGE, "trace batch too large (%zu workgroups)", L.nblocks);
     L.o_params = cv.take((size_t)L.nscenes * sizeof(TraceParams));
-    L.o_blocks = cv.take(L.nscenes > 1 ? L.nblocks * sizeof(uint2) : 0);
+    L.o_blocks = cv.take(!morton && L.nscenes > 1 ? L.nblocks * sizeof(uint2) : 0);
+    L.o_seg = cv.take(morton ? ((size_t)L.nscenes + 1) * sizeof(uint32_t) : 0);
+    max_entries = std::max(max_entries, L.entries);
     launches.push_back(L);
   }
   HIP_TRY(hipSetDevice(ctx->device));
@@ -871,20 +885,33 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   vgi_workspace* w = workspace(ctx, st);
   if (!w) return fail(VGI_E_HIP, "workspace creation failed");
   std::lock_guard<std::mutex> wl(w->mu);
+  if (morton) {  // sort buffers for the largest launch (grown; queued work drained first)
+    size_t tmp_need = 0;
+    HIP_TRY(sort_pixels(nullptr, &tmp_need, nullptr, nullptr, nullptr, nullptr, (uint32_t)max_entries, 64, st));
+    if (w->sort_keys_bytes < max_entries * 16 || w->sort_tmp_bytes < tmp_need) HIP_TRY(hipStreamSynchronize(st));
+    TRY(grow_bytes(&w->sort_keys, &w->sort_keys_bytes, max_entries * 16, "sort keys"));
+    TRY(grow_bytes(&w->sort_pix, &w->sort_pix_bytes, max_entries * 8, "sort pixels"));
+    TRY(grow_bytes(&w->sort_tmp, &w->sort_tmp_bytes, std::max<size_t>(tmp_need, 1), "sort scratch"));
+  }
   Slot* sl = nullptr;
   TRY(acquire_slot(w, cv.off, &sl));
   for (const Launch& L : launches) {
     TraceParams* hp = (TraceParams*)(sl->h + L.o_params);
     uint2* hb = (uint2*)(sl->h + L.o_blocks);
-    size_t b = 0;
+    uint32_t* hs = (uint32_t*)(sl->h + L.o_seg);
+    size_t b = 0, e = 0;
     for (uint32_t j = 0; j < L.nscenes; j++) {
       const vgi_trace_item& it = items[group[L.prec][j]];
       fill_trace_params(hp[j], it, tn);
-      if (L.nscenes > 1) {
+      if (morton) {
+        hs[j] = (uint32_t)e;
+        e += align_up((size_t)it.gbuffer.width * it.gbuffer.height, kSortBlock);
+      } else if (L.nscenes > 1) {
         const uint32_t nt = hp[j].tiles_x * hp[j].tiles_y;
         for (uint32_t t = 0; t < nt; t++) hb[b++] = make_uint2(j, t);
       }
     }
+    if (morton) hs[L.nscenes] = (uint32_t)e;
   }
   TRY(upload(w, sl, 0, cv.off));
   const bool prof = ctx->profiling != 0;
@@ -895,13 +922,24 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   for (const Launch& L : launches) {
     TraceBatch B{};
     B.scenes = (const TraceParams*)(sl->d + L.o_params);
-    B.blocks = L.nscenes > 1 ? (const uint2*)(sl->d + L.o_blocks) : nullptr;
+    B.blocks = !morton && L.nscenes > 1 ? (const uint2*)(sl->d + L.o_blocks) : nullptr;
     B.counter = prof ? ctx->d_counter : nullptr;
     B.nblocks = (uint32_t)L.nblocks;
     B.tile_w = tn.tile_w;
     B.tile_h = tn.tile_h;
     B.prec = L.prec;
     B.nscenes = L.nscenes;
+    if (morton && L.entries > 0) {
+      unsigned long long* keys = (unsigned long long*)w->sort_keys;
+      uint32_t* pix = (uint32_t*)w->sort_pix;
+      B.seg = (const uint32_t*)(sl->d + L.o_seg);
+      HIP_TRY(launch_morton_keys(B, keys, pix, (uint32_t)L.entries, st));
+      const int end_bit = 32 + (int)ilog2(std::max<uint32_t>(L.nscenes - 1, 1)) + 1;
+      HIP_TRY(sort_pixels(w->sort_tmp, &w->sort_tmp_bytes, keys, keys + max_entries, pix, pix + max_entries,
+                          (uint32_t)L.entries, std::min(end_bit, 64), st));
+      B.skeys = keys + max_entries;
+      B.spix = pix + max_entries;
+    }
     HIP_TRY(launch_trace(B, L.buffer_path, st));
   }
   if (prof) {
@@ -1003,6 +1041,7 @@ int vgi_set_tuning(vgi_context* ctx, const vgi_tuning* t) {
                        (t->tile_w == 32 && t->tile_h == 8) || (t->tile_w == 16 && t->tile_h == 8) ||
                        (t->tile_w == 8 && t->tile_h == 16);
   if (!ok_tile) return fail(VGI_E_INVALID, "tile %ux%u not supported", t->tile_w, t->tile_h);
+  if (t->pixel_order > VGI_ORDER_MORTON) return fail(VGI_E_INVALID, "pixel_order %u", t->pixel_order);
   std::lock_guard<std::mutex> lk(ctx->mu);
   ctx->tuning = *t;
   return VGI_OK;

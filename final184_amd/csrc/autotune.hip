@@ -4,7 +4,8 @@
 // Bin size trades the front end's work (more, smaller bins: more (triangle, bin) pairs
 // and slot atomics) against the voxelize kernel's (bigger bins: fewer, longer work items
 // and bigger LDS tiles swept per item); the screen tile trades workgroup size against
-// tail effects and the G-buffer's spatial coherence. Neither has a closed-form optimum
+// tail effects and the G-buffer's spatial coherence, and Morton order (a sort before
+// the march) pays off only when the G-buffer is not coherent. None has a closed-form optimum
 // across scene mixes, so both are measured. The two sweeps are independent: the trace
 // reads only the mip levels, which are the same bits for every bin size, and voxelize and
 // mips never read the tile shape.
@@ -17,7 +18,11 @@
 namespace {
 
 constexpr uint32_t kBins[3] = {8, 16, 32};
-constexpr uint32_t kTiles[3][2] = {{8, 8}, {16, 16}, {32, 8}};
+// pixel schedules: screen tiles of three shapes, and Morton order (256-pixel runs)
+constexpr uint32_t kSched[4][3] = {{8, 8, VGI_ORDER_TILES},
+                                   {16, 16, VGI_ORDER_TILES},
+                                   {32, 8, VGI_ORDER_TILES},
+                                   {16, 16, VGI_ORDER_MORTON}};
 
 int time_build(vgi_context* ctx, const vgi_voxelize_item* vox, const vgi_mips_item* mips, uint32_t count,
                uint32_t precision, void* stream, float* ms) {
@@ -66,12 +71,12 @@ extern "C" int vgi_autotune_batch(vgi_context* ctx, const vgi_voxelize_item* vox
     }
     r.build_ms[b] = best;
   }
-  for (int j = 0; j < 3; j++) {
+  for (int j = 0; j < 4; j++) {
     vgi_tuning t = base;
-    t.tile_w = kTiles[j][0];
-    t.tile_h = kTiles[j][1];
-    r.tiles[j][0] = kTiles[j][0];
-    r.tiles[j][1] = kTiles[j][1];
+    t.tile_w = kSched[j][0];
+    t.tile_h = kSched[j][1];
+    t.pixel_order = kSched[j][2];
+    for (int c = 0; c < 3; c++) r.tiles[j][c] = kSched[j][c];
     if ((rc = vgi_set_tuning(ctx, &t)) != VGI_OK) return rc;
     float best = FLT_MAX, ms = 0.f;
     for (uint32_t k = 0; k <= reps; k++) {
@@ -85,7 +90,7 @@ extern "C" int vgi_autotune_batch(vgi_context* ctx, const vgi_voxelize_item* vox
   }
   int bb = 0, bt = 0;
   for (int b = 0; b < 3; b++)
-    for (int j = 0; j < 3; j++) {
+    for (int j = 0; j < 4; j++) {
       r.frame_ms[b][j] = r.build_ms[b] + r.trace_ms[j];
       if (r.frame_ms[b][j] < r.frame_ms[bb][bt]) {
         bb = b;
@@ -94,8 +99,9 @@ extern "C" int vgi_autotune_batch(vgi_context* ctx, const vgi_voxelize_item* vox
     }
   r.best = base;
   r.best.bin_size = kBins[bb];
-  r.best.tile_w = kTiles[bt][0];
-  r.best.tile_h = kTiles[bt][1];
+  r.best.tile_w = kSched[bt][0];
+  r.best.tile_h = kSched[bt][1];
+  r.best.pixel_order = kSched[bt][2];
   if ((rc = vgi_set_tuning(ctx, &r.best)) != VGI_OK) return rc;
   // leave the scenes built with the chosen bin size
   if ((rc = vgi_voxelize_batch(ctx, vox, count, stream)) != VGI_OK) return rc;

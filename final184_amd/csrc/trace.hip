@@ -23,6 +23,15 @@
 // texels of a level come from one VGPR byte offset (plus SGPR pitch and immediate +1
 // texel), four 16-byte loads per level for FP16 (texel x and x+1 share a load), eight for
 // FP32. The per-pixel-roughness specular cone uses 64-bit global loads.
+//
+// Morton order (for G-buffers that are not spatially coherent, e.g. points sampled on
+// triangles): k_morton_keys keys every pixel by (scene << 32 | 30-bit Morton code of its
+// position quantized to 1024^3 over the grid), one radix sort orders the batch, and a
+// workgroup then takes 256 consecutive sorted pixels of one scene, so the cones of a
+// wavefront start close together and share cache lines. Every pixel's result is the
+// same in either order.
+#include <hipcub/hipcub.hpp>
+
 #include "vgi_internal.hpp"
 
 namespace vgi {
@@ -224,27 +233,44 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
   return make_float4(cr, cg, cb, a);
 }
 
-template <int PREC, bool BUF>
+template <int PREC, bool BUF, bool SORTED>
 __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams* __restrict__ scenes,
-                                               const uint2* __restrict__ blocks) {
+                                               const uint2* __restrict__ blocks,
+                                               const unsigned long long* __restrict__ skeys,
+                                               const uint32_t* __restrict__ spix) {
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous band of the
-  // batch's tile list (tiles of one scene are consecutive, row-major).
+  // batch's work list (tiles of one scene are consecutive, row-major; sorted runs are
+  // consecutive in Morton order).
   const uint32_t b = blockIdx.x;
   const uint32_t xcd = b & 7u, slot = b >> 3;
   const uint32_t full = Bt.nblocks >> 3, rem = Bt.nblocks & 7u;
   const uint32_t idx = xcd * full + (xcd < rem ? xcd : rem) + slot;
-  const uint2 st = Bt.nscenes == 1 ? make_uint2(0u, idx) : blocks[idx];
-  const TraceParams& P = scenes[st.x];
-  const uint32_t tile = st.y;
-  const uint32_t tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t wpr = Bt.tile_w >> 3;  // 8x8 wave blocks per tile row
-  const uint32_t px = tx * Bt.tile_w + (wave % wpr) * 8 + (lane & 7u);
-  const uint32_t py = ty * Bt.tile_h + (wave / wpr) * 8 + (lane >> 3);
-  const bool active = px < P.width && py < P.height;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t scene;
+  size_t p;
+  bool active;
+  if constexpr (SORTED) {
+    const size_t e = (size_t)idx * kSortBlock + threadIdx.x;
+    scene = (uint32_t)(skeys[(size_t)idx * kSortBlock] >> 32);  // a run never spans two scenes
+    const uint32_t q = gld(spix + e);
+    active = q != kNoPixel;
+    p = q;
+  } else {
+    const uint2 st = Bt.nscenes == 1 ? make_uint2(0u, idx) : blocks[idx];
+    scene = st.x;
+    const TraceParams& T = scenes[scene];
+    const uint32_t tile = st.y;
+    const uint32_t tx = tile % T.tiles_x, ty = tile / T.tiles_x;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wpr = Bt.tile_w >> 3;  // 8x8 wave blocks per tile row
+    const uint32_t px = tx * Bt.tile_w + (wave % wpr) * 8 + (lane & 7u);
+    const uint32_t py = ty * Bt.tile_h + (wave / wpr) * 8 + (lane >> 3);
+    active = px < T.width && py < T.height;
+    p = (size_t)py * T.width + px;
+  }
+  const TraceParams& P = scenes[scene];
   uint32_t count = 0;
   if (active) {
-    const size_t p = (size_t)py * P.width + px;
     const float Px = gld(P.pos + p * 3), Py = gld(P.pos + p * 3 + 1), Pz = gld(P.pos + p * 3 + 2);
     const float Nx = gld(P.nrm + p * 3), Ny = gld(P.nrm + p * 3 + 1), Nz = gld(P.nrm + p * 3 + 2);
     const float len2 = (Nx * Nx + Ny * Ny) + Nz * Nz;
@@ -303,26 +329,98 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams*
   }
 }
 
+template <int PREC, bool SORTED>
+static void launch_trace_prec(const TraceBatch& B, bool buffer_path, dim3 grid, dim3 block, hipStream_t s) {
+  if (buffer_path)
+    hipLaunchKernelGGL((k_trace<PREC, true, SORTED>), grid, block, 0, s, B, B.scenes, B.blocks, B.skeys, B.spix);
+  else
+    hipLaunchKernelGGL((k_trace<PREC, false, SORTED>), grid, block, 0, s, B, B.scenes, B.blocks, B.skeys, B.spix);
+}
+
 hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s) {
   if (B.nblocks == 0) return hipSuccess;
-  const dim3 block(B.tile_w * B.tile_h), grid(B.nblocks);
+  const bool sorted = B.skeys != nullptr;
+  const dim3 block(sorted ? kSortBlock : B.tile_w * B.tile_h), grid(B.nblocks);
   switch (B.prec) {
     case VGI_PREC_FP32:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, true>), grid, block, 0, s, B, B.scenes, B.blocks);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP32, false>), grid, block, 0, s, B, B.scenes, B.blocks);
+      if (sorted) launch_trace_prec<VGI_PREC_FP32, true>(B, buffer_path, grid, block, s);
+      else launch_trace_prec<VGI_PREC_FP32, false>(B, buffer_path, grid, block, s);
       break;
     case VGI_PREC_FP16:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, true>), grid, block, 0, s, B, B.scenes, B.blocks);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_FP16, false>), grid, block, 0, s, B, B.scenes, B.blocks);
+      if (sorted) launch_trace_prec<VGI_PREC_FP16, true>(B, buffer_path, grid, block, s);
+      else launch_trace_prec<VGI_PREC_FP16, false>(B, buffer_path, grid, block, s);
       break;
     case VGI_PREC_BF16:
-      if (buffer_path) hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, true>), grid, block, 0, s, B, B.scenes, B.blocks);
-      else hipLaunchKernelGGL((k_trace<VGI_PREC_BF16, false>), grid, block, 0, s, B, B.scenes, B.blocks);
+      if (sorted) launch_trace_prec<VGI_PREC_BF16, true>(B, buffer_path, grid, block, s);
+      else launch_trace_prec<VGI_PREC_BF16, false>(B, buffer_path, grid, block, s);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Morton order
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// One thread per entry of the batch's padded pixel list; a workgroup's 256 entries belong
+// to one scene (segments are multiples of 256).
+__global__ __launch_bounds__(256) void k_morton_keys(const TraceParams* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ seg, uint32_t nscenes,
+                                                     unsigned long long* __restrict__ keys,
+                                                     uint32_t* __restrict__ pix) {
+  const uint32_t e0 = blockIdx.x * kSortBlock;
+  uint32_t lo = 0, hi = nscenes - 1;  // last scene whose segment starts at or before e0
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (seg[mid] <= e0) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t sc = lo;
+  const TraceParams& P = scenes[sc];
+  const uint32_t e = e0 + threadIdx.x;
+  const uint32_t i = e - seg[sc];
+  const uint32_t npx = P.width * P.height;
+  unsigned long long key = ((unsigned long long)sc << 32) | 0xFFFFFFFFull;  // padding sorts last
+  uint32_t q = kNoPixel;
+  if (i < npx) {
+    const float s = 1024.0f / (float)P.n * P.inv_h;
+    uint32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float u = (gld(P.pos + (size_t)i * 3 + a) - P.origin[a]) * s;
+      c[a] = (uint32_t)fminf(fmaxf(u, 0.0f), 1023.0f);  // NaN -> 0
+    }
+    key = ((unsigned long long)sc << 32) | (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
+    q = i;
+  }
+  gst(keys + e, key);
+  gst(pix + e, q);
+}
+
+hipError_t launch_morton_keys(const TraceBatch& B, unsigned long long* keys, uint32_t* pix, uint32_t entries,
+                              hipStream_t s) {
+  if (entries == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton_keys, dim3(entries / kSortBlock), dim3(kSortBlock), 0, s, B.scenes, B.seg,
+                     B.nscenes, keys, pix);
+  return hipGetLastError();
+}
+
+hipError_t sort_pixels(void* tmp, size_t* tmp_bytes, const unsigned long long* keys_in,
+                       unsigned long long* keys_out, const uint32_t* pix_in, uint32_t* pix_out, uint32_t n,
+                       int end_bit, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, keys_in, keys_out, pix_in, pix_out, (int)n, 0,
+                                            end_bit, s);
 }
 
 }  // namespace vgi

@@ -160,13 +160,22 @@ struct TraceParams {
 
 struct TraceBatch {
   const TraceParams* scenes;
-  const uint2* blocks;            // (scene, tile) per workgroup, in XCD-band order
+  const uint2* blocks;            // (scene, tile) per workgroup (screen-tile order, > 1 scene)
   unsigned long long* counter;    // trilinear gathers (may be null)
+  // Morton order (null in screen-tile order): the batch's pixels sorted by
+  // (scene << 32 | Morton code of the position); scene s owns entries [seg[s], seg[s+1]),
+  // a multiple of 256, padded with pixel index kNoPixel after its sorted pixels.
+  const unsigned long long* skeys;
+  const uint32_t* spix;
+  const uint32_t* seg;
   uint32_t nblocks;
   uint32_t tile_w, tile_h;
   uint32_t prec;
   uint32_t nscenes;
 };
+
+constexpr uint32_t kNoPixel = 0xFFFFFFFFu;
+constexpr uint32_t kSortBlock = 256;  // sorted pixels per workgroup
 
 // Texel bytes per precision.
 inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u : 8u; }
@@ -258,5 +267,11 @@ hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_block
                            hipStream_t s);
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s);
 hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s);
+// Morton keys of every pixel of a batch (entries seg[0]..seg[nscenes], padding included).
+hipError_t launch_morton_keys(const TraceBatch& B, unsigned long long* keys, uint32_t* pix, uint32_t entries,
+                              hipStream_t s);
+hipError_t sort_pixels(void* tmp, size_t* tmp_bytes, const unsigned long long* keys_in,
+                       unsigned long long* keys_out, const uint32_t* pix_in, uint32_t* pix_out, uint32_t n,
+                       int end_bit, hipStream_t s);
 
 }  // namespace vgi

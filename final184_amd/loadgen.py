@@ -10,9 +10,10 @@ serving: a Poisson request stream into vgi.Server. Each request names a scene ke
 follow a Zipf popularity over a population of scenes whose geometry comes from a pool of
 seeded triangle soups (triangle counts lognormal, median 50k) placed in per-key grids
 (128^3 or 256^3, per-key origin shift, so every key has its own volume). A request asks
-for 1-8 frames at 512x512 .. 1920x1080 (G-buffer windows of its soup's pool G-buffer,
-seeded eye per frame); 10% of requests re-voxelize (dynamic geometry). Latency is
-arrival (the scheduled Poisson time) to completion of the request's last frame.
+for 1-8 frames at 512x512 .. 1920x1080, each a ray-cast G-buffer of its soup seen from a
+seeded camera pose (9 diffuse cones + a roughness-driven specular cone); 10% of
+requests re-voxelize (dynamic geometry). Latency is arrival (the scheduled Poisson time)
+to completion of the request's last frame.
 """
 from __future__ import annotations
 
@@ -52,6 +53,7 @@ def run_batched(ctx, vgi, count=64, steps=5, reps=2, log=print, seed=sg.SEED_BAT
     log(f"batched: {count} scenes generated in {time.perf_counter() - t0:.1f}s")
     sc = [_BatchScene(ctx, vgi, w) for w in ws]
     prec = vgi.PREC_FP16
+    prev = ctx.get_tuning()
     vox = [(s.scene, [s.mesh], s.grid) for s in sc]
     mips = [(s.scene, s.lights) for s in sc]
     trace = [(s.scene, s.gb, s.cones, s.od, s.os) for s in sc]
@@ -96,7 +98,7 @@ def run_batched(ctx, vgi, count=64, steps=5, reps=2, log=print, seed=sg.SEED_BAT
         autotune=tuned.as_dict())
     for s in sc:
         s.scene.release()
-    ctx.set_tuning()
+    ctx.put_tuning(prev)
     return out
 
 
@@ -109,31 +111,26 @@ POOL_PIXELS = 1920 * 1080
 
 
 class _Soup:
-    """A seeded triangle soup and a 1920x1080 pool G-buffer of points on it, on the device."""
+    """A seeded triangle soup (per-triangle roughness 0.05-0.5) and, for one seeded pinhole
+    camera inside it, a ray-cast G-buffer per frame size (centred crops of one 1920x1080
+    image), all on the device."""
 
-    def __init__(self, ctx, vgi, pos, alb, gp, gn):
-        self.ntri = pos.shape[0]
+    def __init__(self, ctx, vgi, seed, ntri):
+        pos, alb = sg.random_triangles(seed, ntri)
+        rough = np.random.default_rng(seed).uniform(0.05, 0.5, ntri).astype(np.float32)
+        cam = sg.camera_pose(seed, (0.2, 0.2, 0.2), (0.8, 0.8, 0.8))
+        gp, gn, gr = sg.gbuffer_camera(pos, rough, cam, 1920, 1080)
+        self.ntri = ntri
+        self.eye = tuple(float(v) for v in cam[:3])
         self.mesh = vgi.MeshData(ctx, pos.reshape(-1, 3), albedo=alb)
-        self.gb = vgi.GBufferData(ctx, gp, gn, POOL_PIXELS, 1)
-
-    def window(self, vgi, offset, width, height):
-        g = vgi.GBuffer()
-        g.position = self.gb.pos.ptr + offset * 12
-        g.normal = self.gb.nrm.ptr + offset * 12
-        g.roughness = None
-        g.width, g.height = width, height
-        return g
-
-
-def _make_soups(ctx, vgi, seed, ntris):
-    def gen(i):
-        pos, alb = sg.random_triangles(seed + 7 * (i + 1), int(ntris[i]))
-        gp, gn = sg.gbuffer_on_triangles(seed + 7 * (i + 1), pos, POOL_PIXELS)
-        return pos, alb, gp, gn
-
-    with ThreadPoolExecutor(max_workers=8) as ex:  # the generators release the GIL
-        host = list(ex.map(gen, range(len(ntris))))
-    return [_Soup(ctx, vgi, *h) for h in host]
+        self.gbs = {}
+        img_p, img_n, img_r = gp.reshape(1080, 1920, 3), gn.reshape(1080, 1920, 3), gr.reshape(1080, 1920)
+        for (w, h) in FRAME_SIZES:
+            y0, x0 = (1080 - h) // 2, (1920 - w) // 2
+            crop = (slice(y0, y0 + h), slice(x0, x0 + w))
+            self.gbs[(w, h)] = vgi.GBufferData(ctx, np.ascontiguousarray(img_p[crop]).reshape(-1, 3),
+                                               np.ascontiguousarray(img_n[crop]).reshape(-1, 3), w, h,
+                                               np.ascontiguousarray(img_r[crop]).reshape(-1))
 
 
 class _OutPool:
@@ -153,14 +150,14 @@ class _OutPool:
         self.free.extend(bufs)
 
 
-def run_serving(ctx, vgi, seconds=60.0, qps=200.0, budget_gb=200.0, keys=1000, soups=32,
+def run_serving(ctx, vgi, seconds=60.0, qps=200.0, budget_gb=200.0, keys=1000, soups=24,
                 zipf_s=1.0, revox=0.1, slots=4, max_batch=16, num_diffuse=9, out_buffers=192,
                 log=print, seed=0xF1840005):
     rng = np.random.default_rng(seed)
     ntris = sg.lognormal_ints(seed, soups, 50_000.0, 1.0, 1_000, 200_000)
     t0 = time.perf_counter()
-    src = _make_soups(ctx, vgi, seed, ntris)
-    log(f"serving: {soups} soups ({int(ntris.sum())} triangles) + pool G-buffers ready in "
+    src = [_Soup(ctx, vgi, seed + 7 * (i + 1), int(ntris[i])) for i in range(soups)]
+    log(f"serving: {soups} soups ({int(ntris.sum())} triangles) + camera G-buffers ready in "
         f"{time.perf_counter() - t0:.1f}s")
 
     # per-key scene definition
@@ -189,10 +186,7 @@ def run_serving(ctx, vgi, seconds=60.0, qps=200.0, budget_gb=200.0, keys=1000, s
         nf = int(rng.integers(1, 9))
         frames = []
         for _ in range(nf):
-            wdt, hgt = FRAME_SIZES[int(rng.integers(len(FRAME_SIZES)))]
-            off = int(rng.integers(0, POOL_PIXELS - wdt * hgt + 1))
-            eye = tuple(float(v) for v in rng.uniform(-0.5, 1.5, 3))
-            frames.append((off, wdt, hgt, eye))
+            frames.append(FRAME_SIZES[int(rng.integers(len(FRAME_SIZES)))])
         reqs.append((key, frames, bool(rng.random() < revox)))
     cone_base = dict(num_diffuse=num_diffuse, specular=True, diffuse_aperture_deg=60.0,
                      diffuse_ring_deg=60.0, specular_aperture_deg=10.0, step_factor=0.5,
@@ -216,13 +210,13 @@ def run_serving(ctx, vgi, seconds=60.0, qps=200.0, budget_gb=200.0, keys=1000, s
                 if bufs is None:
                     break  # out of output buffers: wait for completions
                 s = src[key % soups]
-                fl = [(s.window(vgi, off, wdt, hgt), vgi.make_cones(**cone_base, eye=eye), od, os_)
-                      for (off, wdt, hgt, eye), (od, os_) in zip(frames, bufs)]
+                fl = [(s.gbs[(wdt, hgt)], vgi.make_cones(**cone_base, eye=s.eye), od, os_)
+                      for (wdt, hgt), (od, os_) in zip(frames, bufs)]
                 arr = t_start + float(arrivals[i])
                 late.append(now - arr)
                 srv.submit(key, [s.mesh], grids[key], key_light[key], fl, tag=i, revoxelize=rv,
                            arrival_s=arr)
-                held[i] = (bufs, sum(w * h for _, w, h, _ in frames), len(frames))
+                held[i] = (bufs, sum(w * h for w, h in frames), len(frames))
                 i += 1
             if i < nreq:  # sleep in the poll until the next arrival (at most 2 ms)
                 wait = min(2000, max(0, int((t_start + arrivals[i] - vgi.now()) * 1e6)))

@@ -67,9 +67,12 @@ class GBuffer(C.Structure):
                 ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
+ORDER_TILES, ORDER_MORTON = 0, 1
+
+
 class Tuning(C.Structure):
     _fields_ = [("bin_size", C.c_uint32), ("chunk_tris", C.c_uint32), ("tile_w", C.c_uint32),
-                ("tile_h", C.c_uint32)]
+                ("tile_h", C.c_uint32), ("pixel_order", C.c_uint32)]
 
 
 class SceneStats(C.Structure):
@@ -102,15 +105,19 @@ class TraceItem(C.Structure):
 
 
 class AutotuneReport(C.Structure):
-    _fields_ = [("best", Tuning), ("bin_sizes", C.c_uint32 * 3), ("tiles", (C.c_uint32 * 2) * 3),
-                ("build_ms", C.c_float * 3), ("trace_ms", C.c_float * 3),
-                ("frame_ms", (C.c_float * 3) * 3)]
+    _fields_ = [("best", Tuning), ("bin_sizes", C.c_uint32 * 3), ("tiles", (C.c_uint32 * 3) * 4),
+                ("build_ms", C.c_float * 3), ("trace_ms", C.c_float * 4),
+                ("frame_ms", (C.c_float * 4) * 3)]
+
+    @staticmethod
+    def schedule_name(w, h, order) -> str:
+        return "morton" if order == ORDER_MORTON else f"{w}x{h}"
 
     def as_dict(self) -> dict:
-        return dict(best=dict(bin_size=self.best.bin_size, tile=[self.best.tile_w, self.best.tile_h]),
-                    build_ms={str(self.bin_sizes[b]): round(self.build_ms[b], 4) for b in range(3)},
-                    trace_ms={f"{self.tiles[j][0]}x{self.tiles[j][1]}": round(self.trace_ms[j], 4)
-                              for j in range(3)})
+        b = self.best
+        return dict(best=dict(bin_size=b.bin_size, schedule=self.schedule_name(b.tile_w, b.tile_h, b.pixel_order)),
+                    build_ms={str(self.bin_sizes[i]): round(self.build_ms[i], 4) for i in range(3)},
+                    trace_ms={self.schedule_name(*self.tiles[j]): round(self.trace_ms[j], 4) for j in range(4)})
 
 
 class ServerConfig(C.Structure):
@@ -474,8 +481,11 @@ class Context:
     def sync(self, stream=None) -> None:
         _check(lib.vgi_sync(self.handle, stream))
 
-    def set_tuning(self, bin_size=8, chunk_tris=64, tile_w=16, tile_h=16) -> None:
-        t = Tuning(bin_size, chunk_tris, tile_w, tile_h)
+    def set_tuning(self, bin_size=8, chunk_tris=64, tile_w=16, tile_h=16, pixel_order=ORDER_TILES) -> None:
+        t = Tuning(bin_size, chunk_tris, tile_w, tile_h, pixel_order)
+        _check(lib.vgi_set_tuning(self.handle, C.byref(t)))
+
+    def put_tuning(self, t: Tuning) -> None:
         _check(lib.vgi_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> Tuning:

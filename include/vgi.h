@@ -139,11 +139,19 @@ typedef struct vgi_gbuffer {
 } vgi_gbuffer;
 
 /* Launch-shape knobs (the autotuner's search space). */
+/* Cone-march pixel order. TILES: workgroups take screen tiles (coherent G-buffers from a
+ * camera). MORTON: the pixels of each G-buffer are first sorted by the Morton code of
+ * their position in the grid and workgroups take runs of 256 sorted pixels, so that the
+ * cones of a wavefront start close together even when the G-buffer is not spatially
+ * coherent (e.g. points sampled on triangles). Results are identical either way. */
+enum { VGI_ORDER_TILES = 0, VGI_ORDER_MORTON = 1 };
+
 typedef struct vgi_tuning {
   uint32_t bin_size;    /* voxelize triangle bin edge: 8, 16 or 32 voxels */
-  uint32_t chunk_tris;  /* triangles per voxelize work item: 16..64 */
-  uint32_t tile_w;      /* cone-march screen tile: 8x8, 16x16 or 32x8 */
+  uint32_t chunk_tris;  /* triangles per voxelize work item: 1..64 */
+  uint32_t tile_w;      /* cone-march screen tile: 8x8, 16x16 or 32x8 (TILES order) */
   uint32_t tile_h;
+  uint32_t pixel_order; /* VGI_ORDER_* */
 } vgi_tuning;
 
 typedef struct vgi_scene_stats {
@@ -254,20 +262,21 @@ int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t coun
 
 /* ---- autotuner -------------------------------------------------------------- */
 /* Times one batch (the same scenes in vox, mips and trace, count each) under every bin
- * size {8,16,32} and every screen tile {8x8,16x16,32x8}, keeps the fastest pair in the
- * context's tuning (chunk_tris unchanged) and reports all timings. The sweep is
- * separable: the bin size changes only voxelize + mips and the tile only the trace (the
- * trace reads mip levels, which are bit-identical for every bin size), so a frame time
- * is build_ms[bin] + trace_ms[tile]. Each candidate runs once untimed, then `reps` timed
- * runs of which the fastest counts. Synchronous; leaves the scenes built and traced.
- * No reference counterpart (the reference has fixed 128^3 draws). */
+ * size {8,16,32} and every pixel schedule (screen tiles 8x8, 16x16, 32x8, or Morton
+ * order), keeps the fastest pair in the context's tuning (chunk_tris unchanged) and
+ * reports all timings. The sweep is separable: the bin size changes only voxelize + mips
+ * and the schedule only the trace (the trace reads mip levels, which are bit-identical
+ * for every bin size), so a frame time is build_ms[bin] + trace_ms[schedule]. Each
+ * candidate runs once untimed, then `reps` timed runs of which the fastest counts.
+ * Synchronous; leaves the scenes built and traced. No reference counterpart (the
+ * reference has fixed 128^3 draws). */
 typedef struct vgi_autotune_report {
   vgi_tuning best;
   uint32_t bin_sizes[3];
-  uint32_t tiles[3][2];
-  float build_ms[3];   /* voxelize + mips of the batch, per bin size */
-  float trace_ms[3];   /* trace of the batch, per tile */
-  float frame_ms[3][3];/* [bin][tile] = build_ms[bin] + trace_ms[tile] */
+  uint32_t tiles[4][3];  /* tile_w, tile_h, pixel_order of each schedule */
+  float build_ms[3];     /* voxelize + mips of the batch, per bin size */
+  float trace_ms[4];     /* trace of the batch, per schedule */
+  float frame_ms[3][4];  /* [bin][schedule] = build_ms[bin] + trace_ms[schedule] */
 } vgi_autotune_report;
 
 int vgi_autotune_batch(vgi_context* ctx, const vgi_voxelize_item* vox, const vgi_mips_item* mips,

@@ -131,12 +131,14 @@ def test_trace_room_vs_fp64(ctx):
     f.release()
 
 
-@pytest.mark.parametrize("tile", [(8, 8), (16, 16), (32, 8)])
-def test_tile_shapes_identical(ctx, soup, tile):
+@pytest.mark.parametrize("tile,order", [((8, 8), 0), ((16, 16), 0), ((32, 8), 0), ((16, 16), 1)])
+def test_tile_shapes_identical(ctx, soup, tile, order):
+    """Screen tiles of every shape and Morton pixel order give the same bits per pixel."""
     w = soup["w"]
+    print(f"case schedule tile={tile} order={order}", flush=True)
     f0 = Frame(ctx, w, "fp32")
     try:
-        ctx.set_tuning(tile_w=tile[0], tile_h=tile[1])
+        ctx.set_tuning(tile_w=tile[0], tile_h=tile[1], pixel_order=order)
         f1 = Frame(ctx, w, "fp32")
     finally:
         ctx.set_tuning()
