@@ -76,6 +76,15 @@ def run_precision(ctx, vgi, w, mesh, gb, prec, steps, warmup):
     return out
 
 
+def _traffic_bytes(traffic, prec):
+    if not traffic or prec not in traffic:
+        return None
+    t = traffic[prec]
+    if "FETCH_SIZE" not in t or "WRITE_SIZE" not in t:
+        return None
+    return t["FETCH_SIZE"] + t["WRITE_SIZE"]
+
+
 def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
     """Oracle FP32 timed on this host: full voxelize + mips, trace on a pixel sample
     scaled to the frame. Also the FP64 error of the GPU outputs on a 4096-pixel subset."""
@@ -111,6 +120,99 @@ def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
                         f"to the frame ({t_tr:.2f}s)")), errs
 
 
+WORKLOAD_ARGS = ("tris", "n", "levels", "width", "height", "cones", "bin", "tile", "order")
+CALIB_MIB = 1024
+
+
+def traffic_probe(args) -> int:
+    """Run under rocprofv3 --pmc by measure_traffic: one 1 GiB k_copy (calibration), then
+    the headline frame's trace kernel three times with FP16 mips and three with FP32."""
+    from final184_amd import scenegen as sg
+    from final184_amd import vgi
+
+    w = sg.room(ntri=args.tris, n=args.n, levels=args.levels, width=args.width, height=args.height,
+                num_diffuse=args.cones)
+    ctx = vgi.Context(0)
+    tw, th = (int(v) for v in args.tile.split("x"))
+    ctx.set_tuning(bin_size=args.bin, chunk_tris=64, tile_w=tw, tile_h=th,
+                   pixel_order=vgi.ORDER_MORTON if args.order == "morton" else vgi.ORDER_TILES)
+    src = vgi.DeviceArray(ctx, CALIB_MIB << 20)
+    dst = vgi.DeviceArray(ctx, CALIB_MIB << 20)
+    src.zero()
+    ctx.copy(dst, src)
+    ctx.sync()
+    src.free()
+    dst.free()
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    scene = ctx.scene()
+    scene.voxelize([mesh], vgi.make_grid(**w.grid))
+    cones = vgi.make_cones(**w.cones)
+    for prec in (vgi.PREC_FP16, vgi.PREC_FP32):
+        scene.build_mips(vgi.make_lights(w.lights), prec)
+        for _ in range(3):
+            scene.trace(gb, cones, od, osp)
+        ctx.sync()
+    scene.release()
+    ctx.close()
+    return 0
+
+
+def _counter_rows(outdir):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(outdir, "**", "*counter_collection*.csv"), recursive=True):
+        with open(f) as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def measure_traffic(args):
+    """HBM bytes per trace launch from two rocprofv3 --pmc runs of the probe (FETCH_SIZE,
+    then WRITE_SIZE), each scaled by what the counter reads for the 1 GiB copy. Runs in
+    child processes before this process touches the GPU."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    wl = []
+    for k in WORKLOAD_ARGS:
+        wl += [f"--{k}", str(getattr(args, k))]
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="vgi_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--kernel-include-regex",
+               "k_trace|k_copy", "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-probe", *wl]
+        log(f"traffic: {' '.join(cmd)}")
+        r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, capture_output=True,
+                           text=True, timeout=200)
+        rows = [x for x in _counter_rows(d) if x.get("Counter_Name") == counter]
+        shutil.rmtree(d, ignore_errors=True)
+        if not rows:
+            return None, f"{counter}: no counter rows (rocprofv3 exit {r.returncode}: {r.stderr[-300:]})"
+        per[counter] = rows
+    out = {}
+    for counter, rows in per.items():
+        copy = [float(x["Counter_Value"]) for x in rows if "k_copy" in x["Kernel_Name"]]
+        if not copy or copy[0] <= 0:
+            return None, f"{counter}: no calibration dispatch"
+        factor = (CALIB_MIB << 20) / (copy[0] * 1024.0)  # counter is in KiB
+        for prec, tag in (("fp16", "1"), ("fp32", "0")):
+            vals = [float(x["Counter_Value"]) for x in rows
+                    if re.search(r"k_trace<" + tag + r"\b", x["Kernel_Name"])]
+            if vals:
+                out.setdefault(prec, {})[counter] = float(np.mean(vals)) * 1024.0 * factor
+        out.setdefault("calibration", {})[counter] = factor
+    return out, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,7 +228,14 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp32,fp16")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched run")
-    ap.add_argument("--no-autotune", action="store_true", help="default tuning for the headline run")
+    ap.add_argument("--bin", type=int, default=8, help="voxelize bin edge (8, 16, 32)")
+    ap.add_argument("--tile", default="32x8", help="cone-march screen tile (8x8, 16x16, 32x8)")
+    ap.add_argument("--order", default="tiles", choices=["tiles", "morton"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner on the headline workload instead of --bin/--tile/--order")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs that measure the trace kernel's HBM bytes")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
@@ -138,6 +247,15 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the path shards by scene; see the design notes)", file=sys.stderr)
         return 2
+    if args.traffic_probe:
+        return traffic_probe(args)
+    traffic, traffic_err = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        try:
+            traffic, traffic_err = measure_traffic(args)
+        except Exception as e:  # the headline line is still printed
+            traffic, traffic_err = None, repr(e)
+        log(f"traffic: {traffic if traffic else traffic_err}")
 
     from final184_amd import scenegen as sg
     from final184_amd import vgi
@@ -161,8 +279,11 @@ def main() -> int:
         dst.free()
     mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
     gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    tw, th = (int(v) for v in args.tile.split("x"))
+    ctx.set_tuning(bin_size=args.bin, chunk_tris=64, tile_w=tw, tile_h=th,
+                   pixel_order=vgi.ORDER_MORTON if args.order == "morton" else vgi.ORDER_TILES)
     tuned = None
-    if not args.no_autotune:  # bin size and pixel schedule for this workload (FP16 mips)
+    if args.autotune:  # search bin size and pixel schedule for this workload (FP16 mips)
         sc = ctx.scene()
         grid, lights, cones = vgi.make_grid(**w.grid), vgi.make_lights(w.lights), vgi.make_cones(**w.cones)
         od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
@@ -217,11 +338,17 @@ def main() -> int:
         "fp32_frame_mpixels_s": px / (base["frame_ms"] * 1e-3) / 1e6,
         "fp16_vs_fp32_frame_speedup": base["frame_ms"] / head["frame_ms"],
         "trilinear_per_pixel": head["trilinear"] / px,
-        "tuning": tuned.as_dict() if tuned is not None else "default",
+        "tuning": tuned.as_dict() if tuned is not None else
+                  {"bin_size": args.bin, "schedule": args.order if args.order == "morton" else args.tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": _traffic_bytes(traffic, head["prec"]),
                      "kernel": f"k_trace<{head['prec']}>",
-                     "fp32_achieved": fp32_achieved, "fp32_frac": fp32_achieved / HBM_PEAK_GBS},
+                     "algorithmic_bytes": alg_bytes,
+                     "fp32_achieved": fp32_achieved, "fp32_frac": fp32_achieved / HBM_PEAK_GBS,
+                     "fp32_algorithmic_bytes": fp32_alg,
+                     "fp32_traffic": _traffic_bytes(traffic, base["prec"]),
+                     "traffic_counters": traffic if traffic else traffic_err},
     }
     if not args.no_batched:
         from final184_amd import loadgen
