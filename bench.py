@@ -76,6 +76,50 @@ def run_precision(ctx, vgi, w, mesh, gb, prec, steps, warmup):
     return out
 
 
+def run_config(ctx, vgi, w, precs, steps, keep=None):
+    """Frames of one workload per mip precision: frame time (voxelize + mips + trace, HIP
+    events), the phase split of one profiled frame, and the traced outputs (all pixels, or
+    the pixel subset `keep`) for the FP64 check in the CPU leg."""
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    grid, lights, cones = vgi.make_grid(**w.grid), vgi.make_lights(w.lights), vgi.make_cones(**w.cones)
+    od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    scene = ctx.scene()
+    out = {}
+    for prec in precs:
+        p = vgi.PREC_NAMES[prec]
+
+        def frame():
+            scene.voxelize([mesh], grid)
+            scene.build_mips(lights, p)
+            scene.trace(gb, cones, od, osp)
+
+        frame()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(steps):
+            frame()
+        ms = ctx.timer_stop() / steps
+        ctx.set_profiling(True)
+        frame()
+        t = ctx.timings()
+        ctx.set_profiling(False)
+        d, sp = od.numpy(), osp.numpy()
+        if keep is not None:
+            d, sp = d[keep], sp[keep]
+        out[prec] = dict(frame_ms=ms, voxelize_ms=t.voxelize_total_ms, mips_ms=t.mips_ms, trace_ms=t.trace_ms,
+                         trilinear_per_pixel=t.trace_trilinear_samples / w.num_pixels, diffuse=d, specular=sp)
+    scene.release()
+    for a in (od, osp, mesh.pos, gb.pos, gb.nrm):
+        a.free()
+    return out
+
+
+def _public(res):
+    return {p: {k: v for k, v in r.items() if k not in ("diffuse", "specular")} for p, r in res.items()}
+
+
 def _traffic_bytes(traffic, prec):
     if not traffic or prec not in traffic:
         return None
@@ -83,6 +127,25 @@ def _traffic_bytes(traffic, prec):
     if "FETCH_SIZE" not in t or "WRITE_SIZE" not in t:
         return None
     return t["FETCH_SIZE"] + t["WRITE_SIZE"]
+
+
+def fp64_errors(w, gpu, pixels):
+    """Relative L2 and max-abs of each GPU form's diffuse + specular RGB against the
+    oracle's FP64 run of the same workload at `pixels` (the CPU leg's checker)."""
+    from oracle import oracle as orc
+
+    acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+    lv64 = orc.inject_mips(acc, w.grid, w.lights, f64=True)
+    del acc
+    d64, s64, _ = orc.trace(lv64, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough,
+                            pixels=pixels, f64=True)
+    del lv64
+    ref = np.concatenate([d64[:, :3], s64[:, :3]], 1)
+    errs = {}
+    for prec, r in gpu.items():
+        got = np.concatenate([r["diffuse"][:, :3], r["specular"][:, :3]], 1)
+        errs[prec] = dict(rel_l2=rel_l2(got, ref), max_abs=float(np.abs(got - ref).max()))
+    return errs
 
 
 def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
@@ -228,6 +291,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp32,fp16")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 10k-triangle soup and the 1M-triangle 512^3 / 4K precision study")
     ap.add_argument("--bin", type=int, default=8, help="voxelize bin edge (8, 16, 32)")
     ap.add_argument("--tile", default="32x8", help="cone-march screen tile (8x8, 16x16, 32x8)")
     ap.add_argument("--order", default="tiles", choices=["tiles", "morton"])
@@ -330,6 +395,10 @@ def main() -> int:
                    "cones": f"{args.cones} diffuse 60deg + 1 specular (roughness)",
                    "mip_storage": head["prec"], "accumulate": "f32", "parallelism": "single GPU"},
         "voxelize_mtris_s": w.num_triangles / (head["vox_ms"] * 1e-3) / 1e6,
+        "voxelize_work": {"bin_pairs": int(head["stats"].bin_pairs), "work_items": int(head["stats"].work_items),
+                          "bin_size": int(head["stats"].bin_size),
+                          "rejected": int(head["stats"].triangles_rejected),
+                          "degenerate": int(head["stats"].triangles_degenerate)},
         "trace_mpixels_s": px / (head["trace_ms"] * 1e-3) / 1e6,
         "mips_gbs": (args.n ** 3 * 32 + mip_levels_bytes * texel_b[head["prec"]]) / (head["mips_ms"] * 1e-3) / 1e9,
         "phases_ms": {r["prec"]: {"voxelize": r["vox_ms"], "voxelize_raster": r["vox_raster_ms"],
@@ -350,6 +419,23 @@ def main() -> int:
                      "fp32_traffic": _traffic_bytes(traffic, base["prec"]),
                      "traffic_counters": traffic if traffic else traffic_err},
     }
+    soup = study = None
+    if not args.no_configs:
+        # 10,000-triangle soup (64^3 / 6 levels, 256x256 points on triangles, 6+1 cones):
+        # every pixel is checked against FP64
+        ws = sg.random_soup()
+        log(f"config {ws.name}")
+        soup = (ws, run_config(ctx, vgi, ws, ("fp32", "fp16", "bf16"), args.steps))
+        # precision study: room at 1,048,576 triangles, 512^3 / 9 levels, 3840x2160, 16+1 cones;
+        # a seeded 4096-pixel subset is checked against FP64
+        t0 = time.perf_counter()
+        wp = sg.room(ntri=1_048_576, n=512, levels=9, width=3840, height=2160, num_diffuse=16)
+        log(f"config {wp.name} generated in {time.perf_counter() - t0:.1f}s")
+        sub = np.random.default_rng(4096).choice(wp.num_pixels, 4096, replace=False)
+        study = (wp, sub, run_config(ctx, vgi, wp, ("fp32", "fp16", "bf16"), max(2, args.steps // 4), keep=sub))
+        for name, res in (("soup", soup[1]), ("precision study", study[2])):
+            log(f"{name}: " + ", ".join(f"{p} frame {r['frame_ms']:.3f} ms (trace {r['trace_ms']:.3f})"
+                                        for p, r in res.items()))
     if not args.no_batched:
         from final184_amd import loadgen
         try:
@@ -378,8 +464,21 @@ def main() -> int:
         cb, errs = cpu_baseline(w, results, args.cpu_sample)
         line["cpu_baseline"] = cb
         line["error_vs_fp64"] = errs
+        if soup is not None:
+            ws, res = soup
+            line["soup_10k"] = dict(workload=ws.name, **_public(res),
+                                    error_vs_fp64=fp64_errors(ws, res, np.arange(ws.num_pixels)))
+            wp, sub, res = study
+            line["precision_study"] = dict(workload=wp.name, **_public(res),
+                                           error_vs_fp64=fp64_errors(wp, res, sub),
+                                           error_pixels="seeded 4096-pixel subset")
+            log(f"soup errors {line['soup_10k']['error_vs_fp64']}")
+            log(f"precision study errors {line['precision_study']['error_vs_fp64']}")
     else:
         line["cpu_baseline"] = None
+        if soup is not None:
+            line["soup_10k"] = dict(workload=soup[0].name, **_public(soup[1]))
+            line["precision_study"] = dict(workload=study[0].name, **_public(study[2]))
     ctx.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -594,7 +594,9 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     HIP_TRY(launch_zero(B, persistent, st));
   }
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
-  const uint32_t grid_blocks = std::min<uint32_t>(total_work, (uint32_t)ctx->num_cus * 8);
+  // one workgroup per work item: item costs vary by orders of magnitude, and the
+  // dispatcher balances them better than a static round-robin over persistent groups
+  const uint32_t grid_blocks = std::min<uint32_t>(total_work, 0x7FFFFFFFu);
   HIP_TRY(launch_voxelize(B, ts, grid_blocks, total_work, st));
   if (prof) {
     HIP_TRY(hipEventRecord(ctx->ev[4], st));

@@ -23,6 +23,10 @@
 namespace vgi {
 
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
+  // Round to FP32 first, then to FP16, as the definition says: without this barrier the
+  // backend may fuse a preceding multiply into the conversion (v_fma_mixlo_f16), which
+  // rounds once and differs from the two roundings by an ulp in rare near-ties.
+  __asm__ volatile("" : "+v"(f));
   _Float16 h = (_Float16)f;
   return __builtin_bit_cast(uint16_t, h);
 }
@@ -100,16 +104,38 @@ __device__ __forceinline__ float4 scale4(float4 a, float s) {
   return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
 }
 
-// Radiance of level-0 voxel (x, y, z) from the accumulator.
-__device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
+// Occupancy of level-0 voxel (x, y, z): its bin holds triangles and its mask bit is set
+// (records of unoccupied voxels are not written; masks of empty bins are not either).
+__device__ __forceinline__ bool occupied(const MipScene& P, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-  if ((uint32_t)gld(P.bin_packed + bin) == 0) return make_float4(0.f, 0.f, 0.f, 0.f);  // empty bin
+  if ((uint32_t)gld(P.bin_packed + bin) == 0) return false;
   const uint32_t nbr = P.n >> 3;
   const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
   const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
-  if (!((gld(P.mask + (size_t)brick * 16u + (local >> 5)) >> (local & 31u)) & 1u))
-    return make_float4(0.f, 0.f, 0.f, 0.f);  // unoccupied voxel: its record is not written
+  return (gld(P.mask + (size_t)brick * 16u + (local >> 5)) >> (local & 31u)) & 1u;
+}
+
+// Occupancy bits of the 2x2x2 group at even (x, y, z): bit dx + 2 dy + 4 dz. The group lies
+// in one bin and one 8^3 brick, and each of its two z-slices in one mask word.
+__device__ __forceinline__ uint32_t group_occupancy(const MipScene& P, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
+  const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
+  if ((uint32_t)gld(P.bin_packed + bin) == 0) return 0u;
+  const uint32_t nbr = P.n >> 3;
+  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
+  const uint32_t l0 = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));  // z-slice 0 of the group
+  const uint32_t* m = P.mask + (size_t)brick * 16u;
+  const uint32_t sh = l0 & 31u;  // bits sh, sh+1, sh+8, sh+9 of each slice's word
+  const uint32_t w0 = gld(m + (l0 >> 5)) >> sh, w1 = gld(m + ((l0 + 64u) >> 5)) >> sh;
+  return (w0 & 3u) | ((w0 >> 6) & 12u) | ((w1 & 3u) << 4) | ((w1 >> 2) & 192u);
+}
+
+// Radiance of an occupied level-0 voxel (x, y, z) from its accumulator record.
+__device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t nbr = P.n >> 3;
+  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
+  const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
   const uint32_t* a = P.acc + ((size_t)brick * 512u + local) * kAccWords;
   const uint4 s0 = gld4(a), s1 = gld4(a + 4);
   const uint32_t c = s0.w;
@@ -180,7 +206,8 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     // single-level chain: resolve level 0 only, one voxel per thread
     for (uint32_t v = g; v < S * S * S; v += 512) {
       const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
-      TX::store(P.level[0], padded_index(x, y, z, ns), TX::round(inject(P, s_lut, x, y, z)));
+      const float4 c = occupied(P, x, y, z) ? TX::round(inject(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
+      TX::store(P.level[0], padded_index(x, y, z, ns), c);
     }
     return;
   }
@@ -188,16 +215,19 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
   if (g < G * G * G) {
     const uint32_t gx = g % G, gy = (g / G) % G, gz = g / (G * G);
     float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t x0 = tx * S + 2 * gx, y0 = ty * S + 2 * gy, z0 = tz * S + 2 * gz;
+    const uint32_t occ = SRC_ACC ? group_occupancy(P, x0, y0, z0) : 0u;
 #pragma unroll
     for (uint32_t dz = 0; dz < 2; dz++)
 #pragma unroll
       for (uint32_t dy = 0; dy < 2; dy++)
 #pragma unroll
         for (uint32_t dx = 0; dx < 2; dx++) {
-          const uint32_t x = tx * S + 2 * gx + dx, y = ty * S + 2 * gy + dy, z = tz * S + 2 * gz + dz;
+          const uint32_t x = x0 + dx, y = y0 + dy, z = z0 + dz;
           float4 c;
           if (SRC_ACC) {
-            c = TX::round(inject(P, s_lut, x, y, z));
+            c = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((occ >> (dx + 2 * dy + 4 * dz)) & 1u) c = TX::round(inject(P, s_lut, x, y, z));
             TX::store(P.level[0], padded_index(x, y, z, ns), c);
           } else {
             c = TX::load(P.level[src], padded_index(x, y, z, ns));

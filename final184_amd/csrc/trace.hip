@@ -22,7 +22,9 @@
 // the level's buffer descriptor and its row/slice pitches live in SGPRs and the eight
 // texels of a level come from one VGPR byte offset (plus SGPR pitch and immediate +1
 // texel), four 16-byte loads per level for FP16 (texel x and x+1 share a load), eight for
-// FP32. The per-pixel-roughness specular cone uses 64-bit global loads.
+// FP32. In the FP32 form a gather whose 2x2x2 block is the same for every active lane
+// is read with scalar loads instead. The per-pixel-roughness specular cone uses per-lane
+// global loads.
 //
 // Morton order (for G-buffers that are not spatially coherent, e.g. points sampled on
 // triangles): k_morton_keys keys every pixel by (scene << 32 | 30-bit Morton code of its
@@ -101,18 +103,44 @@ __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, flo
 }
 
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
+// FP32: when every active lane's footprint is the same 2x2x2 texel block (usual at coarse
+// levels, where a texel spans many pixels' cone positions), the block is read once with
+// scalar loads and broadcast, so the texture-address path does no work for it. That path
+// bounds the FP32 form (eight 16-byte loads per gather); the FP16 form, with half the
+// bytes, is bound by VALU instead, and the vote would only add instructions there. The
+// weighted sum is the same either way.
+typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
+#define VGI_CONSTANT __attribute__((address_space(4)))
+
 template <int PREC>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
                                                float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
-  const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)P.level_bytes[k], 0x00020000);
   const Foot f = footprint(k, p1, ux, uy, uz, wl);
   const uint32_t off = f.idx * TB;  // levels on this path are < 2^31 bytes
   const int soff[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+  if constexpr (PREC == VGI_PREC_FP32) {
+    const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
+    if (__all(off == off0)) {
+      const char* base = (const char*)P.level[k] + off0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float wz = q < 2 ? f.wz0 : f.wz1;
+        const u32x8_t v = *(const VGI_CONSTANT u32x8_t*)(base + soff[q]);
+        const uint32_t a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        const uint32_t b0 = v[4], b1 = v[5], b2 = v[6], b3 = v[7];
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
+        abits |= a3 | b3;
+      }
+      return;
+    }
+  }
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)P.level_bytes[k], 0x00020000);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const float wz = q < 2 ? f.wz0 : f.wz1;
@@ -155,11 +183,10 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
       Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
       abits |= a.w | b.w;
     } else {
-      const uint2 a = gld2(base + offs[q]);
-      const uint2 b = gld2(base + offs[q] + 8);
+      const uint4 a = gld4(base + offs[q]);  // texels x and x+1 (8-byte aligned)
       Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
-      Acc<PREC>::add(acc, wxy[q][1] * wz, b.x, b.y);
-      abits |= (a.y | b.y) & 0xFFFF0000u;
+      Acc<PREC>::add(acc, wxy[q][1] * wz, a.z, a.w);
+      abits |= (a.y | a.w) & 0xFFFF0000u;
     }
   }
 }

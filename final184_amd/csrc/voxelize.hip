@@ -14,8 +14,8 @@
 //                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
 //                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
 //                   (bin = B^3 voxels) tests each bin exactly and takes a slot in every
-//                   overlapped bin with a wave-aggregated atomic (the slot is kept);
-//                   larger triangles go to a list.
+//                   overlapped bin (ranked per workgroup in LDS, one global atomic per
+//                   distinct bin; the slot is kept); larger triangles go to a list.
 //   2 k_big<false>  one workgroup per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
 //                   packed as (items << 32 | pairs); bins with > 1 item are listed.
@@ -25,7 +25,7 @@
 //     k_emit_small  scatter small triangles to their kept slots (no atomics);
 //     k_big<true>   large triangles append after the small ones of each bin.
 //   6 k_zero        clear the accumulator and occupancy mask of multi-item bins.
-//   7 k_voxelize    persistent loop over the batch's work items; per item: chunk
+//   7 k_voxelize    one workgroup per work item of the batch; per item: chunk
 //                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
 //                   projection spread over the workgroup, each column tests its 1-4
 //                   candidate voxels with the exact separating-axis test and adds packed
@@ -188,29 +188,6 @@ __device__ __forceinline__ uint32_t first_item(const VoxBatch& B, const VoxScene
   return (uint32_t)(gld(B.scan_all + S.bin_base + b) >> 32);
 }
 
-// Wave-aggregated atomic increment with per-lane keys: every active lane gets a distinct
-// slot in counter[key] (old value + its rank among the lanes sharing the key). Up to two
-// shared keys are served with one atomic each (neighbouring triangles of a mesh usually
-// share bins); lanes left over take one atomic each, so the number of dependent atomic
-// round trips per call stays at three when keys are all distinct.
-__device__ __forceinline__ uint32_t agg_slot(uint32_t* counter, uint32_t key, bool active) {
-  const uint32_t lane = __lane_id();
-  uint64_t pending = __ballot(active);
-  uint32_t slot = 0;
-  for (int round = 0; round < 2 && pending; round++) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t lkey = __shfl(key, leader);
-    const uint64_t same = __ballot(active && key == lkey) & pending;
-    uint32_t base = 0;
-    if (lane == (uint32_t)leader) base = gatomic_add(&counter[lkey], (uint32_t)__popcll(same));
-    base = __shfl(base, leader);
-    if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-    pending &= ~same;
-  }
-  if ((pending >> lane) & 1ull) slot = gatomic_add(&counter[key], 1u);
-  return slot;
-}
-
 // ---------------------------------------------------------------------------
 // 1 setup
 // ---------------------------------------------------------------------------
@@ -298,13 +275,28 @@ __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
   R.rgb = rgb | (status << 24);
 }
 
+// Slots of small triangles in their bins, aggregated per workgroup: each (triangle, bin)
+// pair takes a rank in an LDS hash table keyed by bin, then one returning global atomic
+// per distinct bin reserves that bin's run of slots, so a workgroup waits for one atomic
+// round trip however many bins its triangles touch (neighbouring triangles of a mesh
+// share most of them). Pairs that find no table entry within kProbe probes take a slot
+// with their own global atomic.
+constexpr uint32_t kSlotTable = 1024;  // distinct bins per workgroup before overflow
+constexpr uint32_t kProbe = 32;
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
 __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  __shared__ uint32_t s_key[kSlotTable], s_cnt[kSlotTable], s_base[kSlotTable];
+  for (uint32_t e = threadIdx.x; e < kSlotTable; e += blockDim.x) {
+    s_key[e] = kNoKey;
+    s_cnt[e] = 0u;
+  }
   const uint2 blk = tri_block(B);  // a workgroup never spans two scenes
   const VoxScene& P = scenes[blk.x];
   const uint32_t t = blk.y + threadIdx.x;
   const bool in = t < P.ntri;
   TriRec R;
-  bool small = false;
+  bool small = false, big = false;
   BinBox bb{0, 0, 0, 0, 0, 0};
   int64_t q[3][3], N[3];
   if (in) {
@@ -316,18 +308,57 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
     if (tri_ok(R)) {
       bb = bin_box(R, P.bin_shift);
       small = bb.count() <= kSmallBins;
-      if (!small) gst(P.big_list + gatomic_add(&P.stats->big_count, 1u), t);
-      else load_q(R, q, N);
+      big = !small;
+      if (small) load_q(R, q, N);
     }
   }
-  // every lane runs the same slot rounds so the aggregated atomics stay convergent
-  uint32_t nslot = 0;
+  {  // large triangles: one list append per wavefront (they are often neighbours in a mesh)
+    const uint64_t bm = __ballot(big);
+    if (bm) {
+      const uint32_t lane = __lane_id();
+      const int leader = __ffsll((unsigned long long)bm) - 1;
+      uint32_t base = 0;
+      if (lane == (uint32_t)leader) base = gatomic_add(&P.stats->big_count, (uint32_t)__popcll(bm));
+      base = __shfl(base, leader);
+      if (big) gst(P.big_list + base + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull)), t);
+    }
+  }
+  __syncthreads();  // table cleared
+  // rank each (triangle, bin) pair within the workgroup
+  uint32_t ent[kSmallBins], rank[kSmallBins];  // table entry (or kNoKey: own atomic) and rank
+  uint32_t hits = 0;                           // bit j: candidate j overlaps its bin
+#pragma unroll
   for (int j = 0; j < kSmallBins; j++) {
-    if (!__ballot(small && j < bb.count())) break;  // wave-uniform exit
+    ent[j] = kNoKey;
+    rank[j] = 0;
     uint32_t bin = 0;
-    const bool hit = small && j < bb.count() && bin_candidate(P, q, N, bb, j, bin);
-    const uint32_t slot = agg_slot(P.bin_small, bin, hit);
-    if (hit) gst(P.tri_slot + (size_t)t * kSmallBins + nslot++, slot);
+    if (small && j < bb.count() && bin_candidate(P, q, N, bb, j, bin)) {
+      hits |= 1u << j;
+      uint32_t h = (bin * 0x9E3779B1u) >> 22;  // 10-bit hash
+      for (uint32_t probe = 0; probe < kProbe; probe++) {
+        const uint32_t k = atomicCAS(&s_key[h], kNoKey, bin);
+        if (k == kNoKey || k == bin) {
+          ent[j] = h;
+          rank[j] = atomicAdd(&s_cnt[h], 1u);
+          break;
+        }
+        h = (h + 1u) & (kSlotTable - 1u);
+      }
+      if (ent[j] == kNoKey) rank[j] = gatomic_add(P.bin_small + bin, 1u);  // table overflow
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < kSlotTable; e += blockDim.x) {
+    const uint32_t key = s_key[e];
+    if (key != kNoKey) s_base[e] = gatomic_add(P.bin_small + key, s_cnt[e]);
+  }
+  __syncthreads();
+  uint32_t nslot = 0;
+#pragma unroll
+  for (int j = 0; j < kSmallBins; j++) {
+    if (!((hits >> j) & 1u)) continue;
+    const uint32_t slot = ent[j] == kNoKey ? rank[j] : s_base[ent[j]] + rank[j];
+    gst(P.tri_slot + (size_t)t * kSmallBins + nslot++, slot);
   }
 }
 
@@ -478,12 +509,6 @@ hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s) 
 // 6 clear the bins that several work items add into
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ size_t acc_index(uint32_t x, uint32_t y, uint32_t z, uint32_t nbr) {
-  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
-  const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
-  return ((size_t)brick * 512u + local) * kAccWords;
-}
-
 __global__ __launch_bounds__(256) void k_zero(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   for (uint32_t sc = 0; sc < Bt.nscenes; sc++) {
     const VoxScene& P = scenes[sc];
@@ -593,19 +618,33 @@ __device__ __forceinline__ bool edge_ok(const TriCoef& C, int u, int va, int vb)
   return ok;
 }
 
+// LDS tile accumulator: two arrays of packed 64-bit increments (RGB+count, normals) with
+// padded pitches, entry x + PY*y + PZ*z. Unpadded power-of-two pitches put lanes that
+// step along y or z (the column direction of X- and Y-dominant triangles) into one LDS
+// bank; with PY = TS+1 and PZ = (TS+1)*TS+1 (odd multiples of 8 bytes, 18 banks apart)
+// sixteen lanes of an 8-byte access hit sixteen distinct bank pairs along any axis.
+template <int TS>
+struct TileLayout {
+  static constexpr int PY = TS + 1;
+  static constexpr int PZ = (TS + 1) * TS + 1;
+  static constexpr int ENTRIES = (TS - 1) * (1 + PY + PZ) + 1;
+};
+
 template <int TS>
 __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes,
                                                   uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
-  constexpr int BT = (TS / 8) * (TS / 8) * (TS / 8);  // 8^3 accumulator bricks per tile
-  __shared__ unsigned long long s_acc[2 * TV];
+  constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
+  using L = TileLayout<TS>;
+  __shared__ unsigned long long s_acc0[L::ENTRIES];
+  __shared__ unsigned long long s_acc1[L::ENTRIES];
   __shared__ TriCoef s_tri[kMaxChunk];
   __shared__ uint32_t s_pref[kMaxChunk + 1];
-  __shared__ uint32_t s_mask[BT * 16];
 
   const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
 
-  // persistent loop over the work items of every scene of the batch
+  // work items of every scene of the batch (normally one per workgroup)
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const uint2 item = gld2(Bt.work_map + w);
     const VoxScene& P = scenes[item.x];
@@ -629,8 +668,10 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* _
       const int tx0 = (int)(bx * B + TS * (st % sub));
       const int ty0 = (int)(by * B + TS * ((st / sub) % sub));
       const int tz0 = (int)(bz * B + TS * (st / (sub * sub)));
-      for (int i = tid; i < 2 * TV; i += 256) s_acc[i] = 0ull;
-      for (int i = tid; i < BT * 16; i += 256) s_mask[i] = 0u;
+      for (int i = tid; i < L::ENTRIES; i += 256) {
+        s_acc0[i] = 0ull;
+        s_acc1[i] = 0ull;
+      }
       // clip to the tile, count columns of the dominant projection, scan (wave 0)
       if (tid < 64) {
         uint32_t cols = 0;
@@ -691,49 +732,50 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* _
           if (s + C.pmin > 0 || s + C.pmax < 0) continue;
           if (!edge_ok(C, u1, sel3(vx, vy, vz, (u1 + 1) % 3), sel3(vx, vy, vz, (u1 + 2) % 3))) continue;
           if (!edge_ok(C, u2, sel3(vx, vy, vz, (u2 + 1) % 3), sel3(vx, vy, vz, (u2 + 2) % 3))) continue;
-          const int li = (vx - tx0) + TS * ((vy - ty0) + TS * (vz - tz0));
-          atomicAdd(&s_acc[2 * li], C.inc0);
-          atomicAdd(&s_acc[2 * li + 1], C.inc1);
+          const int pi = (vx - tx0) + L::PY * (vy - ty0) + L::PZ * (vz - tz0);
+          atomicAdd(&s_acc0[pi], C.inc0);
+          atomicAdd(&s_acc1[pi], C.inc1);
         }
       }
       __syncthreads();
-      // flush the tile: occupancy mask (per 8^3 brick) + records of occupied voxels
-      for (int li = tid; li < TV; li += 256) {
-        const unsigned long long w0 = s_acc[2 * li];
+      // flush the tile in brick-major order (64 consecutive entries of a wavefront are one
+      // 64-voxel slab of one 8^3 brick): records of occupied voxels, and the brick's
+      // occupancy mask, two words per wavefront from a ballot
+      for (int i = tid; i < TV; i += 256) {  // trip count is the same for every lane
+        const uint32_t bk = (uint32_t)i >> 9, l = (uint32_t)i & 511u;
+        const uint32_t bkx = bk % NBK, bky = (bk / NBK) % NBK, bkz = bk / (NBK * NBK);
+        const uint32_t lx = bkx * 8 + (l & 7u), ly = bky * 8 + ((l >> 3) & 7u), lz = bkz * 8 + (l >> 6);
+        const int pi = (int)lx + L::PY * (int)ly + L::PZ * (int)lz;
+        const unsigned long long w0 = s_acc0[pi];
         const uint32_t c = (uint32_t)(w0 >> 48);
-        if (c == 0) continue;
-        const unsigned long long w1 = s_acc[2 * li + 1];
-        const uint32_t lx = li % TS, ly = (li / TS) % TS, lz = li / (TS * TS);
-        const uint32_t lb = (lx >> 3) + (TS / 8) * ((ly >> 3) + (TS / 8) * (lz >> 3));
-        const uint32_t bit = (lx & 7u) + 8u * ((ly & 7u) + 8u * (lz & 7u));
-        atomicOr(&s_mask[lb * 16 + (bit >> 5)], 1u << (bit & 31u));
-        uint32_t* dst = P.acc + acc_index(tx0 + lx, ty0 + ly, tz0 + lz, nbr);
-        const uint32_t r = (uint32_t)(w0 & 0xFFFFu), g = (uint32_t)((w0 >> 16) & 0xFFFFu),
-                       b = (uint32_t)((w0 >> 32) & 0xFFFFu);
-        const uint32_t nx = (uint32_t)(w1 & 0xFFFFFu), ny = (uint32_t)((w1 >> 20) & 0xFFFFFu),
-                       nz = (uint32_t)((w1 >> 40) & 0xFFFFFu);
-        if (nch == 1) {
-          gst4(dst, make_uint4(r, g, b, c));
-          gst4(dst + 4, make_uint4(nx, ny, nz, 0u));
-        } else {
-          gatomic_add(dst + 0, r);
-          gatomic_add(dst + 1, g);
-          gatomic_add(dst + 2, b);
-          gatomic_add(dst + 3, c);
-          gatomic_add(dst + 4, nx);
-          gatomic_add(dst + 5, ny);
-          gatomic_add(dst + 6, nz);
+        const uint32_t brick = ((uint32_t)tx0 >> 3) + bkx + nbr * (((uint32_t)ty0 >> 3) + bky + nbr * (((uint32_t)tz0 >> 3) + bkz));
+        if (c != 0) {
+          const unsigned long long w1 = s_acc1[pi];
+          uint32_t* dst = P.acc + ((size_t)brick * 512u + l) * kAccWords;
+          const uint32_t r = (uint32_t)(w0 & 0xFFFFu), g = (uint32_t)((w0 >> 16) & 0xFFFFu),
+                         b = (uint32_t)((w0 >> 32) & 0xFFFFu);
+          const uint32_t nx = (uint32_t)(w1 & 0xFFFFFu), ny = (uint32_t)((w1 >> 20) & 0xFFFFFu),
+                         nz = (uint32_t)((w1 >> 40) & 0xFFFFFu);
+          if (nch == 1) {
+            gst4(dst, make_uint4(r, g, b, c));
+            gst4(dst + 4, make_uint4(nx, ny, nz, 0u));
+          } else {
+            gatomic_add(dst + 0, r);
+            gatomic_add(dst + 1, g);
+            gatomic_add(dst + 2, b);
+            gatomic_add(dst + 3, c);
+            gatomic_add(dst + 4, nx);
+            gatomic_add(dst + 5, ny);
+            gatomic_add(dst + 6, nz);
+          }
         }
-      }
-      __syncthreads();
-      for (int i = tid; i < BT * 16; i += 256) {
-        const uint32_t lb = i >> 4;
-        const uint32_t bxx = (tx0 >> 3) + lb % (TS / 8), byy = (ty0 >> 3) + (lb / (TS / 8)) % (TS / 8),
-                       bzz = (tz0 >> 3) + lb / ((TS / 8) * (TS / 8));
-        uint32_t* m = P.mask + (size_t)(bxx + nbr * (byy + nbr * bzz)) * 16u + (i & 15);
-        const uint32_t v = s_mask[i];
-        if (nch == 1) gst(m, v);
-        else if (v) gatomic_or(m, v);
+        const unsigned long long bits = __ballot(c != 0);  // bit j: voxel l - lane + j
+        if (lane < 2) {
+          const uint32_t word = lane == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+          uint32_t* m = P.mask + (size_t)brick * 16u + ((l - lane) >> 5) + lane;
+          if (nch == 1) gst(m, word);  // single-item bin: every word is written, zeros too
+          else if (word) gatomic_or(m, word);
+        }
       }
       __syncthreads();
     }
@@ -743,6 +785,7 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* _
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s) {
   if (grid_blocks == 0 || nwork == 0) return hipSuccess;
+  static_assert(TileLayout<8>::ENTRIES == 582 && TileLayout<16>::ENTRIES == 4366, "tile layout");
   if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
   else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
   return hipGetLastError();

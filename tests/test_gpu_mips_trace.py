@@ -78,9 +78,14 @@ def test_mips_room_levels(ctx):
     print(f"case {w.name} mips", flush=True)
     f = Frame(ctx, w, "fp16")
     acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+    got_acc = f.scene.read_voxels()
+    assert np.array_equal(got_acc, acc), f"voxels: {(got_acc != acc).any(axis=-1).sum()} records differ"
     ref = orc.inject_mips(acc, w.grid, w.lights, "fp16")
     for k in range(w.grid["levels"]):
-        assert np.array_equal(f.scene.read_level(k), ref[k]), f"level {k}"
+        got = f.scene.read_level(k)
+        bad = (got != ref[k]).any(axis=-1)
+        assert not bad.any(), (f"level {k}: {bad.sum()} texels differ, first at {np.argwhere(bad)[:3].tolist()}, "
+                               f"got {got[bad][:3].tolist()} want {ref[k][bad][:3].tolist()}")
     # mass conservation per level (exact in FP64; FP16 rounding bounded)
     l0 = ref[0].astype(np.float64).sum(axis=(0, 1, 2))
     for k in range(1, w.grid["levels"]):
