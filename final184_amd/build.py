@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "abi.hip", "autotune.hip", "serve.hip"]
+HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "screen.hip", "abi.hip", "autotune.hip", "serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1186,6 +1186,85 @@ int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t coun
   return trace_batch(ctx, items, count, stream);
 }
 
+// ---- screen-space passes ----------------------------------------------------
+
+namespace {
+
+constexpr uint32_t kMaxScreenEdge = 16384;
+
+bool image_format_ok(uint32_t f) { return f == VGI_IMAGE_RGBA32F || f == VGI_IMAGE_RGBA16F; }
+size_t image_texel_bytes(uint32_t f) { return f == VGI_IMAGE_RGBA16F ? 8 : 16; }
+bool aligned(const void* p, size_t a) { return ((uintptr_t)p % a) == 0; }
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
+
+int check_screen(uint32_t w, uint32_t h) {
+  if (w > kMaxScreenEdge || h > kMaxScreenEdge)
+    return fail(VGI_E_RANGE, "image %ux%u exceeds %u per edge", w, h, kMaxScreenEdge);
+  return VGI_OK;
+}
+
+}  // namespace
+
+int vgi_gbuffer_decode(vgi_context* ctx, const vgi_gbuffer_ref* in, float* out_position, float* out_normal,
+                       float* out_view_depth, void* stream) {
+  if (!ctx || !in || !out_position || !out_normal) return fail(VGI_E_INVALID, "NULL argument");
+  TRY(check_screen(in->width, in->height));
+  if (!in->width || !in->height) return VGI_OK;
+  if (!in->depth || !in->normals) return fail(VGI_E_INVALID, "NULL depth or normal target");
+  if (!aligned(in->depth, 4) || !aligned(in->normals, 8) || !aligned(out_position, 4) ||
+      !aligned(out_normal, 4) || !aligned(out_view_depth, 4))
+    return fail(VGI_E_INVALID, "misaligned buffer (depth 4 B, normals 8 B, outputs 4 B)");
+  const size_t npx = (size_t)in->width * in->height;
+  if (overlaps(out_position, npx * 12, out_normal, npx * 12) ||
+      (out_view_depth && (overlaps(out_view_depth, npx * 4, out_position, npx * 12) ||
+                          overlaps(out_view_depth, npx * 4, out_normal, npx * 12))))
+    return fail(VGI_E_INVALID, "output buffers overlap");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_gbuffer_decode(*in, out_position, out_normal, out_view_depth, pick(ctx, stream)));
+  return VGI_OK;
+}
+
+int vgi_temporal_resolve(vgi_context* ctx, const vgi_temporal_params* p, void* stream) {
+  if (!ctx || !p) return fail(VGI_E_INVALID, "NULL argument");
+  TRY(check_screen(p->width, p->height));
+  if (!p->width || !p->height) return VGI_OK;
+  if (!p->indirect || !p->position || !p->view_depth || !p->history || !p->out)
+    return fail(VGI_E_INVALID, "NULL buffer");
+  if (!image_format_ok(p->history_format) || !image_format_ok(p->out_format))
+    return fail(VGI_E_INVALID, "unknown image format");
+  if (!aligned(p->indirect, 16) || !aligned(p->position, 4) || !aligned(p->view_depth, 4) ||
+      !aligned(p->history, image_texel_bytes(p->history_format)) || !aligned(p->out, image_texel_bytes(p->out_format)))
+    return fail(VGI_E_INVALID, "misaligned buffer (one texel for images, 4 B otherwise)");
+  const size_t npx = (size_t)p->width * p->height;
+  if (overlaps(p->out, npx * image_texel_bytes(p->out_format), p->history, npx * image_texel_bytes(p->history_format)))
+    return fail(VGI_E_INVALID, "out overlaps history");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_temporal(*p, pick(ctx, stream)));
+  return VGI_OK;
+}
+
+int vgi_bilateral_blur(vgi_context* ctx, const vgi_blur_params* p, void* stream) {
+  if (!ctx || !p) return fail(VGI_E_INVALID, "NULL argument");
+  TRY(check_screen(p->width, p->height));
+  if (!p->width || !p->height) return VGI_OK;
+  if (!p->color || !p->depth || !p->out) return fail(VGI_E_INVALID, "NULL buffer");
+  if (!image_format_ok(p->color_format) || !image_format_ok(p->out_format))
+    return fail(VGI_E_INVALID, "unknown image format");
+  if (p->axis > 1) return fail(VGI_E_INVALID, "axis %u (0: x, 1: y)", p->axis);
+  if (!aligned(p->color, image_texel_bytes(p->color_format)) || !aligned(p->out, image_texel_bytes(p->out_format)) ||
+      !aligned(p->depth, 4))
+    return fail(VGI_E_INVALID, "misaligned buffer (one texel for images, 4 B for depth)");
+  const size_t npx = (size_t)p->width * p->height;
+  if (overlaps(p->out, npx * image_texel_bytes(p->out_format), p->color, npx * image_texel_bytes(p->color_format)))
+    return fail(VGI_E_INVALID, "out overlaps color");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_blur(*p, pick(ctx, stream)));
+  return VGI_OK;
+}
+
 int vgi_scene_read_voxels(vgi_context* ctx, const vgi_scene* s, uint32_t* host_out) {
   if (!ctx || !s || !host_out) return fail(VGI_E_INVALID, "NULL argument");
   if (!s->has_voxels) return fail(VGI_E_NOT_READY, "scene has not been voxelized");

@@ -22,17 +22,6 @@
 
 namespace vgi {
 
-__device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
-  // Round to FP32 first, then to FP16, as the definition says: without this barrier the
-  // backend may fuse a preceding multiply into the conversion (v_fma_mixlo_f16), which
-  // rounds once and differs from the two roundings by an ulp in rare near-ties.
-  __asm__ volatile("" : "+v"(f));
-  _Float16 h = (_Float16)f;
-  return __builtin_bit_cast(uint16_t, h);
-}
-__device__ __forceinline__ float f16_bits_to_f32(uint16_t b) {
-  return (float)__builtin_bit_cast(_Float16, b);
-}
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
   uint32_t x = __builtin_bit_cast(uint32_t, f);
   x += 0x7fffu + ((x >> 16) & 1u);

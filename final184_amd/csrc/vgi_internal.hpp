@@ -244,6 +244,19 @@ __device__ __forceinline__ void gatomic_or(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_or((VGI_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// FP32 -> FP16 bits, round to nearest even. The value is rounded to FP32 first, then to
+// FP16: without the register barrier the backend may fuse a preceding multiply into the
+// conversion (v_fma_mixlo_f16), which rounds once and differs from the two roundings by
+// an ulp in rare near-ties.
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
+  __asm__ volatile("" : "+v"(f));
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint16_t b) {
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+
 // Sets the calling thread's vgi_last_error() message and returns code (defined in abi.hip).
 int set_error(int code, const char* fmt, ...);
 
@@ -273,5 +286,11 @@ hipError_t launch_morton_keys(const TraceBatch& B, unsigned long long* keys, uin
 hipError_t sort_pixels(void* tmp, size_t* tmp_bytes, const unsigned long long* keys_in,
                        unsigned long long* keys_out, const uint32_t* pix_in, uint32_t* pix_out, uint32_t n,
                        int end_bit, hipStream_t s);
+
+// ---- screen-space passes (screen.hip); arguments validated by the caller ----
+hipError_t launch_gbuffer_decode(const vgi_gbuffer_ref& in, float* pos, float* nrm, float* view_depth,
+                                 hipStream_t s);
+hipError_t launch_temporal(const vgi_temporal_params& p, hipStream_t s);
+hipError_t launch_blur(const vgi_blur_params& p, hipStream_t s);
 
 }  // namespace vgi

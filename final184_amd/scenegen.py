@@ -104,6 +104,59 @@ def gbuffer_camera(pos, tri_rough, cam, width, height, vfov_deg=60.0):
     return gp, gn, gr
 
 
+def camera_view_matrix(cam) -> np.ndarray:
+    """World -> view (float64 4x4, row-major) of a camera_pose (eye, forward, right, up):
+    view x = right, y = up, z = -forward (the camera looks down -z)."""
+    c = np.asarray(cam, np.float64)
+    e, f, r, u = c[0:3], c[3:6], c[6:9], c[9:12]
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = r, u, -f
+    m[:3, 3] = -(m[:3, :3] @ e)
+    return m
+
+
+def perspective_matrix(vfov_deg: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """The reference's CCamera::CalcPerspective (Foreground/SceneGraph/Camera.cpp:85-101),
+    float64 4x4 row-major: OpenGL-style depth terms, y negated for Vulkan's NDC (row 0 of
+    the screen is +y in view space)."""
+    ys = math.tan(math.radians(vfov_deg) / 2.0)
+    xs = ys * aspect
+    a, b = 1.0 / xs, 1.0 / ys
+    c = (far + near) / (near - far)
+    d = -2.0 * far * near / (far - near)
+    return np.array([[a, 0, 0, 0], [0, -b, 0, 0], [0, 0, c, d], [0, 0, -1, 0]], np.float64)
+
+
+def glsl_mat4(m: np.ndarray) -> np.ndarray:
+    """16 float32 in GLSL's column-major order (m[column * 4 + row])."""
+    return np.ascontiguousarray(np.asarray(m, np.float64).T.reshape(16), np.float32)
+
+
+def reference_gbuffer(gb_pos, gb_nrm, cam, width: int, height: int, vfov_deg: float = 60.0,
+                      near: float = 0.05, far: float = 10.0):
+    """A world-space G-buffer (camera_pose + gbuffer_camera) in the reference's render-target
+    form: D32 depth = z_ndc of the reference's projection, RGBA16 UNORM view-space normals
+    (n * 0.5 + 0.5, rounded), and the column-major InvProj / InvModelView the reference
+    uploads (SceneView.cpp:28-30, MegaPipeline.cpp:242). Pixels without a surface get depth 1
+    and normal 0. Returns depth (H*W float32), normals (H*W x 4 uint16), inv_proj,
+    inv_model_view (16 float32 each), view and proj (float64 4x4)."""
+    V = camera_view_matrix(cam)
+    P = perspective_matrix(vfov_deg, width / height, near, far)
+    pos = np.asarray(gb_pos, np.float64).reshape(-1, 3)
+    nrm = np.asarray(gb_nrm, np.float64).reshape(-1, 3)
+    hit = np.any(nrm != 0, axis=1)
+    vp = pos @ V[:3, :3].T + V[:3, 3]
+    clip_z = vp[:, 2] * P[2, 2] + P[2, 3]
+    clip_w = -vp[:, 2]
+    depth = np.where(hit, clip_z / np.where(hit, clip_w, 1.0), 1.0).astype(np.float32)
+    vn = nrm @ V[:3, :3].T
+    q = np.rint((vn * 0.5 + 0.5) * 65535.0)
+    normals = np.zeros((pos.shape[0], 4), np.uint16)
+    normals[:, :3] = np.where(hit[:, None], np.clip(q, 0, 65535), 0).astype(np.uint16)
+    normals[:, 3] = 65535
+    return depth, normals, glsl_mat4(np.linalg.inv(P)), glsl_mat4(np.linalg.inv(V)), V, P
+
+
 def point_lights(seed: int, count: int, lo, hi, imin=1.0, imax=5.0):
     out = np.empty((count, 4), np.float32)
     lo_a, hi_a = _f32(lo), _f32(hi)  # keep the arrays alive across the call

@@ -68,6 +68,27 @@ class GBuffer(C.Structure):
 
 
 ORDER_TILES, ORDER_MORTON = 0, 1
+IMAGE_RGBA32F, IMAGE_RGBA16F = 0, 1
+
+
+class GBufferRef(C.Structure):
+    """The reference's G-buffer: D32 depth, RGBA16 UNORM view-space normals and the
+    column-major inverse projection / inverse model-view."""
+    _fields_ = [("depth", C.c_void_p), ("normals", C.c_void_p), ("inv_proj", C.c_float * 16),
+                ("inv_model_view", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class TemporalParams(C.Structure):
+    _fields_ = [("indirect", C.c_void_p), ("position", C.c_void_p), ("view_depth", C.c_void_p),
+                ("history", C.c_void_p), ("out", C.c_void_p), ("history_format", C.c_uint32),
+                ("out_format", C.c_uint32), ("prev_projection", C.c_float * 16),
+                ("prev_model_view", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class BlurParams(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("depth", C.c_void_p), ("out", C.c_void_p),
+                ("color_format", C.c_uint32), ("out_format", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("axis", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -175,6 +196,9 @@ SIGNATURES = [
     ("vgi_voxelize_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.c_uint32, _P]),
     ("vgi_build_mips_batch", C.c_int, [_P, C.POINTER(MipsItem), C.c_uint32, C.c_uint32, _P]),
     ("vgi_trace_batch", C.c_int, [_P, C.POINTER(TraceItem), C.c_uint32, _P]),
+    ("vgi_gbuffer_decode", C.c_int, [_P, C.POINTER(GBufferRef), _P, _P, _P, _P]),
+    ("vgi_temporal_resolve", C.c_int, [_P, C.POINTER(TemporalParams), _P]),
+    ("vgi_bilateral_blur", C.c_int, [_P, C.POINTER(BlurParams), _P]),
     ("vgi_autotune_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.POINTER(MipsItem),
                                      C.POINTER(TraceItem), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(AutotuneReport), _P]),
@@ -547,6 +571,39 @@ class Context:
         for scene, _, grid in vox_jobs:
             scene.grid = grid
         return r
+
+    # ---- screen-space passes (device buffers in, device buffers out) ----
+    def gbuffer_decode(self, depth: DeviceArray, normals: DeviceArray, inv_proj, inv_model_view,
+                       width: int, height: int, out_position: DeviceArray, out_normal: DeviceArray,
+                       out_view_depth: DeviceArray | None = None, stream=None) -> None:
+        """inv_proj / inv_model_view: 4x4 arrays as GLSL stores them (column-major, 16 floats)."""
+        g = GBufferRef()
+        g.depth, g.normals = depth.ptr, normals.ptr
+        g.inv_proj[:] = [float(v) for v in np.asarray(inv_proj, np.float32).reshape(16)]
+        g.inv_model_view[:] = [float(v) for v in np.asarray(inv_model_view, np.float32).reshape(16)]
+        g.width, g.height = width, height
+        _check(lib.vgi_gbuffer_decode(self.handle, C.byref(g), out_position.ptr, out_normal.ptr,
+                                      out_view_depth.ptr if out_view_depth is not None else None,
+                                      stream))
+
+    def temporal_resolve(self, indirect: DeviceArray, position: DeviceArray, view_depth: DeviceArray,
+                         history: DeviceArray, out: DeviceArray, prev_projection, prev_model_view,
+                         width: int, height: int, history_format: int = IMAGE_RGBA32F,
+                         out_format: int = IMAGE_RGBA32F, stream=None) -> None:
+        t = TemporalParams()
+        t.indirect, t.position, t.view_depth = indirect.ptr, position.ptr, view_depth.ptr
+        t.history, t.out = history.ptr, out.ptr
+        t.history_format, t.out_format = history_format, out_format
+        t.prev_projection[:] = [float(v) for v in np.asarray(prev_projection, np.float32).reshape(16)]
+        t.prev_model_view[:] = [float(v) for v in np.asarray(prev_model_view, np.float32).reshape(16)]
+        t.width, t.height = width, height
+        _check(lib.vgi_temporal_resolve(self.handle, C.byref(t), stream))
+
+    def bilateral_blur(self, color: DeviceArray, depth: DeviceArray, out: DeviceArray, width: int,
+                       height: int, axis: int, color_format: int = IMAGE_RGBA32F,
+                       out_format: int = IMAGE_RGBA32F, stream=None) -> None:
+        b = BlurParams(color.ptr, depth.ptr, out.ptr, color_format, out_format, width, height, axis)
+        _check(lib.vgi_bilateral_blur(self.handle, C.byref(b), stream))
 
     def timer_start(self, stream=None) -> None:
         _check(lib.vgi_timer_start(self.handle, stream))

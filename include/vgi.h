@@ -17,6 +17,10 @@
  *   - the lighting_indirect fullscreen pass     Foreground/Renderer/MegaPipeline.cpp:252-268
  *     (Shader/Lighting/indirect.frag:104-243)
  *   - the voxel image ownership                 Foreground/Renderer/MegaPipeline.cpp:470-490
+ *   - G-buffer reconstruction in that pass      Shader/Lighting/indirect.frag:44-58,189-196
+ *   - its temporal blend                        Shader/Lighting/indirect.frag:224-242
+ *   - the indirect_blurX / blurY passes         Foreground/Renderer/MegaPipeline.cpp:270-284
+ *     (Shader/Lighting/bilateralBlur.inc:15-72)
  *
  * Input/output buffers passed to vgi_voxelize / vgi_trace are DEVICE pointers
  * (the reference hands GPU vertex/index buffers and images to its passes); the
@@ -259,6 +263,79 @@ int vgi_build_mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t 
                          uint32_t precision, void* stream);
 /* Scenes whose mips have different precisions are traced in one launch per precision. */
 int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream);
+
+/* ---- screen-space passes around the cone march ------------------------------ */
+/* The reference's indirect-lighting frame is: reconstruct world position and normal from
+ * the depth and normal targets, march the cones, blend with last frame's result
+ * (temporal), then a cross-bilateral blur along y and along x (MegaPipeline.cpp:252-284).
+ * These entry points are those steps on device buffers, one thread per pixel, in FP32
+ * with a fixed operation order (no fused multiply-add contraction), so that results are
+ * reproducible bit for bit. Images are row-major width x height, 4 channels per texel,
+ * stored as fp32 or fp16 (the reference's render targets are RGBA16F). */
+enum { VGI_IMAGE_RGBA32F = 0, VGI_IMAGE_RGBA16F = 1 };
+
+/* The reference's G-buffer inputs: the D32 depth target and the RGBA16 UNORM view-space
+ * normal target (rgb = n * 0.5 + 0.5), with the inverse projection and inverse
+ * model-view matrices (GLSL mat4, column-major: m[column * 4 + row]). */
+typedef struct vgi_gbuffer_ref {
+  const float* depth;      /* width x height */
+  const uint16_t* normals; /* width x height x 4 */
+  float inv_proj[16];
+  float inv_model_view[16];
+  uint32_t width;
+  uint32_t height;
+} vgi_gbuffer_ref;
+
+/* World position and normal (float32 x 3 per pixel, the vgi_gbuffer layout) and view
+ * depth (-z in view space, float per pixel; may be NULL) of every pixel. Pixel (x, y) is
+ * taken at uv = ((x + 0.5) / width, (y + 0.5) / height): view position =
+ * InvProj * (uv * 2 - 1, depth, 1) divided by its w; world position = InvModelView *
+ * (view position, 1); normal = mat3(InvModelView) * normalize(unorm * 2 - 1), or 0 where
+ * the stored normal has zero length (the trace skips such pixels). Replaces
+ * Shader/Lighting/indirect.frag:44-58 (getDepth, getCSpos, getNormal) and :189-196. */
+int vgi_gbuffer_decode(vgi_context* ctx, const vgi_gbuffer_ref* in, float* out_position,
+                       float* out_normal, float* out_view_depth, void* stream);
+
+/* Temporal accumulation: each pixel's world position is reprojected with last frame's
+ * projection and model-view; where it lands inside the screen, last frame's result
+ * (bilinear, wrapping at the edges as the reference's repeat sampler does) is blended in
+ * with weight 0.95 * smoothstep(0, 1, 1 - |history depth - view depth|) and the colour is
+ * clamped to [0, 16]. out = (colour, view depth), the next frame's history. Replaces
+ * Shader/Lighting/indirect.frag:224-242. Device pointers; out must not alias history. */
+typedef struct vgi_temporal_params {
+  const float* indirect;   /* float4 per pixel, rgb used (e.g. vgi_trace's out_diffuse) */
+  const float* position;   /* world position float32 x 3 per pixel */
+  const float* view_depth; /* float per pixel */
+  const void* history;     /* last frame's out, history_format */
+  void* out;               /* out_format */
+  uint32_t history_format;
+  uint32_t out_format;
+  float prev_projection[16];  /* column-major */
+  float prev_model_view[16];  /* column-major */
+  uint32_t width;
+  uint32_t height;
+} vgi_temporal_params;
+int vgi_temporal_resolve(vgi_context* ctx, const vgi_temporal_params* params, void* stream);
+
+/* Cross-bilateral blur along one screen axis: the centre texel plus taps at +-2, 4, 6, 8
+ * pixels (distance weights for r = 1..4) and +-11, 15 pixels (r = 5, 7), each weighted by
+ * exp2(-r^2 / 32 - ((d_centre - d_tap) * 512)^2) on the raw depth, edges clamped;
+ * out = (weighted mean rgb, 0). axis 0 blurs along x (the reference's blurY.frag), axis 1
+ * along y (blurX.frag); the reference runs axis 1 then axis 0. exp2 is evaluated as
+ * 2^k * p(f) with k = rint(x), f = x - k and p the degree-7 Taylor polynomial of 2^f,
+ * 0 below 2^-125. Replaces Shader/Lighting/bilateralBlur.inc:15-72. out must not alias
+ * color. */
+typedef struct vgi_blur_params {
+  const void* color;  /* color_format, rgb used */
+  const float* depth; /* the D32 depth target */
+  void* out;          /* out_format */
+  uint32_t color_format;
+  uint32_t out_format;
+  uint32_t width;
+  uint32_t height;
+  uint32_t axis;
+} vgi_blur_params;
+int vgi_bilateral_blur(vgi_context* ctx, const vgi_blur_params* params, void* stream);
 
 /* ---- autotuner -------------------------------------------------------------- */
 /* Times one batch (the same scenes in vox, mips and trace, count each) under every bin

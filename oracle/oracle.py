@@ -60,6 +60,11 @@ def _load():
         "orc_ref_hemisphere_cos": (None, [C.c_float, C.c_float, P]),
         "orc_ref_make_coord_space": (None, [P, P]),
         "orc_ref_get_indirect": (C.c_int, [P, P, P, P, C.c_float, C.c_float, P, P, P, P, P]),
+        "orc_gbuffer_decode_f32": (None, [P, P, P, P, C.c_uint32, C.c_uint32, P, P, P]),
+        "orc_gbuffer_decode_f64": (None, [P, P, P, P, C.c_uint32, C.c_uint32, P, P, P]),
+        "orc_temporal": (None, [P, P, P, P, P, P, C.c_uint32, C.c_uint32, P]),
+        "orc_exp2_poly": (C.c_float, [C.c_float]),
+        "orc_bilateral_blur": (None, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -266,3 +271,51 @@ def ref_get_indirect(voxels_rg16, w2voxel_colmajor, wpos, wnorm, r0, r1, sun_dir
     hit = lib.orc_ref_get_indirect(_p(vox), _p(m), _p(wp), _p(wn), r0, r1, _p(sd), _p(sl), _p(Lo),
                                    _p(hp), C.byref(steps))
     return bool(hit), Lo, hp, int(steps.value)
+
+
+def gbuffer_decode(depth, normals, inv_proj, inv_mv, width, height, f64=False):
+    """World position, normal (float32 x 3 per pixel) and view depth (-z) from the reference's
+    G-buffer: D32 depth, RGBA16 UNORM view-space normals, column-major InvProj and
+    InvModelView (indirect.frag:44-58, 189-196)."""
+    npx = width * height
+    depth = np.ascontiguousarray(depth, np.float32).reshape(npx)
+    normals = np.ascontiguousarray(normals, np.uint16).reshape(npx, 4)
+    ip = np.ascontiguousarray(inv_proj, np.float32).reshape(16)
+    im = np.ascontiguousarray(inv_mv, np.float32).reshape(16)
+    pos = np.empty((npx, 3), np.float32)
+    nrm = np.empty((npx, 3), np.float32)
+    vd = np.empty(npx, np.float32)
+    fn = lib.orc_gbuffer_decode_f64 if f64 else lib.orc_gbuffer_decode_f32
+    fn(_p(depth), _p(normals), _p(ip), _p(im), width, height, _p(pos), _p(nrm), _p(vd))
+    return pos, nrm, vd
+
+
+def temporal(indirect, position, view_depth, history, prev_proj, prev_mv, width, height):
+    """Temporal blend of one frame (indirect.frag:224-242): float32 (H*W, 4) out, rgb =
+    blended colour, a = view depth. history is float32 (H*W, 4) (an FP16 history is
+    widened by the caller, exactly)."""
+    npx = width * height
+    ind = np.ascontiguousarray(indirect, np.float32).reshape(npx, 4)
+    pos = np.ascontiguousarray(position, np.float32).reshape(npx, 3)
+    vd = np.ascontiguousarray(view_depth, np.float32).reshape(npx)
+    hist = np.ascontiguousarray(history, np.float32).reshape(npx, 4)
+    pp = np.ascontiguousarray(prev_proj, np.float32).reshape(16)
+    pm = np.ascontiguousarray(prev_mv, np.float32).reshape(16)
+    out = np.empty((npx, 4), np.float32)
+    lib.orc_temporal(_p(ind), _p(pos), _p(vd), _p(hist), _p(pp), _p(pm), width, height, _p(out))
+    return out
+
+
+def exp2_poly(x):
+    return float(lib.orc_exp2_poly(float(x)))
+
+
+def bilateral_blur(color, depth, width, height, axis):
+    """One cross-bilateral blur pass (bilateralBlur.inc:15-72); axis 0 along x (blurY.frag),
+    1 along y (blurX.frag). float32 (H*W, 4) in (rgb used) and out (a = 0)."""
+    npx = width * height
+    col = np.ascontiguousarray(color, np.float32).reshape(npx, 4)
+    dep = np.ascontiguousarray(depth, np.float32).reshape(npx)
+    out = np.empty((npx, 4), np.float32)
+    lib.orc_bilateral_blur(_p(col), _p(dep), width, height, axis, _p(out))
+    return out

@@ -35,6 +35,18 @@
 //      indirect.frag's unpackColor565/unpackNormal565 (:88-102), make_coord_space
 //      (:71-86), hemisphereSample_cos (Shader/math.inc:76-81) and getIndirect's ray
 //      march (indirect.frag:104-184) over a nearest-voxel RG16UI volume.
+//  (5) orc_gbuffer_decode_* — the reference's G-buffer reconstruction
+//      (Shader/Lighting/indirect.frag:44-58 getDepth/getCSpos/getNormal, :189-196): D32
+//      depth at the pixel, view position = InvProj * (uv*2-1, depth, 1) divided by w,
+//      world position and normal through InvModelView, normal = normalize(unorm*2-1).
+//  (6) orc_temporal — the reference's temporal blend (indirect.frag:224-242): reproject
+//      the world position with last frame's matrices, bilinear (repeat-addressed)
+//      history sample, weight 0.95 * smoothstep(0, 1, 1 - |prevDepth + cspos.z|),
+//      clamp [0, 16], out = (colour, -cspos.z).
+//  (7) orc_bilateral_blur — bilateralBlur.inc:15-72 with blurX.frag / blurY.frag's
+//      directions: taps at +-2i (i = 1..4) and +-(2i+1) (i = 5, 7) pixels (invres is
+//      2 / resolution, so every tap lands on a texel centre), clamp-to-edge,
+//      w = exp2(-r^2/32 - ((d0 - d) * 512)^2), out = (sum w*c / sum w, 0).
 //
 // Data conventions shared with the product are the spec, restated here:
 //  * grid cube [origin, origin+extent]^3, n^3 voxels, voxel (i,j,k) linear index
@@ -820,3 +832,150 @@ OAPI int orc_ref_get_indirect(const uint16_t* voxels, const float* w2voxel, cons
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// (5) G-buffer reconstruction (indirect.frag:44-58, 189-196)
+// ---------------------------------------------------------------------------
+// Matrices are GLSL mat4, column-major: m[c * 4 + r]. Pixel (x, y) is sampled at its
+// centre uv = ((x + 0.5) / W, (y + 0.5) / H). Sums run in the order written; the FP32
+// form is the arithmetic the product performs (IEEE division and square root).
+template <typename T>
+static void gbuffer_decode(const float* depth, const uint16_t* normals, const float* inv_proj,
+                           const float* inv_mv, uint32_t W, uint32_t H, float* out_pos, float* out_nrm,
+                           float* out_view_depth) {
+  T P[16], M[16];
+  for (int i = 0; i < 16; i++) {
+    P[i] = (T)inv_proj[i];
+    M[i] = (T)inv_mv[i];
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t y = 0; y < (int64_t)H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      const size_t i = (size_t)y * W + x;
+      const T u = ((T)x + (T)0.5) / (T)W, v = ((T)y + (T)0.5) / (T)H;
+      const T p[4] = {u * (T)2 - (T)1, v * (T)2 - (T)1, (T)depth[i], (T)1};
+      T c[4];
+      for (int r = 0; r < 4; r++) c[r] = ((P[r] * p[0] + P[4 + r] * p[1]) + P[8 + r] * p[2]) + P[12 + r] * p[3];
+      const T cs[3] = {c[0] / c[3], c[1] / c[3], c[2] / c[3]};
+      for (int r = 0; r < 3; r++)
+        out_pos[i * 3 + r] = (float)(((M[r] * cs[0] + M[4 + r] * cs[1]) + M[8 + r] * cs[2]) + M[12 + r]);
+      T n[3];
+      for (int k = 0; k < 3; k++) n[k] = std::fma((T)normals[i * 4 + k] / (T)65535, (T)2, (T)-1);
+      const T len2 = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
+      if (len2 > (T)0) {
+        const T len = std::sqrt(len2);
+        for (int k = 0; k < 3; k++) n[k] = n[k] / len;
+      } else {
+        n[0] = n[1] = n[2] = (T)0;  // no surface: the trace skips zero normals
+      }
+      for (int r = 0; r < 3; r++) out_nrm[i * 3 + r] = (float)((M[r] * n[0] + M[4 + r] * n[1]) + M[8 + r] * n[2]);
+      if (out_view_depth) out_view_depth[i] = (float)(-cs[2]);
+    }
+}
+
+OAPI void orc_gbuffer_decode_f32(const float* depth, const uint16_t* normals, const float* inv_proj,
+                                 const float* inv_mv, uint32_t W, uint32_t H, float* out_pos, float* out_nrm,
+                                 float* out_view_depth) {
+  gbuffer_decode<float>(depth, normals, inv_proj, inv_mv, W, H, out_pos, out_nrm, out_view_depth);
+}
+
+OAPI void orc_gbuffer_decode_f64(const float* depth, const uint16_t* normals, const float* inv_proj,
+                                 const float* inv_mv, uint32_t W, uint32_t H, float* out_pos, float* out_nrm,
+                                 float* out_view_depth) {
+  gbuffer_decode<double>(depth, normals, inv_proj, inv_mv, W, H, out_pos, out_nrm, out_view_depth);
+}
+
+// ---------------------------------------------------------------------------
+// (6) temporal blend (indirect.frag:224-242), FP32 in the written order
+// ---------------------------------------------------------------------------
+static inline float mrow(const float* m, int r, float a, float b, float c, float d) {
+  return ((m[r] * a + m[4 + r] * b) + m[8 + r] * c) + m[12 + r] * d;
+}
+
+OAPI void orc_temporal(const float* indirect, const float* pos, const float* view_depth, const float* history,
+                       const float* prev_proj, const float* prev_mv, uint32_t W, uint32_t H, float* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)W * H; i++) {
+    const float* wp = pos + i * 3;
+    const float csz = -view_depth[i];
+    float c[4];
+    for (int r = 0; r < 4; r++) c[r] = mrow(prev_mv, r, wp[0], wp[1], wp[2], 1.0f);
+    float q[4];
+    for (int r = 0; r < 4; r++) q[r] = mrow(prev_proj, r, c[0], c[1], c[2], c[3]);
+    // prevProjPos /= prevProjPos.w; st * 0.5 + 0.5
+    const float u = (q[0] / q[3]) * 0.5f + 0.5f, v = (q[1] / q[3]) * 0.5f + 0.5f;
+    float col[3] = {indirect[i * 4], indirect[i * 4 + 1], indirect[i * 4 + 2]};
+    if (u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f) {
+      // GL_REPEAT bilinear: texel centres at (k + 0.5) / size, indices taken modulo size
+      const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+      const float fx0 = std::floor(x), fy0 = std::floor(y);
+      const float fx = x - fx0, fy = y - fy0;
+      auto wrap = [](int64_t k, int64_t n) { return ((k % n) + n) % n; };
+      const int64_t xa = wrap((int64_t)fx0, W), xb = wrap((int64_t)fx0 + 1, W);
+      const int64_t ya = wrap((int64_t)fy0, H), yb = wrap((int64_t)fy0 + 1, H);
+      const float* t00 = history + (ya * W + xa) * 4;
+      const float* t10 = history + (ya * W + xb) * 4;
+      const float* t01 = history + (yb * W + xa) * 4;
+      const float* t11 = history + (yb * W + xb) * 4;
+      float h[4];
+      for (int k = 0; k < 4; k++)
+        h[k] = (t00[k] * (1.0f - fx) + t10[k] * fx) * (1.0f - fy) + (t01[k] * (1.0f - fx) + t11[k] * fx) * fy;
+      // smoothstep(0, 1, s) = t * t * (3 - 2t), t = clamp(s, 0, 1); mix(a, b, w) = a(1-w) + bw
+      float t = 1.0f - std::fabs(h[3] + csz);
+      t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+      const float blend = 0.95f * ((t * t) * (3.0f - 2.0f * t));
+      for (int k = 0; k < 3; k++) col[k] = std::fmin(std::fmax(col[k] * (1.0f - blend) + h[k] * blend, 0.0f), 16.0f);
+    }
+    float* o = out + i * 4;
+    o[0] = col[0];
+    o[1] = col[1];
+    o[2] = col[2];
+    o[3] = -csz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (7) cross-bilateral blur (bilateralBlur.inc:15-72)
+// ---------------------------------------------------------------------------
+// exp2 as the product defines it: 2^rint(x) * P7(x - rint(x)), P7 the Taylor polynomial
+// of 2^f with FP32 coefficients ln2^j / j!, Horner with fused multiply-adds; 0 for
+// x <= -125 or NaN.
+OAPI float orc_exp2_poly(float x) {
+  if (!(x > -125.0f)) return 0.0f;
+  const float k = std::nearbyint(x);
+  const float f = x - k;
+  static const float c[8] = {1.0f, 6.931471825e-01f, 2.402265072e-01f, 5.550410971e-02f,
+                             9.618128650e-03f, 1.333355787e-03f, 1.540352969e-04f, 1.525273365e-05f};
+  float q = c[7];
+  for (int j = 6; j >= 0; j--) q = std::fma(q, f, c[j]);
+  return std::ldexp(q, (int)k);
+}
+
+OAPI void orc_bilateral_blur(const float* color, const float* depth, uint32_t W, uint32_t H, uint32_t axis,
+                             float* out) {
+  // (offset in pixels, r): SampleColor(DIR(i)) is i * 2 / resolution away in uv
+  static const int off[12] = {2, -2, 4, -4, 6, -6, 8, -8, 11, -11, 15, -15};
+  static const int rad[12] = {1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 7, 7};
+  const float sigma = 8.0f * 0.5f, falloff = 1.0f / (2.0f * sigma * sigma);  // KERNEL_RADIUS 8
+#pragma omp parallel for schedule(static)
+  for (int64_t y = 0; y < (int64_t)H; y++)
+    for (int64_t x = 0; x < (int64_t)W; x++) {
+      const int64_t i = y * W + x;
+      const float d0 = depth[i];
+      float tot[3] = {color[i * 4], color[i * 4 + 1], color[i * 4 + 2]}, tw = 1.0f;
+      for (int j = 0; j < 12; j++) {
+        const int64_t sx = axis == 0 ? std::min<int64_t>(std::max<int64_t>(x + off[j], 0), W - 1) : x;
+        const int64_t sy = axis == 1 ? std::min<int64_t>(std::max<int64_t>(y + off[j], 0), H - 1) : y;
+        const int64_t si = sy * W + sx;
+        const float r = (float)rad[j];
+        const float dz = (d0 - depth[si]) * 512.0f;
+        const float w = orc_exp2_poly(-r * r * falloff - dz * dz);
+        for (int k = 0; k < 3; k++) tot[k] = tot[k] + color[si * 4 + k] * w;
+        tw = tw + w;
+      }
+      out[i * 4] = tot[0] / tw;
+      out[i * 4 + 1] = tot[1] / tw;
+      out[i * 4 + 2] = tot[2] / tw;
+      out[i * 4 + 3] = 0.0f;
+    }
+}

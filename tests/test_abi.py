@@ -51,6 +51,9 @@ STRUCTS = {
     "vgi_request": (vgi.Request, [f for f, _ in vgi.Request._fields_]),
     "vgi_completion": (vgi.Completion, [f for f, _ in vgi.Completion._fields_]),
     "vgi_server_stats": (vgi.ServerStats, [f for f, _ in vgi.ServerStats._fields_]),
+    "vgi_gbuffer_ref": (vgi.GBufferRef, [f for f, _ in vgi.GBufferRef._fields_]),
+    "vgi_temporal_params": (vgi.TemporalParams, [f for f, _ in vgi.TemporalParams._fields_]),
+    "vgi_blur_params": (vgi.BlurParams, [f for f, _ in vgi.BlurParams._fields_]),
 }
 
 
@@ -104,6 +107,9 @@ def test_null_arguments_are_rejected_with_a_message():
     assert vgi.lib.vgi_autotune_batch(None, None, None, None, 0, 0, 1, None, None) == -1
     assert vgi.lib.vgi_server_create(None, None, None) == -1
     assert vgi.lib.vgi_server_submit(None, None) == -1
+    assert vgi.lib.vgi_gbuffer_decode(None, None, None, None, None, None) == -1
+    assert vgi.lib.vgi_temporal_resolve(None, None, None) == -1
+    assert vgi.lib.vgi_bilateral_blur(None, None, None) == -1
     assert vgi.lib.vgi_server_destroy(None) == 0
     assert vgi.lib.vgi_now() > 0
 
