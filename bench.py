@@ -116,6 +116,92 @@ def run_config(ctx, vgi, w, precs, steps, keep=None):
     return out
 
 
+def run_screen(ctx, vgi, sg, w, mesh, gb_rough, steps):
+    """The screen-space passes of the reference's frame at the headline size, each timed
+    alone (HIP events around `steps` launches on the context stream) with its algorithmic
+    bytes per pixel: decode 40 (D32 depth 4 + RGBA16 normal 8 in; position 12, normal 12,
+    view depth 4 out), temporal 48 (indirect float4 16, position 12, view depth 4, one
+    RGBA16F history texel 8 in; RGBA16F 8 out), blur 20 per axis (RGBA16F 8 + depth 4
+    in; RGBA16F 8 out). Then the reference's whole frame on the decoded G-buffer: decode,
+    voxelize, FP16 mips, trace, temporal, blur along y, blur along x."""
+    W, H = w.width, w.height
+    px = W * H
+    cam = np.asarray(w.meta["camera"], np.float32)
+    depth, normals, ip, im, V, P = sg.reference_gbuffer(w.gb_pos, w.gb_nrm, cam, W, H)
+    d_depth = vgi.DeviceArray.from_numpy(ctx, depth)
+    d_nrm16 = vgi.DeviceArray.from_numpy(ctx, normals)
+    pos = vgi.DeviceArray.empty(ctx, (px, 3), np.float32)
+    nrm = vgi.DeviceArray.empty(ctx, (px, 3), np.float32)
+    vd = vgi.DeviceArray.empty(ctx, (px,), np.float32)
+    ind = vgi.DeviceArray.empty(ctx, (px, 4), np.float32)
+    spec = vgi.DeviceArray.empty(ctx, (px, 4), np.float32)
+    hist = [vgi.DeviceArray.empty(ctx, (px, 4), np.float16) for _ in range(2)]
+    for h in hist:
+        h.zero()
+    bx = vgi.DeviceArray.empty(ctx, (px, 4), np.float16)
+    by = vgi.DeviceArray.empty(ctx, (px, 4), np.float16)
+    pp, pmv = sg.glsl_mat4(P), sg.glsl_mat4(V)
+    F16 = vgi.IMAGE_RGBA16F
+    grid, lights, cones = vgi.make_grid(**w.grid), vgi.make_lights(w.lights), vgi.make_cones(**w.cones)
+    scene = ctx.scene()
+    gbd = vgi.GBuffer(pos.ptr, nrm.ptr, gb_rough.ptr, W, H)
+
+    def decode():
+        ctx.gbuffer_decode(d_depth, d_nrm16, ip, im, W, H, pos, nrm, vd)
+
+    def temporal(k=0):
+        ctx.temporal_resolve(ind, pos, vd, hist[k & 1], hist[(k + 1) & 1], pp, pmv, W, H, F16, F16)
+
+    def blur_y():
+        ctx.bilateral_blur(hist[0], d_depth, bx, W, H, 1, F16, F16)
+
+    def blur_x():
+        ctx.bilateral_blur(bx, d_depth, by, W, H, 0, F16, F16)
+
+    def timed(fn, n):
+        fn()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(n):
+            fn()
+        return ctx.timer_stop() / n
+
+    out = {}
+    decode()
+    scene.voxelize([mesh], grid)
+    scene.build_mips(lights, vgi.PREC_FP16)
+    vgi._check(vgi.lib.vgi_trace(ctx.handle, scene.handle, vgi.C.byref(gbd), vgi.C.byref(cones), ind.ptr,
+                                 spec.ptr, None))
+    n = max(20, steps * 5)
+    for name, fn, bpp in (("decode", decode, 40), ("temporal", temporal, 48), ("blur_y", blur_y, 20),
+                          ("blur_x", blur_x, 20)):
+        ms = timed(fn, n)
+        gbs = px * bpp / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "bytes_per_pixel": bpp, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        log(f"screen {name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%} of HBM peak)")
+    k = [0]
+
+    def frame():
+        decode()
+        scene.voxelize([mesh], grid)
+        scene.build_mips(lights, vgi.PREC_FP16)
+        vgi._check(vgi.lib.vgi_trace(ctx.handle, scene.handle, vgi.C.byref(gbd), vgi.C.byref(cones), ind.ptr,
+                                     spec.ptr, None))
+        temporal(k[0])
+        k[0] += 1
+        blur_y()
+        blur_x()
+
+    ms = timed(frame, steps)
+    out["reference_frame"] = {"ms": ms, "mpixels_s": px / (ms * 1e-3) / 1e6,
+                              "stages": "decode, voxelize, fp16 mips, trace, temporal, blur y, blur x"}
+    log(f"reference frame (decode .. blur x): {ms:.3f} ms")
+    scene.release()
+    for a in (d_depth, d_nrm16, pos, nrm, vd, ind, spec, bx, by, *hist):
+        a.free()
+    return out
+
+
 def _public(res):
     return {p: {k: v for k, v in r.items() if k not in ("diffuse", "specular")} for p, r in res.items()}
 
@@ -419,6 +505,11 @@ def main() -> int:
                      "fp32_traffic": _traffic_bytes(traffic, base["prec"]),
                      "traffic_counters": traffic if traffic else traffic_err},
     }
+    try:
+        line["screen_passes"] = run_screen(ctx, vgi, sg, w, mesh, gb.rough, args.steps)
+    except Exception as e:  # the headline line is still printed
+        log(f"screen passes failed: {e!r}")
+        line["screen_passes"] = {"error": repr(e)}
     soup = study = None
     if not args.no_configs:
         # 10,000-triangle soup (64^3 / 6 levels, 256x256 points on triangles, 6+1 cones):

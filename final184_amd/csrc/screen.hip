@@ -10,9 +10,11 @@
 // its 13 taps from LDS; workgroups walk the tiles along the blur axis within each XCD
 // so that the reach, re-read by the neighbouring tile, is an L2 hit.
 //
-// Arithmetic is FP32 in the order the definitions write it, without fused
-// multiply-add contraction, IEEE division and square root, and a polynomial exp2, so
-// the results are reproducible bit for bit on any IEEE machine.
+// Arithmetic is FP32 in the order the definitions write it, with fused multiply-adds
+// only where written (the blur's weighted sums and its depth term), IEEE division and
+// square root, and a polynomial exp2, so the results are reproducible bit for bit on
+// any IEEE machine. The blur is VALU-bound (about 300 operations per pixel, mostly the
+// 12 exp2 evaluations), the other two passes HBM-bound.
 #include "vgi_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -172,9 +174,11 @@ __global__ __launch_bounds__(256) void k_temporal(vgi_temporal_params p) {
 // truncation error is below 6e-9 relative) evaluated by Horner's rule with FMAs;
 // 0 for x <= -125 and NaN, so every non-zero result is a normal number.
 __device__ __forceinline__ float exp2_poly(float x) {
-  if (!(x > -125.0f)) return 0.0f;
-  const float k = rintf(x);
-  const float f = x - k;
+  // Branch-free: the polynomial runs on max(x, -126) (NaN -> -126) and a select returns
+  // 0 below -125, so the taps of a pixel keep their loads in flight together.
+  const float xc = fmaxf(x, -126.0f);
+  const float k = rintf(xc);
+  const float f = xc - k;
   float q = 1.525273365e-05f;
   q = __builtin_fmaf(q, f, 1.540352969e-04f);
   q = __builtin_fmaf(q, f, 1.333355787e-03f);
@@ -183,7 +187,8 @@ __device__ __forceinline__ float exp2_poly(float x) {
   q = __builtin_fmaf(q, f, 2.402265072e-01f);
   q = __builtin_fmaf(q, f, 6.931471825e-01f);
   q = __builtin_fmaf(q, f, 1.0f);
-  return ldexpf(q, (int)k);
+  const float r = ldexpf(q, (int)k);
+  return x > -125.0f ? r : 0.0f;
 }
 
 constexpr int kBlurReach = 15;  // farthest tap from the centre, pixels
@@ -208,9 +213,10 @@ __device__ __forceinline__ float blur_dist() {
   return (-(float)(R * R)) * 0.03125f;
 }
 
+// exp2(-r^2/32 - dz^2) with dz = (d0 - d) * 512; the square is fused into the subtraction.
 __device__ __forceinline__ float blur_weight(float dist, float d, float d0) {
   const float dz = (d0 - d) * 512.0f;
-  return exp2_poly(dist - dz * dz);
+  return exp2_poly(__builtin_fmaf(-dz, dz, dist));
 }
 
 template <int AXIS, bool IN_HALF, bool OUT_HALF>
@@ -251,9 +257,9 @@ __global__ __launch_bounds__(256) void k_blur(vgi_blur_params p, uint32_t ntx, u
     {                                                                  \
       const float4 s = s_tex[lc + kTapOffset[J] * T::STEP];            \
       const float w = blur_weight(blur_dist<R>(), s.w, d0);            \
-      tr = tr + s.x * w;                                               \
-      tg = tg + s.y * w;                                               \
-      tb = tb + s.z * w;                                               \
+      tr = __builtin_fmaf(s.x, w, tr);                                 \
+      tg = __builtin_fmaf(s.y, w, tg);                                 \
+      tb = __builtin_fmaf(s.z, w, tb);                                 \
       tw = tw + w;                                                     \
     }
     VGI_TAP(0, 1) VGI_TAP(1, 1) VGI_TAP(2, 2) VGI_TAP(3, 2) VGI_TAP(4, 3) VGI_TAP(5, 3)

@@ -323,8 +323,8 @@ int vgi_temporal_resolve(vgi_context* ctx, const vgi_temporal_params* params, vo
  * out = (weighted mean rgb, 0). axis 0 blurs along x (the reference's blurY.frag), axis 1
  * along y (blurX.frag); the reference runs axis 1 then axis 0. exp2 is evaluated as
  * 2^k * p(f) with k = rint(x), f = x - k and p the degree-7 Taylor polynomial of 2^f,
- * 0 below 2^-125. Replaces Shader/Lighting/bilateralBlur.inc:15-72. out must not alias
- * color. */
+ * 0 for x <= -125; the depth square and the weighted sums are fused multiply-adds.
+ * Replaces Shader/Lighting/bilateralBlur.inc:15-72. out must not alias color. */
 typedef struct vgi_blur_params {
   const void* color;  /* color_format, rgb used */
   const float* depth; /* the D32 depth target */

@@ -46,7 +46,8 @@
 //  (7) orc_bilateral_blur — bilateralBlur.inc:15-72 with blurX.frag / blurY.frag's
 //      directions: taps at +-2i (i = 1..4) and +-(2i+1) (i = 5, 7) pixels (invres is
 //      2 / resolution, so every tap lands on a texel centre), clamp-to-edge,
-//      w = exp2(-r^2/32 - ((d0 - d) * 512)^2), out = (sum w*c / sum w, 0).
+//      w = exp2(-r^2/32 - ((d0 - d) * 512)^2), out = (sum w*c / sum w, 0); the square
+//      and the weighted sums are fused multiply-adds (what a shader compiler emits).
 //
 // Data conventions shared with the product are the spec, restated here:
 //  * grid cube [origin, origin+extent]^3, n^3 voxels, voxel (i,j,k) linear index
@@ -968,9 +969,10 @@ OAPI void orc_bilateral_blur(const float* color, const float* depth, uint32_t W,
         const int64_t sy = axis == 1 ? std::min<int64_t>(std::max<int64_t>(y + off[j], 0), H - 1) : y;
         const int64_t si = sy * W + sx;
         const float r = (float)rad[j];
+        // -r*r*falloff - dz*dz and total += c * w, each as one fused multiply-add
         const float dz = (d0 - depth[si]) * 512.0f;
-        const float w = orc_exp2_poly(-r * r * falloff - dz * dz);
-        for (int k = 0; k < 3; k++) tot[k] = tot[k] + color[si * 4 + k] * w;
+        const float w = orc_exp2_poly(std::fma(-dz, dz, -r * r * falloff));
+        for (int k = 0; k < 3; k++) tot[k] = std::fma(color[si * 4 + k], w, tot[k]);
         tw = tw + w;
       }
       out[i * 4] = tot[0] / tw;
