@@ -6,7 +6,9 @@
 //       SetRenderPass, RenderList(context, modelMats, primitives),
 //       PreparePrimitiveResources, ClearResourceCache, GarbageCollectResourceCache
 //   Foreground::CIndirectPass       <- the lighting_indirect CMaterial pass,
-//       Foreground/Renderer/MegaPipeline.cpp:252-268 (Shader/Lighting/indirect.frag)
+//       Foreground/Renderer/MegaPipeline.cpp:252-268 (Shader/Lighting/indirect.frag),
+//       and with RenderFrame the chain around it: the G-buffer decode, the temporal
+//       blend and the indirect_blurX / indirect_blurY passes (MegaPipeline.cpp:196-284)
 //   Foreground::CTriangleMesh / CBasicMaterial / CPrimitive
 //                                   <- Foreground/Shape/TriangleMesh.h:26-103,
 //                                      Foreground/Material/BasicMaterial.h,
@@ -184,11 +186,34 @@ private:
   std::string Error;
 };
 
+// The G-buffer targets the indirect pass samples (MegaPipeline.cpp:254-255): GBufferDepth
+// (D32) and GBuffer1 (RGBA16 UNORM view-space normals), device memory, width x height.
+struct CGBufferTargets {
+  const float* Depth = nullptr;
+  const uint16_t* Normals = nullptr;
+  const float* Roughness = nullptr;  // optional per-pixel specular roughness
+  uint32_t Width = 0;
+  uint32_t Height = 0;
+};
+
+// The matrices of one frame as the reference uploads them, GLSL column-major: InvProj
+// (SceneView.cpp:30), InvModelView (MegaPipeline.cpp:242) and last frame's ProjMat and
+// ViewMat (the prevProj block, MegaPipeline.cpp:70-71,321-322).
+struct CIndirectFrameConstants {
+  float InvProj[16];
+  float InvModelView[16];
+  float PrevProjection[16];
+  float PrevModelView[16];
+};
+
 // The lighting_indirect pass: direct light injected into the volume, the mip chain, and
 // the cone trace of every G-buffer pixel into the indirect buffer.
 class CIndirectPass {
 public:
   CIndirectPass(vgi_context* ctx, vgi_scene* scene) : Ctx(ctx), Scene(scene) {}
+  CIndirectPass(const CIndirectPass&) = delete;
+  CIndirectPass& operator=(const CIndirectPass&) = delete;
+  ~CIndirectPass() { FreeImages(); }
 
   void SetLights(const vgi_lights& lights) { Lights = lights; }
   void SetCones(const vgi_cone_params& cones) { Cones = cones; }
@@ -203,10 +228,115 @@ public:
     if (Status != VGI_OK) Error = vgi_last_error();
   }
 
+  // The reference's whole indirect chain for one frame (MegaPipeline.cpp:196-284): the
+  // targets are decoded to world position and normal (indirect.frag:44-58,189-196), the
+  // cones traced, the result blended with the previous frame's (indirect.frag:224-242),
+  // then blurred along y (indirect_blurX) and along x (indirect_blurY). The history is
+  // this pass's previous blended output, which the reference copies into
+  // indirectTemporalImage at the start of a frame; here two images alternate instead.
+  // Images are RGBA16F like the reference's render targets, owned by the pass and
+  // re-created (history cleared, as on the reference's first frame) when the target
+  // size changes. Returns the final image (device, RGBA16F) or NULL on error.
+  const uint16_t* RenderFrame(CRenderContext& context, const CGBufferTargets& targets,
+                              const CIndirectFrameConstants& m) {
+    Status = EnsureImages(targets.Width, targets.Height);
+    if (Status == VGI_OK) {
+      vgi_gbuffer_ref ref{targets.Depth, targets.Normals, {}, {}, targets.Width, targets.Height};
+      for (int i = 0; i < 16; i++) {
+        ref.inv_proj[i] = m.InvProj[i];
+        ref.inv_model_view[i] = m.InvModelView[i];
+      }
+      Status = vgi_gbuffer_decode(Ctx, &ref, Position, Normal, ViewDepth, context.Stream);
+    }
+    if (Status == VGI_OK) {
+      const vgi_gbuffer gb{Position, Normal, targets.Roughness, Width, Height};
+      Render(context, gb, Diffuse, nullptr);
+    }
+    if (Status == VGI_OK) {
+      vgi_temporal_params t{};
+      t.indirect = Diffuse;
+      t.position = Position;
+      t.view_depth = ViewDepth;
+      t.history = Indirect[Frame & 1];
+      t.out = Indirect[(Frame + 1) & 1];
+      t.history_format = VGI_IMAGE_RGBA16F;
+      t.out_format = VGI_IMAGE_RGBA16F;
+      for (int i = 0; i < 16; i++) {
+        t.prev_projection[i] = m.PrevProjection[i];
+        t.prev_model_view[i] = m.PrevModelView[i];
+      }
+      t.width = Width;
+      t.height = Height;
+      Status = vgi_temporal_resolve(Ctx, &t, context.Stream);
+    }
+    if (Status == VGI_OK) {
+      const vgi_blur_params bx{Indirect[(Frame + 1) & 1], targets.Depth, Blur, VGI_IMAGE_RGBA16F,
+                               VGI_IMAGE_RGBA16F, Width, Height, 1};
+      Status = vgi_bilateral_blur(Ctx, &bx, context.Stream);
+    }
+    if (Status == VGI_OK) {
+      const vgi_blur_params by{Blur, targets.Depth, Final, VGI_IMAGE_RGBA16F, VGI_IMAGE_RGBA16F,
+                               Width, Height, 0};
+      Status = vgi_bilateral_blur(Ctx, &by, context.Stream);
+    }
+    if (Status != VGI_OK) {
+      Error = vgi_last_error();
+      return nullptr;
+    }
+    Frame++;
+    return Final;
+  }
+
+  // The reference's first-frame clear of the history (MegaPipeline.cpp:197-204).
+  void ResetHistory() {
+    if (Indirect[0]) Status = vgi_memset(Ctx, Indirect[Frame & 1], 0, (size_t)Width * Height * 8);
+  }
+
+  // Intermediate images of the last RenderFrame (device memory): world position and normal
+  // (float x3), view depth, the traced irradiance (float4), the blended result (RGBA16F,
+  // the next frame's history) and the image blurred along y (RGBA16F).
+  const float* PositionImage() const { return Position; }
+  const float* NormalImage() const { return Normal; }
+  const float* ViewDepthImage() const { return ViewDepth; }
+  const float* DiffuseImage() const { return Diffuse; }
+  const uint16_t* TemporalImage() const { return Indirect[Frame & 1]; }
+  const uint16_t* BlurImage() const { return Blur; }
+
   int LastStatus() const { return Status; }
   const std::string& LastError() const { return Error; }
 
 private:
+  int EnsureImages(uint32_t w, uint32_t h) {
+    if (!Ctx) return VGI_E_INVALID;
+    if (w == Width && h == Height && Final) return VGI_OK;
+    FreeImages();
+    const size_t px = (size_t)w * h;
+    const size_t bytes[8] = {px * 12, px * 12, px * 4, px * 16, px * 8, px * 8, px * 8, px * 8};
+    void** dst[8] = {(void**)&Position, (void**)&Normal, (void**)&ViewDepth, (void**)&Diffuse,
+                     (void**)&Indirect[0], (void**)&Indirect[1], (void**)&Blur, (void**)&Final};
+    for (int i = 0; i < 8; i++) {
+      const int rc = vgi_malloc(Ctx, dst[i], bytes[i] ? bytes[i] : 1);
+      if (rc != VGI_OK) {
+        FreeImages();
+        return rc;
+      }
+    }
+    Width = w;
+    Height = h;
+    Frame = 0;
+    const int rc = vgi_memset(Ctx, Indirect[0], 0, px * 8);
+    return rc != VGI_OK ? rc : vgi_memset(Ctx, Indirect[1], 0, px * 8);
+  }
+
+  void FreeImages() {
+    void* all[8] = {Position, Normal, ViewDepth, Diffuse, Indirect[0], Indirect[1], Blur, Final};
+    for (void* p : all)
+      if (p && Ctx) vgi_free(Ctx, p);
+    Position = Normal = ViewDepth = Diffuse = nullptr;
+    Indirect[0] = Indirect[1] = Blur = Final = nullptr;
+    Width = Height = 0;
+  }
+
   vgi_context* Ctx;
   vgi_scene* Scene;
   vgi_lights Lights{};
@@ -214,6 +344,15 @@ private:
   vgi_precision Precision = VGI_PREC_FP16;
   int Status = VGI_OK;
   std::string Error;
+  uint32_t Width = 0, Height = 0;
+  uint64_t Frame = 0;
+  float* Position = nullptr;
+  float* Normal = nullptr;
+  float* ViewDepth = nullptr;
+  float* Diffuse = nullptr;
+  uint16_t* Indirect[2] = {nullptr, nullptr};
+  uint16_t* Blur = nullptr;
+  uint16_t* Final = nullptr;
 };
 
 }  // namespace Foreground

@@ -1,7 +1,8 @@
 // Drives include/vgi_foreground.hpp the way Foreground/Renderer/MegaPipeline.cpp drives its
 // passes (RenderList with per-primitive model matrices, then the indirect pass).
 //   adapter_test nodevice : cache and error behaviour without a GPU
-//   adapter_test gpu      : two primitives voxelized through the adapter == the oracle
+//   adapter_test gpu      : two primitives voxelized through the adapter == the oracle, then
+//                           two RenderFrame calls whose every stage == the oracle chain
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +16,13 @@ extern "C" {
 struct orc_vox_stats { uint64_t a, b, c, d; };
 int orc_voxelize(const float*, uint32_t, const uint32_t*, uint32_t, const uint8_t*, const uint8_t*,
                  const float*, const float*, float, uint32_t, uint32_t*, orc_vox_stats*);
+void orc_gbuffer_decode_f32(const float*, const uint16_t*, const float*, const float*, uint32_t, uint32_t,
+                            float*, float*, float*);
+void orc_temporal(const float*, const float*, const float*, const float*, const float*, const float*, uint32_t,
+                  uint32_t, float*);
+void orc_bilateral_blur(const float*, const float*, uint32_t, uint32_t, uint32_t, float*);
+uint16_t orc_float_to_half(float);
+float orc_half_to_float(uint16_t);
 }
 
 #define CHECK(c)                                                      \
@@ -66,6 +74,115 @@ static void* dev_copy(vgi_context* c, const void* src, size_t bytes) {
   if (vgi_malloc(c, &p, bytes) != VGI_OK) return nullptr;
   vgi_memcpy_h2d(c, p, src, bytes);
   return p;
+}
+
+static std::vector<float> widen(const std::vector<uint16_t>& h) {
+  std::vector<float> f(h.size());
+  for (size_t i = 0; i < h.size(); i++) f[i] = orc_half_to_float(h[i]);
+  return f;
+}
+
+static std::vector<uint16_t> narrow(const std::vector<float>& f) {
+  std::vector<uint16_t> h(f.size());
+  for (size_t i = 0; i < f.size(); i++) h[i] = orc_float_to_half(f[i]);
+  return h;
+}
+
+template <class T>
+static std::vector<T> readback(vgi_context* c, const T* d, size_t count) {
+  std::vector<T> h(count);
+  vgi_memcpy_d2h(c, h.data(), d, count * sizeof(T));
+  return h;
+}
+
+// Two frames of CIndirectPass::RenderFrame on a camera in front of the voxelized
+// primitives (the reference's projection form, Camera.cpp:85-101, and a view that only
+// translates). Every stage must equal the oracle on that stage's inputs, and the second
+// frame must take the first frame's blended image as its history.
+static int frames(vgi_context* c, vgi_scene* s, const vgi_lights& L) {
+  const uint32_t W = 48, H = 32;
+  const size_t px = (size_t)W * H;
+  // depth 0.87 .. 0.90 puts the surface at view z -1.35 .. -1.68, inside the grid; normals
+  // face the camera with some noise
+  std::mt19937 rng(11);
+  std::uniform_real_distribution<float> N(-0.0005f, 0.0005f);
+  std::uniform_int_distribution<int> Q(-6000, 6000);
+  std::vector<float> depth(px);
+  std::vector<uint16_t> nrm(px * 4);
+  for (size_t i = 0; i < px; i++) {
+    depth[i] = 0.87f + 0.03f * (float)(i % W) / W + N(rng);
+    nrm[i * 4 + 0] = (uint16_t)(32768 + Q(rng));
+    nrm[i * 4 + 1] = (uint16_t)(32768 + Q(rng));
+    nrm[i * 4 + 2] = 65535;
+    nrm[i * 4 + 3] = 65535;
+  }
+  // P = [[a,0,0,0],[0,-b,0,0],[0,0,c,d],[0,0,-1,0]]; its inverse, column-major
+  const float yslope = std::tan(0.5f * 60.f * 3.14159265f / 180.f), aspect = (float)W / H;
+  const float n = 0.1f, f = 10.f, a = 1.f / (yslope * aspect), b = 1.f / yslope;
+  const float cc = (f + n) / (n - f), dd = -2.f * f * n / (f - n);
+  CIndirectFrameConstants m{};
+  m.InvProj[0] = 1.f / a;
+  m.InvProj[5] = -1.f / b;
+  m.InvProj[11] = 1.f / dd;  // row 3, column 2
+  m.InvProj[14] = -1.f;      // row 2, column 3
+  m.InvProj[15] = cc / dd;
+  const float eye[3] = {0.f, 0.f, 1.5f};
+  for (int i = 0; i < 16; i++) m.InvModelView[i] = (i % 5 == 0) ? 1.f : 0.f;
+  for (int k = 0; k < 3; k++) m.InvModelView[12 + k] = eye[k];
+  // last frame's camera: the same projection, the eye 0.02 to the left
+  for (int i = 0; i < 16; i++) m.PrevProjection[i] = m.PrevModelView[i] = 0.f;
+  m.PrevProjection[0] = a;
+  m.PrevProjection[5] = -b;
+  m.PrevProjection[10] = cc;
+  m.PrevProjection[11] = -1.f;
+  m.PrevProjection[14] = dd;
+  for (int i = 0; i < 4; i++) m.PrevModelView[i * 5] = 1.f;
+  m.PrevModelView[12] = -(eye[0] - 0.02f);
+  m.PrevModelView[13] = -eye[1];
+  m.PrevModelView[14] = -eye[2];
+
+  CGBufferTargets t;
+  t.Depth = (const float*)dev_copy(c, depth.data(), px * 4);
+  t.Normals = (const uint16_t*)dev_copy(c, nrm.data(), px * 8);
+  t.Width = W;
+  t.Height = H;
+  CIndirectPass ip(c, s);
+  ip.SetLights(L);
+  vgi_cone_params cones{6, 0, 60.f, 60.f, 10.f, 0.5f, 0.8f, 0.99f, 1.f, 1.f, {eye[0], eye[1], eye[2]}};
+  ip.SetCones(cones);
+  CRenderContext ctx;
+  std::vector<float> history(px * 4, 0.f);  // the first frame's history is cleared
+  for (int frame = 0; frame < 2; frame++) {
+    const uint16_t* fin = ip.RenderFrame(ctx, t, m);
+    CHECK(fin != nullptr && ip.LastStatus() == VGI_OK);
+    CHECK(vgi_sync(c, nullptr) == VGI_OK);
+    std::vector<float> rp(px * 3), rn(px * 3), rv(px);
+    orc_gbuffer_decode_f32(depth.data(), nrm.data(), m.InvProj, m.InvModelView, W, H, rp.data(), rn.data(),
+                           rv.data());
+    CHECK(readback(c, ip.PositionImage(), px * 3) == rp);
+    CHECK(readback(c, ip.NormalImage(), px * 3) == rn);
+    CHECK(readback(c, ip.ViewDepthImage(), px) == rv);
+    const std::vector<float> diffuse = readback(c, ip.DiffuseImage(), px * 4);
+    std::vector<float> taa(px * 4);
+    orc_temporal(diffuse.data(), rp.data(), rv.data(), history.data(), m.PrevProjection, m.PrevModelView, W,
+                 H, taa.data());
+    const std::vector<uint16_t> taa16 = narrow(taa);
+    CHECK(readback(c, ip.TemporalImage(), px * 4) == taa16);
+    std::vector<float> bx(px * 4), by(px * 4);
+    orc_bilateral_blur(widen(taa16).data(), depth.data(), W, H, 1, bx.data());
+    const std::vector<uint16_t> bx16 = narrow(bx);
+    CHECK(readback(c, ip.BlurImage(), px * 4) == bx16);
+    orc_bilateral_blur(widen(bx16).data(), depth.data(), W, H, 0, by.data());
+    CHECK(readback(c, fin, px * 4) == narrow(by));
+    size_t blended = 0;
+    for (size_t i = 0; i < px; i++) blended += taa[i * 4] != diffuse[i * 4];
+    std::printf("frame %d: %zu of %zu pixels blended with the history\n", frame, blended, px);
+    CHECK(frame == 0 || blended > px / 2);
+    history = widen(taa16);
+  }
+  vgi_free(c, (void*)t.Depth);
+  vgi_free(c, (void*)t.Normals);
+  return 0;
 }
 
 static int gpu() {
@@ -162,6 +279,7 @@ static int gpu() {
   std::printf("indirect alpha: min %.9g max %.9g non-finite %zu\n", amin, amax, nonfinite);
   // the diffuse alpha is a weighted mean of per-cone opacities in [0, 1] (weights rounded)
   CHECK(nonfinite == 0 && amin >= 0.f && amax <= 1.f + 1e-6f);
+  CHECK(frames(c, s, L) == 0);
   vgi_scene_release(s);
   vgi_destroy(c);
   std::printf("ADAPTER GPU OK\n");
