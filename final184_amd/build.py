@@ -22,6 +22,8 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "screen.hip", "abi.hip", "autotune.hip", "serve.hip"]
 
+HOST_SOURCES = ["gltf.cpp"]  # host-only C++ linked into libvgi.so
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -56,6 +58,10 @@ def build_vgi() -> str:
         obj = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(obj)
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        cmds.append([GXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     out = os.path.join(PKG, "libvgi.so")

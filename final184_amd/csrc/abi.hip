@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "gltf.hpp"
 #include "vgi_internal.hpp"
 
 using namespace vgi;
@@ -1184,6 +1185,148 @@ int vgi_trace(vgi_context* ctx, const vgi_scene* s, const vgi_gbuffer* gb, const
 
 int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream) {
   return trace_batch(ctx, items, count, stream);
+}
+
+// ---- glTF ingestion -----------------------------------------------------------
+
+struct vgi_gltf {
+  vgi::GltfScene scene;
+  vgi_context* ctx = nullptr;               // device copies belong to this context's device
+  std::vector<void*> dev_pos, dev_idx;      // per geometry
+};
+
+namespace {
+
+int gltf_finish(int rc, const std::string& err, vgi::GltfScene& scene, vgi_gltf** out) {
+  if (rc != VGI_OK) return fail(rc, "glTF: %s", err.c_str());
+  auto* g = new (std::nothrow) vgi_gltf;
+  if (!g) return fail(VGI_E_NOMEM, "out of host memory");
+  g->scene = std::move(scene);
+  *out = g;
+  return VGI_OK;
+}
+
+void gltf_free_device(vgi_gltf* g) {
+  if (!g->ctx) return;
+  // best effort on release: a failure here leaves nothing the caller could act on
+  (void)hipSetDevice(g->ctx->device);
+  for (void* p : g->dev_pos) (void)hipFree(p);
+  for (void* p : g->dev_idx) (void)hipFree(p);
+  g->dev_pos.clear();
+  g->dev_idx.clear();
+  g->ctx = nullptr;
+}
+
+}  // namespace
+
+int vgi_gltf_load_file(const char* path, uint32_t albedo_mode, vgi_gltf** out) {
+  if (!path || !out) return fail(VGI_E_INVALID, "NULL argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(VGI_E_INVALID, "glTF: cannot open %s", path);
+  std::vector<uint8_t> data;
+  uint8_t buf[1 << 16];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) data.insert(data.end(), buf, buf + n);
+  std::fclose(f);
+  std::string dir(path);
+  const size_t slash = dir.find_last_of('/');
+  dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash == 0 ? 1 : slash);
+  vgi::GltfScene scene;
+  std::string err;
+  const int rc = vgi::gltf_parse(data.data(), data.size(), dir, albedo_mode, scene, err);
+  return gltf_finish(rc, err, scene, out);
+}
+
+int vgi_gltf_load_memory(const void* data, size_t bytes, const char* base_dir, uint32_t albedo_mode,
+                         vgi_gltf** out) {
+  if (!data || !out) return fail(VGI_E_INVALID, "NULL argument");
+  *out = nullptr;
+  vgi::GltfScene scene;
+  std::string err;
+  const int rc = vgi::gltf_parse((const uint8_t*)data, bytes, base_dir ? base_dir : "", albedo_mode, scene, err);
+  return gltf_finish(rc, err, scene, out);
+}
+
+int vgi_gltf_get_info(const vgi_gltf* g, vgi_gltf_info* out) {
+  if (!g || !out) return fail(VGI_E_INVALID, "NULL argument");
+  *out = vgi_gltf_info{};
+  out->num_primitives = (uint32_t)g->scene.instances.size();
+  out->num_geometries = (uint32_t)g->scene.geometry.size();
+  out->skipped_primitives = g->scene.skipped_primitives;
+  for (const auto& in : g->scene.instances) out->num_triangles += g->scene.geometry[in.geometry].indices.size() / 3;
+  return VGI_OK;
+}
+
+int vgi_gltf_get_primitive(const vgi_gltf* g, uint32_t index, vgi_gltf_primitive* out) {
+  if (!g || !out) return fail(VGI_E_INVALID, "NULL argument");
+  if (index >= g->scene.instances.size())
+    return fail(VGI_E_INVALID, "primitive %u >= %zu", index, g->scene.instances.size());
+  const vgi::GltfInstance& in = g->scene.instances[index];
+  const vgi::GltfGeometry& ge = g->scene.geometry[in.geometry];
+  out->positions = ge.positions.data();
+  out->indices = ge.indices.data();
+  out->num_vertices = (uint32_t)(ge.positions.size() / 3);
+  out->num_triangles = (uint32_t)(ge.indices.size() / 3);
+  std::memcpy(out->model, in.model, sizeof(in.model));
+  std::memcpy(out->base_color, in.base_color, 4);
+  out->node = in.node;
+  out->mesh = ge.mesh;
+  out->primitive = ge.primitive;
+  out->material = in.material;
+  return VGI_OK;
+}
+
+int vgi_gltf_upload(vgi_context* ctx, vgi_gltf* g, vgi_mesh* meshes, uint32_t capacity, uint32_t* count) {
+  if (!ctx || !g || !count || (capacity && !meshes)) return fail(VGI_E_INVALID, "NULL argument");
+  const auto& sc = g->scene;
+  *count = (uint32_t)sc.instances.size();
+  if (capacity < sc.instances.size())
+    return fail(VGI_E_RANGE, "%zu primitives do not fit in %u entries", sc.instances.size(), capacity);
+  if (g->ctx && g->ctx != ctx) gltf_free_device(g);
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!g->ctx) {
+    g->dev_pos.assign(sc.geometry.size(), nullptr);
+    g->dev_idx.assign(sc.geometry.size(), nullptr);
+    g->ctx = ctx;
+    for (size_t i = 0; i < sc.geometry.size(); i++) {
+      const auto& ge = sc.geometry[i];
+      const size_t pb = std::max<size_t>(ge.positions.size() * 4, 4), ib = std::max<size_t>(ge.indices.size() * 4, 4);
+      hipError_t e = hipMalloc(&g->dev_pos[i], pb);
+      if (e == hipSuccess) e = hipMalloc(&g->dev_idx[i], ib);
+      if (e == hipSuccess && !ge.positions.empty())
+        e = hipMemcpy(g->dev_pos[i], ge.positions.data(), ge.positions.size() * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess && !ge.indices.empty())
+        e = hipMemcpy(g->dev_idx[i], ge.indices.data(), ge.indices.size() * 4, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        gltf_free_device(g);
+        return fail(e == hipErrorOutOfMemory ? VGI_E_NOMEM : VGI_E_HIP, "glTF upload: %s", hipGetErrorString(e));
+      }
+    }
+  }
+  for (size_t i = 0; i < sc.instances.size(); i++) {
+    const auto& in = sc.instances[i];
+    const auto& ge = sc.geometry[in.geometry];
+    vgi_mesh m{};
+    m.positions = (const float*)g->dev_pos[in.geometry];
+    m.position_stride = 12;
+    m.num_vertices = (uint32_t)(ge.positions.size() / 3);
+    m.indices = g->dev_idx[in.geometry];
+    m.index_type = VGI_INDEX_U32;
+    m.num_triangles = (uint32_t)(ge.indices.size() / 3);
+    m.albedo_rgb8 = nullptr;
+    std::memcpy(m.base_color, in.base_color, 4);
+    std::memcpy(m.model, in.model, sizeof(m.model));
+    meshes[i] = m;
+  }
+  return VGI_OK;
+}
+
+int vgi_gltf_release(vgi_gltf* g) {
+  if (!g) return fail(VGI_E_INVALID, "NULL argument");
+  gltf_free_device(g);
+  delete g;
+  return VGI_OK;
 }
 
 // ---- screen-space passes ----------------------------------------------------

@@ -433,6 +433,57 @@ int vgi_server_get_stats(vgi_server* server, vgi_server_stats* out);
 /* Drains, stops the threads and releases the resident scenes and streams. */
 int vgi_server_destroy(vgi_server* server);
 
+/* ---- glTF 2.0 ingestion (host side) ------------------------------------------ */
+/* Reads a .gltf (external or base64 data-URI buffers) or .glb file: the default scene's
+ * node tree (matrix or TRS, composed parent * local in double, rounded to float), every
+ * triangle primitive's POSITION (float VEC3, any byteStride) and indices (u8/u16/u32 or
+ * none), with strips and fans assembled into lists as Vulkan assembles them (strip
+ * triangle i = (i, i+1+i%2, i+2-i%2), fan = (i+1, i+2, 0)); points and lines are
+ * skipped. A mesh placed by several nodes is stored once. Replaces the mesh and node part
+ * of CglTFSceneImporter::ImportFile (Foreground/SceneGraph/glTFSceneImporter.cpp:210-326
+ * GetMesh, :333-372 Visit, :387-425 ImportFile).
+ * Albedo: VGI_GLTF_ALBEDO_REFERENCE reproduces the reference importer, whose material
+ * lookup of "baseColor" (:180) never matches glTF's "baseColorFactor", so every material
+ * is white, and whose voxelizer draws only primitives that have a material
+ * (VoxelizeRenderer.cpp:40,103); VGI_GLTF_ALBEDO_FACTOR uses baseColorFactor (glTF's
+ * default material, white, for primitives without one). Colours are rounded to RGB8. */
+typedef struct vgi_gltf vgi_gltf;
+enum { VGI_GLTF_ALBEDO_REFERENCE = 0, VGI_GLTF_ALBEDO_FACTOR = 1 };
+
+typedef struct vgi_gltf_info {
+  uint32_t num_primitives;     /* placed primitives (node x mesh primitive) */
+  uint32_t num_geometries;     /* distinct mesh primitives */
+  uint32_t skipped_primitives; /* points / lines */
+  uint32_t pad;
+  uint64_t num_triangles;      /* over placed primitives */
+} vgi_gltf_info;
+
+typedef struct vgi_gltf_primitive {
+  const float* positions;   /* host memory owned by the handle: xyz per vertex, object space */
+  const uint32_t* indices;  /* host memory owned by the handle: 3 per triangle */
+  uint32_t num_vertices;
+  uint32_t num_triangles;
+  float model[12];          /* row-major 3x4 world transform of the node */
+  uint8_t base_color[4];    /* RGBA8 */
+  int32_t node;
+  int32_t mesh;
+  int32_t primitive;
+  int32_t material;         /* -1: none */
+} vgi_gltf_primitive;
+
+int vgi_gltf_load_file(const char* path, uint32_t albedo_mode, vgi_gltf** out);
+/* base_dir: directory for relative buffer URIs (NULL: only embedded buffers). */
+int vgi_gltf_load_memory(const void* data, size_t bytes, const char* base_dir, uint32_t albedo_mode,
+                         vgi_gltf** out);
+int vgi_gltf_get_info(const vgi_gltf* g, vgi_gltf_info* out);
+int vgi_gltf_get_primitive(const vgi_gltf* g, uint32_t index, vgi_gltf_primitive* out);
+/* Copies every geometry to device memory of ctx's device (kept by the handle until
+ * vgi_gltf_release; a second call with the same ctx reuses it) and writes one vgi_mesh
+ * per placed primitive (u32 indices, the node's model matrix, base colour) into
+ * meshes[0..capacity); *count receives the number of primitives. Synchronous. */
+int vgi_gltf_upload(vgi_context* ctx, vgi_gltf* g, vgi_mesh* meshes, uint32_t capacity, uint32_t* count);
+int vgi_gltf_release(vgi_gltf* g);
+
 /* ---- parity readback (synchronous) ----------------------------------------- */
 /* Accumulator as n^3 voxels x 8 uint32 (x fastest): R sum, G sum, B sum, count,
  * sum(nx+1024), sum(ny+1024), sum(nz+1024), 0 (normals quantized to 1/1024). */
