@@ -85,6 +85,20 @@ class TemporalParams(C.Structure):
                 ("prev_model_view", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
+GLTF_ALBEDO_REFERENCE, GLTF_ALBEDO_FACTOR = 0, 1
+
+
+class GltfInfo(C.Structure):
+    _fields_ = [("num_primitives", C.c_uint32), ("num_geometries", C.c_uint32),
+                ("skipped_primitives", C.c_uint32), ("pad", C.c_uint32), ("num_triangles", C.c_uint64)]
+
+
+class GltfPrimitive(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("num_vertices", C.c_uint32),
+                ("num_triangles", C.c_uint32), ("model", C.c_float * 12), ("base_color", C.c_uint8 * 4),
+                ("node", C.c_int32), ("mesh", C.c_int32), ("primitive", C.c_int32), ("material", C.c_int32)]
+
+
 class BlurParams(C.Structure):
     _fields_ = [("color", C.c_void_p), ("depth", C.c_void_p), ("out", C.c_void_p),
                 ("color_format", C.c_uint32), ("out_format", C.c_uint32), ("width", C.c_uint32),
@@ -199,6 +213,12 @@ SIGNATURES = [
     ("vgi_gbuffer_decode", C.c_int, [_P, C.POINTER(GBufferRef), _P, _P, _P, _P]),
     ("vgi_temporal_resolve", C.c_int, [_P, C.POINTER(TemporalParams), _P]),
     ("vgi_bilateral_blur", C.c_int, [_P, C.POINTER(BlurParams), _P]),
+    ("vgi_gltf_load_file", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
+    ("vgi_gltf_load_memory", C.c_int, [_P, C.c_size_t, C.c_char_p, C.c_uint32, C.POINTER(_P)]),
+    ("vgi_gltf_get_info", C.c_int, [_P, C.POINTER(GltfInfo)]),
+    ("vgi_gltf_get_primitive", C.c_int, [_P, C.c_uint32, C.POINTER(GltfPrimitive)]),
+    ("vgi_gltf_upload", C.c_int, [_P, _P, C.POINTER(Mesh), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("vgi_gltf_release", C.c_int, [_P]),
     ("vgi_autotune_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.POINTER(MipsItem),
                                      C.POINTER(TraceItem), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.POINTER(AutotuneReport), _P]),
@@ -480,6 +500,61 @@ def _trace_items(jobs):
         items[i].out_diffuse = out_d.ptr
         items[i].out_specular = out_s.ptr if out_s is not None else None
     return items
+
+
+class Gltf:
+    """A glTF 2.0 file read by the library (host side; no device needed until upload)."""
+
+    def __init__(self, path: str | None = None, data: bytes | None = None, base_dir: str | None = None,
+                 albedo_mode: int = GLTF_ALBEDO_REFERENCE):
+        h = C.c_void_p()
+        if path is not None:
+            _check(lib.vgi_gltf_load_file(path.encode(), albedo_mode, C.byref(h)))
+        else:
+            buf = C.create_string_buffer(bytes(data), len(data))
+            _check(lib.vgi_gltf_load_memory(buf, len(data), base_dir.encode() if base_dir else None,
+                                            albedo_mode, C.byref(h)))
+        self.handle = h.value
+
+    def info(self) -> GltfInfo:
+        i = GltfInfo()
+        _check(lib.vgi_gltf_get_info(self.handle, C.byref(i)))
+        return i
+
+    def primitive(self, index: int) -> dict:
+        """Copies of one placed primitive's arrays and attributes."""
+        p = GltfPrimitive()
+        _check(lib.vgi_gltf_get_primitive(self.handle, index, C.byref(p)))
+        pos = np.ctypeslib.as_array(C.cast(p.positions, C.POINTER(C.c_float)), (p.num_vertices * 3,)).copy() \
+            if p.num_vertices else np.zeros(0, np.float32)
+        idx = np.ctypeslib.as_array(C.cast(p.indices, C.POINTER(C.c_uint32)), (p.num_triangles * 3,)).copy() \
+            if p.num_triangles else np.zeros(0, np.uint32)
+        return dict(positions=pos.reshape(-1, 3), indices=idx.reshape(-1, 3), model=np.array(p.model[:], np.float32),
+                    base_color=tuple(p.base_color[:]), node=p.node, mesh=p.mesh, primitive=p.primitive,
+                    material=p.material)
+
+    def primitives(self) -> list:
+        return [self.primitive(i) for i in range(self.info().num_primitives)]
+
+    def upload(self, ctx: "Context") -> list:
+        """Device copies of the geometry and one vgi_mesh per placed primitive."""
+        n = self.info().num_primitives
+        arr = (Mesh * max(1, n))()
+        cnt = C.c_uint32()
+        _check(lib.vgi_gltf_upload(ctx.handle, self.handle, arr, n, C.byref(cnt)))
+        return [arr[i] for i in range(cnt.value)]
+
+    def close(self) -> None:
+        if self.handle:
+            _check(lib.vgi_gltf_release(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.vgi_gltf_release(self.handle)
+        except Exception:
+            pass
 
 
 class Context:

@@ -54,6 +54,8 @@ STRUCTS = {
     "vgi_gbuffer_ref": (vgi.GBufferRef, [f for f, _ in vgi.GBufferRef._fields_]),
     "vgi_temporal_params": (vgi.TemporalParams, [f for f, _ in vgi.TemporalParams._fields_]),
     "vgi_blur_params": (vgi.BlurParams, [f for f, _ in vgi.BlurParams._fields_]),
+    "vgi_gltf_info": (vgi.GltfInfo, [f for f, _ in vgi.GltfInfo._fields_]),
+    "vgi_gltf_primitive": (vgi.GltfPrimitive, [f for f, _ in vgi.GltfPrimitive._fields_]),
 }
 
 
@@ -110,6 +112,11 @@ def test_null_arguments_are_rejected_with_a_message():
     assert vgi.lib.vgi_gbuffer_decode(None, None, None, None, None, None) == -1
     assert vgi.lib.vgi_temporal_resolve(None, None, None) == -1
     assert vgi.lib.vgi_bilateral_blur(None, None, None) == -1
+    assert vgi.lib.vgi_gltf_load_file(None, 0, None) == -1
+    assert vgi.lib.vgi_gltf_load_memory(None, 0, None, 0, None) == -1
+    assert vgi.lib.vgi_gltf_get_info(None, None) == -1
+    assert vgi.lib.vgi_gltf_upload(None, None, None, 0, None) == -1
+    assert vgi.lib.vgi_gltf_release(None) == -1
     assert vgi.lib.vgi_server_destroy(None) == 0
     assert vgi.lib.vgi_now() > 0
 
