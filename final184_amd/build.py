@@ -91,11 +91,33 @@ def build_oracle() -> str:
     return out
 
 
+REFERENCE_GLTF = "/root/reference/Foreground/SceneGraph"
+
+
+def build_gltf_ref() -> str | None:
+    """The reference's own glTF parser (tinygltf + json.hpp + stb_image, vendored in the
+    reference) compiled from where it lies, with oracle/gltf_ref.cpp driving it as the
+    reference's importer does: a checker for the product's glTF reader. Built into
+    oracle/_ref/ only where the reference sources exist; tests skip without it."""
+    src = os.path.join(REFERENCE_GLTF, "tiny_gltf.cpp")
+    if not os.path.exists(src):
+        return None
+    rdir = os.path.join(ORACLE_DIR, "_ref")
+    os.makedirs(rdir, exist_ok=True)
+    out = os.path.join(rdir, "libgltf_ref.so")
+    tmp = out + ".tmp"
+    _run([GXX, "-O1", "-std=c++14", "-fPIC", "-shared", "-w", "-I", REFERENCE_GLTF,
+          os.path.join(ORACLE_DIR, "gltf_ref.cpp"), src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_scenegen()
     build_oracle()
+    build_gltf_ref()
     build_vgi()
 
 

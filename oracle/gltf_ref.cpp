@@ -1,0 +1,217 @@
+// gltf_ref.cpp — test infrastructure: the reference's own glTF parser (tinygltf, vendored
+// in the reference at Foreground/SceneGraph/tiny_gltf.{h,cpp} with json.hpp and stb_image)
+// driven the way the reference's importer drives it, flattened to the same form as the
+// product's vgi_gltf_* entry points, so that tests can compare the two on the same files.
+// Built only where the reference sources exist (final184_amd/build.py: build_gltf_ref,
+// output oracle/_ref/libgltf_ref.so); never linked into the product.
+//
+// Restated from the importer (Foreground/SceneGraph/glTFSceneImporter.cpp):
+//   ImportFile (:387-425): .glb -> LoadBinaryFromFile, else LoadASCIIFromFile; the default
+//     scene's root nodes are visited in order. (A file without "scene" leaves tinygltf's
+//     defaultScene at -1, which the importer would index; here scene 0 is used.)
+//   Visit (:333-372): a node's transform is its matrix, else scale, rotation
+//     (w, x, y, z), translation; children are visited depth first, in order.
+//   GetMesh (:210-326): per primitive, the POSITION accessor read through its bufferView
+//     with the view's byteStride or the accessor's packed stride, the index accessor if
+//     any (element count = index count), the primitive's topology, and the material.
+//   GetMaterial (:170-210): the albedo lookup is values["baseColor"], which tinygltf never
+//     fills (it stores glTF's "baseColorFactor"), so the albedo is (1, 1, 1, 1).
+//   CVoxelizeRenderer::Render (Foreground/Renderer/VoxelizeRenderer.cpp:103) draws only
+//     primitives whose material is a CBasicMaterial, i.e. that have a material.
+// Vertex assembly of lists, strips and fans follows the Vulkan specification (the
+// reference draws through Vulkan): strip triangle i = (i, i+1+(i&1), i+2-(i&1)), fan
+// triangle i = (i+1, i+2, 0).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tiny_gltf.h"
+
+namespace {
+
+struct Prim {
+  std::vector<float> pos;
+  std::vector<uint32_t> idx;
+  float model[12];
+  uint8_t rgba[4];
+  int material;
+};
+
+struct Scene {
+  std::vector<Prim> prims;
+};
+
+// 3x4 affine transforms as 4x4 row-major doubles
+struct M4 {
+  double a[4][4];
+};
+
+M4 identity() {
+  M4 m{};
+  for (int i = 0; i < 4; i++) m.a[i][i] = 1;
+  return m;
+}
+
+M4 mul(const M4& x, const M4& y) {
+  M4 r{};
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++) r.a[i][j] += x.a[i][k] * y.a[k][j];
+  return r;
+}
+
+// rotation of a unit quaternion (w, v): R = I + 2w[v]x + 2[v]x^2
+M4 rotation(double w, double x, double y, double z) {
+  const double K[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+  M4 r = identity();
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double k2 = 0;
+      for (int k = 0; k < 3; k++) k2 += K[i][k] * K[k][j];
+      r.a[i][j] += 2 * w * K[i][j] + 2 * k2;
+    }
+  return r;
+}
+
+M4 local_transform(const tinygltf::Node& n) {
+  M4 m = identity();
+  if (!n.matrix.empty()) {  // column-major in the file
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 4; r++) m.a[r][c] = n.matrix[c * 4 + r];
+    return m;
+  }
+  M4 s = identity(), rot = identity(), t = identity();
+  if (!n.scale.empty())
+    for (int i = 0; i < 3; i++) s.a[i][i] = n.scale[i];
+  if (!n.rotation.empty()) rot = rotation(n.rotation[3], n.rotation[0], n.rotation[1], n.rotation[2]);
+  if (!n.translation.empty())
+    for (int i = 0; i < 3; i++) t.a[i][3] = n.translation[i];
+  return mul(t, mul(rot, s));
+}
+
+uint32_t read_index(const unsigned char* p, int comp) {
+  if (comp == TINYGLTF_COMPONENT_TYPE_UNSIGNED_BYTE) return p[0];
+  if (comp == TINYGLTF_COMPONENT_TYPE_UNSIGNED_SHORT) {
+    uint16_t v;
+    std::memcpy(&v, p, 2);
+    return v;
+  }
+  uint32_t v;
+  std::memcpy(&v, p, 4);
+  return v;
+}
+
+struct Visitor {
+  const tinygltf::Model& model;
+  int albedo_mode;
+  Scene& out;
+
+  void visit(int ni, const M4& parent) {
+    const tinygltf::Node& n = model.nodes[ni];
+    const M4 world = mul(parent, local_transform(n));
+    if (n.mesh >= 0) {
+      const tinygltf::Mesh& mesh = model.meshes[n.mesh];
+      for (const tinygltf::Primitive& p : mesh.primitives) {
+        if (p.mode < TINYGLTF_MODE_TRIANGLES) continue;  // no triangles to voxelize
+        if (albedo_mode == 0 && p.material < 0) continue;
+        Prim pr;
+        pr.material = p.material;
+        // POSITION through its binding
+        const tinygltf::Accessor& acc = model.accessors[p.attributes.at("POSITION")];
+        const tinygltf::BufferView& bv = model.bufferViews[acc.bufferView];
+        size_t stride = bv.byteStride ? bv.byteStride : (size_t)acc.ByteStride(bv);
+        const unsigned char* base = model.buffers[bv.buffer].data.data() + bv.byteOffset + acc.byteOffset;
+        pr.pos.resize(acc.count * 3);
+        for (size_t e = 0; e < acc.count; e++) std::memcpy(&pr.pos[e * 3], base + e * stride, 12);
+        std::vector<uint32_t> order;
+        if (p.indices >= 0) {
+          const tinygltf::Accessor& ia = model.accessors[p.indices];
+          const tinygltf::BufferView& iv = model.bufferViews[ia.bufferView];
+          const unsigned char* ib = model.buffers[iv.buffer].data.data() + iv.byteOffset + ia.byteOffset;
+          const size_t isz = tinygltf::GetComponentSizeInBytes(ia.componentType);
+          for (size_t e = 0; e < ia.count; e++) order.push_back(read_index(ib + e * isz, ia.componentType));
+        } else {
+          for (size_t e = 0; e < acc.count; e++) order.push_back((uint32_t)e);
+        }
+        const size_t nv = order.size();
+        if (p.mode == TINYGLTF_MODE_TRIANGLES) {
+          for (size_t t = 0; t + 3 <= nv; t += 3) {
+            pr.idx.push_back(order[t]);
+            pr.idx.push_back(order[t + 1]);
+            pr.idx.push_back(order[t + 2]);
+          }
+        } else {
+          for (size_t i = 0; i + 2 < nv; i++) {
+            uint32_t v[3];
+            if (p.mode == TINYGLTF_MODE_TRIANGLE_STRIP) {
+              v[0] = order[i];
+              v[1] = order[i + 1 + (i & 1)];
+              v[2] = order[i + 2 - (i & 1)];
+            } else {
+              v[0] = order[i + 1];
+              v[1] = order[i + 2];
+              v[2] = order[0];
+            }
+            pr.idx.insert(pr.idx.end(), v, v + 3);
+          }
+        }
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 4; c++) pr.model[r * 4 + c] = (float)world.a[r][c];
+        double f[4] = {1, 1, 1, 1};
+        if (p.material >= 0) {
+          const tinygltf::Material& m = model.materials[p.material];
+          const auto it = m.values.find(albedo_mode == 0 ? "baseColor" : "baseColorFactor");
+          if (it != m.values.end() && it->second.number_array.size() == 4)
+            for (int k = 0; k < 4; k++) f[k] = it->second.number_array[k];
+        }
+        for (int k = 0; k < 4; k++) pr.rgba[k] = (uint8_t)std::lround(std::fmin(std::fmax(f[k], 0.0), 1.0) * 255.0);
+        out.prims.push_back(std::move(pr));
+      }
+    }
+    for (int c : n.children) visit(c, world);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* gref_load(const char* path, int albedo_mode, char* err, int errlen) {
+  tinygltf::Model model;
+  tinygltf::TinyGLTF loader;
+  std::string e, w;
+  const std::string p(path);
+  const bool glb = p.size() >= 3 && p.compare(p.size() - 3, 3, "glb") == 0;
+  const bool ok = glb ? loader.LoadBinaryFromFile(&model, &e, &w, p) : loader.LoadASCIIFromFile(&model, &e, &w, p);
+  if (!ok || !e.empty()) {
+    if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", e.c_str());
+    return nullptr;
+  }
+  auto* s = new Scene;
+  if (!model.scenes.empty()) {
+    const int si = model.defaultScene >= 0 ? model.defaultScene : 0;
+    Visitor v{model, albedo_mode, *s};
+    for (int n : model.scenes[si].nodes) v.visit(n, identity());
+  }
+  return s;
+}
+
+int gref_count(void* h) { return (int)static_cast<Scene*>(h)->prims.size(); }
+
+int gref_get(void* h, int i, const float** pos, int* nv, const uint32_t** idx, int* nt, float* model,
+             uint8_t* rgba, int* material) {
+  const Prim& p = static_cast<Scene*>(h)->prims.at((size_t)i);
+  *pos = p.pos.data();
+  *nv = (int)(p.pos.size() / 3);
+  *idx = p.idx.data();
+  *nt = (int)(p.idx.size() / 3);
+  std::memcpy(model, p.model, sizeof(p.model));
+  std::memcpy(rgba, p.rgba, 4);
+  *material = p.material;
+  return 0;
+}
+
+void gref_free(void* h) { delete static_cast<Scene*>(h); }
+
+}  // extern "C"
