@@ -24,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# one wave64 FP32 VALU instruction per CU per clock (4 SIMD16 units, 4 cycles each),
+# 256 CUs at 2.4 GHz; the 157.3 TFLOP/s vector peak counts packed FP32 (two FMAs a lane)
+VALU_PEAK_GIPS = 256 * 2.4
 
 
 def log(msg: str) -> None:
@@ -321,8 +324,9 @@ def _counter_rows(outdir):
 
 def measure_traffic(args):
     """HBM bytes per trace launch from two rocprofv3 --pmc runs of the probe (FETCH_SIZE,
-    then WRITE_SIZE), each scaled by what the counter reads for the 1 GiB copy. Runs in
-    child processes before this process touches the GPU."""
+    then WRITE_SIZE), each scaled by what the counter reads for the 1 GiB copy, and a third
+    run for the launch's VALU instruction count and L2 hit rate. Runs in child processes
+    before this process touches the GPU."""
     import re
     import shutil
     import subprocess
@@ -334,20 +338,33 @@ def measure_traffic(args):
     for k in WORKLOAD_ARGS:
         wl += [f"--{k}", str(getattr(args, k))]
     per = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    # one pass per traffic counter, then the VALU instruction count and L2 hits / misses
+    for counters in (("FETCH_SIZE",), ("WRITE_SIZE",), ("SQ_INSTS_VALU", "TCC_HIT_sum", "TCC_MISS_sum")):
         d = tempfile.mkdtemp(prefix="vgi_pmc_", dir="/tmp")
-        cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--kernel-include-regex",
+        cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", *counters, "--kernel-include-regex",
                "k_trace|k_copy", "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                os.path.abspath(__file__), "--traffic-probe", *wl]
         log(f"traffic: {' '.join(cmd)}")
         r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, capture_output=True,
                            text=True, timeout=200)
-        rows = [x for x in _counter_rows(d) if x.get("Counter_Name") == counter]
+        all_rows = _counter_rows(d)
         shutil.rmtree(d, ignore_errors=True)
-        if not rows:
-            return None, f"{counter}: no counter rows (rocprofv3 exit {r.returncode}: {r.stderr[-300:]})"
-        per[counter] = rows
+        for counter in counters:
+            rows = [x for x in all_rows if x.get("Counter_Name") == counter]
+            if not rows:
+                return None, f"{counter}: no counter rows (rocprofv3 exit {r.returncode}: {r.stderr[-300:]})"
+            per[counter] = rows
     out = {}
+    for counter in ("SQ_INSTS_VALU", "TCC_HIT_sum", "TCC_MISS_sum"):
+        rows = per.pop(counter)
+        for prec, tag in (("fp16", "1"), ("fp32", "0")):
+            vals = [float(x["Counter_Value"]) for x in rows if re.search(r"k_trace<" + tag + r"\b", x["Kernel_Name"])]
+            if vals:
+                out.setdefault(prec, {})[counter] = float(np.mean(vals))
+    for prec in ("fp16", "fp32"):
+        o = out.get(prec, {})
+        if "TCC_HIT_sum" in o and "TCC_MISS_sum" in o:
+            o["l2_hit_rate"] = o["TCC_HIT_sum"] / max(o["TCC_HIT_sum"] + o["TCC_MISS_sum"], 1.0)
     for counter, rows in per.items():
         copy = [float(x["Counter_Value"]) for x in rows if "k_copy" in x["Kernel_Name"]]
         if not copy or copy[0] <= 0:
@@ -505,6 +522,13 @@ def main() -> int:
                      "fp32_traffic": _traffic_bytes(traffic, base["prec"]),
                      "traffic_counters": traffic if traffic else traffic_err},
     }
+    for prec, r, key in ((head["prec"], head, "valu"), (base["prec"], base, "fp32_valu")):
+        t = (traffic or {}).get(prec, {})
+        if "SQ_INSTS_VALU" in t:
+            rate = t["SQ_INSTS_VALU"] / (r["trace_ms"] * 1e-3) / 1e9
+            line["roofline"][key] = {"bound": "valu", "instructions": t["SQ_INSTS_VALU"], "achieved": rate,
+                                     "peak": VALU_PEAK_GIPS, "unit": "G wave64 instr/s", "frac": rate / VALU_PEAK_GIPS,
+                                     "l2_hit_rate": t.get("l2_hit_rate")}
     try:
         line["screen_passes"] = run_screen(ctx, vgi, sg, w, mesh, gb.rough, args.steps)
     except Exception as e:  # the headline line is still printed

@@ -213,6 +213,23 @@ __device__ __forceinline__ void gst4(void* p, uint4 v) {
   *(VGI_GLOBAL vgi_u32x4*)p = vgi_u32x4{v.x, v.y, v.z, v.w};
 }
 // whole records (size a multiple of 16 bytes, 16-byte aligned) as 16-byte accesses
+// A record at a wave-uniform address, read through the scalar data cache (s_load):
+// the constant address space tells the compiler the data is read-only for the kernel.
+template <class T>
+__device__ __forceinline__ T sld_rec(const T* p) {
+  static_assert(sizeof(T) % 16 == 0, "record size");
+  struct Raw {
+    uint4 w[sizeof(T) / 16];
+  } r;
+  const __attribute__((address_space(4))) vgi_u32x4* q = (const __attribute__((address_space(4))) vgi_u32x4*)p;
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 16; i++) {
+    const vgi_u32x4 v = q[i];
+    r.w[i] = make_uint4(v.x, v.y, v.z, v.w);
+  }
+  return __builtin_bit_cast(T, r);
+}
+
 template <class T>
 __device__ __forceinline__ T gld_rec(const T* p) {
   static_assert(sizeof(T) % 16 == 0, "record size");

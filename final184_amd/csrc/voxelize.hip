@@ -193,7 +193,13 @@ __device__ __forceinline__ uint32_t first_item(const VoxBatch& B, const VoxScene
 // ---------------------------------------------------------------------------
 
 __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
-  const MeshDev M = gld_rec(P.meshes + find_mesh(P.meshes, P.nmesh, t));
+  // the mesh descriptor: through the scalar cache when the wavefront's triangles share
+  // one mesh (the usual case), which shortens the fetch chain ahead of the index loads
+  const uint32_t mi = find_mesh(P.meshes, P.nmesh, t);
+  const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
+  MeshDev M;
+  if (__all(mi == mi0)) M = sld_rec(P.meshes + mi0);
+  else M = gld_rec(P.meshes + mi);
   const uint32_t lt = t - M.tri_base;
   uint32_t vid[3];
   if (M.idx_type == VGI_INDEX_U32) {
@@ -348,9 +354,18 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
     }
   }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < kSlotTable; e += blockDim.x) {
-    const uint32_t key = s_key[e];
-    if (key != kNoKey) s_base[e] = gatomic_add(P.bin_small + key, s_cnt[e]);
+  {  // reserve every listed bin's run: all four returning atomics in flight together
+    static_assert(kSlotTable == 4 * kBlockTris, "four table entries per thread");
+    uint32_t key[4], base[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const uint32_t e = threadIdx.x + (uint32_t)r * kBlockTris;
+      key[r] = s_key[e];
+      base[r] = key[r] != kNoKey ? gatomic_add(P.bin_small + key[r], s_cnt[e]) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (key[r] != kNoKey) s_base[threadIdx.x + (uint32_t)r * kBlockTris] = base[r];
   }
   __syncthreads();
   uint32_t nslot = 0;
@@ -630,9 +645,13 @@ struct TileLayout {
   static constexpr int ENTRIES = (TS - 1) * (1 + PY + PZ) + 1;
 };
 
+// Occupancy: a work item spends most of its time waiting on its chain of dependent loads
+// (work map, bin tables, pair ids, triangle records), so resident workgroups are what
+// hides it. The 8^3 tile needs 25.5 KB of LDS (6 workgroups per CU); capping registers at
+// 80 (6 waves per SIMD, no spills) lets all 6 be resident instead of 4 at 103 registers.
 template <int TS>
-__global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes,
-                                                  uint32_t nwork) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 6 : 1)))
+void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
   using L = TileLayout<TS>;
@@ -760,13 +779,14 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxBatch Bt, const VoxScene* _
             gst4(dst, make_uint4(r, g, b, c));
             gst4(dst + 4, make_uint4(nx, ny, nz, 0u));
           } else {
-            gatomic_add(dst + 0, r);
-            gatomic_add(dst + 1, g);
-            gatomic_add(dst + 2, b);
-            gatomic_add(dst + 3, c);
-            gatomic_add(dst + 4, nx);
-            gatomic_add(dst + 5, ny);
-            gatomic_add(dst + 6, nz);
+            // word pairs as 64-bit adds: with fewer than 2^21 triangles per voxel no 32-bit
+            // word's total reaches 2^32 (R, G, B <= 255 each, normals <= 2048 each), so no
+            // carry crosses into the upper word and the pairs add exactly as single words
+            unsigned long long* d64 = (unsigned long long*)dst;
+            gatomic_add(d64 + 0, (unsigned long long)r | ((unsigned long long)g << 32));
+            gatomic_add(d64 + 1, (unsigned long long)b | ((unsigned long long)c << 32));
+            gatomic_add(d64 + 2, (unsigned long long)nx | ((unsigned long long)ny << 32));
+            gatomic_add(d64 + 3, (unsigned long long)nz);
           }
         }
         const unsigned long long bits = __ballot(c != 0);  // bit j: voxel l - lane + j
