@@ -194,7 +194,7 @@ struct vgi_scene {
   uint32_t* mask = nullptr; // 16 words per 8^3 brick
   size_t acc_bytes = 0;
   size_t mask_bytes = 0;
-  void* bin_packed = nullptr;  // items << 32 | pairs per bin
+  void* bin_packed = nullptr;  // work units << 32 | pairs per bin
   size_t bin_packed_bytes = 0;
   uint32_t prec = 0;
   void* level[VGI_MAX_LEVELS] = {};
@@ -534,6 +534,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     S.chunk = tn.chunk_tris;
     S.pairs_cap = 0;
     S.bin_base = (uint32_t)p.bin_base;
+    S.unit_items = (ts == 8 && p.bin_shift == 3) ? kUnitItems : 1u;
     if (multi) {
       for (uint32_t b = 0; b < p.tri_blocks; b++) htb[tb++] = make_uint2(i, b * kBlockTris);
       for (uint32_t b = 0; b < p.bin_blocks; b++) hbb[bb++] = make_uint2(i, b * kBlockBins);

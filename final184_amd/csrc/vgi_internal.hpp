@@ -20,6 +20,7 @@ constexpr int kNormalBias = 1024;      // quantized normal offset in the accumul
 constexpr int kSmallBins = 8;          // a triangle whose bin box has <= 8 bins keeps its slots
 constexpr uint32_t kBlockTris = 256;   // triangles per setup / emit workgroup
 constexpr uint32_t kBlockBins = 256;   // bins per bins / workmap workgroup
+constexpr uint32_t kUnitItems = 16;    // chunks one voxelize workgroup takes in turn (8^3 bins)
 
 // Per-mesh descriptor, device resident.
 struct MeshDev {
@@ -69,7 +70,7 @@ struct VoxScene {
   uint32_t* bin_big;               // nbins: pairs from large triangles
   uint32_t* big_cursor;            // nbins
   uint32_t* zero_list;             // bins with more than one work item
-  unsigned long long* bin_packed;  // nbins: work items << 32 | pairs (kept with the scene)
+  unsigned long long* bin_packed;  // nbins: work units << 32 | pairs (kept with the scene)
   uint32_t* pairs;                 // triangle ids grouped by bin
   uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
   uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
@@ -84,6 +85,8 @@ struct VoxScene {
   uint32_t chunk;                  // triangles per work item (<= kMaxChunk)
   uint32_t pairs_cap;
   uint32_t bin_base;               // first bin of this scene in the batch-wide arrays
+  uint32_t unit_items;             // work items (of chunk triangles) per work unit: 1, or
+                                   // kUnitItems when the bin is one 8^3 tile
 };
 
 struct VoxBatch {
@@ -92,8 +95,8 @@ struct VoxBatch {
   const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
   unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
   unsigned long long* scan_all;     // exclusive scan of packed_all
-  unsigned long long* totals;       // per scene: items << 32 | pairs; [nscenes] = batch total
-  uint2* work_map;                  // (scene, bin) of every work item
+  unsigned long long* totals;       // per scene: units << 32 | pairs; [nscenes] = batch total
+  uint2* work_map;                  // (scene, bin) of every work unit
   uint32_t nscenes;
   uint32_t total_bins;
 };

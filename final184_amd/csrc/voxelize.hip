@@ -18,20 +18,23 @@
 //                   distinct bin; the slot is kept); larger triangles go to a list.
 //   2 k_big<false>  one workgroup per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
-//                   packed as (items << 32 | pairs); bins with > 1 item are listed.
+//                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
+//                   bins with > 1 unit are listed.
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
-//                   offset and its first work item at once; k_totals per scene.
-//   5 k_workmap     work item -> (scene, bin);
+//                   offset and its first work unit at once; k_totals per scene.
+//   5 k_workmap     work unit -> (scene, bin);
 //     k_emit_small  scatter small triangles to their kept slots (no atomics);
 //     k_big<true>   large triangles append after the small ones of each bin.
-//   6 k_zero        clear the accumulator and occupancy mask of multi-item bins.
-//   7 k_voxelize    one workgroup per work item of the batch; per item: chunk
+//   6 k_zero        clear the accumulator and occupancy mask of bins with several units.
+//   7 k_voxelize    one workgroup per work unit of the batch; per work item (chunk) of it:
 //                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
 //                   projection spread over the workgroup, each column tests its 1-4
 //                   candidate voxels with the exact separating-axis test and adds packed
-//                   64-bit increments into a per-tile LDS accumulator; flush: occupancy
-//                   mask + the records of occupied voxels, by plain stores for single-item
-//                   bins and by integer atomics otherwise.
+//                   64-bit increments into a per-tile LDS accumulator. An 8^3 bin adds its
+//                   chunks' tiles into registers and is flushed once per unit; larger bins
+//                   flush every tile. Flush: occupancy mask + the records of occupied
+//                   voxels, by plain stores for bins of one unit (most bins: a unit is up
+//                   to kUnitItems chunks) and by 64-bit integer atomics otherwise.
 //
 // All coverage decisions are exact integer arithmetic on snapped vertices, so the
 // voxel set and the integer sums do not depend on order, rounding or thread mapping.
@@ -433,10 +436,11 @@ __global__ __launch_bounds__(256) void k_bins(VoxBatch B, const VoxScene* __rest
   if (b < nbins) {
     const uint32_t pairs = gld(P.bin_small + b) + gld(P.bin_big + b);
     const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
-    const unsigned long long packed = ((unsigned long long)items << 32) | pairs;
+    const uint32_t units = (items + P.unit_items - 1) / P.unit_items;
+    const unsigned long long packed = ((unsigned long long)units << 32) | pairs;
     gst(P.bin_packed + b, packed);
     gst(B.packed_all + P.bin_base + b, packed);
-    if (items > 1) gst(P.zero_list + gatomic_add(&P.stats->zero_count, 1u), b);
+    if (units > 1) gst(P.zero_list + gatomic_add(&P.stats->zero_count, 1u), b);
     total = pairs;
   }
   // exact 64-bit pair count per scene (the packed scan keeps only 32 bits of it)
@@ -455,7 +459,7 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
   return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
 }
 
-// per scene: (items << 32 | pairs) of its bins, and the batch total in [nscenes]
+// per scene: (units << 32 | pairs) of its bins, and the batch total in [nscenes]
 __global__ void k_totals(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc > B.nscenes) return;
@@ -638,6 +642,38 @@ __device__ __forceinline__ bool edge_ok(const TriCoef& C, int u, int va, int vb)
 // step along y or z (the column direction of X- and Y-dominant triangles) into one LDS
 // bank; with PY = TS+1 and PZ = (TS+1)*TS+1 (odd multiples of 8 bytes, 18 banks apart)
 // sixteen lanes of an 8-byte access hit sixteen distinct bank pairs along any axis.
+// Writes one voxel's record (R, G, B sums, count, three normal sums) into its 8^3 brick,
+// and with the wavefront's ballot the brick's occupancy words for these 64 voxels (l is
+// the voxel's index in the brick; lanes hold 64 consecutive indices). Alone in its bin
+// (multi false) the record and both mask words are stored; otherwise they are added
+// (the bin was cleared before): word pairs as 64-bit atomics, which is exact with fewer
+// than 2^21 triangles per voxel (no 32-bit word's total reaches 2^32, so no carry
+// crosses a word: R, G, B add <= 255 each, normals <= 2048 each).
+__device__ __forceinline__ void flush_voxel(const VoxScene& P, uint32_t brick, uint32_t l, const uint32_t (&rec)[7],
+                                            bool multi, uint32_t lane) {
+  const uint32_t c = rec[3];
+  if (c != 0) {
+    uint32_t* dst = P.acc + ((size_t)brick * 512u + l) * kAccWords;
+    if (!multi) {
+      gst4(dst, make_uint4(rec[0], rec[1], rec[2], c));
+      gst4(dst + 4, make_uint4(rec[4], rec[5], rec[6], 0u));
+    } else {
+      unsigned long long* d64 = (unsigned long long*)dst;
+      gatomic_add(d64 + 0, (unsigned long long)rec[0] | ((unsigned long long)rec[1] << 32));
+      gatomic_add(d64 + 1, (unsigned long long)rec[2] | ((unsigned long long)c << 32));
+      gatomic_add(d64 + 2, (unsigned long long)rec[4] | ((unsigned long long)rec[5] << 32));
+      gatomic_add(d64 + 3, (unsigned long long)rec[6]);
+    }
+  }
+  const unsigned long long bits = __ballot(c != 0);  // bit j: voxel l - lane + j
+  if (lane < 2) {
+    const uint32_t word = lane == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+    uint32_t* m = P.mask + (size_t)brick * 16u + ((l - lane) >> 5) + lane;
+    if (!multi) gst(m, word);  // alone in its bin: every word is written, zeros too
+    else if (word) gatomic_or(m, word);
+  }
+}
+
 template <int TS>
 struct TileLayout {
   static constexpr int PY = TS + 1;
@@ -645,12 +681,10 @@ struct TileLayout {
   static constexpr int ENTRIES = (TS - 1) * (1 + PY + PZ) + 1;
 };
 
-// Occupancy: a work item spends most of its time waiting on its chain of dependent loads
-// (work map, bin tables, pair ids, triangle records), so resident workgroups are what
-// hides it. The 8^3 tile needs 25.5 KB of LDS (6 workgroups per CU); capping registers at
-// 80 (6 waves per SIMD, no spills) lets all 6 be resident instead of 4 at 103 registers.
+// Registers: the 8^3 tile's unit accumulators (14 per thread) fit without spills at 4
+// waves per SIMD (4 resident workgroups per CU; LDS, 25.5 KB each, would allow 6).
 template <int TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 6 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 4 : 1)))
 void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
@@ -663,7 +697,7 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
 
-  // work items of every scene of the batch (normally one per workgroup)
+  // work units of every scene of the batch (normally one per workgroup)
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const uint2 item = gld2(Bt.work_map + w);
     const VoxScene& P = scenes[item.x];
@@ -671,133 +705,148 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
     const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
     const uint32_t sub = B / TS;  // sub-tiles per bin edge
     const unsigned long long packed = gld(P.bin_packed + bin);
-    const uint32_t cnt_bin = (uint32_t)packed, nch = (uint32_t)(packed >> 32);
-    const uint32_t chunk = w - first_item(Bt, P, bin);
-    const uint32_t first = pair_base(Bt, P, bin) + chunk * P.chunk;
-    const uint32_t cnt = min(P.chunk, cnt_bin - chunk * P.chunk);
+    const uint32_t cnt_bin = (uint32_t)packed, units = (uint32_t)(packed >> 32);
+    const uint32_t unit = w - first_item(Bt, P, bin);
+    const uint32_t base = pair_base(Bt, P, bin);
+    const uint32_t items_bin = (cnt_bin + P.chunk - 1) / P.chunk;
+    const uint32_t it0 = unit * P.unit_items, it1 = min(it0 + P.unit_items, items_bin);
+    const bool multi = units > 1;  // other workgroups add into this bin: atomics
     const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
-
-    if (tid < cnt) {
-      const uint32_t t = gld(P.pairs + first + tid);
-      const TriRec R = gld_rec(P.tris + t);
-      make_coef(R, s_tri[tid]);
-    }
-
-    for (uint32_t st = 0; st < sub * sub * sub; st++) {
-      const int tx0 = (int)(bx * B + TS * (st % sub));
-      const int ty0 = (int)(by * B + TS * ((st / sub) % sub));
-      const int tz0 = (int)(bz * B + TS * (st / (sub * sub)));
-      for (int i = tid; i < L::ENTRIES; i += 256) {
-        s_acc0[i] = 0ull;
-        s_acc1[i] = 0ull;
-      }
-      // clip to the tile, count columns of the dominant projection, scan (wave 0)
-      if (tid < 64) {
-        uint32_t cols = 0;
-        if (tid < cnt) {
-          TriCoef& C = s_tri[tid];
-          const int t0[3] = {tx0, ty0, tz0};
-          bool emp = false;
+    // A bin that is one 8^3 tile keeps its sums in registers across the chunks of its unit
+    // (each thread owns tile entries tid and tid + 256) and is flushed once.
+    const bool regs = TS == 8 && sub == 1;
+    uint32_t racc[2][7];
 #pragma unroll
-          for (int a = 0; a < 3; a++) {
-            const int lo = max((int)C.lo[a], t0[a]), hi = min((int)C.hi[a], t0[a] + TS - 1);
-            C.clo[a] = (int16_t)lo;
-            C.chi[a] = (int16_t)hi;
-            emp = emp || lo > hi;
-          }
-          if (!emp) {
-            const int u = (int)C.axis, A = (u + 1) % 3, Bx = (u + 2) % 3;
-            cols = (uint32_t)(C.chi[A] - C.clo[A] + 1) * (uint32_t)(C.chi[Bx] - C.clo[Bx] + 1);
-          }
-        }
-        uint32_t incl = cols;
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t v = __shfl_up(incl, o);
-          if ((int)tid >= o) incl += v;
-        }
-        s_pref[tid + 1] = incl;
-        if (tid == 0) s_pref[0] = 0;
+      for (int k = 0; k < 7; k++) racc[r][k] = 0u;
+
+    for (uint32_t it = it0; it < it1; it++) {
+      const uint32_t first = base + it * P.chunk;
+      const uint32_t cnt = min(P.chunk, cnt_bin - it * P.chunk);
+      if (tid < cnt) {
+        const uint32_t t = gld(P.pairs + first + tid);
+        const TriRec R = gld_rec(P.tris + t);
+        make_coef(R, s_tri[tid]);
       }
-      __syncthreads();
-      const uint32_t total = s_pref[cnt];
-      for (uint32_t wi = tid; wi < total; wi += 256) {
-        int lo = 0, hi = (int)cnt - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_pref[mid] <= wi) lo = mid;
-          else hi = mid - 1;
+
+      for (uint32_t st = 0; st < sub * sub * sub; st++) {
+        const int tx0 = (int)(bx * B + TS * (st % sub));
+        const int ty0 = (int)(by * B + TS * ((st / sub) % sub));
+        const int tz0 = (int)(bz * B + TS * (st / (sub * sub)));
+        for (int i = tid; i < L::ENTRIES; i += 256) {
+          s_acc0[i] = 0ull;
+          s_acc1[i] = 0ull;
         }
-        const TriCoef& C = s_tri[lo];
-        const uint32_t local = wi - s_pref[lo];
-        const int u = (int)C.axis, A = (u + 1) % 3, Bx = (u + 2) % 3;
-        const int wa = C.chi[A] - C.clo[A] + 1;
-        const int va = C.clo[A] + (int)(local % (uint32_t)wa);
-        const int vb = C.clo[Bx] + (int)(local / (uint32_t)wa);
-        if (!edge_ok(C, u, va, vb)) continue;
-        const float zc = C.zq0 - C.ra * ((float)(va * kFixedOne + kFixedOne / 2) - C.q0a) -
-                         C.rb * ((float)(vb * kFixedOne + kFixedOne / 2) - C.q0b);
-        const float wd = 0.5f * (float)kFixedOne * (fabsf(C.ra) + fabsf(C.rb));
-        int k0 = (int)floorf((zc - wd - 4.0f) * (1.0f / kFixedOne));
-        int k1 = (int)floorf((zc + wd + 4.0f) * (1.0f / kFixedOne));
-        k0 = max(k0, (int)C.clo[u]);
-        k1 = min(k1, (int)C.chi[u]);
-        const int u1 = (u + 1) % 3, u2 = (u + 2) % 3;
-        for (int k = k0; k <= k1; k++) {
-          const int vx = (u == 0) ? k : (A == 0 ? va : vb);
-          const int vy = (u == 1) ? k : (A == 1 ? va : vb);
-          const int vz = (u == 2) ? k : (A == 2 ? va : vb);
-          const int64_t s = C.N256[0] * vx + C.N256[1] * vy + C.N256[2] * vz;
-          if (s + C.pmin > 0 || s + C.pmax < 0) continue;
-          if (!edge_ok(C, u1, sel3(vx, vy, vz, (u1 + 1) % 3), sel3(vx, vy, vz, (u1 + 2) % 3))) continue;
-          if (!edge_ok(C, u2, sel3(vx, vy, vz, (u2 + 1) % 3), sel3(vx, vy, vz, (u2 + 2) % 3))) continue;
-          const int pi = (vx - tx0) + L::PY * (vy - ty0) + L::PZ * (vz - tz0);
-          atomicAdd(&s_acc0[pi], C.inc0);
-          atomicAdd(&s_acc1[pi], C.inc1);
+        // clip to the tile, count columns of the dominant projection, scan (wave 0)
+        if (tid < 64) {
+          uint32_t cols = 0;
+          if (tid < cnt) {
+            TriCoef& C = s_tri[tid];
+            const int t0[3] = {tx0, ty0, tz0};
+            bool emp = false;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+              const int lo = max((int)C.lo[a], t0[a]), hi = min((int)C.hi[a], t0[a] + TS - 1);
+              C.clo[a] = (int16_t)lo;
+              C.chi[a] = (int16_t)hi;
+              emp = emp || lo > hi;
+            }
+            if (!emp) {
+              const int u = (int)C.axis, A = (u + 1) % 3, Bx = (u + 2) % 3;
+              cols = (uint32_t)(C.chi[A] - C.clo[A] + 1) * (uint32_t)(C.chi[Bx] - C.clo[Bx] + 1);
+            }
+          }
+          uint32_t incl = cols;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if ((int)tid >= o) incl += v;
+          }
+          s_pref[tid + 1] = incl;
+          if (tid == 0) s_pref[0] = 0;
         }
-      }
-      __syncthreads();
-      // flush the tile in brick-major order (64 consecutive entries of a wavefront are one
-      // 64-voxel slab of one 8^3 brick): records of occupied voxels, and the brick's
-      // occupancy mask, two words per wavefront from a ballot
-      for (int i = tid; i < TV; i += 256) {  // trip count is the same for every lane
-        const uint32_t bk = (uint32_t)i >> 9, l = (uint32_t)i & 511u;
-        const uint32_t bkx = bk % NBK, bky = (bk / NBK) % NBK, bkz = bk / (NBK * NBK);
-        const uint32_t lx = bkx * 8 + (l & 7u), ly = bky * 8 + ((l >> 3) & 7u), lz = bkz * 8 + (l >> 6);
-        const int pi = (int)lx + L::PY * (int)ly + L::PZ * (int)lz;
-        const unsigned long long w0 = s_acc0[pi];
-        const uint32_t c = (uint32_t)(w0 >> 48);
-        const uint32_t brick = ((uint32_t)tx0 >> 3) + bkx + nbr * (((uint32_t)ty0 >> 3) + bky + nbr * (((uint32_t)tz0 >> 3) + bkz));
-        if (c != 0) {
+        __syncthreads();
+        const uint32_t total = s_pref[cnt];
+        for (uint32_t wi = tid; wi < total; wi += 256) {
+          int lo = 0, hi = (int)cnt - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pref[mid] <= wi) lo = mid;
+            else hi = mid - 1;
+          }
+          const TriCoef& C = s_tri[lo];
+          const uint32_t local = wi - s_pref[lo];
+          const int u = (int)C.axis, A = (u + 1) % 3, Bx = (u + 2) % 3;
+          const int wa = C.chi[A] - C.clo[A] + 1;
+          const int va = C.clo[A] + (int)(local % (uint32_t)wa);
+          const int vb = C.clo[Bx] + (int)(local / (uint32_t)wa);
+          if (!edge_ok(C, u, va, vb)) continue;
+          const float zc = C.zq0 - C.ra * ((float)(va * kFixedOne + kFixedOne / 2) - C.q0a) -
+                           C.rb * ((float)(vb * kFixedOne + kFixedOne / 2) - C.q0b);
+          const float wd = 0.5f * (float)kFixedOne * (fabsf(C.ra) + fabsf(C.rb));
+          int k0 = (int)floorf((zc - wd - 4.0f) * (1.0f / kFixedOne));
+          int k1 = (int)floorf((zc + wd + 4.0f) * (1.0f / kFixedOne));
+          k0 = max(k0, (int)C.clo[u]);
+          k1 = min(k1, (int)C.chi[u]);
+          const int u1 = (u + 1) % 3, u2 = (u + 2) % 3;
+          for (int k = k0; k <= k1; k++) {
+            const int vx = (u == 0) ? k : (A == 0 ? va : vb);
+            const int vy = (u == 1) ? k : (A == 1 ? va : vb);
+            const int vz = (u == 2) ? k : (A == 2 ? va : vb);
+            const int64_t sv = C.N256[0] * vx + C.N256[1] * vy + C.N256[2] * vz;
+            if (sv + C.pmin > 0 || sv + C.pmax < 0) continue;
+            if (!edge_ok(C, u1, sel3(vx, vy, vz, (u1 + 1) % 3), sel3(vx, vy, vz, (u1 + 2) % 3))) continue;
+            if (!edge_ok(C, u2, sel3(vx, vy, vz, (u2 + 1) % 3), sel3(vx, vy, vz, (u2 + 2) % 3))) continue;
+            const int pi = (vx - tx0) + L::PY * (vy - ty0) + L::PZ * (vz - tz0);
+            atomicAdd(&s_acc0[pi], C.inc0);
+            atomicAdd(&s_acc1[pi], C.inc1);
+          }
+        }
+        __syncthreads();
+        if (regs) {
+          // add this chunk's tile into the registers (entries in brick order, as flushed)
+#pragma unroll
+          for (int r = 0; r < 2; r++) {
+            const uint32_t l = tid + 256u * (uint32_t)r;
+            const int pi = (int)(l & 7u) + L::PY * (int)((l >> 3) & 7u) + L::PZ * (int)(l >> 6);
+            const unsigned long long w0 = s_acc0[pi], w1 = s_acc1[pi];
+            racc[r][0] += (uint32_t)(w0 & 0xFFFFu);
+            racc[r][1] += (uint32_t)((w0 >> 16) & 0xFFFFu);
+            racc[r][2] += (uint32_t)((w0 >> 32) & 0xFFFFu);
+            racc[r][3] += (uint32_t)(w0 >> 48);
+            racc[r][4] += (uint32_t)(w1 & 0xFFFFFu);
+            racc[r][5] += (uint32_t)((w1 >> 20) & 0xFFFFFu);
+            racc[r][6] += (uint32_t)((w1 >> 40) & 0xFFFFFu);
+          }
+          __syncthreads();  // the next chunk clears the tile and overwrites the coefficients
+          continue;
+        }
+        // flush the tile in brick-major order (64 consecutive entries of a wavefront are one
+        // 64-voxel slab of one 8^3 brick): records of occupied voxels, and the brick's
+        // occupancy mask, two words per wavefront from a ballot
+        for (int i = tid; i < TV; i += 256) {  // trip count is the same for every lane
+          const uint32_t bk = (uint32_t)i >> 9, l = (uint32_t)i & 511u;
+          const uint32_t bkx = bk % NBK, bky = (bk / NBK) % NBK, bkz = bk / (NBK * NBK);
+          const uint32_t lx = bkx * 8 + (l & 7u), ly = bky * 8 + ((l >> 3) & 7u), lz = bkz * 8 + (l >> 6);
+          const int pi = (int)lx + L::PY * (int)ly + L::PZ * (int)lz;
+          const unsigned long long w0 = s_acc0[pi];
           const unsigned long long w1 = s_acc1[pi];
-          uint32_t* dst = P.acc + ((size_t)brick * 512u + l) * kAccWords;
-          const uint32_t r = (uint32_t)(w0 & 0xFFFFu), g = (uint32_t)((w0 >> 16) & 0xFFFFu),
-                         b = (uint32_t)((w0 >> 32) & 0xFFFFu);
-          const uint32_t nx = (uint32_t)(w1 & 0xFFFFFu), ny = (uint32_t)((w1 >> 20) & 0xFFFFFu),
-                         nz = (uint32_t)((w1 >> 40) & 0xFFFFFu);
-          if (nch == 1) {
-            gst4(dst, make_uint4(r, g, b, c));
-            gst4(dst + 4, make_uint4(nx, ny, nz, 0u));
-          } else {
-            // word pairs as 64-bit adds: with fewer than 2^21 triangles per voxel no 32-bit
-            // word's total reaches 2^32 (R, G, B <= 255 each, normals <= 2048 each), so no
-            // carry crosses into the upper word and the pairs add exactly as single words
-            unsigned long long* d64 = (unsigned long long*)dst;
-            gatomic_add(d64 + 0, (unsigned long long)r | ((unsigned long long)g << 32));
-            gatomic_add(d64 + 1, (unsigned long long)b | ((unsigned long long)c << 32));
-            gatomic_add(d64 + 2, (unsigned long long)nx | ((unsigned long long)ny << 32));
-            gatomic_add(d64 + 3, (unsigned long long)nz);
-          }
+          const uint32_t rec[7] = {(uint32_t)(w0 & 0xFFFFu),  (uint32_t)((w0 >> 16) & 0xFFFFu),
+                                   (uint32_t)((w0 >> 32) & 0xFFFFu), (uint32_t)(w0 >> 48),
+                                   (uint32_t)(w1 & 0xFFFFFu), (uint32_t)((w1 >> 20) & 0xFFFFFu),
+                                   (uint32_t)((w1 >> 40) & 0xFFFFFu)};
+          const uint32_t brick = ((uint32_t)tx0 >> 3) + bkx +
+                                 nbr * (((uint32_t)ty0 >> 3) + bky + nbr * (((uint32_t)tz0 >> 3) + bkz));
+          flush_voxel(P, brick, l, rec, multi, lane);
         }
-        const unsigned long long bits = __ballot(c != 0);  // bit j: voxel l - lane + j
-        if (lane < 2) {
-          const uint32_t word = lane == 0 ? (uint32_t)bits : (uint32_t)(bits >> 32);
-          uint32_t* m = P.mask + (size_t)brick * 16u + ((l - lane) >> 5) + lane;
-          if (nch == 1) gst(m, word);  // single-item bin: every word is written, zeros too
-          else if (word) gatomic_or(m, word);
-        }
+        __syncthreads();
       }
-      __syncthreads();
+    }
+    if (regs) {  // one flush per unit
+      const uint32_t brick = bx + nbr * (by + nbr * bz);  // the bin is one brick
+#pragma unroll
+      for (int r = 0; r < 2; r++) flush_voxel(P, brick, tid + 256u * (uint32_t)r, racc[r], multi, lane);
     }
   }
 }

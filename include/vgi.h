@@ -163,7 +163,7 @@ typedef struct vgi_scene_stats {
   uint64_t triangles_rejected; /* non-finite, outside the guard band or bad index */
   uint64_t triangles_degenerate; /* zero area after snapping */
   uint64_t bin_pairs;          /* (triangle, bin) pairs after the bin-level overlap test */
-  uint64_t work_items;         /* voxelize work items (bin, chunk) */
+  uint64_t work_items;         /* voxelize work units (bin, up to 16 chunks of chunk_tris triangles) */
   uint64_t device_bytes;       /* device memory held by the scene */
   uint32_t n;
   uint32_t levels;
