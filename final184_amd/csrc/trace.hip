@@ -215,7 +215,8 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
     const float ux = fmaf(udx, t, uox);
     const float uy = fmaf(udy, t, uoy);
     const float uz = fmaf(udz, t, uoz);
-    if (!(ux >= 0.f && ux <= nf && uy >= 0.f && uy <= nf && uz >= 0.f && uz <= nf)) break;
+    // inside [0, n]^3 (false for NaN): one min3/max3 pair instead of six compares
+    if (!(fminf(fminf(ux, uy), uz) >= 0.f && fmaxf(fmaxf(ux, uy), uz) <= nf)) break;
     float diam = fmaxf(P.h, 2.0f * t * tan_half);
     float lvl = fmaxf(0.0f, __log2f(diam * P.inv_h));
     if (UNIFORM) {
@@ -225,8 +226,7 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t abits = 0;
     if (lvl >= top) {
-      if (UNIFORM) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
-      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
+      gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // the level is uniform
       count += 1;
     } else {
       const float fl = floorf(lvl);
@@ -241,8 +241,16 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
         gather_uniform<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
         if (abits) gather_uniform<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       } else {
-        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-        if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+        // per-lane schedules (the roughness-driven specular cone) usually still agree on the
+        // level across the wavefront: then the level's descriptor and pitches are scalars
+        const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
+        if (__all(l0 == lu)) {
+          gather_uniform<PREC>(P, lu + 1, ux, uy, uz, fr, s, abits);
+          if (abits) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
+        } else {
+          gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
+          if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+        }
       }
       count += 2;
     }
@@ -348,11 +356,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams*
       gst4(P.out_s + p, make_uint4(__builtin_bit_cast(uint32_t, os.x), __builtin_bit_cast(uint32_t, os.y),
                                    __builtin_bit_cast(uint32_t, os.z), __builtin_bit_cast(uint32_t, os.w)));
   }
-  if (Bt.counter) {
+  if (Bt.counter) {  // one global atomic per workgroup (all add to one address)
+    __shared__ unsigned long long s_count[4];
     unsigned long long c = count;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if (lane == 0) atomicAdd(Bt.counter, c);
+    if (lane == 0) s_count[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // workgroups of 64 (8x8 tiles) or 256 threads
+      unsigned long long t = 0;
+      for (uint32_t i = 0; i < (blockDim.x >> 6); i++) t += s_count[i];
+      atomicAdd(Bt.counter, t);
+    }
   }
 }
 
