@@ -24,9 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# one wave64 FP32 VALU instruction per CU per clock (4 SIMD16 units, 4 cycles each),
-# 256 CUs at 2.4 GHz; the 157.3 TFLOP/s vector peak counts packed FP32 (two FMAs a lane)
-VALU_PEAK_GIPS = 256 * 2.4
+# wave64 VALU instructions per CU per clock: a probe of independent v_fma_f32 chains
+# issues 0.98 (4 SIMD16 units, 4 cycles per wave64 FMA; the 157.3 TFLOP/s vector peak
+# counts packed FP32). 256 CUs at 2.4 GHz.
+CU_CLOCKS_PER_S = 256 * 2.4e9
 
 
 def log(msg: str) -> None:
@@ -525,9 +526,11 @@ def main() -> int:
     for prec, r, key in ((head["prec"], head, "valu"), (base["prec"], base, "fp32_valu")):
         t = (traffic or {}).get(prec, {})
         if "SQ_INSTS_VALU" in t:
-            rate = t["SQ_INSTS_VALU"] / (r["trace_ms"] * 1e-3) / 1e9
-            line["roofline"][key] = {"bound": "valu", "instructions": t["SQ_INSTS_VALU"], "achieved": rate,
-                                     "peak": VALU_PEAK_GIPS, "unit": "G wave64 instr/s", "frac": rate / VALU_PEAK_GIPS,
+            rate = t["SQ_INSTS_VALU"] / (r["trace_ms"] * 1e-3)
+            line["roofline"][key] = {"bound": "valu", "instructions": t["SQ_INSTS_VALU"],
+                                     "achieved": rate / 1e9, "unit": "G wave64 instr/s",
+                                     "per_cu_per_clock": rate / CU_CLOCKS_PER_S,
+                                     "fma_probe_per_cu_per_clock": 0.98,
                                      "l2_hit_rate": t.get("l2_hit_rate")}
     try:
         line["screen_passes"] = run_screen(ctx, vgi, sg, w, mesh, gb.rough, args.steps)
