@@ -80,12 +80,20 @@ struct Foot {
   float wz0, wz1;            // z weights x level weight
 };
 
+// floor(x) as an integer in one instruction (the compiler emits floor, then convert); x is
+// finite and within the padded grid here (positions outside [0, n]^3 end the march first)
+__device__ __forceinline__ int cvt_floor(float x) {
+  int r;
+  __asm__("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, float uy, float uz,
                                           float wl) {
   const float sc = __builtin_ldexpf(1.0f, -(int)k);
   const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
   // integer part by a flooring convert, fraction by v_fract (x - floor(x))
-  const int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+  const int ix = cvt_floor(gx), iy = cvt_floor(gy), iz = cvt_floor(gz);
   const float fx = __builtin_amdgcn_fractf(gx), fy = __builtin_amdgcn_fractf(gy),
               fz = __builtin_amdgcn_fractf(gz);
   const float ex = 1.0f - fx, ey = 1.0f - fy;
@@ -211,14 +219,19 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
   const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
   const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
   const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
-  for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= P.max_dist; it++) {
+  // loop invariants, read once (otherwise reloaded from the descriptor every sample)
+  const float h = P.h, inv_h = P.inv_h, step = P.step, cutoff = P.cutoff, max_dist = P.max_dist;
+  for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= max_dist; it++) {
     const float ux = fmaf(udx, t, uox);
     const float uy = fmaf(udy, t, uoy);
     const float uz = fmaf(udz, t, uoz);
-    // inside [0, n]^3 (false for NaN): one min3/max3 pair instead of six compares
-    if (!(fminf(fminf(ux, uy), uz) >= 0.f && fmaxf(fmaxf(ux, uy), uz) <= nf)) break;
-    float diam = fmaxf(P.h, 2.0f * t * tan_half);
-    float lvl = fmaxf(0.0f, __log2f(diam * P.inv_h));
+    // inside [0, n]^3: one minimum3/maximum3 pair instead of six compares; these propagate
+    // NaN (fminf/fmaxf would drop it), so a NaN in any coordinate ends the march
+    const float umin = __builtin_elementwise_minimum(__builtin_elementwise_minimum(ux, uy), uz);
+    const float umax = __builtin_elementwise_maximum(__builtin_elementwise_maximum(ux, uy), uz);
+    if (!(umin >= 0.f && umax <= nf)) break;
+    float diam = fmaxf(h, 2.0f * t * tan_half);
+    float lvl = fmaxf(0.0f, __log2f(diam * inv_h));
     if (UNIFORM) {
       diam = uniform_f(diam);
       lvl = uniform_f(lvl);
@@ -262,8 +275,8 @@ __device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, flo
     cg = fmaf(w, s.y, cg);
     cb = fmaf(w, s.z, cb);
     a = fmaf(w, s.w, a);
-    if (a >= P.cutoff) break;
-    t = t + P.step * diam;
+    if (a >= cutoff) break;
+    t = t + step * diam;
   }
   return make_float4(cr, cg, cb, a);
 }

@@ -183,3 +183,32 @@ def test_empty_volume_gives_zero_irradiance(ctx):
     assert not d[:, :3].any() and np.all(d[:, 3] == 1.0)
     assert not sp.any()
     s.release()
+
+
+def test_nan_position_component_ends_march(ctx):
+    """A NaN in one coordinate of a G-buffer position ends that pixel's cones at their first
+    sample, as the oracle's per-axis bounds test does; other pixels are unaffected."""
+    vgi = _vgi()
+    w = sg.random_soup(ntri=2000, n=32, levels=5, width=16, height=16)
+    pos = w.gb_pos.copy()
+    bad = np.array([3, 40, 77, 130, 201])
+    for k, p in enumerate(bad):
+        pos[p, k % 3] = np.nan
+    print("case nan-position trace", flush=True)
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    s = ctx.scene()
+    s.voxelize([mesh], vgi.make_grid(**w.grid))
+    s.build_mips(vgi.make_lights(w.lights), vgi.PREC_FP32)
+    gb = vgi.GBufferData(ctx, pos, w.gb_nrm, w.width, w.height)
+    od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    s.trace(gb, vgi.make_cones(**w.cones), od, osp)
+    d, sp = od.numpy(), osp.numpy()
+    s.release()
+    acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+    lv32 = orc.inject_mips(acc, w.grid, w.lights, "fp32")
+    d32, s32, _ = orc.trace(lv32, w.grid, pos, w.gb_nrm, w.cones)
+    assert np.isfinite(d).all() and np.isfinite(sp).all()
+    assert np.array_equal(d[bad], d32[bad]) and np.array_equal(sp[bad], s32[bad])
+    good = np.setdiff1d(np.arange(w.num_pixels), bad)
+    assert rel_l2(_irr(d[good], sp[good]), _irr(d32[good], s32[good])) <= 1e-5
