@@ -96,16 +96,26 @@ __device__ __forceinline__ uint32_t find_mesh(const MeshDev* __restrict__ m, uin
   return lo;
 }
 
-// Exact closed-box test of a triangle against [b, b + d]^3 (fixed-point units), with
-// the bounding boxes already known to overlap. Magnitudes: |q| <= 2^18, |edge| <= 2^19,
-// |N| <= 2^39, d <= 2^13: every product and sum fits in int64.
+// Exact closed-box test of a triangle against [b, b + d]^3 (fixed-point units, d = 2^dshift),
+// with the bounding boxes already known to overlap. Magnitudes: |q| <= 2^18 (the snap guard),
+// 0 <= b <= 2^18, so |b - q| <= 2^19 and |edge| <= 2^19 fit int32; |N| <= 2^39, d <= 2^13.
+// Edge-axis products are 32 x 32 -> 64-bit multiplies, products with d are shifts, and the
+// normal-axis sums stay below 2^61: every value is the exact integer.
+__device__ __forceinline__ int64_t shl64(int64_t v, uint32_t s) { return (int64_t)((uint64_t)v << s); }
+__device__ __forceinline__ int64_t mul32x32(int32_t a, int32_t b) { return (int64_t)a * (int64_t)b; }
+
 __device__ __forceinline__ bool sat_box(const int64_t (&q)[3][3], const int64_t (&N)[3],
-                                        const int64_t (&b)[3], int64_t d) {
+                                        const int32_t (&b)[3], uint32_t dshift) {
+  int32_t qi[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) qi[i][a] = (int32_t)q[i][a];
   int64_t s_lo = 0, s_hi = 0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    int64_t t0 = N[a] * (b[a] - q[0][a]);
-    int64_t dd = N[a] * d;
+    const int64_t t0 = N[a] * (int64_t)(b[a] - qi[0][a]);
+    const int64_t dd = shl64(N[a], dshift);
     s_lo += t0 + imin64(0, dd);
     s_hi += t0 + imax64(0, dd);
   }
@@ -113,14 +123,15 @@ __device__ __forceinline__ bool sat_box(const int64_t (&q)[3][3], const int64_t 
 #pragma unroll
   for (int u = 0; u < 3; u++) {
     const int A = (u + 1) % 3, B = (u + 2) % 3;
-    const int64_t sgn = N[u] < 0 ? -1 : 1;
+    const bool neg = N[u] < 0;  // edge normal (-dB, dA) * sign(N[u])
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const int j = (i + 1) % 3;
-      int64_t na = -(q[j][B] - q[i][B]) * sgn;
-      int64_t nb = (q[j][A] - q[i][A]) * sgn;
-      int64_t e = na * (b[A] - q[i][A]) + nb * (b[B] - q[i][B]) + imax64(0, na * d) +
-                  imax64(0, nb * d);
+      const int32_t dB = qi[j][B] - qi[i][B], dA = qi[j][A] - qi[i][A];
+      const int32_t na = neg ? dB : -dB;
+      const int32_t nb = neg ? -dA : dA;
+      const int64_t e = mul32x32(na, b[A] - qi[i][A]) + mul32x32(nb, b[B] - qi[i][B]) +
+                        imax64(0, shl64(na, dshift)) + imax64(0, shl64(nb, dshift));
       if (e < 0) return false;
     }
   }
@@ -132,15 +143,15 @@ __device__ __forceinline__ void load_q(const TriRec& R, int64_t (&q)[3][3], int6
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int a = 0; a < 3; a++) q[i][a] = R.q[i * 3 + a];
-  int64_t e0[3], e1[3];
+  int32_t e0[3], e1[3];  // |edge| <= 2^19
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    e0[a] = q[1][a] - q[0][a];
-    e1[a] = q[2][a] - q[0][a];
+    e0[a] = R.q[3 + a] - R.q[a];
+    e1[a] = R.q[6 + a] - R.q[a];
   }
-  N[0] = e0[1] * e1[2] - e0[2] * e1[1];
-  N[1] = e0[2] * e1[0] - e0[0] * e1[2];
-  N[2] = e0[0] * e1[1] - e0[1] * e1[0];
+  N[0] = mul32x32(e0[1], e1[2]) - mul32x32(e0[2], e1[1]);
+  N[1] = mul32x32(e0[2], e1[0]) - mul32x32(e0[0], e1[2]);
+  N[2] = mul32x32(e0[0], e1[1]) - mul32x32(e0[1], e1[0]);
 }
 
 __device__ __forceinline__ bool tri_ok(const TriRec& R) {
@@ -168,10 +179,10 @@ __device__ __forceinline__ bool bin_candidate(const VoxScene& S, const int64_t (
                                               const int64_t (&N)[3], const BinBox& bb, int j,
                                               uint32_t& bin) {
   const int bx = bb.x0 + j % bb.nx, by = bb.y0 + (j / bb.nx) % bb.ny, bz = bb.z0 + j / (bb.nx * bb.ny);
-  const int64_t d = (int64_t)kFixedOne << S.bin_shift;
-  const int64_t b[3] = {(int64_t)bx * d, (int64_t)by * d, (int64_t)bz * d};
+  const uint32_t dshift = VGI_FIXED_BITS + S.bin_shift;
+  const int32_t b[3] = {bx << dshift, by << dshift, bz << dshift};  // <= n * 256 <= 2^18
   bin = (uint32_t)bx + S.nbins_axis * ((uint32_t)by + S.nbins_axis * (uint32_t)bz);
-  return sat_box(q, N, b, d);
+  return sat_box(q, N, b, dshift);
 }
 
 // Workgroup -> (scene, first item) for per-triangle and per-bin kernels.
@@ -591,16 +602,18 @@ __device__ void make_coef(const TriRec& R, TriCoef& C) {
 #pragma unroll
   for (int u = 0; u < 3; u++) {
     const int A = (u + 1) % 3, B = (u + 2) % 3;
-    const int64_t sgn = N[u] < 0 ? -1 : 1;
+    const bool neg = N[u] < 0;  // edge normal (-dB, dA) * sign(N[u]), |dA|, |dB| <= 2^19
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const int j = (i + 1) % 3;
-      const int64_t na = -(q[j][B] - q[i][B]) * sgn;
-      const int64_t nb = (q[j][A] - q[i][A]) * sgn;
-      const int64_t ea = na * kFixedOne, eb = nb * kFixedOne;
+      const int32_t dB = R.q[j * 3 + B] - R.q[i * 3 + B], dA = R.q[j * 3 + A] - R.q[i * 3 + A];
+      const int32_t na = neg ? dB : -dB;
+      const int32_t nb = neg ? -dA : dA;
+      const int64_t ea = shl64(na, VGI_FIXED_BITS), eb = shl64(nb, VGI_FIXED_BITS);
       C.eA[u * 3 + i] = (int32_t)ea;
       C.eB[u * 3 + i] = (int32_t)eb;
-      C.eC[u * 3 + i] = -(na * q[i][A] + nb * q[i][B]) + imax64(0, ea) + imax64(0, eb);
+      C.eC[u * 3 + i] = -(mul32x32(na, R.q[i * 3 + A]) + mul32x32(nb, R.q[i * 3 + B])) + imax64(0, ea) +
+                        imax64(0, eb);
     }
   }
   const int u = R.axis;
