@@ -66,9 +66,10 @@ struct VoxScene {
   TriRec* tris;
   uint32_t* tri_slot;              // kSmallBins slots per triangle (small triangles)
   uint32_t* big_list;              // ids of large triangles
-  uint32_t* bin_small;             // nbins: pairs from small triangles (slot counters)
-  uint32_t* bin_big;               // nbins: pairs from large triangles
-  uint32_t* big_cursor;            // nbins
+  // per-bin counters (nbins each), bin b at a hashed index (voxelize.hip: ctr)
+  uint32_t* bin_small;             // pairs from small triangles (slot counters)
+  uint32_t* bin_big;               // pairs from large triangles
+  uint32_t* big_cursor;            // large-triangle pairs placed so far
   uint32_t* zero_list;             // bins with more than one work item
   unsigned long long* bin_packed;  // nbins: work units << 32 | pairs (kept with the scene)
   uint32_t* pairs;                 // triangle ids grouped by bin

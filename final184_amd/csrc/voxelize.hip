@@ -185,6 +185,15 @@ __device__ __forceinline__ bool bin_candidate(const VoxScene& S, const int64_t (
   return sat_box(q, N, b, dshift);
 }
 
+// Index of bin b's counter in bin_small / bin_big / big_cursor. Bins are scattered over the
+// counter array by an odd multiplier (a bijection modulo the power-of-two bin count), so
+// spatially adjacent bins -- a wall's run of bins, each hit by many workgroups -- do not
+// share a 64-byte line, on which returning atomics from different workgroups serialize.
+__device__ __forceinline__ uint32_t ctr(const VoxScene& S, uint32_t bin) {
+  const uint32_t nb = S.nbins_axis * S.nbins_axis * S.nbins_axis;
+  return (bin * 0x9E3779B1u) & (nb - 1u);
+}
+
 // Workgroup -> (scene, first item) for per-triangle and per-bin kernels.
 __device__ __forceinline__ uint2 tri_block(const VoxBatch& B) {
   return B.nscenes == 1 ? make_uint2(0u, blockIdx.x * kBlockTris) : B.tri_blocks[blockIdx.x];
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
         }
         h = (h + 1u) & (kSlotTable - 1u);
       }
-      if (ent[j] == kNoKey) rank[j] = gatomic_add(P.bin_small + bin, 1u);  // table overflow
+      if (ent[j] == kNoKey) rank[j] = gatomic_add(P.bin_small + ctr(P, bin), 1u);  // table overflow
     }
   }
   __syncthreads();
@@ -375,7 +384,7 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
     for (int r = 0; r < 4; r++) {
       const uint32_t e = threadIdx.x + (uint32_t)r * kBlockTris;
       key[r] = s_key[e];
-      base[r] = key[r] != kNoKey ? gatomic_add(P.bin_small + key[r], s_cnt[e]) : 0u;
+      base[r] = key[r] != kNoKey ? gatomic_add(P.bin_small + ctr(P, key[r]), s_cnt[e]) : 0u;
     }
 #pragma unroll
     for (int r = 0; r < 4; r++)
@@ -417,9 +426,10 @@ __global__ __launch_bounds__(256) void k_big(VoxBatch B, const VoxScene* __restr
         uint32_t bin;
         if (!bin_candidate(P, q, N, bb, j, bin)) continue;
         if (!EMIT) {
-          gatomic_add(&P.bin_big[bin], 1u);
+          gatomic_add(P.bin_big + ctr(P, bin), 1u);
         } else {
-          const uint32_t pos = pair_base(B, P, bin) + gld(P.bin_small + bin) + gatomic_add(&P.big_cursor[bin], 1u);
+          const uint32_t c = ctr(P, bin);
+          const uint32_t pos = pair_base(B, P, bin) + gld(P.bin_small + c) + gatomic_add(P.big_cursor + c, 1u);
           if (pos < P.pairs_cap) gst(P.pairs + pos, t);
         }
       }
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(256) void k_bins(VoxBatch B, const VoxScene* __rest
   const uint32_t b = blk.y + threadIdx.x;
   unsigned long long total = 0;
   if (b < nbins) {
-    const uint32_t pairs = gld(P.bin_small + b) + gld(P.bin_big + b);
+    const uint32_t pairs = gld(P.bin_small + ctr(P, b)) + gld(P.bin_big + ctr(P, b));
     const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
     const uint32_t units = (items + P.unit_items - 1) / P.unit_items;
     const unsigned long long packed = ((unsigned long long)units << 32) | pairs;
