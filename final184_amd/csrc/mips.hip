@@ -31,32 +31,39 @@ __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
 }
 
+// A texel in its storage precision: Packed is the stored form (lossless for a value that
+// round() has already rounded), load/store move it to and from a level.
 template <int PREC> struct Texel;
 
 template <> struct Texel<VGI_PREC_FP32> {
-  __device__ static float4 load(const void* base, size_t i) {
-    const uint4 u = gld4((const float4*)base + i);
+  using Packed = uint4;
+  __device__ static Packed pack(float4 v) {
+    return make_uint4(__builtin_bit_cast(uint32_t, v.x), __builtin_bit_cast(uint32_t, v.y),
+                      __builtin_bit_cast(uint32_t, v.z), __builtin_bit_cast(uint32_t, v.w));
+  }
+  __device__ static float4 unpack(Packed u) {
     return make_float4(__builtin_bit_cast(float, u.x), __builtin_bit_cast(float, u.y),
                        __builtin_bit_cast(float, u.z), __builtin_bit_cast(float, u.w));
   }
-  __device__ static void store(void* base, size_t i, float4 v) {
-    gst4((float4*)base + i, make_uint4(__builtin_bit_cast(uint32_t, v.x), __builtin_bit_cast(uint32_t, v.y),
-                                       __builtin_bit_cast(uint32_t, v.z), __builtin_bit_cast(uint32_t, v.w)));
-  }
+  __device__ static float4 load(const void* base, size_t i) { return unpack(gld4((const uint4*)base + i)); }
+  __device__ static void store_packed(void* base, size_t i, Packed u) { gst4((uint4*)base + i, u); }
+  __device__ static void store(void* base, size_t i, float4 v) { store_packed(base, i, pack(v)); }
   __device__ static float4 round(float4 v) { return v; }
 };
 
 template <> struct Texel<VGI_PREC_FP16> {
-  __device__ static float4 load(const void* base, size_t i) {
-    const uint2 u = gld2((const uint2*)base + i);
-    return make_float4(f16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.x >> 16)),
-                       f16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
-  }
-  __device__ static uint2 pack(float4 v) {
+  using Packed = uint2;
+  __device__ static Packed pack(float4 v) {
     return make_uint2((uint32_t)f32_to_f16_bits(v.x) | ((uint32_t)f32_to_f16_bits(v.y) << 16),
                       (uint32_t)f32_to_f16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
   }
-  __device__ static void store(void* base, size_t i, float4 v) { gst2((uint2*)base + i, pack(v)); }
+  __device__ static float4 unpack(Packed u) {
+    return make_float4(f16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.x >> 16)),
+                       f16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
+  }
+  __device__ static float4 load(const void* base, size_t i) { return unpack(gld2((const uint2*)base + i)); }
+  __device__ static void store_packed(void* base, size_t i, Packed u) { gst2((uint2*)base + i, u); }
+  __device__ static void store(void* base, size_t i, float4 v) { store_packed(base, i, pack(v)); }
   __device__ static float4 round(float4 v) {
     return make_float4(f16_bits_to_f32(f32_to_f16_bits(v.x)), f16_bits_to_f32(f32_to_f16_bits(v.y)),
                        f16_bits_to_f32(f32_to_f16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
@@ -65,16 +72,18 @@ template <> struct Texel<VGI_PREC_FP16> {
 
 // RGB in bf16, opacity in fp16.
 template <> struct Texel<VGI_PREC_BF16> {
-  __device__ static float4 load(const void* base, size_t i) {
-    const uint2 u = gld2((const uint2*)base + i);
+  using Packed = uint2;
+  __device__ static Packed pack(float4 v) {
+    return make_uint2((uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
+                      (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16));
+  }
+  __device__ static float4 unpack(Packed u) {
     return make_float4(bf16_bits_to_f32((uint16_t)(u.x & 0xFFFFu)), bf16_bits_to_f32((uint16_t)(u.x >> 16)),
                        bf16_bits_to_f32((uint16_t)(u.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(u.y >> 16)));
   }
-  __device__ static void store(void* base, size_t i, float4 v) {
-    gst2((uint2*)base + i,
-         make_uint2((uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
-                    (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_f16_bits(v.w) << 16)));
-  }
+  __device__ static float4 load(const void* base, size_t i) { return unpack(gld2((const uint2*)base + i)); }
+  __device__ static void store_packed(void* base, size_t i, Packed u) { gst2((uint2*)base + i, u); }
+  __device__ static void store(void* base, size_t i, float4 v) { store_packed(base, i, pack(v)); }
   __device__ static float4 round(float4 v) {
     return make_float4(bf16_bits_to_f32(f32_to_bf16_bits(v.x)), bf16_bits_to_f32(f32_to_bf16_bits(v.y)),
                        bf16_bits_to_f32(f32_to_bf16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
@@ -179,54 +188,98 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
   const MipScene& P = scenes[blk.scene];
   const uint32_t src = blk.src, nout = blk.nout;
   const uint32_t ns = P.nk[src];
-  const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge
+  const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge (a power of two)
   const uint32_t tpa = ns / S;             // tiles per axis
   const uint32_t tile = blk.tile;
   const uint32_t tx = tile % tpa, ty = (tile / tpa) % tpa, tz = tile / (tpa * tpa);
   const uint32_t G = S >> 1;  // groups per tile edge (>= 1 when nout >= 1)
+  const uint32_t gsh = G > 1 ? (uint32_t)__builtin_ctz(G) : 0u;
   const uint32_t g = threadIdx.x;
-  __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
-  if (SRC_ACC) {
+  const uint32_t gx = g & (G - 1u), gy = (g >> gsh) & (G - 1u), gz = g >> (2 * gsh);
+  const bool grp = g < G * G * G;
+
+  if constexpr (SRC_ACC) {
+    __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
     if (g < 256) s_lut[g] = lut[g];
-    __syncthreads();
-  }
-
-  if (SRC_ACC && nout == 0) {
-    // single-level chain: resolve level 0 only, one voxel per thread
-    for (uint32_t v = g; v < S * S * S; v += 512) {
-      const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
-      const float4 c = occupied(P, x, y, z) ? TX::round(inject(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
-      TX::store(P.level[0], padded_index(x, y, z, ns), c);
+    if (nout == 0) {
+      __syncthreads();
+      // single-level chain: resolve level 0 only, one voxel per thread
+      for (uint32_t v = g; v < S * S * S; v += 512) {
+        const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
+        const float4 c = occupied(P, x, y, z) ? TX::round(inject(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
+        TX::store(P.level[0], padded_index(x, y, z, ns), c);
+      }
+      return;
     }
-    return;
-  }
-
-  if (g < G * G * G) {
-    const uint32_t gx = g % G, gy = (g / G) % G, gz = g / (G * G);
-    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t x0 = tx * S + 2 * gx, y0 = ty * S + 2 * gy, z0 = tz * S + 2 * gz;
-    const uint32_t occ = SRC_ACC ? group_occupancy(P, x0, y0, z0) : 0u;
+    // Level 0 of the tile in LDS (x fastest), zero where unoccupied. The occupied voxels
+    // are listed first and injected by all threads in turn, so a wavefront's lanes all do
+    // useful work however sparse the surface is; the level is then stored row by row and
+    // each 2x2x2 group summed from LDS.
+    __shared__ typename TX::Packed s_val[4096];
+    __shared__ uint32_t s_nocc;
+    uint16_t* s_item = (uint16_t*)s_a;  // occupied voxel list (group * 8 + bit), before s_a is used
+    static_assert(sizeof(s_a) >= 4096 * sizeof(uint16_t), "item list fits in s_a");
+    const uint32_t ssh = gsh + 1u, NV = S * S * S;
+    const uint32_t lane = g & 63u;
+    for (uint32_t v = g; v < NV; v += 512) s_val[v] = TX::pack(make_float4(0.f, 0.f, 0.f, 0.f));
+    if (g == 0) s_nocc = 0u;
+    __syncthreads();
+    const uint32_t occ = grp ? group_occupancy(P, tx * S + 2 * gx, ty * S + 2 * gy, tz * S + 2 * gz) : 0u;
+    const uint32_t cnt = (uint32_t)__builtin_popcount(occ);
+    uint32_t incl = cnt;  // inclusive scan of the counts over the wavefront
 #pragma unroll
-    for (uint32_t dz = 0; dz < 2; dz++)
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    uint32_t wbase = 0;
+    if (lane == 63u) wbase = atomicAdd(&s_nocc, incl);
+    wbase = __shfl(wbase, 63);
+    uint32_t k = wbase + incl - cnt;
+    for (uint32_t m = occ; m != 0u; m &= m - 1u) s_item[k++] = (uint16_t)(g * 8u + (uint32_t)__builtin_ctz(m));
+    __syncthreads();
+    const uint32_t nocc = s_nocc;
+    for (uint32_t it = g; it < nocc; it += 512) {
+      const uint32_t e = s_item[it], gi = e >> 3, bit = e & 7u;
+      const uint32_t lx = 2 * (gi & (G - 1u)) + (bit & 1u);
+      const uint32_t ly = 2 * ((gi >> gsh) & (G - 1u)) + ((bit >> 1) & 1u);
+      const uint32_t lz = 2 * (gi >> (2 * gsh)) + (bit >> 2);
+      const float4 c = TX::round(inject(P, s_lut, tx * S + lx, ty * S + ly, tz * S + lz));
+      s_val[lx + (ly << ssh) + (lz << (2 * ssh))] = TX::pack(c);
+    }
+    __syncthreads();
+    for (uint32_t v = g; v < NV; v += 512) {
+      const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
+      TX::store_packed(P.level[0], padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns), s_val[v]);
+    }
+    if (grp) {
+      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (uint32_t dy = 0; dy < 2; dy++)
+      for (uint32_t dz = 0; dz < 2; dz++)
 #pragma unroll
-        for (uint32_t dx = 0; dx < 2; dx++) {
-          const uint32_t x = x0 + dx, y = y0 + dy, z = z0 + dz;
-          float4 c;
-          if (SRC_ACC) {
-            c = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((occ >> (dx + 2 * dy + 4 * dz)) & 1u) c = TX::round(inject(P, s_lut, x, y, z));
-            TX::store(P.level[0], padded_index(x, y, z, ns), c);
-          } else {
-            c = TX::load(P.level[src], padded_index(x, y, z, ns));
-          }
-          sum = add4(sum, c);
-        }
-    const float4 v = TX::round(scale4(sum, 0.125f));
-    const uint32_t nd = P.nk[src + 1];
-    TX::store(P.level[src + 1], padded_index(tx * G + gx, ty * G + gy, tz * G + gz, nd), v);
-    s_a[g] = v;
+        for (uint32_t dy = 0; dy < 2; dy++)
+#pragma unroll
+          for (uint32_t dx = 0; dx < 2; dx++)
+            sum = add4(sum, TX::unpack(s_val[(2 * gx + dx) + ((2 * gy + dy) << ssh) + ((2 * gz + dz) << (2 * ssh))]));
+      const float4 v = TX::round(scale4(sum, 0.125f));
+      TX::store(P.level[src + 1], padded_index(tx * G + gx, ty * G + gy, tz * G + gz, P.nk[src + 1]), v);
+      s_a[g] = v;  // the item list in s_a was last read before the barrier above
+    }
+  } else {
+    if (grp) {
+      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint32_t x0 = tx * S + 2 * gx, y0 = ty * S + 2 * gy, z0 = tz * S + 2 * gz;
+#pragma unroll
+      for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (uint32_t dy = 0; dy < 2; dy++)
+#pragma unroll
+          for (uint32_t dx = 0; dx < 2; dx++)
+            sum = add4(sum, TX::load(P.level[src], padded_index(x0 + dx, y0 + dy, z0 + dz, ns)));
+      const float4 v = TX::round(scale4(sum, 0.125f));
+      TX::store(P.level[src + 1], padded_index(tx * G + gx, ty * G + gy, tz * G + gz, P.nk[src + 1]), v);
+      s_a[g] = v;
+    }
   }
   __syncthreads();
   float4* in = s_a;

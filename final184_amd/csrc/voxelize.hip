@@ -16,7 +16,7 @@
 //                   (bin = B^3 voxels) tests each bin exactly and takes a slot in every
 //                   overlapped bin (ranked per workgroup in LDS, one global atomic per
 //                   distinct bin; the slot is kept); larger triangles go to a list.
-//   2 k_big<false>  one workgroup per listed triangle: its bins, tested cooperatively.
+//   2 k_big<false>  one wavefront per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
 //                   bins with > 1 unit are listed.
@@ -407,22 +407,26 @@ hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// 2 large triangles: count (then scatter) their bins, one workgroup per triangle
+// 2 large triangles: count (then scatter) their bins, one wavefront per triangle
 // ---------------------------------------------------------------------------
 
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_big(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  // one wavefront per triangle: most large triangles span a few dozen bins
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  const int lane = (int)(threadIdx.x & 63u);
   for (uint32_t sc = 0; sc < B.nscenes; sc++) {
     const VoxScene& P = scenes[sc];
     const uint32_t nbig = gld(&P.stats->big_count);
-    for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+    for (uint32_t i = wave; i < nbig; i += nwaves) {
       const uint32_t t = gld(P.big_list + i);
       const TriRec R = gld_rec(P.tris + t);
       int64_t q[3][3], N[3];
       load_q(R, q, N);
       const BinBox bb = bin_box(R, P.bin_shift);
       const int nbb = bb.count();
-      for (int j = threadIdx.x; j < nbb; j += blockDim.x) {
+      for (int j = lane; j < nbb; j += 64) {
         uint32_t bin;
         if (!bin_candidate(P, q, N, bb, j, bin)) continue;
         if (!EMIT) {
