@@ -18,6 +18,7 @@
 // host, uploads them with one copy from a pinned staging slot, and launches the
 // batch-native kernels.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -155,8 +156,9 @@ struct vgi_workspace {
   size_t sort_pix_bytes = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  uint8_t* h_read = nullptr;  // pinned readback of DevStats[] + totals[]
+  uint8_t* h_read = nullptr;  // pinned readback of DevStats[] + totals[], then the handoff flag
   size_t h_read_cap = 0;
+  uint32_t pub_seq = 0;  // last handoff sequence number (never 0)
 
   size_t device_bytes() const {
     size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
@@ -281,6 +283,32 @@ int upload(vgi_workspace* w, Slot* s, size_t offset, size_t bytes) {
   if (bytes) HIP_TRY(hipMemcpyAsync(s->d + offset, s->h + offset, bytes, hipMemcpyHostToDevice, w->stream));
   HIP_TRY(hipEventRecord(s->done, w->stream));
   s->pending = true;
+  return VGI_OK;
+}
+
+// Device-to-host handoff of a small region (at most h_read_cap - 64 bytes): k_publish
+// copies it into the pinned buffer w->h_read and then raises a sequence number just past
+// it, which the host spins on. Stream order makes the copy follow all earlier work on the
+// stream, so seeing the number means the region is final. This replaces a device-to-host
+// copy plus a stream synchronize, whose wake-up costs tens of microseconds. A wait longer
+// than kHandoffSpinSeconds falls back to a stream synchronize, which reports a device error
+// if one occurred.
+constexpr int kHandoffSpinSeconds = 5;
+
+int publish_and_wait(vgi_workspace* w, const void* src, size_t bytes, hipStream_t st) {
+  uint32_t* flag = (uint32_t*)(w->h_read + align_up(bytes, 64));
+  if (++w->pub_seq == 0u) w->pub_seq = 1u;
+  const uint32_t seq = w->pub_seq;
+  HIP_TRY(launch_publish(src, w->h_read, bytes, flag, seq, st));
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(kHandoffSpinSeconds)) {
+      HIP_TRY(hipStreamSynchronize(st));
+      if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) return fail(VGI_E_HIP, "device-to-host handoff did not arrive");
+      break;
+    }
+    __builtin_ia32_pause();
+  }
   return VGI_OK;
 }
 
@@ -464,12 +492,14 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
-  if (w->h_read_cap < read_bytes) {
+  if (w->h_read_cap < align_up(read_bytes, 64) + 64) {
     if (w->h_read) (void)hipHostFree(w->h_read);
     w->h_read = nullptr;
     w->h_read_cap = 0;
-    HIP_TRY(hipHostMalloc((void**)&w->h_read, read_bytes * 2, hipHostMallocDefault));
-    w->h_read_cap = read_bytes * 2;
+    const size_t cap = align_up(read_bytes * 2, 64) + 64;
+    HIP_TRY(hipHostMalloc((void**)&w->h_read, cap, hipHostMallocCoherent));
+    std::memset(w->h_read, 0, cap);  // the handoff flag starts at 0, which no handoff uses
+    w->h_read_cap = cap;
   }
 
   // ---- descriptor tables: scenes | meshes | triangle blocks | bin blocks ----
@@ -564,8 +594,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
   HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
   HIP_TRY(launch_totals(B, st));
-  HIP_TRY(hipMemcpyAsync(w->h_read, dstats, read_bytes, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  TRY(publish_and_wait(w, dstats, read_bytes, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
 
   // ---- size the pair lists and the work map from the per-scene totals ----

@@ -294,6 +294,8 @@ hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
 hipError_t launch_totals(const VoxBatch& B, hipStream_t s);
+hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_t* flag_host, uint32_t seq,
+                          hipStream_t s);
 hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s);

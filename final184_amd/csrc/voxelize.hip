@@ -502,6 +502,24 @@ hipError_t launch_totals(const VoxBatch& B, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Copies `words` words to pinned host memory and then stores `seq` into the host flag.
+// One wavefront: the system-scope release orders every lane's copy before the flag, so a
+// host thread that sees the flag (acquire) reads the whole copy. The host waits on the
+// flag instead of a stream synchronize and a device-to-host copy.
+__global__ __launch_bounds__(64) void k_publish(const uint32_t* __restrict__ src, uint32_t* dst, uint32_t words,
+                                                uint32_t* flag, uint32_t seq) {
+  for (uint32_t i = threadIdx.x; i < words; i += 64) dst[i] = gld(src + i);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_t* flag_host, uint32_t seq,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, (const uint32_t*)src, (uint32_t*)dst_host,
+                     (uint32_t)(bytes / 4), flag_host, seq);
+  return hipGetLastError();
+}
+
 // work item -> (scene, bin), from the scanned item offsets
 __global__ __launch_bounds__(256) void k_workmap(VoxBatch B, const VoxScene* __restrict__ scenes) {
   const uint2 blk = bin_block(B);
