@@ -550,7 +550,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     S.big_cursor = zb + 2 * total_bins + p.bin_base;
     S.zero_list = (uint32_t*)w->zero_list + p.bin_base;
     S.bin_packed = (unsigned long long*)p.s->bin_packed;
-    S.pairs = nullptr;
     S.acc = p.s->acc;
     S.mask = p.s->mask;
     S.stats = dstats + i;
@@ -562,7 +561,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     S.bin_shift = p.bin_shift;
     S.nbins_axis = p.nbins_axis;
     S.chunk = tn.chunk_tris;
-    S.pairs_cap = 0;
     S.bin_base = (uint32_t)p.bin_base;
     S.unit_items = (ts == 8 && p.bin_shift == 3) ? kUnitItems : 1u;
     if (multi) {
@@ -589,22 +587,22 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   // bin_small, bin_big, big_cursor and the per-scene counters start at zero
   HIP_TRY(launch_fill_u32(zb, 0u, totals_off / 4, st));
   HIP_TRY(launch_setup(B, (uint32_t)nblk_t, st));
-  HIP_TRY(launch_big(B, false, persistent, st));
+  HIP_TRY(launch_big_count(B, persistent, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
   HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
   HIP_TRY(launch_totals(B, st));
+  HIP_TRY(launch_zero(B, persistent, st));  // bins that several units add into (listed by k_bins)
   TRY(publish_and_wait(w, dstats, read_bytes, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
 
   // ---- size the pair lists and the work map from the per-scene totals ----
   const DevStats* hst = (const DevStats*)w->h_read;
   const unsigned long long* htot = (const unsigned long long*)(w->h_read + (totals_off - stats_off));
-  uint64_t pairs_sum = 0;
-  std::vector<size_t> pair_off(count);
+  uint64_t pairs_sum = 0, big_sum = 0;
   for (uint32_t i = 0; i < count; i++) {
-    pair_off[i] = (size_t)pairs_sum;
     pairs_sum += hst[i].pairs;
+    big_sum += hst[i].big_count;
   }
   if (pairs_sum >= (1ull << 32))
     return fail(VGI_E_RANGE, "batch produces %llu (triangle, bin) pairs; the limit is 2^32 - split the batch",
@@ -612,17 +610,13 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const uint32_t total_work = (uint32_t)(htot[count] >> 32);
   TRY(grow_bytes(&w->pairs, &w->pairs_bytes, std::max<uint64_t>(pairs_sum, 1) * 4, "bin pairs"));
   TRY(grow_bytes(&w->work_map, &w->work_bytes, std::max<size_t>(total_work, 1) * sizeof(uint2), "work map"));
-  for (uint32_t i = 0; i < count; i++) {
-    hs[i].pairs = (uint32_t*)w->pairs + pair_off[i];
-    hs[i].pairs_cap = (uint32_t)hst[i].pairs;
-  }
-  TRY(upload(w, sl, o_scenes, (size_t)count * sizeof(VoxScene)));
+  // the pair list and the work map are kernel arguments: nothing is uploaded after the handoff
   B.work_map = (uint2*)w->work_map;
+  B.pairs_all = (uint32_t*)w->pairs;
+  B.pairs_total = (uint32_t)pairs_sum;
   if (pairs_sum > 0) {
-    HIP_TRY(launch_workmap(B, (uint32_t)nblk_b, st));
-    HIP_TRY(launch_emit_small(B, (uint32_t)nblk_t, st));
-    HIP_TRY(launch_big(B, true, persistent, st));
-    HIP_TRY(launch_zero(B, persistent, st));
+    const uint32_t big_blocks = (uint32_t)std::min<uint64_t>(persistent, (big_sum + 3) / 4);  // 4 waves each
+    HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, st));
   }
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
   // one workgroup per work item: item costs vary by orders of magnitude, and the
@@ -728,11 +722,13 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
       if (src >= g.levels - 1 || nout == 0) break;
     }
   }
+  // one scene: workgroup b of a stage is tile b, so no block table is uploaded
+  const bool table = count > 1;
   size_t nblocks = 0;
   for (const auto& v : stages) nblocks += v.size();
   Carve cv;
   const size_t o_scenes = cv.take((size_t)count * sizeof(MipScene));
-  const size_t o_blocks = cv.take(nblocks * sizeof(MipBlock));
+  const size_t o_blocks = cv.take(table ? nblocks * sizeof(MipBlock) : 0);
   Slot* sl = nullptr;
   TRY(acquire_slot(w, cv.off, &sl));
   MipScene* hs = (MipScene*)(sl->h + o_scenes);
@@ -767,10 +763,11 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   }
   MipBlock* hb = (MipBlock*)(sl->h + o_blocks);
   size_t off = 0;
-  for (const auto& v : stages) {
-    std::memcpy(hb + off, v.data(), v.size() * sizeof(MipBlock));
-    off += v.size();
-  }
+  if (table)
+    for (const auto& v : stages) {
+      std::memcpy(hb + off, v.data(), v.size() * sizeof(MipBlock));
+      off += v.size();
+    }
   TRY(upload(w, sl, 0, cv.off));
   const bool prof = ctx->profiling != 0;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[5], st));
@@ -778,8 +775,10 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   for (size_t j = 0; j < stages.size(); j++) {
     MipBatch B{};
     B.scenes = (const MipScene*)(sl->d + o_scenes);
-    B.blocks = (const MipBlock*)(sl->d + o_blocks) + off;
+    B.blocks = table ? (const MipBlock*)(sl->d + o_blocks) + off : nullptr;
     B.lut = ctx->d_lut;
+    B.src = stages[j][0].src;  // one scene: every block of the stage shares src and nout
+    B.nout = stages[j][0].nout;
     HIP_TRY(launch_mips(B, precision, j == 0, (uint32_t)stages[j].size(), st));
     off += stages[j].size();
   }

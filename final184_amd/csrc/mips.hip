@@ -180,11 +180,11 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
 template <int PREC, bool SRC_ACC>
 __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scenes,
                                                const MipBlock* __restrict__ blocks,
-                                               const float* __restrict__ lut) {
+                                               const float* __restrict__ lut, uint32_t src1, uint32_t nout1) {
   using TX = Texel<PREC>;
   __shared__ float4 s_a[512];
   __shared__ float4 s_b[64];
-  const MipBlock blk = blocks[blockIdx.x];
+  const MipBlock blk = blocks ? blocks[blockIdx.x] : MipBlock{0u, blockIdx.x, src1, nout1};
   const MipScene& P = scenes[blk.scene];
   const uint32_t src = blk.src, nout = blk.nout;
   const uint32_t ns = P.nk[src];
@@ -315,16 +315,16 @@ hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t
   const dim3 g(blocks), b(512);
   switch (prec) {
     case VGI_PREC_FP32:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
       break;
     case VGI_PREC_FP16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
       break;
     case VGI_PREC_BF16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut);
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
       break;
     default:
       return hipErrorInvalidValue;

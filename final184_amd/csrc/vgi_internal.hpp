@@ -72,7 +72,6 @@ struct VoxScene {
   uint32_t* big_cursor;            // large-triangle pairs placed so far
   uint32_t* zero_list;             // bins with more than one work item
   unsigned long long* bin_packed;  // nbins: work units << 32 | pairs (kept with the scene)
-  uint32_t* pairs;                 // triangle ids grouped by bin
   uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
   uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
   DevStats* stats;
@@ -84,7 +83,6 @@ struct VoxScene {
   uint32_t bin_shift;              // log2(bin edge in voxels)
   uint32_t nbins_axis;             // n >> bin_shift
   uint32_t chunk;                  // triangles per work item (<= kMaxChunk)
-  uint32_t pairs_cap;
   uint32_t bin_base;               // first bin of this scene in the batch-wide arrays
   uint32_t unit_items;             // work items (of chunk triangles) per work unit: 1, or
                                    // kUnitItems when the bin is one 8^3 tile
@@ -98,6 +96,8 @@ struct VoxBatch {
   unsigned long long* scan_all;     // exclusive scan of packed_all
   unsigned long long* totals;       // per scene: units << 32 | pairs; [nscenes] = batch total
   uint2* work_map;                  // (scene, bin) of every work unit
+  uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn
+  uint32_t pairs_total;             // entries of pairs_all
   uint32_t nscenes;
   uint32_t total_bins;
 };
@@ -133,8 +133,10 @@ struct MipBlock {
 
 struct MipBatch {
   const MipScene* scenes;
-  const MipBlock* blocks;
-  const float* lut;  // 256 entries: (i/255)^2.2
+  const MipBlock* blocks;  // per workgroup; null for one scene: then block b is tile b of
+                           // scene 0 with the launch's src and nout
+  const float* lut;        // 256 entries: (i/255)^2.2
+  uint32_t src, nout;
 };
 
 // One scene of a trace batch (device memory).
@@ -289,15 +291,14 @@ int check_request_input(const vgi_mesh* meshes, uint32_t num_meshes, const vgi_g
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s);
-hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t s);
+hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
 hipError_t launch_totals(const VoxBatch& B, hipStream_t s);
 hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_t* flag_host, uint32_t seq,
                           hipStream_t s);
-hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s);
-hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks, hipStream_t s);
 hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s);

@@ -8,24 +8,24 @@
 // Every kernel serves a batch of scenes (one scene is a batch of one): per-triangle and
 // per-bin kernels map workgroups to (scene, first item) through a block table, and the
 // scenes' bins are concatenated so that one scan and one persistent voxelize launch
-// cover the whole batch. Pipeline (one stream; the host reads the per-scene totals
-// between steps 4 and 5 to size the pair lists):
+// cover the whole batch. Pipeline (one stream; after step 5 k_publish copies the per-scene
+// totals to pinned host memory and raises a flag the host waits on, and the host sizes the
+// pair list and the work map):
 //   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
 //                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
 //                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
 //                   (bin = B^3 voxels) tests each bin exactly and takes a slot in every
 //                   overlapped bin (ranked per workgroup in LDS, one global atomic per
 //                   distinct bin; the slot is kept); larger triangles go to a list.
-//   2 k_big<false>  one wavefront per listed triangle: its bins, tested cooperatively.
+//   2 k_big_count   one wavefront per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
 //                   bins with > 1 unit are listed.
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
 //                   offset and its first work unit at once; k_totals per scene.
-//   5 k_workmap     work unit -> (scene, bin);
-//     k_emit_small  scatter small triangles to their kept slots (no atomics);
-//     k_big<true>   large triangles append after the small ones of each bin.
-//   6 k_zero        clear the accumulator and occupancy mask of bins with several units.
+//   5 k_zero        clear the accumulator and occupancy mask of bins with several units.
+//   6 k_scatter     one launch: work unit -> (scene, bin); small triangles to their kept
+//                   slots (no atomics); large triangles after the small ones of each bin.
 //   7 k_voxelize    one workgroup per work unit of the batch; per work item (chunk) of it:
 //                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
 //                   projection spread over the workgroup, each column tests its 1-4
@@ -195,17 +195,20 @@ __device__ __forceinline__ uint32_t ctr(const VoxScene& S, uint32_t bin) {
 }
 
 // Workgroup -> (scene, first item) for per-triangle and per-bin kernels.
-__device__ __forceinline__ uint2 tri_block(const VoxBatch& B) {
-  return B.nscenes == 1 ? make_uint2(0u, blockIdx.x * kBlockTris) : B.tri_blocks[blockIdx.x];
+__device__ __forceinline__ uint2 tri_block(const VoxBatch& B, uint32_t blk) {
+  return B.nscenes == 1 ? make_uint2(0u, blk * kBlockTris) : B.tri_blocks[blk];
 }
-__device__ __forceinline__ uint2 bin_block(const VoxBatch& B) {
-  return B.nscenes == 1 ? make_uint2(0u, blockIdx.x * kBlockBins) : B.bin_blocks[blockIdx.x];
+__device__ __forceinline__ uint2 bin_block(const VoxBatch& B, uint32_t blk) {
+  return B.nscenes == 1 ? make_uint2(0u, blk * kBlockBins) : B.bin_blocks[blk];
 }
+__device__ __forceinline__ uint2 tri_block(const VoxBatch& B) { return tri_block(B, blockIdx.x); }
+__device__ __forceinline__ uint2 bin_block(const VoxBatch& B) { return bin_block(B, blockIdx.x); }
 
-// Offset of bin b's first pair within its scene's pair list, and its first work item
-// within the batch.
+// Offset of bin b's first pair in the batch's pair list (the scenes' lists follow each
+// other in bin order, so this is the low half of the batch-wide exclusive scan), and its
+// first work unit within the batch (the high half).
 __device__ __forceinline__ uint32_t pair_base(const VoxBatch& B, const VoxScene& S, uint32_t b) {
-  return (uint32_t)gld(B.scan_all + S.bin_base + b) - (uint32_t)gld(B.scan_all + S.bin_base);
+  return (uint32_t)gld(B.scan_all + S.bin_base + b);
 }
 __device__ __forceinline__ uint32_t first_item(const VoxBatch& B, const VoxScene& S, uint32_t b) {
   return (uint32_t)(gld(B.scan_all + S.bin_base + b) >> 32);
@@ -410,11 +413,12 @@ hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 // 2 large triangles: count (then scatter) their bins, one wavefront per triangle
 // ---------------------------------------------------------------------------
 
+// Large triangles with wavefront ids wave, wave + nwaves, ...; EMIT false counts their
+// pairs per bin, EMIT true places them after the bin's small-triangle pairs.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_big(VoxBatch B, const VoxScene* __restrict__ scenes) {
+__device__ __forceinline__ void big_pass(const VoxBatch& B, const VoxScene* __restrict__ scenes, uint32_t wave,
+                                         uint32_t nwaves) {
   // one wavefront per triangle: most large triangles span a few dozen bins
-  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   const int lane = (int)(threadIdx.x & 63u);
   for (uint32_t sc = 0; sc < B.nscenes; sc++) {
     const VoxScene& P = scenes[sc];
@@ -434,16 +438,19 @@ __global__ __launch_bounds__(256) void k_big(VoxBatch B, const VoxScene* __restr
         } else {
           const uint32_t c = ctr(P, bin);
           const uint32_t pos = pair_base(B, P, bin) + gld(P.bin_small + c) + gatomic_add(P.big_cursor + c, 1u);
-          if (pos < P.pairs_cap) gst(P.pairs + pos, t);
+          if (pos < B.pairs_total) gst(B.pairs_all + pos, t);
         }
       }
     }
   }
 }
 
-hipError_t launch_big(const VoxBatch& B, bool emit, uint32_t blocks, hipStream_t s) {
-  if (emit) hipLaunchKernelGGL(k_big<true>, dim3(blocks), dim3(256), 0, s, B, B.scenes);
-  else hipLaunchKernelGGL(k_big<false>, dim3(blocks), dim3(256), 0, s, B, B.scenes);
+__global__ __launch_bounds__(256) void k_big_count(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  big_pass<false>(B, scenes, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+}
+
+hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_big_count, dim3(blocks), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -520,9 +527,9 @@ hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_
   return hipGetLastError();
 }
 
-// work item -> (scene, bin), from the scanned item offsets
-__global__ __launch_bounds__(256) void k_workmap(VoxBatch B, const VoxScene* __restrict__ scenes) {
-  const uint2 blk = bin_block(B);
+// work unit -> (scene, bin), from the scanned unit offsets; workgroup blk of the bins
+__device__ __forceinline__ void workmap_block(const VoxBatch& B, const VoxScene* __restrict__ scenes, uint32_t blk_id) {
+  const uint2 blk = bin_block(B, blk_id);
   const VoxScene& P = scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   const uint32_t b = blk.y + threadIdx.x;
@@ -532,17 +539,12 @@ __global__ __launch_bounds__(256) void k_workmap(VoxBatch B, const VoxScene* __r
   for (uint32_t i = 0; i < items; i++) gst2(B.work_map + first + i, make_uint2(blk.x, b));
 }
 
-hipError_t launch_workmap(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_workmap, dim3(blocks), dim3(kBlockBins), 0, s, B, B.scenes);
-  return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // 5 scatter small triangles into their slots
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_emit_small(VoxBatch B, const VoxScene* __restrict__ scenes) {
-  const uint2 blk = tri_block(B);
+__device__ __forceinline__ void emit_small_block(const VoxBatch& B, const VoxScene* __restrict__ scenes, uint32_t blk_id) {
+  const uint2 blk = tri_block(B, blk_id);
   const VoxScene& P = scenes[blk.x];
   const uint32_t t = blk.y + threadIdx.x;
   if (t >= P.ntri) return;
@@ -557,13 +559,30 @@ __global__ __launch_bounds__(256) void k_emit_small(VoxBatch B, const VoxScene* 
     uint32_t bin;
     if (!bin_candidate(P, q, N, bb, j, bin)) continue;
     const uint32_t pos = pair_base(B, P, bin) + gld(P.tri_slot + (size_t)t * kSmallBins + nslot++);
-    if (pos < P.pairs_cap) gst(P.pairs + pos, t);
+    if (pos < B.pairs_total) gst(B.pairs_all + pos, t);
   }
 }
 
-hipError_t launch_emit_small(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
+// One launch for the three steps that need the sized pair list and work map: workgroups
+// [0, nblk_b) map work units, [nblk_b, nblk_b + nblk_t) scatter small triangles, the rest
+// place large triangles (one wavefront each). They write disjoint outputs.
+__global__ __launch_bounds__(256) void k_scatter(VoxBatch B, const VoxScene* __restrict__ scenes, uint32_t nblk_b,
+                                                 uint32_t nblk_t) {
+  const uint32_t b = blockIdx.x;
+  if (b < nblk_b) {
+    workmap_block(B, scenes, b);
+  } else if (b < nblk_b + nblk_t) {
+    emit_small_block(B, scenes, b - nblk_b);
+  } else {
+    const uint32_t w0 = b - nblk_b - nblk_t, nbig_blocks = gridDim.x - nblk_b - nblk_t;
+    big_pass<true>(B, scenes, w0 * (blockDim.x >> 6) + (threadIdx.x >> 6), nbig_blocks * (blockDim.x >> 6));
+  }
+}
+
+hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks, hipStream_t s) {
+  const uint32_t blocks = nblk_b + nblk_t + big_blocks;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit_small, dim3(blocks), dim3(kBlockTris), 0, s, B, B.scenes);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, B, B.scenes, nblk_b, nblk_t);
   return hipGetLastError();
 }
 
@@ -770,7 +789,7 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
       const uint32_t first = base + it * P.chunk;
       const uint32_t cnt = min(P.chunk, cnt_bin - it * P.chunk);
       if (tid < cnt) {
-        const uint32_t t = gld(P.pairs + first + tid);
+        const uint32_t t = gld(Bt.pairs_all + first + tid);
         const TriRec R = gld_rec(P.tris + t);
         make_coef(R, s_tri[tid]);
       }
