@@ -88,6 +88,17 @@ __device__ __forceinline__ int cvt_floor(float x) {
   return r;
 }
 
+// a * b + c with 24-bit signed factors, one full-rate instruction (a 32-bit multiply issues
+// at a quarter of the rate); B_SGPR: b is wave-uniform and held in an SGPR
+template <bool B_SGPR>
+__device__ __forceinline__ uint32_t mad_i24(int a, uint32_t b, uint32_t c) {
+  uint32_t d;
+  if constexpr (B_SGPR) __asm__("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  else __asm__("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+template <bool UNIFORM_LEVEL>
 __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, float uy, float uz,
                                           float wl) {
   const float sc = __builtin_ldexpf(1.0f, -(int)k);
@@ -98,8 +109,9 @@ __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, flo
               fz = __builtin_amdgcn_fractf(gz);
   const float ex = 1.0f - fx, ey = 1.0f - fy;
   Foot f;
-  // +1: padded border; every factor is < 2^11, so 24-bit multiplies are exact
-  const uint32_t row = __umul24(p1, (uint32_t)(iz + 1)) + (uint32_t)(iy + 1);
+  // +1: padded border; iz >= -1 and every factor is < 2^11, so 24-bit multiplies are exact.
+  // row = (iz + 1) p1 + iy + 1, written as iz p1 + (p1 + iy + 1) for one multiply-add
+  const uint32_t row = mad_i24<UNIFORM_LEVEL>(iz, p1, p1 + (uint32_t)(iy + 1));
   f.idx = __umul24(p1, row) + (uint32_t)(ix + 1);  // < 1026^3 < 2^31
   f.w00 = ex * ey;
   f.w10 = fx * ey;
@@ -126,7 +138,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
-  const Foot f = footprint(k, p1, ux, uy, uz, wl);
+  const Foot f = footprint<true>(k, p1, ux, uy, uz, wl);
   const uint32_t off = f.idx * TB;  // levels on this path are < 2^31 bytes
   const int soff[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
@@ -177,7 +189,7 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
-  const Foot f = footprint(k, p1, ux, uy, uz, wl);
+  const Foot f = footprint<false>(k, p1, ux, uy, uz, wl);
   const char* base = (const char*)P.level[k] + (size_t)f.idx * TB;
   const size_t offs[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
