@@ -875,6 +875,37 @@ void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tunin
   P.out_s = (float4*)it.out_specular;
 }
 
+// March schedule of a cone whose sample distances do not depend on the pixel (a fixed
+// aperture): the per-pixel recurrence of the cone march evaluated once, in FP32 -- diameter
+// d = max(h, 2 t tan), level max(0, log2(d / h)), next t = t + step d -- while
+// t <= max_distance, for at most VGI_MAX_MARCH_STEPS samples.
+void march_schedule(const TraceParams& P, float tan_half, std::vector<SchedEntry>& out) {
+  out.clear();
+  const float top = (float)(P.levels - 1);
+  float t = P.start_off * P.h;
+  for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= P.max_dist; it++) {
+    const float twice = 2.0f * t;
+    const float diam = std::max(P.h, twice * tan_half);
+    const float lvl = std::max(0.0f, std::log2(diam * P.inv_h));
+    SchedEntry e;
+    if (lvl >= top) {
+      e.t = t;
+      e.fr = 1.0f;
+      e.gr = 0.0f;
+      e.code = kSchedTop | (P.levels - 1);
+    } else {
+      const float fl = std::floor(lvl);
+      e.t = t;
+      e.fr = lvl - fl;
+      e.gr = 1.0f - e.fr;
+      e.code = (uint32_t)fl;
+    }
+    out.push_back(e);
+    const float adv = P.step * diam;
+    t = t + adv;
+  }
+}
+
 int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, void* stream) {
   if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
   if (count == 0) return VGI_OK;
@@ -887,16 +918,25 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   for (uint32_t i = 0; i < count; i++) group[items[i].scene->prec].push_back(i);
   struct Launch {
     uint32_t prec, nscenes;
-    size_t o_params, o_blocks, o_seg, nblocks, entries;
+    size_t o_params, o_blocks, o_seg, o_sched, nblocks, entries, nsched;
     bool buffer_path;
   };
+  // per item: its parameters and the schedules of its fixed-aperture cones
+  std::vector<TraceParams> tp(count);
+  std::vector<std::vector<SchedEntry>> sched(2 * (size_t)count);
+  for (uint32_t i = 0; i < count; i++) {
+    fill_trace_params(tp[i], items[i], tn);
+    if (tp[i].num_diffuse > 0) march_schedule(tp[i], tp[i].tan_d, sched[2 * i]);
+    if (tp[i].specular && !tp[i].rough) march_schedule(tp[i], tp[i].tan_s, sched[2 * i + 1]);
+  }
   std::vector<Launch> launches;
   Carve cv;
   size_t max_entries = 0;
   for (uint32_t p = 0; p < 3; p++) {
     if (group[p].empty()) continue;
-    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, true};
+    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, 0, 0, true};
     for (uint32_t i : group[p]) {
+      L.nsched += sched[2 * i].size() + sched[2 * i + 1].size();
       const vgi_gbuffer& gb = items[i].gbuffer;
       if (morton) L.entries += align_up((size_t)gb.width * gb.height, kSortBlock);
       else L.nblocks += (size_t)ceil_div(gb.width, tn.tile_w) * ceil_div(gb.height, tn.tile_h);
@@ -909,6 +949,7 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
     L.o_params = cv.take((size_t)L.nscenes * sizeof(TraceParams));
     L.o_blocks = cv.take(!morton && L.nscenes > 1 ? L.nblocks * sizeof(uint2) : 0);
     L.o_seg = cv.take(morton ? ((size_t)L.nscenes + 1) * sizeof(uint32_t) : 0);
+    L.o_sched = cv.take(L.nsched * sizeof(SchedEntry));
     max_entries = std::max(max_entries, L.entries);
     launches.push_back(L);
   }
@@ -931,10 +972,20 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
     TraceParams* hp = (TraceParams*)(sl->h + L.o_params);
     uint2* hb = (uint2*)(sl->h + L.o_blocks);
     uint32_t* hs = (uint32_t*)(sl->h + L.o_seg);
-    size_t b = 0, e = 0;
+    SchedEntry* hsch = (SchedEntry*)(sl->h + L.o_sched);
+    const SchedEntry* dsch = (const SchedEntry*)(sl->d + L.o_sched);
+    size_t b = 0, e = 0, ns = 0;
     for (uint32_t j = 0; j < L.nscenes; j++) {
-      const vgi_trace_item& it = items[group[L.prec][j]];
-      fill_trace_params(hp[j], it, tn);
+      const uint32_t i = group[L.prec][j];
+      const vgi_trace_item& it = items[i];
+      hp[j] = tp[i];
+      for (int c = 0; c < 2; c++) {
+        const std::vector<SchedEntry>& v = sched[2 * i + c];
+        if (!v.empty()) std::memcpy(hsch + ns, v.data(), v.size() * sizeof(SchedEntry));
+        hp[j].sched[c] = dsch + ns;
+        hp[j].sched_len[c] = (uint32_t)v.size();
+        ns += v.size();
+      }
       if (morton) {
         hs[j] = (uint32_t)e;
         e += align_up((size_t)it.gbuffer.width * it.gbuffer.height, kSortBlock);

@@ -211,82 +211,125 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
   }
 }
 
-__device__ __forceinline__ float uniform_f(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+// Compositing of one sample (front to back). The sample's opacity is a convex combination
+// of values in [0, 1]; its rounded weights can sum to 1 + ulp, and the clamp keeps the
+// accumulated a within [0, 1].
+__device__ __forceinline__ void composite(float4 s, float& cr, float& cg, float& cb, float& a) {
+  s.w = fminf(s.w, 1.0f);
+  const float w = 1.0f - a;
+  cr = fmaf(w, s.x, cr);
+  cg = fmaf(w, s.y, cg);
+  cb = fmaf(w, s.z, cb);
+  a = fmaf(w, s.w, a);
 }
 
-// One cone. UNIFORM: the schedule (t, diameter, level) is the same for every active lane
-// of the wavefront (it depends only on start, step, aperture and voxel size).
-template <int PREC, bool UNIFORM>
-__device__ __forceinline__ float4 cone_march(const TraceParams& P, float ox, float oy, float oz,
-                                             float dx, float dy, float dz, float tan_half,
-                                             uint32_t& count) {
+// inside [0, n]^3: one minimum3/maximum3 pair instead of six compares; these propagate NaN
+// (fminf/fmaxf would drop it), so a NaN in any coordinate ends the march
+__device__ __forceinline__ bool inside_grid(float ux, float uy, float uz, float nf) {
+  const float umin = __builtin_elementwise_minimum(__builtin_elementwise_minimum(ux, uy), uz);
+  const float umax = __builtin_elementwise_maximum(__builtin_elementwise_maximum(ux, uy), uz);
+  return umin >= 0.f && umax <= nf;
+}
+
+// One fixed-aperture cone: its samples (distance, levels, level weights) come from the
+// schedule the host computed once for the aperture, read with scalar loads, so the whole
+// wavefront takes each sample at the same distance and level with no per-lane schedule
+// arithmetic. Sample: quadrilinear gather (levels l0 + 1 and l0, or the top level alone),
+// then front-to-back compositing; the march ends at the schedule's end (max distance),
+// the grid boundary or opacity >= cutoff.
+//
+// The coarse footprint (level l0 + 1) holds the parents of every fine (level l0) footprint
+// texel; a parent with zero opacity has an all-zero subtree (texels are box averages of
+// non-negative values and opacity cannot underflow to zero), so when the coarse
+// footprint's opacity is zero the fine gather is exactly zero and is skipped.
+template <int PREC>
+__device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const SchedEntry* __restrict__ sched,
+                                                   uint32_t nsched, float ox, float oy, float oz, float dx,
+                                                   float dy, float dz, uint32_t& count) {
+  float cr = 0.f, cg = 0.f, cb = 0.f, a = 0.f;
+  const float nf = (float)P.n;
+  // cone origin and direction in level-0 voxel units: u(t) = uo + ud t
+  const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
+  const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
+  const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
+  const float cutoff = P.cutoff;
+  // Wave-uniform loop control: the sample index is the same for every lane (the entry is
+  // read with scalar loads), lanes whose march has ended idle under the exec mask, and the
+  // loop ends when none is left.
+  bool alive = true;
+  for (uint32_t it = 0; it < nsched; it++) {
+    const SchedEntry e = sld_rec(sched + it);  // scalar load: the index is wave-uniform
+    const float ux = fmaf(udx, e.t, uox);
+    const float uy = fmaf(udy, e.t, uoy);
+    const float uz = fmaf(udz, e.t, uoz);
+    alive = alive && inside_grid(ux, uy, uz, nf);
+    if (alive) {
+      float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t abits = 0;
+      if (e.code & kSchedTop) {
+        gather_uniform<PREC>(P, e.code & ~kSchedTop, ux, uy, uz, 1.0f, s, abits);
+        count += 1;
+      } else {
+        gather_uniform<PREC>(P, e.code + 1, ux, uy, uz, e.fr, s, abits);
+        if (abits) gather_uniform<PREC>(P, e.code, ux, uy, uz, e.gr, s, abits);
+        count += 2;
+      }
+      composite(s, cr, cg, cb, a);
+      alive = a < cutoff;
+    }
+    if (!__any(alive)) break;
+  }
+  return make_float4(cr, cg, cb, a);
+}
+
+// One cone with a per-pixel aperture (the roughness-driven specular cone), or any cone when
+// a level is too large for 32-bit buffer offsets (BUF false: every gather then uses 64-bit
+// addresses): sample diameter d(t) = max(h, 2 t tan), level log2(d / h), step
+// t += step_factor * d, computed per lane.
+template <int PREC, bool BUF>
+__device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox, float oy, float oz, float dx,
+                                                  float dy, float dz, float tan_half, uint32_t& count) {
   float cr = 0.f, cg = 0.f, cb = 0.f, a = 0.f;
   float t = P.start_off * P.h;
   const float nf = (float)P.n;
   const float top = (float)(P.levels - 1);
-  // the step cap bounds the loop even when t stops growing in float (zero aperture,
-  // zero direction, huge max distance); no workload comes near it
-  // cone origin and direction in level-0 voxel units: u(t) = uo + ud t
   const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
   const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
   const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
   // loop invariants, read once (otherwise reloaded from the descriptor every sample)
   const float h = P.h, inv_h = P.inv_h, step = P.step, cutoff = P.cutoff, max_dist = P.max_dist;
+  // the step cap bounds the loop even when t stops growing in float (zero aperture,
+  // zero direction, huge max distance); no workload comes near it
   for (int it = 0; it < VGI_MAX_MARCH_STEPS && t <= max_dist; it++) {
     const float ux = fmaf(udx, t, uox);
     const float uy = fmaf(udy, t, uoy);
     const float uz = fmaf(udz, t, uoz);
-    // inside [0, n]^3: one minimum3/maximum3 pair instead of six compares; these propagate
-    // NaN (fminf/fmaxf would drop it), so a NaN in any coordinate ends the march
-    const float umin = __builtin_elementwise_minimum(__builtin_elementwise_minimum(ux, uy), uz);
-    const float umax = __builtin_elementwise_maximum(__builtin_elementwise_maximum(ux, uy), uz);
-    if (!(umin >= 0.f && umax <= nf)) break;
-    float diam = fmaxf(h, 2.0f * t * tan_half);
-    float lvl = fmaxf(0.0f, __log2f(diam * inv_h));
-    if (UNIFORM) {
-      diam = uniform_f(diam);
-      lvl = uniform_f(lvl);
-    }
+    if (!inside_grid(ux, uy, uz, nf)) break;
+    const float diam = fmaxf(h, 2.0f * t * tan_half);
+    const float lvl = fmaxf(0.0f, __log2f(diam * inv_h));
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t abits = 0;
     if (lvl >= top) {
-      gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // the level is uniform
+      if constexpr (BUF) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // uniform level
+      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
       count += 1;
     } else {
       const float fl = floorf(lvl);
       const float fr = lvl - fl;
-      uint32_t l0 = (uint32_t)fl;
-      // The coarse footprint (level l0+1) holds the parents of every fine (level l0)
-      // footprint texel; a parent with zero opacity has an all-zero subtree (texels are
-      // box averages of non-negative values and opacity cannot underflow to zero), so
-      // when the coarse footprint's opacity is zero the fine gather is exactly zero.
-      if (UNIFORM) {
-        l0 = __builtin_amdgcn_readfirstlane(l0);
-        gather_uniform<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-        if (abits) gather_uniform<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+      const uint32_t l0 = (uint32_t)fl;
+      // per-lane schedules usually still agree on the level across the wavefront: then the
+      // level's descriptor and pitches are scalars
+      const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
+      if (BUF && __all(l0 == lu)) {
+        gather_uniform<PREC>(P, lu + 1, ux, uy, uz, fr, s, abits);
+        if (abits) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
       } else {
-        // per-lane schedules (the roughness-driven specular cone) usually still agree on the
-        // level across the wavefront: then the level's descriptor and pitches are scalars
-        const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
-        if (__all(l0 == lu)) {
-          gather_uniform<PREC>(P, lu + 1, ux, uy, uz, fr, s, abits);
-          if (abits) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
-        } else {
-          gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-          if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
-        }
+        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
+        if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       }
       count += 2;
     }
-    // the sample's opacity is a convex combination of values in [0, 1]; its rounded
-    // weights can sum to 1 + ulp, and the clamp keeps the front-to-back a within [0, 1]
-    s.w = fminf(s.w, 1.0f);
-    const float w = 1.0f - a;
-    cr = fmaf(w, s.x, cr);
-    cg = fmaf(w, s.y, cg);
-    cb = fmaf(w, s.z, cb);
-    a = fmaf(w, s.w, a);
+    composite(s, cr, cg, cb, a);
     if (a >= cutoff) break;
     t = t + step * diam;
   }
@@ -352,7 +395,8 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams*
         const float dx = (t0 * lx + b0 * ly) + nx * lz;
         const float dy = (t1 * lx + b1 * ly) + ny * lz;
         const float dz = (t2 * lx + b2 * ly) + nz * lz;
-        const float4 r = cone_march<PREC, BUF>(P, ox, oy, oz, dx, dy, dz, P.tan_d, count);
+        const float4 r = BUF ? cone_march_fixed<PREC>(P, P.sched[0], P.sched_len[0], ox, oy, oz, dx, dy, dz, count)
+                             : cone_march_lane<PREC, BUF>(P, ox, oy, oz, dx, dy, dz, P.tan_d, count);
         const float wc = P.w[c];
         ar = fmaf(wc, r.x, ar);
         ag = fmaf(wc, r.y, ag);
@@ -369,9 +413,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams*
         const float rx = ix - 2.0f * dn * nx, ry = iy - 2.0f * dn * ny, rz = iz - 2.0f * dn * nz;
         if (P.rough) {
           const float th = fminf(fmaxf(gld(P.rough + p), 0.0087f), 1.0f);
-          os = cone_march<PREC, false>(P, ox, oy, oz, rx, ry, rz, th, count);
+          os = cone_march_lane<PREC, BUF>(P, ox, oy, oz, rx, ry, rz, th, count);
         } else {
-          os = cone_march<PREC, BUF>(P, ox, oy, oz, rx, ry, rz, P.tan_s, count);
+          os = BUF ? cone_march_fixed<PREC>(P, P.sched[1], P.sched_len[1], ox, oy, oz, rx, ry, rz, count)
+                   : cone_march_lane<PREC, BUF>(P, ox, oy, oz, rx, ry, rz, P.tan_s, count);
         }
       }
     }

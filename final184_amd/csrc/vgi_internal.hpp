@@ -140,6 +140,16 @@ struct MipBatch {
 };
 
 // One scene of a trace batch (device memory).
+// One sample of a fixed-aperture cone's march (the same for every pixel): distance t, the
+// level fraction fr and 1 - fr, and the lower level l0, or kSchedTop | top level for a
+// sample taken at the top level alone.
+struct SchedEntry {
+  float t, fr, gr;
+  uint32_t code;
+};
+constexpr uint32_t kSchedTop = 0x80000000u;
+static_assert(sizeof(SchedEntry) == 16, "schedule entries are read as 16-byte scalar loads");
+
 struct TraceParams {
   const void* level[VGI_MAX_LEVELS];
   uint64_t level_bytes[VGI_MAX_LEVELS];
@@ -157,6 +167,8 @@ struct TraceParams {
   float eye[3];
   float dirs[VGI_MAX_DIFFUSE_CONES][3];
   float w[VGI_MAX_DIFFUSE_CONES];
+  const SchedEntry* sched[2];  // march schedules: [0] diffuse cones, [1] specular cone
+  uint32_t sched_len[2];       // (no roughness buffer); filled by the host
   const float* pos;
   const float* nrm;
   const float* rough;
