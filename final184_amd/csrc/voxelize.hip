@@ -745,10 +745,12 @@ struct TileLayout {
   static constexpr int ENTRIES = (TS - 1) * (1 + PY + PZ) + 1;
 };
 
-// Registers: the 8^3 tile's unit accumulators (14 per thread) fit without spills at 4
-// waves per SIMD (4 resident workgroups per CU; LDS, 25.5 KB each, would allow 6).
+// Registers: the 8^3 tile's unit accumulators (14 per thread) fit without spills at 5
+// waves per SIMD (96 VGPRs; 5 resident workgroups per CU; LDS, 25.5 KB each, would allow
+// 6, where the registers would spill). The kernel is latency-bound (dependent loads of
+// pair ids and triangle records, and barriers per chunk), so resident workgroups count.
 template <int TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 4 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 5 : 1)))
 void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
