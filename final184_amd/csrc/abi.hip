@@ -941,7 +941,8 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
       if (morton) L.entries += align_up((size_t)gb.width * gb.height, kSortBlock);
       else L.nblocks += (size_t)ceil_div(gb.width, tn.tile_w) * ceil_div(gb.height, tn.tile_h);
       for (uint32_t k = 0; k < items[i].scene->grid.levels; k++)
-        L.buffer_path = L.buffer_path && items[i].scene->level_bytes[k] < 0x7FFFFFF0ull;  // 31-bit offsets
+        // buffer gathers: unsigned 32-bit byte offsets and a 32-bit record count
+        L.buffer_path = L.buffer_path && items[i].scene->level_bytes[k] < 0xFFFFFFF0ull;
     }
     if (morton) L.nblocks = L.entries / kSortBlock;
     if (L.nblocks >= (1ull << 31) || L.entries >= (1ull << 32))

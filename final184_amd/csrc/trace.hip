@@ -139,7 +139,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
   const uint32_t p1 = P.nk[k] + 2;
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
   const Foot f = footprint<true>(k, p1, ux, uy, uz, wl);
-  const uint32_t off = f.idx * TB;  // levels on this path are < 2^31 bytes
+  const uint32_t off = f.idx * TB;  // levels on this path are < 2^32 bytes
   const int soff[4] = {0, rowb, slab, rowb + slab};
   const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
   if constexpr (PREC == VGI_PREC_FP32) {
@@ -160,7 +160,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
     }
   }
   const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)P.level_bytes[k], 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)(uint32_t)P.level_bytes[k], 0x00020000);  // unsigned field
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const float wz = q < 2 ? f.wz0 : f.wz1;
