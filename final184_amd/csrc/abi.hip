@@ -286,20 +286,21 @@ int upload(vgi_workspace* w, Slot* s, size_t offset, size_t bytes) {
   return VGI_OK;
 }
 
-// Device-to-host handoff of a small region (at most h_read_cap - 64 bytes): k_publish
-// copies it into the pinned buffer w->h_read and then raises a sequence number just past
-// it, which the host spins on. Stream order makes the copy follow all earlier work on the
-// stream, so seeing the number means the region is final. This replaces a device-to-host
-// copy plus a stream synchronize, whose wake-up costs tens of microseconds. A wait longer
-// than kHandoffSpinSeconds falls back to a stream synchronize, which reports a device error
-// if one occurred.
+// Device-to-host handoff of the voxelize totals: k_publish writes the DevStats words and
+// the per-scene totals into the pinned buffer w->h_read (stats at 0, totals at
+// totals_at) and then raises a sequence number at flag_at, which the host spins on. Stream
+// order makes k_publish follow all earlier work on the stream, so seeing the number means
+// the values are final. This replaces a device-to-host copy plus a stream synchronize,
+// whose wake-up costs tens of microseconds. A wait longer than kHandoffSpinSeconds falls
+// back to a stream synchronize, which reports a device error if one occurred.
 constexpr int kHandoffSpinSeconds = 5;
 
-int publish_and_wait(vgi_workspace* w, const void* src, size_t bytes, hipStream_t st) {
-  uint32_t* flag = (uint32_t*)(w->h_read + align_up(bytes, 64));
+int publish_and_wait(vgi_workspace* w, const VoxBatch& B, const void* dstats, size_t stats_bytes, size_t totals_at,
+                     size_t flag_at, hipStream_t st) {
+  uint32_t* flag = (uint32_t*)(w->h_read + flag_at);
   if (++w->pub_seq == 0u) w->pub_seq = 1u;
   const uint32_t seq = w->pub_seq;
-  HIP_TRY(launch_publish(src, w->h_read, bytes, flag, seq, st));
+  HIP_TRY(launch_publish(B, dstats, stats_bytes, w->h_read, w->h_read + totals_at, flag, seq, st));
   const auto t0 = std::chrono::steady_clock::now();
   while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
     if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(kHandoffSpinSeconds)) {
@@ -474,10 +475,14 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
 
   // ---- scratch sized for this batch (grown; queued work on this stream is drained
   // first so nothing still reads a block being replaced) ----
+  // device: the three per-bin counter arrays, then DevStats[count] (all cleared per call);
+  // host handoff buffer: DevStats[count], the totals[count + 1], the flag
   const size_t stats_off = align_up(3 * total_bins * 4, 256);
-  const size_t totals_off = stats_off + (size_t)count * sizeof(DevStats);
-  const size_t zeroed_need = totals_off + ((size_t)count + 1) * 8;
-  const size_t read_bytes = zeroed_need - stats_off;
+  const size_t stats_bytes = (size_t)count * sizeof(DevStats);
+  const size_t zeroed_need = stats_off + stats_bytes;
+  const size_t host_totals_at = stats_bytes;
+  const size_t read_bytes = stats_bytes + ((size_t)count + 1) * 8;
+  const size_t flag_at = align_up(read_bytes, 64);
   size_t scan_need = 0;
   HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(total_bins + 1), st));
   const size_t tneed = std::max<size_t>(total_tri, 1);
@@ -492,7 +497,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
-  if (w->h_read_cap < align_up(read_bytes, 64) + 64) {
+  if (w->h_read_cap < flag_at + 64) {
     if (w->h_read) (void)hipHostFree(w->h_read);
     w->h_read = nullptr;
     w->h_read_cap = 0;
@@ -517,7 +522,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   uint2* hbb = (uint2*)(sl->h + o_bblk);
   uint32_t* zb = (uint32_t*)w->zeroed;
   DevStats* dstats = (DevStats*)((uint8_t*)w->zeroed + stats_off);
-  unsigned long long* dtotals = (unsigned long long*)((uint8_t*)w->zeroed + totals_off);
   size_t tb = 0, bb = 0;
   for (uint32_t i = 0; i < count; i++) {
     const VoxPlan& p = plan[i];
@@ -576,7 +580,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.bin_blocks = multi ? (const uint2*)(sl->d + o_bblk) : nullptr;
   B.packed_all = (unsigned long long*)w->packed_all;
   B.scan_all = (unsigned long long*)w->scan_all;
-  B.totals = dtotals;
   B.work_map = nullptr;
   B.nscenes = count;
   B.total_bins = (uint32_t)total_bins;
@@ -585,24 +588,23 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const bool prof = ctx->profiling != 0;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
   // bin_small, bin_big, big_cursor and the per-scene counters start at zero
-  HIP_TRY(launch_fill_u32(zb, 0u, totals_off / 4, st));
+  HIP_TRY(launch_fill_u32(zb, 0u, zeroed_need / 4, st));
   HIP_TRY(launch_setup(B, (uint32_t)nblk_t, st));
   HIP_TRY(launch_big_count(B, persistent, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
   HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
-  HIP_TRY(launch_totals(B, st));
-  HIP_TRY(launch_zero(B, persistent, st));  // bins that several units add into (listed by k_bins)
-  TRY(publish_and_wait(w, dstats, read_bytes, st));
+  TRY(publish_and_wait(w, B, dstats, stats_bytes, host_totals_at, flag_at, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
 
   // ---- size the pair lists and the work map from the per-scene totals ----
   const DevStats* hst = (const DevStats*)w->h_read;
-  const unsigned long long* htot = (const unsigned long long*)(w->h_read + (totals_off - stats_off));
-  uint64_t pairs_sum = 0, big_sum = 0;
+  const unsigned long long* htot = (const unsigned long long*)(w->h_read + host_totals_at);
+  uint64_t pairs_sum = 0, big_sum = 0, zero_sum = 0;
   for (uint32_t i = 0; i < count; i++) {
     pairs_sum += hst[i].pairs;
     big_sum += hst[i].big_count;
+    zero_sum += hst[i].zero_count;
   }
   if (pairs_sum >= (1ull << 32))
     return fail(VGI_E_RANGE, "batch produces %llu (triangle, bin) pairs; the limit is 2^32 - split the batch",
@@ -616,7 +618,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.pairs_total = (uint32_t)pairs_sum;
   if (pairs_sum > 0) {
     const uint32_t big_blocks = (uint32_t)std::min<uint64_t>(persistent, (big_sum + 3) / 4);  // 4 waves each
-    HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, st));
+    const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(persistent, zero_sum);      // bins to clear
+    HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
   }
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
   // one workgroup per work item: item costs vary by orders of magnitude, and the

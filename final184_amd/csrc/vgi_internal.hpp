@@ -94,7 +94,6 @@ struct VoxBatch {
   const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
   unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
   unsigned long long* scan_all;     // exclusive scan of packed_all
-  unsigned long long* totals;       // per scene: units << 32 | pairs; [nscenes] = batch total
   uint2* work_map;                  // (scene, bin) of every work unit
   uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn
   uint32_t pairs_total;             // entries of pairs_all
@@ -307,11 +306,10 @@ hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
-hipError_t launch_totals(const VoxBatch& B, hipStream_t s);
-hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_t* flag_host, uint32_t seq,
-                          hipStream_t s);
-hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks, hipStream_t s);
-hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s);
+hipError_t launch_publish(const VoxBatch& B, const void* stats, size_t stats_bytes, void* dst_stats,
+                          void* dst_totals, uint32_t* flag_host, uint32_t seq, hipStream_t s);
+hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,
+                          uint32_t zero_blocks, hipStream_t s);
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
                            hipStream_t s);
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s);

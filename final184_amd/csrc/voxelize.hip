@@ -8,7 +8,7 @@
 // Every kernel serves a batch of scenes (one scene is a batch of one): per-triangle and
 // per-bin kernels map workgroups to (scene, first item) through a block table, and the
 // scenes' bins are concatenated so that one scan and one persistent voxelize launch
-// cover the whole batch. Pipeline (one stream; after step 5 k_publish copies the per-scene
+// cover the whole batch. Pipeline (one stream; after step 4 k_publish writes the per-scene
 // totals to pinned host memory and raises a flag the host waits on, and the host sizes the
 // pair list and the work map):
 //   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
@@ -22,11 +22,12 @@
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
 //                   bins with > 1 unit are listed.
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
-//                   offset and its first work unit at once; k_totals per scene.
-//   5 k_zero        clear the accumulator and occupancy mask of bins with several units.
-//   6 k_scatter     one launch: work unit -> (scene, bin); small triangles to their kept
-//                   slots (no atomics); large triangles after the small ones of each bin.
-//   7 k_voxelize    one workgroup per work unit of the batch; per work item (chunk) of it:
+//                   offset and its first work unit at once; k_publish adds the per-scene
+//                   totals and hands them to the host.
+//   5 k_scatter     one launch: work unit -> (scene, bin); small triangles to their kept
+//                   slots (no atomics); large triangles after the small ones of each bin;
+//                   clear the accumulator and occupancy mask of bins with several units.
+//   6 k_voxelize    one workgroup per work unit of the batch; per work item (chunk) of it:
 //                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
 //                   projection spread over the workgroup, each column tests its 1-4
 //                   candidate voxels with the exact separating-axis test and adds packed
@@ -491,39 +492,37 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
   return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
 }
 
-// per scene: (units << 32 | pairs) of its bins, and the batch total in [nscenes]
-__global__ void k_totals(VoxBatch B, const VoxScene* __restrict__ scenes) {
-  const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sc > B.nscenes) return;
-  if (sc == B.nscenes) {
-    B.totals[sc] = B.scan_all[B.total_bins];
-    return;
-  }
-  const VoxScene& P = scenes[sc];
-  const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-  B.totals[sc] = B.scan_all[P.bin_base + nbins] - B.scan_all[P.bin_base];
-}
-
-hipError_t launch_totals(const VoxBatch& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_totals, dim3((B.nscenes + 1 + 255) / 256), dim3(256), 0, s, B, B.scenes);
-  return hipGetLastError();
-}
-
-// Copies `words` words to pinned host memory and then stores `seq` into the host flag.
-// One wavefront: the system-scope release orders every lane's copy before the flag, so a
-// host thread that sees the flag (acquire) reads the whole copy. The host waits on the
-// flag instead of a stream synchronize and a device-to-host copy.
-__global__ __launch_bounds__(64) void k_publish(const uint32_t* __restrict__ src, uint32_t* dst, uint32_t words,
+// Hands the per-scene results of steps 1-4 to the host: copies the DevStats words to pinned
+// host memory, writes each scene's total (units << 32 | pairs of its bins, from the scan)
+// and the batch total at [nscenes] beside them, then stores `seq` into the host flag. One
+// wavefront: the system-scope release orders every lane's stores before the flag, so a host
+// thread that sees the flag (acquire) reads all of them. The host waits on the flag instead
+// of a stream synchronize and a device-to-host copy.
+__global__ __launch_bounds__(64) void k_publish(VoxBatch B, const VoxScene* __restrict__ scenes,
+                                                const uint32_t* __restrict__ stats, uint32_t stats_words,
+                                                uint32_t* dst_stats, unsigned long long* dst_totals,
                                                 uint32_t* flag, uint32_t seq) {
-  for (uint32_t i = threadIdx.x; i < words; i += 64) dst[i] = gld(src + i);
+  for (uint32_t i = threadIdx.x; i < stats_words; i += 64) dst_stats[i] = gld(stats + i);
+  for (uint32_t sc = threadIdx.x; sc <= B.nscenes; sc += 64) {
+    unsigned long long t;
+    if (sc == B.nscenes) {
+      t = gld(B.scan_all + B.total_bins);
+    } else {
+      const VoxScene& P = scenes[sc];
+      const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+      t = gld(B.scan_all + P.bin_base + nbins) - gld(B.scan_all + P.bin_base);
+    }
+    dst_totals[sc] = t;
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
   if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-hipError_t launch_publish(const void* src, void* dst_host, size_t bytes, uint32_t* flag_host, uint32_t seq,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, (const uint32_t*)src, (uint32_t*)dst_host,
-                     (uint32_t)(bytes / 4), flag_host, seq);
+hipError_t launch_publish(const VoxBatch& B, const void* stats, size_t stats_bytes, void* dst_stats,
+                          void* dst_totals, uint32_t* flag_host, uint32_t seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, B, B.scenes, (const uint32_t*)stats,
+                     (uint32_t)(stats_bytes / 4), (uint32_t*)dst_stats, (unsigned long long*)dst_totals, flag_host,
+                     seq);
   return hipGetLastError();
 }
 
@@ -563,39 +562,14 @@ __device__ __forceinline__ void emit_small_block(const VoxBatch& B, const VoxSce
   }
 }
 
-// One launch for the three steps that need the sized pair list and work map: workgroups
-// [0, nblk_b) map work units, [nblk_b, nblk_b + nblk_t) scatter small triangles, the rest
-// place large triangles (one wavefront each). They write disjoint outputs.
-__global__ __launch_bounds__(256) void k_scatter(VoxBatch B, const VoxScene* __restrict__ scenes, uint32_t nblk_b,
-                                                 uint32_t nblk_t) {
-  const uint32_t b = blockIdx.x;
-  if (b < nblk_b) {
-    workmap_block(B, scenes, b);
-  } else if (b < nblk_b + nblk_t) {
-    emit_small_block(B, scenes, b - nblk_b);
-  } else {
-    const uint32_t w0 = b - nblk_b - nblk_t, nbig_blocks = gridDim.x - nblk_b - nblk_t;
-    big_pass<true>(B, scenes, w0 * (blockDim.x >> 6) + (threadIdx.x >> 6), nbig_blocks * (blockDim.x >> 6));
-  }
-}
-
-hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks, hipStream_t s) {
-  const uint32_t blocks = nblk_b + nblk_t + big_blocks;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, B, B.scenes, nblk_b, nblk_t);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// 6 clear the bins that several work items add into
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_zero(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
+// Workgroups blk, blk + nblk, ... of the listed bins clear them.
+__device__ __forceinline__ void zero_pass(const VoxBatch& Bt, const VoxScene* __restrict__ scenes, uint32_t blk,
+                                          uint32_t nblk) {
   for (uint32_t sc = 0; sc < Bt.nscenes; sc++) {
     const VoxScene& P = scenes[sc];
     const uint32_t nz = gld(&P.stats->zero_count);
     const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
-    for (uint32_t i = blockIdx.x; i < nz; i += gridDim.x) {
+    for (uint32_t i = blk; i < nz; i += nblk) {
       const uint32_t bin = gld(P.zero_list + i);
       const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
       for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
@@ -609,10 +583,39 @@ __global__ __launch_bounds__(256) void k_zero(VoxBatch Bt, const VoxScene* __res
   }
 }
 
-hipError_t launch_zero(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, B, B.scenes);
+// One launch for the steps after the host handoff, before k_voxelize: workgroups
+// [0, nblk_b) map work units, the next nblk_t scatter small triangles, the next nblk_big
+// place large triangles (one wavefront each), the rest clear the bins that several work
+// units add into (none on most scenes). They write disjoint outputs.
+__global__ __launch_bounds__(256) void k_scatter(VoxBatch B, const VoxScene* __restrict__ scenes, uint32_t nblk_b,
+                                                 uint32_t nblk_t, uint32_t nblk_big) {
+  const uint32_t b = blockIdx.x;
+  if (b < nblk_b) {
+    workmap_block(B, scenes, b);
+  } else if (b < nblk_b + nblk_t) {
+    emit_small_block(B, scenes, b - nblk_b);
+  } else if (b < nblk_b + nblk_t + nblk_big) {
+    const uint32_t w0 = b - nblk_b - nblk_t;
+    big_pass<true>(B, scenes, w0 * (blockDim.x >> 6) + (threadIdx.x >> 6), nblk_big * (blockDim.x >> 6));
+  } else {
+    const uint32_t first = nblk_b + nblk_t + nblk_big;
+    zero_pass(B, scenes, b - first, gridDim.x - first);
+  }
+}
+
+hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,
+                          uint32_t zero_blocks, hipStream_t s) {
+  const uint32_t blocks = nblk_b + nblk_t + big_blocks + zero_blocks;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, B, B.scenes, nblk_b, nblk_t, big_blocks);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// clearing the bins that several work items add into (a k_scatter role)
+// ---------------------------------------------------------------------------
+
+
 
 // ---------------------------------------------------------------------------
 // 7 brick voxelize
