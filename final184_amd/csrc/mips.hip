@@ -119,14 +119,17 @@ __device__ __forceinline__ bool occupied(const MipScene& P, uint32_t x, uint32_t
 __device__ __forceinline__ uint32_t group_occupancy(const MipScene& P, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
   const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-  if ((uint32_t)gld(P.bin_packed + bin) == 0) return 0u;
   const uint32_t nbr = P.n >> 3;
   const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
   const uint32_t l0 = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));  // z-slice 0 of the group
   const uint32_t* m = P.mask + (size_t)brick * 16u;
   const uint32_t sh = l0 & 31u;  // bits sh, sh+1, sh+8, sh+9 of each slice's word
+  // the three loads are issued together: every brick's mask words exist (those of an empty
+  // bin hold stale bits, discarded here), so the mask need not wait for the bin's entry
+  const uint32_t pairs = (uint32_t)gld(P.bin_packed + bin);
   const uint32_t w0 = gld(m + (l0 >> 5)) >> sh, w1 = gld(m + ((l0 + 64u) >> 5)) >> sh;
-  return (w0 & 3u) | ((w0 >> 6) & 12u) | ((w1 & 3u) << 4) | ((w1 >> 2) & 192u);
+  const uint32_t bits = (w0 & 3u) | ((w0 >> 6) & 12u) | ((w1 & 3u) << 4) | ((w1 >> 2) & 192u);
+  return pairs != 0 ? bits : 0u;  // a select, not a branch, so the mask loads stay with the first
 }
 
 // Radiance of an occupied level-0 voxel (x, y, z) from its accumulator record.
@@ -200,8 +203,11 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
 
   if constexpr (SRC_ACC) {
     __shared__ float s_lut[256];  // (i/255)^2.2, read once per workgroup
-    if (g < 256) s_lut[g] = lut[g];
+    // loaded now, stored to LDS just before the inject loop's barrier: barriers wait on LDS
+    // operations only, so the load's latency overlaps the occupancy pass
+    const float lut_v = g < 256 ? lut[g] : 0.0f;
     if (nout == 0) {
+      if (g < 256) s_lut[g] = lut_v;
       __syncthreads();
       // single-level chain: resolve level 0 only, one voxel per thread
       for (uint32_t v = g; v < S * S * S; v += 512) {
@@ -237,6 +243,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     wbase = __shfl(wbase, 63);
     uint32_t k = wbase + incl - cnt;
     for (uint32_t m = occ; m != 0u; m &= m - 1u) s_item[k++] = (uint16_t)(g * 8u + (uint32_t)__builtin_ctz(m));
+    if (g < 256) s_lut[g] = lut_v;
     __syncthreads();
     const uint32_t nocc = s_nocc;
     for (uint32_t it = g; it < nocc; it += 512) {
