@@ -173,7 +173,7 @@ struct vgi_context {
   int device = 0;
   int num_cus = 256;
   hipStream_t stream = nullptr;
-  vgi_tuning tuning{8, 64, 16, 16, VGI_ORDER_TILES};
+  vgi_tuning tuning{8, 64, 16, 16, VGI_ORDER_TILES, VGI_ADDRESS_AUTO};
   int profiling = 0;
   float* d_lut = nullptr;
   unsigned long long* d_counter = nullptr;
@@ -937,7 +937,7 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   size_t max_entries = 0;
   for (uint32_t p = 0; p < 3; p++) {
     if (group[p].empty()) continue;
-    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, 0, 0, true};
+    Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, 0, 0, tn.address_mode == VGI_ADDRESS_AUTO};
     for (uint32_t i : group[p]) {
       L.nsched += sched[2 * i].size() + sched[2 * i + 1].size();
       const vgi_gbuffer& gb = items[i].gbuffer;
@@ -1129,6 +1129,7 @@ int vgi_set_tuning(vgi_context* ctx, const vgi_tuning* t) {
                        (t->tile_w == 8 && t->tile_h == 16);
   if (!ok_tile) return fail(VGI_E_INVALID, "tile %ux%u not supported", t->tile_w, t->tile_h);
   if (t->pixel_order > VGI_ORDER_MORTON) return fail(VGI_E_INVALID, "pixel_order %u", t->pixel_order);
+  if (t->address_mode > VGI_ADDRESS_WIDE) return fail(VGI_E_INVALID, "address_mode %u", t->address_mode);
   std::lock_guard<std::mutex> lk(ctx->mu);
   ctx->tuning = *t;
   return VGI_OK;

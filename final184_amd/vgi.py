@@ -68,6 +68,7 @@ class GBuffer(C.Structure):
 
 
 ORDER_TILES, ORDER_MORTON = 0, 1
+ADDRESS_AUTO, ADDRESS_WIDE = 0, 1  # cone-march texel addressing (vgi.h VGI_ADDRESS_*)
 IMAGE_RGBA32F, IMAGE_RGBA16F = 0, 1
 
 
@@ -107,7 +108,7 @@ class BlurParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("bin_size", C.c_uint32), ("chunk_tris", C.c_uint32), ("tile_w", C.c_uint32),
-                ("tile_h", C.c_uint32), ("pixel_order", C.c_uint32)]
+                ("tile_h", C.c_uint32), ("pixel_order", C.c_uint32), ("address_mode", C.c_uint32)]
 
 
 class SceneStats(C.Structure):
@@ -580,8 +581,9 @@ class Context:
     def sync(self, stream=None) -> None:
         _check(lib.vgi_sync(self.handle, stream))
 
-    def set_tuning(self, bin_size=8, chunk_tris=64, tile_w=16, tile_h=16, pixel_order=ORDER_TILES) -> None:
-        t = Tuning(bin_size, chunk_tris, tile_w, tile_h, pixel_order)
+    def set_tuning(self, bin_size=8, chunk_tris=64, tile_w=16, tile_h=16, pixel_order=ORDER_TILES,
+                   address_mode=ADDRESS_AUTO) -> None:
+        t = Tuning(bin_size, chunk_tris, tile_w, tile_h, pixel_order, address_mode)
         _check(lib.vgi_set_tuning(self.handle, C.byref(t)))
 
     def put_tuning(self, t: Tuning) -> None:

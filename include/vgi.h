@@ -149,6 +149,12 @@ typedef struct vgi_gbuffer {
  * cones of a wavefront start close together even when the G-buffer is not spatially
  * coherent (e.g. points sampled on triangles). Results are identical either way. */
 enum { VGI_ORDER_TILES = 0, VGI_ORDER_MORTON = 1 };
+/* Cone-march texel addressing: AUTO uses 32-bit buffer offsets when every mip level of a
+ * launch is under 4 GiB and 64-bit addresses otherwise (grids of 1024^3); WIDE uses 64-bit
+ * addresses always. The results agree within FP32 rounding (the fixed-aperture march
+ * schedule is computed per lane instead of once per aperture); WIDE exists to exercise and
+ * compare the large-grid form on small scenes. */
+enum { VGI_ADDRESS_AUTO = 0, VGI_ADDRESS_WIDE = 1 };
 
 typedef struct vgi_tuning {
   uint32_t bin_size;    /* voxelize triangle bin edge: 8, 16 or 32 voxels */
@@ -156,6 +162,7 @@ typedef struct vgi_tuning {
   uint32_t tile_w;      /* cone-march screen tile: 8x8, 16x16 or 32x8 (TILES order) */
   uint32_t tile_h;
   uint32_t pixel_order; /* VGI_ORDER_* */
+  uint32_t address_mode; /* VGI_ADDRESS_* */
 } vgi_tuning;
 
 typedef struct vgi_scene_stats {

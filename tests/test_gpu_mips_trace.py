@@ -212,3 +212,28 @@ def test_nan_position_component_ends_march(ctx):
     assert np.array_equal(d[bad], d32[bad]) and np.array_equal(sp[bad], s32[bad])
     good = np.setdiff1d(np.arange(w.num_pixels), bad)
     assert rel_l2(_irr(d[good], sp[good]), _irr(d32[good], s32[good])) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_wide_addressing_matches(ctx, soup, prec):
+    """The 64-bit-address form of the cone march (what levels of 4 GiB and more use) gives
+    the same irradiance as the buffer-offset form, and both are within 1e-5 of the oracle's
+    same-precision run."""
+    vgi = _vgi()
+    w = soup["w"]
+    print(f"case wide addressing {prec}", flush=True)
+    f0 = Frame(ctx, w, prec)
+    try:
+        ctx.set_tuning(address_mode=vgi.ADDRESS_WIDE)
+        f1 = Frame(ctx, w, prec)
+    finally:
+        ctx.set_tuning()
+    lvp = orc.inject_mips(soup["acc"], w.grid, w.lights, prec)
+    dp, sp, _ = orc.trace(lvp, w.grid, w.gb_pos, w.gb_nrm, w.cones)
+    ref = _irr(dp, sp)
+    e0, e1 = rel_l2(_irr(f0.d, f0.s), ref), rel_l2(_irr(f1.d, f1.s), ref)
+    print(f"{prec}: buffer form {e0:.2e}, wide form {e1:.2e} vs the oracle")
+    assert e0 <= 1e-5 and e1 <= 1e-5
+    assert rel_l2(_irr(f1.d, f1.s), _irr(f0.d, f0.s)) <= 1e-5
+    f0.release()
+    f1.release()
