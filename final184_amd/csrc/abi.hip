@@ -181,6 +181,7 @@ struct vgi_context {
   hipEvent_t ev[10] = {};
   bool ev_vox = false, ev_mips = false, ev_trace = false;
   hipEvent_t tstart = nullptr, tstop = nullptr;
+  hipEvent_t dep = nullptr;  // vgi_stream_wait (a wait captures the record it follows)
   vgi_timings timings{};
   std::mutex mu;  // tuning, the workspace map
   std::map<hipStream_t, std::unique_ptr<vgi_workspace>> ws;
@@ -1096,6 +1097,7 @@ int vgi_create(vgi_context** out, int device) {
   for (auto& ev : ctx->ev) HIP_TRY(hipEventCreate(&ev));
   HIP_TRY(hipEventCreate(&ctx->tstart));
   HIP_TRY(hipEventCreate(&ctx->tstop));
+  HIP_TRY(hipEventCreateWithFlags(&ctx->dep, hipEventDisableTiming));
   *out = ctx;
   return VGI_OK;
 }
@@ -1113,6 +1115,7 @@ int vgi_destroy(vgi_context* ctx) {
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->tstart) (void)hipEventDestroy(ctx->tstart);
   if (ctx->tstop) (void)hipEventDestroy(ctx->tstop);
+  if (ctx->dep) (void)hipEventDestroy(ctx->dep);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return VGI_OK;
@@ -1607,6 +1610,19 @@ int vgi_stream_create(vgi_context* ctx, void** out) {
   hipStream_t s = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   *out = (void*)s;
+  return VGI_OK;
+}
+
+int vgi_stream_wait(vgi_context* ctx, void* waiting, void* signalling) {
+  if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipStream_t w = pick(ctx, waiting), sg = pick(ctx, signalling);
+  if (w == sg) return VGI_OK;  // stream order already holds
+  // one event serves every call: hipStreamWaitEvent waits for the record made just before
+  // it, and a later record of the same event does not change an enqueued wait
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipEventRecord(ctx->dep, sg));
+  HIP_TRY(hipStreamWaitEvent(w, ctx->dep, 0));
   return VGI_OK;
 }
 

@@ -241,6 +241,7 @@ SIGNATURES = [
     ("vgi_memcpy_d2d", C.c_int, [_P, _P, _P, C.c_size_t, _P]),
     ("vgi_stream_create", C.c_int, [_P, C.POINTER(_P)]),
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
+    ("vgi_stream_wait", C.c_int, [_P, _P, _P]),
     ("vgi_workspace_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("vgi_timer_start", C.c_int, [_P, _P]),
     ("vgi_timer_stop", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
@@ -613,6 +614,10 @@ class Context:
 
     def stream_destroy(self, s) -> None:
         _check(lib.vgi_stream_destroy(self.handle, s))
+
+    def stream_wait(self, waiting, signalling) -> None:
+        """Later work on `waiting` waits for the work enqueued on `signalling` so far."""
+        _check(lib.vgi_stream_wait(self.handle, waiting, signalling))
 
     def workspace_bytes(self) -> int:
         b = C.c_uint64()

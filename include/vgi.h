@@ -511,6 +511,14 @@ int vgi_memcpy_d2d(vgi_context* ctx, void* dst, const void* src, size_t bytes, v
 int vgi_stream_create(vgi_context* ctx, void** out);
 /* Synchronizes and destroys the stream, and frees the scratch the context held for it. */
 int vgi_stream_destroy(vgi_context* ctx, void* stream);
+/* Makes the work enqueued on `waiting` from now on wait for the work enqueued on
+ * `signalling` so far (an event recorded on one, waited on by the other; the host does not
+ * block). NULL selects the context's stream. No counterpart in the reference, which runs
+ * one Vulkan queue: it lets a caller overlap stages of different frames on two streams,
+ * e.g. voxelize frame k+1 while frame k is traced (a scene's voxelize must not overlap its
+ * own mip build, which reads the accumulator, and its mip build must not overlap its
+ * trace, which reads the levels). */
+int vgi_stream_wait(vgi_context* ctx, void* waiting, void* signalling);
 /* Device bytes of voxelize scratch and descriptor staging the context holds, summed over
  * the streams it has run on (each stream keeps its own, grown to its largest batch). */
 int vgi_workspace_bytes(vgi_context* ctx, uint64_t* out);

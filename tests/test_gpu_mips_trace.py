@@ -237,3 +237,34 @@ def test_wide_addressing_matches(ctx, soup, prec):
     assert rel_l2(_irr(f1.d, f1.s), _irr(f0.d, f0.s)) <= 1e-5
     f0.release()
     f1.release()
+
+
+def test_two_stream_frames_match(ctx, soup):
+    """Frames overlapped on two streams (voxelize of frame k+1 on one stream while frame k
+    is traced on the other, ordered by vgi_stream_wait) give the same bits as one stream."""
+    vgi = _vgi()
+    w = soup["w"]
+    print("case two-stream frames", flush=True)
+    f0 = Frame(ctx, w, "fp16")  # one stream
+    sa, sb = ctx.stream_create(), ctx.stream_create()
+    try:
+        mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+        grid, lights, cones = vgi.make_grid(**w.grid), vgi.make_lights(w.lights), vgi.make_cones(**w.cones)
+        gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+        od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+        sc = ctx.scene()
+        for _ in range(3):
+            sc.voxelize([mesh], grid, stream=sb)
+            ctx.stream_wait(sa, sb)   # mips after the voxelize
+            sc.build_mips(lights, vgi.PREC_FP16, stream=sa)
+            ctx.stream_wait(sb, sa)   # the next voxelize after this mip build (not the trace)
+            sc.trace(gb, cones, od, osp, stream=sa)
+        ctx.sync(sa)
+        ctx.sync(sb)
+        assert np.array_equal(od.numpy(), f0.d) and np.array_equal(osp.numpy(), f0.s)
+        sc.release()
+    finally:
+        ctx.stream_destroy(sa)
+        ctx.stream_destroy(sb)
+        f0.release()
