@@ -18,7 +18,9 @@
 // Gather: a quadrilinear sample is 16 weighted texels, sum_i w_i c_i with the level blend
 // folded into the weights, accumulated in FP32 (FP16 texels feed v_fma_mix directly; no
 // unpacking). For cones whose schedule t_i does not depend on the pixel (all diffuse
-// cones, and the specular cone without a roughness buffer) the level is wave-uniform:
+// cones, and the specular cone without a roughness buffer) the host computes the schedule
+// once per aperture (distance, level, level weights per sample) and the march reads it
+// with scalar loads; the level is then wave-uniform:
 // the level's buffer descriptor and its row/slice pitches live in SGPRs and the eight
 // texels of a level come from one VGPR byte offset (plus SGPR pitch and immediate +1
 // texel), four 16-byte loads per level for FP16 (texel x and x+1 share a load), eight for
