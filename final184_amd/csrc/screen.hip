@@ -204,8 +204,8 @@ struct BlurTile<0> {  // along x: 256 x 2 pixel tiles, 286 x 2 staged
   static constexpr int TX = 256, TY = 2, LX = TX + 2 * kBlurReach, LY = TY, STEP = 1;
 };
 template <>
-struct BlurTile<1> {  // along y: 32 x 64 pixel tiles, 32 x 94 staged
-  static constexpr int TX = 32, TY = 64, LX = TX, LY = TY + 2 * kBlurReach, STEP = LX;
+struct BlurTile<1> {  // along y: 32 x 32 pixel tiles, 32 x 62 staged (31 KB: 5 workgroups per CU)
+  static constexpr int TX = 32, TY = 32, LX = TX, LY = TY + 2 * kBlurReach, STEP = LX;
 };
 
 template <int R>
