@@ -268,3 +268,26 @@ def test_two_stream_frames_match(ctx, soup):
         ctx.stream_destroy(sa)
         ctx.stream_destroy(sb)
         f0.release()
+
+
+def test_changed_and_repeated_descriptors(ctx, soup):
+    """Descriptor tables are re-uploaded when a call changes them and reused when a call
+    repeats them: traces with cone sets A, B, A give A's bits twice and B's in between."""
+    vgi = _vgi()
+    w = soup["w"]
+    print("case descriptor reuse", flush=True)
+    f = Frame(ctx, w, "fp32")
+    cones_a = vgi.make_cones(**w.cones)
+    cb = dict(w.cones)
+    cb["num_diffuse"] = 3
+    cones_b = vgi.make_cones(**cb)
+    outs = []
+    for cones in (cones_a, cones_b, cones_a):
+        f.scene.trace(f.gb, cones, f.od, f.os)
+        outs.append((f.od.numpy().copy(), f.os.numpy().copy()))
+    assert np.array_equal(outs[0][0], f.d) and np.array_equal(outs[0][1], f.s)
+    assert np.array_equal(outs[2][0], outs[0][0]) and np.array_equal(outs[2][1], outs[0][1])
+    assert not np.array_equal(outs[1][0], outs[0][0])
+    d3, s3, _ = orc.trace(orc.inject_mips(soup["acc"], w.grid, w.lights, "fp32"), w.grid, w.gb_pos, w.gb_nrm, cb)
+    assert rel_l2(_irr(outs[1][0], outs[1][1]), _irr(d3, s3)) <= 1e-5
+    f.release()
