@@ -441,7 +441,7 @@ def main() -> int:
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 10k-triangle soup and the 1M-triangle 512^3 / 4K precision study")
     ap.add_argument("--bin", type=int, default=8, help="voxelize bin edge (8, 16, 32)")
-    ap.add_argument("--tile", default="32x8", help="cone-march screen tile (8x8, 16x16, 32x8)")
+    ap.add_argument("--tile", default="8x8", help="cone-march screen tile (8x8, 16x16, 32x8, 16x8, 8x16)")
     ap.add_argument("--order", default="tiles", choices=["tiles", "morton"])
     ap.add_argument("--autotune", action="store_true",
                     help="run the autotuner on the headline workload instead of --bin/--tile/--order")
