@@ -103,7 +103,9 @@ __device__ __forceinline__ uint32_t mad_i24(int a, uint32_t b, uint32_t c) {
 template <bool UNIFORM_LEVEL>
 __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, float uy, float uz,
                                           float wl) {
-  const float sc = __builtin_ldexpf(1.0f, -(int)k);
+  // 2^-k: for a wave-uniform level built from its bit pattern (k <= 126) by scalar
+  // instructions, per lane by one ldexp
+  const float sc = UNIFORM_LEVEL ? __builtin_bit_cast(float, (127u - k) << 23) : __builtin_ldexpf(1.0f, -(int)k);
   const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
   // integer part by a flooring convert, fraction by v_fract (x - floor(x))
   const int ix = cvt_floor(gx), iy = cvt_floor(gy), iz = cvt_floor(gz);
@@ -217,7 +219,7 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
 // of values in [0, 1]; its rounded weights can sum to 1 + ulp, and the clamp keeps the
 // accumulated a within [0, 1].
 __device__ __forceinline__ void composite(float4 s, float& cr, float& cg, float& cb, float& a) {
-  s.w = fminf(s.w, 1.0f);
+  s.w = __builtin_amdgcn_fmed3f(s.w, 0.0f, 1.0f);  // min(s.w, 1) for s.w >= 0, one instruction
   const float w = 1.0f - a;
   cr = fmaf(w, s.x, cr);
   cg = fmaf(w, s.y, cg);
