@@ -214,6 +214,8 @@ struct vgi_scene {
   uint32_t prec = 0;
   void* level[VGI_MAX_LEVELS] = {};
   size_t level_bytes[VGI_MAX_LEVELS] = {};
+  uint32_t* live = nullptr;  // one word per 16^3 tile of level 0 (the mip kernel's skip state)
+  size_t live_bytes = 0;
   uint32_t alloc_levels = 0, alloc_n = 0, alloc_prec = 0;
   bool has_voxels = false, has_mips = false;
   uint64_t tris_in = 0, num_pairs = 0, work_items = 0, rejected = 0, degenerate = 0;
@@ -222,7 +224,7 @@ struct vgi_scene {
 namespace {
 
 size_t scene_device_bytes(const vgi_scene* s) {
-  size_t b = s->acc_bytes + s->mask_bytes + s->bin_packed_bytes;
+  size_t b = s->acc_bytes + s->mask_bytes + s->bin_packed_bytes + s->live_bytes;
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) b += s->level_bytes[k];
   return b;
 }
@@ -698,7 +700,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
 int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
   const vgi_grid& g = s->grid;
   if (s->alloc_n == g.n && s->alloc_levels == g.levels && s->alloc_prec == precision) return VGI_OK;
-  bool held = false;
+  bool held = s->live != nullptr;
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) held = held || s->level[k];
   if (held) HIP_TRY(hipDeviceSynchronize());  // queued work may still read the old levels
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++) {
@@ -706,7 +708,24 @@ int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
     s->level[k] = nullptr;
     s->level_bytes[k] = 0;
   }
+  if (s->live) (void)hipFree(s->live);
+  s->live = nullptr;
+  s->live_bytes = 0;
   s->alloc_n = s->alloc_levels = 0;
+  {
+    // The levels start all zero and a tile's texels are stored only while the tile or its
+    // previous build held occupied voxels, so every texel of the levels stays exact.
+    const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u);
+    const size_t bytes = tpa * tpa * tpa * sizeof(uint32_t);
+    hipError_t e = hipMalloc((void**)&s->live, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      s->live = nullptr;
+      return fail(VGI_E_NOMEM, "tile state allocation (%zu bytes) failed", bytes);
+    }
+    s->live_bytes = bytes;
+    HIP_TRY(launch_fill_u32(s->live, 0u, bytes / 4, st));
+  }
   for (uint32_t k = 0; k < g.levels; k++) {
     const size_t p = (size_t)(g.n >> k) + 2;
     const size_t bytes = p * p * p * texel_bytes(precision);
@@ -787,6 +806,7 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     P.acc = s->acc;
     P.mask = s->mask;
     P.bin_packed = (const unsigned long long*)s->bin_packed;
+    P.live = s->live;
     for (uint32_t k = 0; k < g.levels; k++) {
       P.level[k] = s->level[k];
       P.nk[k] = g.n >> k;
@@ -1253,6 +1273,7 @@ int vgi_scene_release(vgi_scene* s) {
     if (p) (void)hipFree(p);
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++)
     if (s->level[k]) (void)hipFree(s->level[k]);
+  if (s->live) (void)hipFree(s->live);
   delete s;
   return VGI_OK;
 }
@@ -1261,7 +1282,9 @@ uint64_t vgi_scene_bytes_estimate(const vgi_grid* g, uint32_t precision, uint64_
   (void)ntri;
   if (!g || !is_pow2(g->n) || g->n < 8 || g->levels == 0 || g->levels > VGI_MAX_LEVELS) return 0;
   const uint64_t n = g->n;
-  uint64_t b = n * n * n * kAccWords * 4 + (n / 8) * (n / 8) * (n / 8) * 64 + (n / 8) * (n / 8) * (n / 8) * 8;
+  const uint64_t tpa = n / std::min<uint64_t>(n, 16);  // 16^3 tiles per axis (mip tile state)
+  uint64_t b = n * n * n * kAccWords * 4 + (n / 8) * (n / 8) * (n / 8) * 64 + (n / 8) * (n / 8) * (n / 8) * 8 +
+               tpa * tpa * tpa * 4;
   for (uint32_t k = 0; k < g->levels; k++) {
     const uint64_t p = (n >> k) + 2;
     b += p * p * p * texel_bytes(precision);

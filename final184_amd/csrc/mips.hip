@@ -15,7 +15,12 @@
 // then z) and every level is rounded to its storage precision before it is reduced, so
 // the result is a pure function of the accumulator.
 //
-// Level k is stored padded: (n_k + 2)^3 texels with a zero border, x fastest.
+// Level k is stored padded: (n_k + 2)^3 texels with a zero border, x fastest. The levels
+// are zero-filled when allocated, and the first launch stores a 16^3 tile (its level-0
+// texels and its blocks of the four levels above) only while the tile holds occupied
+// voxels, plus once, as zeros, when it has lost them; a per-tile word records which. A
+// surface leaves most tiles empty (70% at 256^3 for the room), so most of the level
+// bytes are not rewritten each build, and every texel still equals the full computation.
 #include "vgi_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -222,7 +227,11 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     // useful work however sparse the surface is; the level is then stored row by row and
     // each 2x2x2 group summed from LDS.
     __shared__ typename TX::Packed s_val[4096];
-    __shared__ uint32_t s_nocc;
+    __shared__ uint32_t s_nocc, s_was;
+    // the tile's state from the previous build, read now so that its latency overlaps the
+    // occupancy pass: 1 while its stored texels (level 0 and its blocks of the levels
+    // produced here) are not all zero
+    const uint32_t was_v = g == 0 ? gld(P.live + tile) : 0u;
     uint16_t* s_item = (uint16_t*)s_a;  // occupied voxel list (group * 8 + bit), before s_a is used
     static_assert(sizeof(s_a) >= 4096 * sizeof(uint16_t), "item list fits in s_a");
     const uint32_t ssh = gsh + 1u, NV = S * S * S;
@@ -244,8 +253,15 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     uint32_t k = wbase + incl - cnt;
     for (uint32_t m = occ; m != 0u; m &= m - 1u) s_item[k++] = (uint16_t)(g * 8u + (uint32_t)__builtin_ctz(m));
     if (g < 256) s_lut[g] = lut_v;
+    if (g == 0) s_was = was_v;
     __syncthreads();
     const uint32_t nocc = s_nocc;
+    // A tile without occupied voxels whose stored texels are already zero (never written
+    // since the levels were allocated and zero-filled, or cleared by an earlier build) is
+    // left as it is; one that held voxels before is stored, as zeros, once.
+    const uint32_t was = s_was;
+    if (nocc == 0u && was == 0u) return;  // uniform: both values come from LDS
+    if (g == 0 && (nocc != 0u) != (was != 0u)) gst(P.live + tile, nocc != 0u ? 1u : 0u);
     for (uint32_t it = g; it < nocc; it += 512) {
       const uint32_t e = s_item[it], gi = e >> 3, bit = e & 7u;
       const uint32_t lx = 2 * (gi & (G - 1u)) + (bit & 1u);

@@ -113,6 +113,7 @@ struct MipScene {
   const uint32_t* acc;
   const uint32_t* mask;
   const unsigned long long* bin_packed;
+  uint32_t* live;  // per 16^3 tile of level 0: 1 while the tile's stored texels are not all zero
   void* level[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t n;

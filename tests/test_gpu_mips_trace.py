@@ -94,6 +94,35 @@ def test_mips_room_levels(ctx):
     f.release()
 
 
+def test_mips_after_geometry_changes(ctx):
+    """Rebuilding one scene's mips after its geometry changed: 16^3 tiles that lose their
+    voxels must read as zeros, tiles that gain voxels as the oracle's texels, at every
+    level, through a change of precision too."""
+    vgi = _vgi()
+    w = sg.room(ntri=20_000, n=64, levels=6, width=16, height=16)
+    pos, alb = w.positions, w.albedo
+    half = (pos.reshape(len(pos), 3, 3)[:, :, 0] < 0.5).all(axis=1)  # wholly in the x < 0.5 half
+    # (24 of the 64 16^3 tiles hold voxels with every triangle, 12 with the half)
+    assert 0 < half.sum() < len(pos)
+    grid = vgi.make_grid(**w.grid)
+    lights = vgi.make_lights(w.lights)
+    sc = ctx.scene()
+    for step, (sel, prec) in enumerate([(None, "fp16"), (half, "fp16"), (None, "fp16"), (half, "fp32")]):
+        p, a = (pos, alb) if sel is None else (pos[sel], alb[sel])
+        print(f"case room 64^3 step {step}: {len(p)} triangles, mips {prec}", flush=True)
+        mesh = vgi.MeshData(ctx, p.reshape(-1, 3), albedo=a)
+        sc.voxelize([mesh], grid)
+        sc.build_mips(lights, vgi.PREC_NAMES[prec])
+        acc, _ = orc.voxelize(p, w.grid, albedo=a)
+        ref = orc.inject_mips(acc, w.grid, w.lights, prec)
+        for k in range(w.grid["levels"]):
+            got = sc.read_level(k)
+            bad = (got != ref[k]).any(axis=-1)
+            assert not bad.any(), (f"step {step} level {k}: {bad.sum()} texels differ, first at "
+                                   f"{np.argwhere(bad)[:3].tolist()}")
+    sc.release()
+
+
 def test_trace_fp32_vs_fp64(ctx, soup):
     w = soup["w"]
     print(f"case {w.name} trace fp32", flush=True)
