@@ -639,9 +639,38 @@ struct TriCoef {
 
 __device__ __forceinline__ int sel3(int a, int b, int c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
 
-__device__ void make_coef(const TriRec& R, TriCoef& C) {
+// The three edge functions of projection axis U (compile-time, so every index is static).
+template <int U>
+__device__ __forceinline__ void make_edges(const TriRec& R, const int64_t (&N)[3], TriCoef& C) {
+  constexpr int A = (U + 1) % 3, B = (U + 2) % 3;
+  const bool neg = N[U] < 0;  // edge normal (-dB, dA) * sign(N[U]), |dA|, |dB| <= 2^19
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int j = (i + 1) % 3;
+    const int32_t dB = R.q[j * 3 + B] - R.q[i * 3 + B], dA = R.q[j * 3 + A] - R.q[i * 3 + A];
+    const int32_t na = neg ? dB : -dB;
+    const int32_t nb = neg ? -dA : dA;
+    const int64_t ea = shl64(na, VGI_FIXED_BITS), eb = shl64(nb, VGI_FIXED_BITS);
+    C.eA[U * 3 + i] = (int32_t)ea;
+    C.eB[U * 3 + i] = (int32_t)eb;
+    C.eC[U * 3 + i] = -(mul32x32(na, R.q[i * 3 + A]) + mul32x32(nb, R.q[i * 3 + B])) + imax64(0, ea) +
+                      imax64(0, eb);
+  }
+}
+
+// Coefficients of one triangle, built by the four wavefronts of a workgroup together
+// (part is wave-uniform): part 0 writes the plane, the dominant-axis depth terms, the box
+// and the increments, part 1 + u the three edge functions of projection axis u. The parts
+// write disjoint fields of C, each with the same arithmetic as one thread computing all.
+__device__ void make_coef(const TriRec& R, TriCoef& C, uint32_t part) {
   int64_t q[3][3], N[3];
   load_q(R, q, N);
+  if (part != 0) {
+    if (part == 1) make_edges<0>(R, N, C);
+    else if (part == 2) make_edges<1>(R, N, C);
+    else make_edges<2>(R, N, C);
+    return;
+  }
   int64_t c0 = -(N[0] * q[0][0] + N[1] * q[0][1] + N[2] * q[0][2]);
   int64_t pmin = c0, pmax = c0;
 #pragma unroll
@@ -653,23 +682,6 @@ __device__ void make_coef(const TriRec& R, TriCoef& C) {
   }
   C.pmin = pmin;
   C.pmax = pmax;
-#pragma unroll
-  for (int u = 0; u < 3; u++) {
-    const int A = (u + 1) % 3, B = (u + 2) % 3;
-    const bool neg = N[u] < 0;  // edge normal (-dB, dA) * sign(N[u]), |dA|, |dB| <= 2^19
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const int j = (i + 1) % 3;
-      const int32_t dB = R.q[j * 3 + B] - R.q[i * 3 + B], dA = R.q[j * 3 + A] - R.q[i * 3 + A];
-      const int32_t na = neg ? dB : -dB;
-      const int32_t nb = neg ? -dA : dA;
-      const int64_t ea = shl64(na, VGI_FIXED_BITS), eb = shl64(nb, VGI_FIXED_BITS);
-      C.eA[u * 3 + i] = (int32_t)ea;
-      C.eB[u * 3 + i] = (int32_t)eb;
-      C.eC[u * 3 + i] = -(mul32x32(na, R.q[i * 3 + A]) + mul32x32(nb, R.q[i * 3 + B])) + imax64(0, ea) +
-                        imax64(0, eb);
-    }
-  }
   const int u = R.axis;
   const int A = (u + 1) % 3, B = (u + 2) % 3;
   const float nu = (float)(u == 0 ? N[0] : (u == 1 ? N[1] : N[2]));
@@ -748,12 +760,13 @@ struct TileLayout {
   static constexpr int ENTRIES = (TS - 1) * (1 + PY + PZ) + 1;
 };
 
-// Registers: the 8^3 tile's unit accumulators (14 per thread) fit without spills at 5
-// waves per SIMD (96 VGPRs; 5 resident workgroups per CU; LDS, 25.5 KB each, would allow
-// 6, where the registers would spill). The kernel is latency-bound (dependent loads of
-// pair ids and triangle records, and barriers per chunk), so resident workgroups count.
+// Registers: the 8^3 tile's unit accumulators (14 per thread) are held at 6 waves per SIMD
+// (80 VGPRs; 6 resident workgroups per CU, as many as LDS allows at 25.5 KB each), which
+// spills one 64-bit value per thread to scratch, reloaded at the flush. The kernel is
+// latency-bound (dependent loads and barriers per chunk), so resident workgroups count:
+// 6 instead of 5 took the room's launch from 80 to 75 us.
 template <int TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 5 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 6 : 1)))
 void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
@@ -762,6 +775,7 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
   __shared__ unsigned long long s_acc1[L::ENTRIES];
   __shared__ TriCoef s_tri[kMaxChunk];
   __shared__ uint32_t s_pref[kMaxChunk + 1];
+  static_assert(kMaxChunk == 64, "a chunk's triangles are the 64 lanes of each of the 4 wavefronts");
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -793,10 +807,10 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
     for (uint32_t it = it0; it < it1; it++) {
       const uint32_t first = base + it * P.chunk;
       const uint32_t cnt = min(P.chunk, cnt_bin - it * P.chunk);
-      if (tid < cnt) {
-        const uint32_t t = gld(Bt.pairs_all + first + tid);
+      if ((tid & 63u) < cnt) {  // triangle tid % 64, coefficient part tid / 64
+        const uint32_t t = gld(Bt.pairs_all + first + (tid & 63u));
         const TriRec R = gld_rec(P.tris + t);
-        make_coef(R, s_tri[tid]);
+        make_coef(R, s_tri[tid & 63u], tid >> 6);
       }
 
       for (uint32_t st = 0; st < sub * sub * sub; st++) {
