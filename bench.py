@@ -24,9 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# wave64 VALU instructions per CU per clock: a probe of independent v_fma_f32 chains
-# issues 0.98 (4 SIMD16 units, 4 cycles per wave64 FMA; the 157.3 TFLOP/s vector peak
-# counts packed FP32). 256 CUs at 2.4 GHz.
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+# 256 CUs at 2.4 GHz: converts instruction rates to wave64 instructions per CU per clock
+# (one wave64 FP32 FMA per CU per clock: 4 SIMD16 units, 4 cycles each; the 157.3 TFLOP/s
+# vector peak counts packed FP32, which this code does not use). The VALU ceiling itself is
+# the FMA probe's measured rate.
 CU_CLOCKS_PER_S = 256 * 2.4e9
 
 
@@ -73,7 +75,7 @@ def run_precision(ctx, vgi, w, mesh, gb, prec, steps, warmup):
         trace_ms.append(ctx.timings().trace_ms)
     ctx.set_profiling(False)
     out = dict(prec=prec, frame_ms=total_ms / steps, vox_ms=t.voxelize_total_ms,
-               vox_raster_ms=t.voxelize_raster_ms, mips_ms=t.mips_ms,
+               vox_raster_ms=t.voxelize_raster_ms, vox_setup_ms=t.voxelize_setup_ms, mips_ms=t.mips_ms,
                trace_ms=float(np.mean(trace_ms)), trilinear=int(t.trace_trilinear_samples),
                stats=scene.stats(), diffuse=od.numpy(), specular=osp.numpy())
     scene.release()
@@ -247,17 +249,74 @@ def run_screen(ctx, vgi, sg, w, mesh, gb_rough, steps):
     return out
 
 
+def trace_roofline(cn, probes, r, px):
+    """Roofline of one cone-march launch. It is bound by VALU issue: `achieved` is its
+    wave64 VALU instruction rate (SQ_INSTS_VALU per launch / its HIP-event time), `peak` the
+    rate the FMA probe reaches on this chip in this process (its SQ_INSTS_VALU / its time).
+    Beside it: HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated) against 8 TB/s, the
+    effective rate of the algorithmic bytes (60 B per pixel + 8 texels per trilinear gather;
+    served mostly from L1/L2, so not an HBM figure), and the gathers' FP32 FMAs (64 flops a
+    gather) against the 157.3 TFLOP/s vector peak."""
+    prec = r["prec"]
+    texel = {"fp32": 16, "fp16": 8, "bf16": 8}[prec]
+    t_s = r["trace_ms"] * 1e-3
+    k = (cn or {}).get(f"trace_{prec}", {})
+    alg = px * (24 + 4 + 32) + r["trilinear"] * 8 * texel
+    d = {"bound": "valu", "achieved": None, "peak": None, "unit": "G wave64 VALU instr/s", "frac": None,
+         "traffic": k.get("hbm_bytes"), "kernel": f"k_trace<{prec}>", "time_ms": r["trace_ms"],
+         "trilinear_gathers": r["trilinear"], "l2_hit_rate": k.get("l2_hit_rate")}
+    valu = k.get("SQ_INSTS_VALU")
+    fma = (cn or {}).get("probe_fma", {}).get("SQ_INSTS_VALU")
+    if valu and fma and probes.get("fma"):
+        ach = valu / t_s / 1e9
+        peak = fma / (probes["fma"] * 1e-3) / 1e9
+        d.update(achieved=ach, peak=peak, frac=ach / peak, valu_instructions=valu,
+                 valu_lane_instructions_per_gather=valu * 64 / max(r["trilinear"], 1),
+                 issue_per_cu_per_clock=ach * 1e9 / CU_CLOCKS_PER_S,
+                 probe_issue_per_cu_per_clock=peak * 1e9 / CU_CLOCKS_PER_S,
+                 salu_instructions=k.get("SQ_INSTS_SALU"))
+    if d["traffic"]:
+        gbs = d["traffic"] / t_s / 1e9
+        d["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    d["effective"] = {"algorithmic_bytes": alg, "achieved": alg / t_s / 1e9, "unit": "GB/s",
+                      "note": "algorithmic bytes / time; gathers hit L1/L2, so this exceeds HBM peak"}
+    tf = r["trilinear"] * 64 / t_s / 1e12
+    d["fma_flops"] = {"achieved": tf, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": tf / FP32_VECTOR_PEAK_TFLOPS}
+    return d
+
+
+def front_end_rooflines(cn, probes, head):
+    """Counter figures of the voxelize front end and the mip chain (FP16 frame): k_setup's
+    returning atomics against the atomic probe's rate, the mip build's HBM bytes."""
+    cn = cn or {}
+    out = {}
+    st, pr = cn.get("setup", {}), cn.get("probe_atomic_ret", {})
+    key = "TCP_TCC_ATOMIC_WITH_RET_REQ_sum"
+    if key in st and key in pr and probes.get("atomic_ret"):
+        rate = st[key] / (head["vox_setup_ms"] * 1e-3) / 1e9
+        peak = pr[key] / (probes["atomic_ret"] * 1e-3) / 1e9
+        out["setup_atomics"] = {"kernel": "k_setup (+ the counter clear and the large-triangle count)",
+                                "returning_atomics": st[key], "time_ms": head["vox_setup_ms"],
+                                "achieved": rate, "peak": peak, "unit": "G atomics/s", "frac": rate / peak,
+                                "peak_noreturn": (cn.get("probe_atomic_noret", {}).get(
+                                    "TCP_TCC_ATOMIC_WITHOUT_RET_REQ_sum", 0.0) / (probes["atomic_noret"] * 1e-3) / 1e9
+                                    if probes.get("atomic_noret") else None)}
+    m = cn.get(f"mips_{head['prec']}", {})
+    if "hbm_bytes" in m:
+        gbs = m["hbm_bytes"] / (head["mips_ms"] * 1e-3) / 1e9
+        out["mips"] = {"kernel": f"k_mips<{head['prec']}> (both launches)", "hbm_bytes": m["hbm_bytes"],
+                       "time_ms": head["mips_ms"], "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": gbs / HBM_PEAK_GBS}
+    v = cn.get("voxelize", {})
+    if "SQ_INSTS_VALU" in v:
+        out["voxelize"] = {"kernel": "k_voxelize<8>", "valu_instructions": v["SQ_INSTS_VALU"],
+                           "hbm_bytes": v.get("hbm_bytes"), "l2_hit_rate": v.get("l2_hit_rate")}
+    return out
+
+
 def _public(res):
     return {p: {k: v for k, v in r.items() if k not in ("diffuse", "specular")} for p, r in res.items()}
-
-
-def _traffic_bytes(traffic, prec):
-    if not traffic or prec not in traffic:
-        return None
-    t = traffic[prec]
-    if "FETCH_SIZE" not in t or "WRITE_SIZE" not in t:
-        return None
-    return t["FETCH_SIZE"] + t["WRITE_SIZE"]
 
 
 def fp64_errors(w, gpu, pixels):
@@ -279,25 +338,42 @@ def fp64_errors(w, gpu, pixels):
     return errs
 
 
-def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
-    """Oracle FP32 timed on this host: full voxelize + mips, trace on a pixel sample
-    scaled to the frame. Also the FP64 error of the GPU outputs on a 4096-pixel subset."""
+def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7, runs: int = 5):
+    """The oracle's FP32 form timed on this host's cores, as the frame it replaces: full
+    voxelize and inject + mips, and the trace of a seeded pixel sample scaled to the frame
+    (per-pixel work is independent). One untimed warm-up, then the median of `runs` timed
+    runs per phase, with one thread and with all the threads OpenMP is given. Also the
+    error of the GPU outputs against the oracle's FP64 run on a seeded 4096-pixel subset."""
     from oracle import oracle as orc
 
-    cores = orc.num_threads()
-    t0 = time.perf_counter()
-    acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
-    t_vox = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    lv = orc.inject_mips(acc, w.grid, w.lights, "fp32")
-    t_mip = time.perf_counter() - t0
+    all_threads = orc.num_threads()
     rng = np.random.default_rng(seed)
     px = rng.choice(w.num_pixels, sample_px, replace=False)
-    t0 = time.perf_counter()
-    orc.trace(lv, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough, pixels=px)
-    t_tr = (time.perf_counter() - t0) * (w.num_pixels / sample_px)
-    frame_s = t_vox + t_mip + t_tr
+
+    def one_run():
+        t0 = time.perf_counter()
+        acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+        t1 = time.perf_counter()
+        lv = orc.inject_mips(acc, w.grid, w.lights, "fp32")
+        t2 = time.perf_counter()
+        orc.trace(lv, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough, pixels=px)
+        t3 = time.perf_counter()
+        return t1 - t0, t2 - t1, (t3 - t2) * (w.num_pixels / sample_px)
+
+    res = {}
+    try:
+        for threads in (1, all_threads):
+            orc.set_threads(threads)
+            one_run()
+            times = np.array([one_run() for _ in range(runs)])
+            vox, mip, tr = (float(np.median(times[:, k])) for k in range(3))
+            frame_s = vox + mip + tr
+            res[threads] = dict(value=w.num_pixels / frame_s / 1e6, frame_s=frame_s, voxelize_s=vox, mips_s=mip,
+                                trace_s=tr, threads=threads)
+    finally:
+        orc.set_threads(all_threads)
     # error vs FP64 on a seeded 4096-pixel subset
+    acc, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
     sub = rng.choice(w.num_pixels, 4096, replace=False)
     lv64 = orc.inject_mips(acc, w.grid, w.lights, f64=True)
     d64, s64, _ = orc.trace(lv64, w.grid, w.gb_pos, w.gb_nrm, w.cones, gb_rough=w.gb_rough,
@@ -307,20 +383,59 @@ def cpu_baseline(w, gpu_results, sample_px: int, seed: int = 7):
     for r in gpu_results:
         got = np.concatenate([r["diffuse"][sub, :3], r["specular"][sub, :3]], 1)
         errs[r["prec"]] = dict(rel_l2=rel_l2(got, ref), max_abs=float(np.abs(got - ref).max()))
-    return dict(value=w.num_pixels / frame_s / 1e6, unit="Mpixels/s", cores=cores,
-                kind="port",
-                sample=(f"oracle FP32 ({cores} threads): full voxelize {t_vox:.2f}s + mips "
-                        f"{t_mip:.2f}s + trace of {sample_px} of {w.num_pixels} pixels scaled "
-                        f"to the frame ({t_tr:.2f}s)")), errs
+    allc = res[all_threads]
+    one = res[1]
+    return dict(value=allc["value"], unit="Mpixels/s", cores=all_threads, kind="port",
+                cpu_model=cpu_model(), nproc=os.cpu_count(),
+                affinity=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+                sample=(f"oracle FP32, C++ -O3: full voxelize + inject/mips, trace of {sample_px} of "
+                        f"{w.num_pixels} pixels scaled to the frame; 1 warm-up + median of {runs} per "
+                        f"phase; {all_threads} threads: voxelize {allc['voxelize_s']:.3f}s, mips "
+                        f"{allc['mips_s']:.3f}s, trace {allc['trace_s']:.2f}s (scaled)"),
+                all_cores=allc, single_thread=one,
+                single_thread_value=one["value"]), errs
 
 
 WORKLOAD_ARGS = ("tris", "n", "levels", "width", "height", "cones", "bin", "tile", "order")
 CALIB_MIB = 1024
 
 
+# Calibration kernels: 1 GiB k_copy (HBM bytes), the FMA probe (VALU issue ceiling) and the
+# atomic probes (returning / non-returning atomics on separate lines).
+PROBE_BLOCKS = 4096          # 1 M lanes: 16 k wavefronts, many per SIMD
+PROBE_FMA_ITERS = 4096       # 8 FMAs per round: 32 k VALU instructions per wavefront
+PROBE_ATOMIC_ITERS = 16      # 16 M atomics over 1 M lines
+
+
+def run_probes(ctx, vgi, reps=1):
+    """Launches the calibration kernels `reps` times each; returns the mean HIP-event time
+    of each kind in ms (the counters come from the same launches under rocprofv3)."""
+    out = {}
+    fbuf = vgi.DeviceArray(ctx, PROBE_BLOCKS * 256 * 4)
+    abuf = vgi.DeviceArray(ctx, PROBE_BLOCKS * 256 * 64)
+    try:
+        for name, kind, buf, iters in (("fma", vgi.PROBE_FMA, fbuf, PROBE_FMA_ITERS),
+                                       ("atomic_ret", vgi.PROBE_ATOMIC_RETURN, abuf, PROBE_ATOMIC_ITERS),
+                                       ("atomic_noret", vgi.PROBE_ATOMIC_NORETURN, abuf, PROBE_ATOMIC_ITERS)):
+            abuf.zero()
+            ctx.probe(kind, buf, PROBE_BLOCKS, iters)  # untimed first launch
+            ctx.sync()
+            ts = []
+            for _ in range(reps):
+                ctx.timer_start()
+                ctx.probe(kind, buf, PROBE_BLOCKS, iters)
+                ts.append(ctx.timer_stop())
+            out[name] = float(np.mean(ts))
+    finally:
+        fbuf.free()
+        abuf.free()
+    return out
+
+
 def traffic_probe(args) -> int:
-    """Run under rocprofv3 --pmc by measure_traffic: one 1 GiB k_copy (calibration), then
-    the headline frame's trace kernel three times with FP16 mips and three with FP32."""
+    """Run under rocprofv3 --pmc by measure_traffic: the calibration kernels (one 1 GiB
+    k_copy, the FMA probe, the two atomic probes), then per mip precision (FP16, then
+    FP32) one headline frame: voxelize, mips, and the trace three times."""
     from final184_amd import scenegen as sg
     from final184_amd import vgi
 
@@ -337,14 +452,18 @@ def traffic_probe(args) -> int:
     ctx.sync()
     src.free()
     dst.free()
+    run_probes(ctx, vgi)
     mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
     gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
     od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
     osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
     scene = ctx.scene()
-    scene.voxelize([mesh], vgi.make_grid(**w.grid))
+    grid = vgi.make_grid(**w.grid)
+    scene.voxelize([mesh], grid)  # sizes the stream's voxelize scratch
+    ctx.sync()
     cones = vgi.make_cones(**w.cones)
     for prec in (vgi.PREC_FP16, vgi.PREC_FP32):
+        scene.voxelize([mesh], grid)
         scene.build_mips(vgi.make_lights(w.lights), prec)
         for _ in range(3):
             scene.trace(gb, cones, od, osp)
@@ -364,11 +483,28 @@ def _counter_rows(outdir):
     return rows
 
 
+# Kernels whose counters are kept, as (key, regex on the kernel name). Per-precision keys
+# take the launches of that precision; the voxelize-front-end keys take the last frame
+# (FP32), whose front end is identical to the FP16 frame's.
+COUNTER_KERNELS = [
+    ("trace_fp16", r"k_trace<1\b"), ("trace_fp32", r"k_trace<0\b"),
+    ("mips_fp16_first", r"k_mips<1, true>"), ("mips_fp16_rest", r"k_mips<1, false>"),
+    ("mips_fp32_first", r"k_mips<0, true>"), ("mips_fp32_rest", r"k_mips<0, false>"),
+    ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8>"),
+    ("probe_fma", r"k_probe_fma"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
+    ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
+]
+COUNTER_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),
+                  ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "TCC_HIT_sum", "TCC_MISS_sum",
+                   "TCP_TCC_ATOMIC_WITH_RET_REQ_sum", "TCP_TCC_ATOMIC_WITHOUT_RET_REQ_sum"))
+
+
 def measure_traffic(args):
-    """HBM bytes per trace launch from two rocprofv3 --pmc runs of the probe (FETCH_SIZE,
-    then WRITE_SIZE), each scaled by what the counter reads for the 1 GiB copy, and a third
-    run for the launch's VALU instruction count and L2 hit rate. Runs in child processes
-    before this process touches the GPU."""
+    """Per-dispatch counters of the path's kernels and of the calibration kernels, from
+    three rocprofv3 --pmc runs of the probe mode (one per pass: FETCH_SIZE, WRITE_SIZE, then
+    VALU/SALU instructions, L2 hits and misses and atomic requests). FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) are scaled by what each reads for the 1 GiB copy in the
+    same run. Runs in child processes before this process touches the GPU."""
     import re
     import shutil
     import subprocess
@@ -379,14 +515,14 @@ def measure_traffic(args):
     wl = []
     for k in WORKLOAD_ARGS:
         wl += [f"--{k}", str(getattr(args, k))]
-    per = {}
-    # one pass per traffic counter, then the VALU instruction count and L2 hits / misses
-    for counters in (("FETCH_SIZE",), ("WRITE_SIZE",), ("SQ_INSTS_VALU", "TCC_HIT_sum", "TCC_MISS_sum")):
+    regex = "k_trace|k_copy|k_mips|k_setup|k_voxelize|k_probe"
+    rows_by_counter = {}
+    for counters in COUNTER_PASSES:
         d = tempfile.mkdtemp(prefix="vgi_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", *counters, "--kernel-include-regex",
-               "k_trace|k_copy", "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               regex, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                os.path.abspath(__file__), "--traffic-probe", *wl]
-        log(f"traffic: {' '.join(cmd)}")
+        log(f"counters: {' '.join(cmd)}")
         r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, capture_output=True,
                            text=True, timeout=200)
         all_rows = _counter_rows(d)
@@ -395,30 +531,47 @@ def measure_traffic(args):
             rows = [x for x in all_rows if x.get("Counter_Name") == counter]
             if not rows:
                 return None, f"{counter}: no counter rows (rocprofv3 exit {r.returncode}: {r.stderr[-300:]})"
-            per[counter] = rows
+            rows_by_counter[counter] = rows
     out = {}
-    for counter in ("SQ_INSTS_VALU", "TCC_HIT_sum", "TCC_MISS_sum"):
-        rows = per.pop(counter)
-        for prec, tag in (("fp16", "1"), ("fp32", "0")):
-            vals = [float(x["Counter_Value"]) for x in rows if re.search(r"k_trace<" + tag + r"\b", x["Kernel_Name"])]
+    for counter, rows in rows_by_counter.items():
+        for key, rx in COUNTER_KERNELS:
+            vals = [float(x["Counter_Value"]) for x in rows if re.search(rx, x["Kernel_Name"])]
+            if key in ("setup", "voxelize"):
+                vals = vals[-1:]  # the FP32 frame's front end
             if vals:
-                out.setdefault(prec, {})[counter] = float(np.mean(vals))
-    for prec in ("fp16", "fp32"):
-        o = out.get(prec, {})
-        if "TCC_HIT_sum" in o and "TCC_MISS_sum" in o:
-            o["l2_hit_rate"] = o["TCC_HIT_sum"] / max(o["TCC_HIT_sum"] + o["TCC_MISS_sum"], 1.0)
-    for counter, rows in per.items():
-        copy = [float(x["Counter_Value"]) for x in rows if "k_copy" in x["Kernel_Name"]]
-        if not copy or copy[0] <= 0:
+                out.setdefault(key, {})[counter] = float(np.mean(vals))
+    calib = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        copy = out.get("copy", {}).get(counter)
+        if not copy or copy <= 0:
             return None, f"{counter}: no calibration dispatch"
-        factor = (CALIB_MIB << 20) / (copy[0] * 1024.0)  # counter is in KiB
-        for prec, tag in (("fp16", "1"), ("fp32", "0")):
-            vals = [float(x["Counter_Value"]) for x in rows
-                    if re.search(r"k_trace<" + tag + r"\b", x["Kernel_Name"])]
-            if vals:
-                out.setdefault(prec, {})[counter] = float(np.mean(vals)) * 1024.0 * factor
-        out.setdefault("calibration", {})[counter] = factor
+        calib[counter] = (CALIB_MIB << 20) / (copy * 1024.0)  # counter is in KiB
+        for key, v in out.items():
+            if counter in v:
+                v[counter + "_bytes"] = v[counter] * 1024.0 * calib[counter]
+    for v in out.values():
+        if "TCC_HIT_sum" in v and "TCC_MISS_sum" in v:
+            v["l2_hit_rate"] = v["TCC_HIT_sum"] / max(v["TCC_HIT_sum"] + v["TCC_MISS_sum"], 1.0)
+        if "FETCH_SIZE_bytes" in v and "WRITE_SIZE_bytes" in v:
+            v["hbm_bytes"] = v["FETCH_SIZE_bytes"] + v["WRITE_SIZE_bytes"]
+    for prec in ("fp16", "fp32"):  # a mip build is its two launches
+        a, b = out.get(f"mips_{prec}_first", {}), out.get(f"mips_{prec}_rest", {})
+        if "hbm_bytes" in a:
+            out[f"mips_{prec}"] = {"hbm_bytes": a["hbm_bytes"] + b.get("hbm_bytes", 0.0),
+                                   "SQ_INSTS_VALU": a.get("SQ_INSTS_VALU", 0.0) + b.get("SQ_INSTS_VALU", 0.0)}
+    out["calibration"] = calib
     return out, None
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main() -> int:
@@ -521,14 +674,13 @@ def main() -> int:
         pipe_ms = run_pipelined(ctx, vgi, w, mesh, gb, head["prec"], max(args.steps, 20), args.warmup)
         log(f"{head['prec']}: two-stream overlapped frames {pipe_ms:.3f} ms per frame "
             f"(sequential {head['frame_ms']:.3f} ms)")
-    texel_b = {"fp32": 16, "fp16": 8, "bf16": 8}
-    # dominant kernel: the cone march
+    # calibration kernels timed in this process (their counters came from the probe runs)
+    probes = run_probes(ctx, vgi, reps=5)
+    log(f"probes (ms): {probes}")
     px = w.num_pixels
-    alg_bytes = px * (24 + 4 + 32) + head["trilinear"] * 8 * texel_b[head["prec"]]
-    achieved = alg_bytes / (head["trace_ms"] * 1e-3) / 1e9
-    fp32_alg = px * (24 + 4 + 32) + base["trilinear"] * 8 * texel_b[base["prec"]]
-    fp32_achieved = fp32_alg / (base["trace_ms"] * 1e-3) / 1e9
-    mip_levels_bytes = sum(((args.n >> k) ** 3) for k in range(args.levels))
+    roof = trace_roofline(traffic, probes, head, px)
+    roof_fp32 = trace_roofline(traffic, probes, base, px)
+    fe = front_end_rooflines(traffic, probes, head)
     line = {
         "metric": "cone-traced Mpixels/s + voxelized Mtris/s; p99 latency @ QPS, FP16 vs FP32",
         "value": px / (head["frame_ms"] * 1e-3) / 1e6,
@@ -552,7 +704,7 @@ def main() -> int:
                           "rejected": int(head["stats"].triangles_rejected),
                           "degenerate": int(head["stats"].triangles_degenerate)},
         "trace_mpixels_s": px / (head["trace_ms"] * 1e-3) / 1e6,
-        "mips_gbs": (args.n ** 3 * 32 + mip_levels_bytes * texel_b[head["prec"]]) / (head["mips_ms"] * 1e-3) / 1e9,
+        "mips_gbs": fe.get("mips", {}).get("achieved"),
         "phases_ms": {r["prec"]: {"voxelize": r["vox_ms"], "voxelize_raster": r["vox_raster_ms"],
                                   "mips": r["mips_ms"], "trace": r["trace_ms"], "frame": r["frame_ms"]}
                       for r in results},
@@ -564,25 +716,12 @@ def main() -> int:
         "trilinear_per_pixel": head["trilinear"] / px,
         "tuning": tuned.as_dict() if tuned is not None else
                   {"bin_size": args.bin, "schedule": args.order if args.order == "morton" else args.tile},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _traffic_bytes(traffic, head["prec"]),
-                     "kernel": f"k_trace<{head['prec']}>",
-                     "algorithmic_bytes": alg_bytes,
-                     "fp32_achieved": fp32_achieved, "fp32_frac": fp32_achieved / HBM_PEAK_GBS,
-                     "fp32_algorithmic_bytes": fp32_alg,
-                     "fp32_traffic": _traffic_bytes(traffic, base["prec"]),
-                     "traffic_counters": traffic if traffic else traffic_err},
+        "roofline": roof,
+        "roofline_fp32": roof_fp32,
+        "front_end": fe,
+        "counters": traffic if traffic else traffic_err,
+        "probes_ms": probes,
     }
-    for prec, r, key in ((head["prec"], head, "valu"), (base["prec"], base, "fp32_valu")):
-        t = (traffic or {}).get(prec, {})
-        if "SQ_INSTS_VALU" in t:
-            rate = t["SQ_INSTS_VALU"] / (r["trace_ms"] * 1e-3)
-            line["roofline"][key] = {"bound": "valu", "instructions": t["SQ_INSTS_VALU"],
-                                     "achieved": rate / 1e9, "unit": "G wave64 instr/s",
-                                     "per_cu_per_clock": rate / CU_CLOCKS_PER_S,
-                                     "fma_probe_per_cu_per_clock": 0.98,
-                                     "l2_hit_rate": t.get("l2_hit_rate")}
     try:
         line["screen_passes"] = run_screen(ctx, vgi, sg, w, mesh, gb.rough, args.steps)
     except Exception as e:  # the headline line is still printed

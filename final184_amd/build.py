@@ -20,7 +20,8 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "screen.hip", "abi.hip", "autotune.hip", "serve.hip"]
+HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "screen.hip", "abi.hip", "autotune.hip", "serve.hip",
+               "probe.hip"]
 
 HOST_SOURCES = ["gltf.cpp"]  # host-only C++ linked into libvgi.so
 

@@ -168,9 +168,9 @@ struct vgi_workspace {
   size_t sort_pix_bytes = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  uint8_t* h_read = nullptr;  // pinned readback of DevStats[] + totals[], then the handoff flag
-  size_t h_read_cap = 0;
-  uint32_t pub_seq = 0;  // last handoff sequence number (never 0)
+  unsigned long long* hist = nullptr;  // pinned: last call's batch pairs, units, call number
+  uint32_t seq = 0;                     // last voxelize call number on this stream (never 0)
+  uint64_t hist_pairs = 0, hist_units = 0;  // largest batch totals seen on this stream
 
   size_t device_bytes() const {
     size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
@@ -218,7 +218,13 @@ struct vgi_scene {
   size_t live_bytes = 0;
   uint32_t alloc_levels = 0, alloc_n = 0, alloc_prec = 0;
   bool has_voxels = false, has_mips = false;
-  uint64_t tris_in = 0, num_pairs = 0, work_items = 0, rejected = 0, degenerate = 0;
+  uint64_t tris_in = 0;
+  // the last voxelize call's statistics: written by the device into the pinned record after
+  // the bin scan (`result`, call number vox_seq); vox_done is recorded on the call's stream
+  // after that, and vgi_scene_get_stats waits for it before reading
+  SceneVoxResult* result = nullptr;
+  hipEvent_t vox_done = nullptr;
+  uint32_t vox_seq = 0;
 };
 
 namespace {
@@ -239,7 +245,7 @@ void free_workspace(vgi_workspace* w) {
     if (s.h) (void)hipHostFree(s.h);
     if (s.done) (void)hipEventDestroy(s.done);
   }
-  if (w->h_read) (void)hipHostFree(w->h_read);
+  if (w->hist) (void)hipHostFree(w->hist);
 }
 
 hipStream_t pick(vgi_context* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
@@ -329,33 +335,6 @@ int stage_descriptors(vgi_workspace* w, UploadSite site, size_t bytes, F&& fill,
   c.gen = sl->gen;
   c.bytes = bytes;
   *out = sl;
-  return VGI_OK;
-}
-
-// Device-to-host handoff of the voxelize totals: k_publish writes the DevStats words and
-// the per-scene totals into the pinned buffer w->h_read (stats at 0, totals at
-// totals_at) and then raises a sequence number at flag_at, which the host spins on. Stream
-// order makes k_publish follow all earlier work on the stream, so seeing the number means
-// the values are final. This replaces a device-to-host copy plus a stream synchronize,
-// whose wake-up costs tens of microseconds. A wait longer than kHandoffSpinSeconds falls
-// back to a stream synchronize, which reports a device error if one occurred.
-constexpr int kHandoffSpinSeconds = 5;
-
-int publish_and_wait(vgi_workspace* w, const VoxBatch& B, const void* dstats, size_t stats_bytes, size_t totals_at,
-                     size_t flag_at, hipStream_t st) {
-  uint32_t* flag = (uint32_t*)(w->h_read + flag_at);
-  if (++w->pub_seq == 0u) w->pub_seq = 1u;
-  const uint32_t seq = w->pub_seq;
-  HIP_TRY(launch_publish(B, dstats, stats_bytes, w->h_read, w->h_read + totals_at, flag, seq, st));
-  const auto t0 = std::chrono::steady_clock::now();
-  while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
-    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(kHandoffSpinSeconds)) {
-      HIP_TRY(hipStreamSynchronize(st));
-      if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) return fail(VGI_E_HIP, "device-to-host handoff did not arrive");
-      break;
-    }
-    __builtin_ia32_pause();
-  }
   return VGI_OK;
 }
 
@@ -519,22 +498,41 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     TRY(scene_reserve(p.s, p.n, p.nbins));
   }
 
+  // ---- history: the largest batch totals of earlier calls on this stream that have
+  // completed (read without waiting; a call still in flight is read by a later one) ----
+  if (w->hist && w->seq != 0 && __atomic_load_n(w->hist + 2, __ATOMIC_ACQUIRE) == (unsigned long long)w->seq) {
+    w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
+    w->hist_units = std::max<uint64_t>(w->hist_units, w->hist[1]);
+  }
+
   // ---- scratch sized for this batch (grown; queued work on this stream is drained
   // first so nothing still reads a block being replaced) ----
-  // device: the three per-bin counter arrays, then DevStats[count] (all cleared per call);
-  // host handoff buffer: DevStats[count], the totals[count + 1], the flag
+  // device: the three per-bin counter arrays, then DevStats[count], then the overflow word
+  // (all cleared per call)
   const size_t stats_off = align_up(3 * total_bins * 4, 256);
   const size_t stats_bytes = (size_t)count * sizeof(DevStats);
-  const size_t zeroed_need = stats_off + stats_bytes;
-  const size_t host_totals_at = stats_bytes;
-  const size_t read_bytes = stats_bytes + ((size_t)count + 1) * 8;
-  const size_t flag_at = align_up(read_bytes, 64);
+  const size_t ovf_off = align_up(stats_off + stats_bytes, 256);
+  const size_t zeroed_need = ovf_off + 256;
+  // The pair list holds 4 pairs per triangle plus 64 Ki (a triangle in at most 8 bins is
+  // the common case; large triangles add more), or 5/4 of the most an earlier call on this
+  // stream needed. A call that needs more is still exact: k_settle sees it and the
+  // per-triangle path runs, and the next call is sized from it.
+  const uint64_t pair_want = std::max<uint64_t>(4ull * total_tri + 65536, w->hist_pairs + w->hist_pairs / 4 + 4096);
+  const uint32_t pair_cap = (uint32_t)std::min<uint64_t>(pair_want, 0xFFFFFFF0ull);
+  // Work units: a bin with pairs has ceil(ceil(pairs / chunk) / unit_items) of them, at
+  // most 1 + pairs / chunk, so total_bins + pair_cap / chunk bounds every batch that fits.
+  const uint64_t work_cap64 = total_bins + pair_cap / std::max(1u, tn.chunk_tris) + 1;
+  if (work_cap64 >= (1ull << 32)) return fail(VGI_E_RANGE, "batch too large (%zu bins)", total_bins);
+  const uint32_t work_cap = (uint32_t)work_cap64;
   size_t scan_need = 0;
   HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(total_bins + 1), st));
   const size_t tneed = std::max<size_t>(total_tri, 1);
   const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
-                    w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need;
+                    w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need ||
+                    w->pairs_bytes < (size_t)pair_cap * 4 || w->work_bytes < (size_t)work_cap * sizeof(uint2);
   if (grow) HIP_TRY(hipStreamSynchronize(st));
+  TRY(grow_bytes(&w->pairs, &w->pairs_bytes, (size_t)pair_cap * 4, "bin pairs"));
+  TRY(grow_bytes(&w->work_map, &w->work_bytes, (size_t)work_cap * sizeof(uint2), "work map"));
   TRY(grow_bytes(&w->tris, &w->tris_bytes, tneed * sizeof(TriRec), "triangle records"));
   TRY(grow_bytes(&w->tri_slot, &w->tri_slot_bytes, tneed * kSmallBins * 4, "triangle slots"));
   TRY(grow_bytes(&w->big_list, &w->big_list_bytes, tneed * 4, "large-triangle list"));
@@ -543,15 +541,11 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
-  if (w->h_read_cap < flag_at + 64) {
-    if (w->h_read) (void)hipHostFree(w->h_read);
-    w->h_read = nullptr;
-    w->h_read_cap = 0;
-    const size_t cap = align_up(read_bytes * 2, 64) + 64;
-    HIP_TRY(hipHostMalloc((void**)&w->h_read, cap, hipHostMallocCoherent));
-    std::memset(w->h_read, 0, cap);  // the handoff flag starts at 0, which no handoff uses
-    w->h_read_cap = cap;
+  if (!w->hist) {
+    HIP_TRY(hipHostMalloc((void**)&w->hist, 4 * sizeof(unsigned long long), hipHostMallocCoherent));
+    std::memset(w->hist, 0, 4 * sizeof(unsigned long long));  // call number 0 is never used
   }
+  if (++w->seq == 0u) w->seq = 1u;
 
   // ---- descriptor tables: scenes | meshes | triangle blocks | bin blocks ----
   const bool multi = count > 1;
@@ -602,6 +596,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     S.acc = p.s->acc;
     S.mask = p.s->mask;
     S.stats = dstats + i;
+    S.result = p.s->result;
     for (int a = 0; a < 3; a++) S.origin[a] = it.grid.origin[a];
     S.snap_scale = (float)(p.n * 256u) / it.grid.extent;
     S.nmesh = it.num_meshes;
@@ -627,7 +622,13 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.bin_blocks = multi ? (const uint2*)(sl->d + o_bblk) : nullptr;
   B.packed_all = (unsigned long long*)w->packed_all;
   B.scan_all = (unsigned long long*)w->scan_all;
-  B.work_map = nullptr;
+  B.work_map = (uint2*)w->work_map;
+  B.pairs_all = (uint32_t*)w->pairs;
+  B.pair_cap = pair_cap;
+  B.work_cap = work_cap;
+  B.overflow = (uint32_t*)((uint8_t*)w->zeroed + ovf_off);
+  B.host_hist = w->hist;
+  B.seq = w->seq;
   B.nscenes = count;
   B.total_bins = (uint32_t)total_bins;
 
@@ -641,38 +642,27 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
   HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
-  TRY(publish_and_wait(w, B, dstats, stats_bytes, host_totals_at, flag_at, st));
+  HIP_TRY(launch_settle(B, st));
+  for (uint32_t i = 0; i < count; i++) {
+    vgi_scene* s = plan[i].s;
+    s->vox_seq = w->seq;
+    HIP_TRY(hipEventRecord(s->vox_done, st));
+  }
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
 
-  // ---- size the pair lists and the work map from the per-scene totals ----
-  const DevStats* hst = (const DevStats*)w->h_read;
-  const unsigned long long* htot = (const unsigned long long*)(w->h_read + host_totals_at);
-  uint64_t pairs_sum = 0, big_sum = 0, zero_sum = 0;
-  for (uint32_t i = 0; i < count; i++) {
-    pairs_sum += hst[i].pairs;
-    big_sum += hst[i].big_count;
-    zero_sum += hst[i].zero_count;
-  }
-  if (pairs_sum >= (1ull << 32))
-    return fail(VGI_E_RANGE, "batch produces %llu (triangle, bin) pairs; the limit is 2^32 - split the batch",
-                (unsigned long long)pairs_sum);
-  const uint32_t total_work = (uint32_t)(htot[count] >> 32);
-  TRY(grow_bytes(&w->pairs, &w->pairs_bytes, std::max<uint64_t>(pairs_sum, 1) * 4, "bin pairs"));
-  TRY(grow_bytes(&w->work_map, &w->work_bytes, std::max<size_t>(total_work, 1) * sizeof(uint2), "work map"));
-  // the pair list and the work map are kernel arguments: nothing is uploaded after the handoff
-  B.work_map = (uint2*)w->work_map;
-  B.pairs_all = (uint32_t*)w->pairs;
-  B.pairs_total = (uint32_t)pairs_sum;
-  if (pairs_sum > 0) {
-    const uint32_t big_blocks = (uint32_t)std::min<uint64_t>(persistent, (big_sum + 3) / 4);  // 4 waves each
-    const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(persistent, zero_sum);      // bins to clear
-    HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
-  }
+  // Everything after the scan reads its sizes on the device: k_scatter's role blocks are
+  // sized from the host's counts (triangle and bin blocks) and persistent for the large
+  // triangles and the split bins, whose counts only the device knows; k_voxelize takes
+  // one work unit per workgroup up to the launch size, which follows the largest unit
+  // count seen on this stream, and loops over the rest.
+  const uint32_t cus = (uint32_t)ctx->num_cus;
+  const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(cus / 4, total_bins);
+  HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, cus, zero_blocks, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
-  // one workgroup per work item: item costs vary by orders of magnitude, and the
-  // dispatcher balances them better than a static round-robin over persistent groups
-  const uint32_t grid_blocks = std::min<uint32_t>(total_work, 0x7FFFFFFFu);
-  HIP_TRY(launch_voxelize(B, ts, grid_blocks, total_work, st));
+  const uint64_t grid_want = w->hist_units ? w->hist_units + w->hist_units / 8 + 64 : (uint64_t)ctx->num_cus * 8;
+  const uint32_t grid_blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid_want, work_cap));
+  HIP_TRY(launch_voxelize(B, ts, grid_blocks, st));
+  HIP_TRY(launch_voxelize_fallback(B, cus, st));
   if (prof) {
     HIP_TRY(hipEventRecord(ctx->ev[4], st));
     ctx->ev_vox = true;
@@ -684,10 +674,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     s->nbins_axis = plan[i].nbins_axis;
     s->chunk = tn.chunk_tris;
     s->tris_in = plan[i].ntri;
-    s->num_pairs = hst[i].pairs;
-    s->work_items = (uint32_t)(htot[i] >> 32);
-    s->rejected = hst[i].rejected;
-    s->degenerate = hst[i].degenerate;
     s->has_voxels = true;
   }
   return VGI_OK;
@@ -1258,8 +1244,18 @@ int vgi_sync(vgi_context* ctx, void* stream) {
 
 int vgi_scene_create(vgi_context* ctx, vgi_scene** out) {
   if (!ctx || !out) return fail(VGI_E_INVALID, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
   vgi_scene* s = new vgi_scene();
   s->ctx = ctx;
+  if (hipHostMalloc((void**)&s->result, sizeof(SceneVoxResult), hipHostMallocCoherent) != hipSuccess ||
+      hipEventCreateWithFlags(&s->vox_done, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (s->result) (void)hipHostFree(s->result);
+    delete s;
+    return fail(VGI_E_NOMEM, "scene record allocation failed");
+  }
+  std::memset(s->result, 0, sizeof(SceneVoxResult));
   *out = s;
   return VGI_OK;
 }
@@ -1274,6 +1270,8 @@ int vgi_scene_release(vgi_scene* s) {
   for (uint32_t k = 0; k < VGI_MAX_LEVELS; k++)
     if (s->level[k]) (void)hipFree(s->level[k]);
   if (s->live) (void)hipFree(s->live);
+  if (s->result) (void)hipHostFree(s->result);
+  if (s->vox_done) (void)hipEventDestroy(s->vox_done);
   delete s;
   return VGI_OK;
 }
@@ -1296,10 +1294,18 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   if (!s || !out) return fail(VGI_E_INVALID, "NULL argument");
   std::memset(out, 0, sizeof(*out));
   out->triangles_in = s->tris_in;
-  out->triangles_rejected = s->rejected;
-  out->triangles_degenerate = s->degenerate;
-  out->bin_pairs = s->num_pairs;
-  out->work_items = s->work_items;
+  if (s->vox_seq != 0) {  // the voxelize call's own statistics, once the device stored them
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipEventSynchronize(s->vox_done));
+    const SceneVoxResult* r = s->result;
+    if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq)
+      return fail(VGI_E_HIP, "voxelize statistics of the scene's last call are missing");
+    out->triangles_rejected = r->st.rejected;
+    out->triangles_degenerate = r->st.degenerate;
+    out->bin_pairs = r->st.pairs;
+    out->work_items = r->totals >> 32;
+    out->voxelize_fallback = r->overflow;
+  }
   out->device_bytes = scene_device_bytes(s);
   out->n = s->grid.n;
   out->levels = s->grid.levels;
@@ -1680,6 +1686,30 @@ int vgi_stream_create(vgi_context* ctx, void** out) {
   HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   *out = (void*)s;
   return VGI_OK;
+}
+
+int vgi_probe(vgi_context* ctx, uint32_t kind, void* buf, size_t buf_bytes, uint32_t blocks, uint32_t iters,
+              void* stream) {
+  if (!ctx || !buf) return fail(VGI_E_INVALID, "NULL argument");
+  if (kind > VGI_PROBE_ATOMIC_NORETURN) return fail(VGI_E_INVALID, "probe kind %u", kind);
+  if (blocks == 0 || blocks > (1u << 20)) return fail(VGI_E_INVALID, "probe blocks %u", blocks);
+  const size_t need = (size_t)blocks * 256u * (kind == VGI_PROBE_FMA ? 4u : 64u);
+  if (buf_bytes < need) return fail(VGI_E_INVALID, "probe buffer of %zu bytes, %zu needed", buf_bytes, need);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(launch_probe(kind, buf, blocks, iters, pick(ctx, stream)));
+  return VGI_OK;
+}
+
+int vgi_stream_query(vgi_context* ctx, void* stream) {
+  if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipError_t e = hipStreamQuery(pick(ctx, stream));
+  if (e == hipSuccess) return VGI_OK;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return VGI_E_NOT_READY;
+  }
+  return fail(VGI_E_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
 }
 
 int vgi_stream_wait(vgi_context* ctx, void* waiting, void* signalling) {

@@ -34,7 +34,7 @@
 // workgroup then takes 256 consecutive sorted pixels of one scene, so the cones of a
 // wavefront start close together and share cache lines. Every pixel's result is the
 // same in either order.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "vgi_internal.hpp"
 
@@ -73,6 +73,20 @@ template <> struct Acc<VGI_PREC_BF16> {  // RGB bf16, opacity fp16
 };
 
 template <int PREC> constexpr uint32_t kTexelBytes = (PREC == VGI_PREC_FP32) ? 16u : 8u;
+
+// Whether a zero-opacity texel of level l0 + 1 proves its whole subtree empty, so that the
+// fine gather at level l0 may be skipped. Occupied level-0 voxels have opacity 1 and a
+// texel is the rounded mean of its 8 children, all non-negative: the sum is at least its
+// largest term and the mean of a nonzero child is >= 8^-k at level k, which round-to-
+// nearest maps to a value >= 8^-k as long as 8^-k is representable. FP32 represents it at
+// every level; FP16 opacity (the FP16 and BF16 forms) down to 2^-24 = 8^-8, so from level
+// 9 on (512^3 grids with 10 levels, 1024^3) a parent can round to zero over a nonzero
+// child, and there the fine gather is always taken.
+constexpr uint32_t kHalfExactSkipTop = 8;
+template <int PREC>
+__device__ __forceinline__ bool skip_fine_ok(uint32_t l0) {
+  return PREC == VGI_PREC_FP32 || l0 + 1 <= kHalfExactSkipTop;
+}
 
 // Trilinear footprint of level k at level-0 voxel coordinates u (texel centres at i+0.5):
 // index of texel (x0, y0, z0) in the padded level, and the 8 weights scaled by wl.
@@ -243,9 +257,8 @@ __device__ __forceinline__ bool inside_grid(float ux, float uy, float uz, float 
 // the grid boundary or opacity >= cutoff.
 //
 // The coarse footprint (level l0 + 1) holds the parents of every fine (level l0) footprint
-// texel; a parent with zero opacity has an all-zero subtree (texels are box averages of
-// non-negative values and opacity cannot underflow to zero), so when the coarse
-// footprint's opacity is zero the fine gather is exactly zero and is skipped.
+// texel. When the coarse footprint's opacity is zero and zero opacity implies an all-zero
+// subtree, the fine gather is exactly zero and is skipped (skip_fine_ok).
 template <int PREC>
 __device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const SchedEntry* __restrict__ sched,
                                                    uint32_t nsched, float ox, float oy, float oz, float dx,
@@ -275,7 +288,7 @@ __device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const S
         count += 1;
       } else {
         gather_uniform<PREC>(P, e.code + 1, ux, uy, uz, e.fr, s, abits);
-        if (abits) gather_uniform<PREC>(P, e.code, ux, uy, uz, e.gr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(e.code)) gather_uniform<PREC>(P, e.code, ux, uy, uz, e.gr, s, abits);
         count += 2;
       }
       composite(s, cr, cg, cb, a);
@@ -326,10 +339,10 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
       const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
       if (BUF && __all(l0 == lu)) {
         gather_uniform<PREC>(P, lu + 1, ux, uy, uz, fr, s, abits);
-        if (abits) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(lu)) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
       } else {
         gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-        if (abits) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(l0)) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       }
       count += 2;
     }
@@ -535,8 +548,8 @@ hipError_t launch_morton_keys(const TraceBatch& B, unsigned long long* keys, uin
 hipError_t sort_pixels(void* tmp, size_t* tmp_bytes, const unsigned long long* keys_in,
                        unsigned long long* keys_out, const uint32_t* pix_in, uint32_t* pix_out, uint32_t n,
                        int end_bit, hipStream_t s) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, keys_in, keys_out, pix_in, pix_out, (int)n, 0,
-                                            end_bit, s);
+  return rocprim::radix_sort_pairs(tmp, *tmp_bytes, keys_in, keys_out, pix_in, pix_out, (size_t)n, 0u,
+                                   (unsigned)end_bit, s);
 }
 
 }  // namespace vgi

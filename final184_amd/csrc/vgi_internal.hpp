@@ -58,6 +58,17 @@ struct DevStats {
 };
 static_assert(sizeof(DevStats) == 32, "DevStats layout");
 
+// A scene's result of one voxelize call, stored by k_settle after the bin scan into the
+// scene's pinned host record (read by vgi_scene_get_stats once the call has completed).
+struct SceneVoxResult {
+  DevStats st;
+  unsigned long long totals;  // work units << 32 | pairs of the scene's bins
+  uint32_t overflow;          // the call's pair list did not fit: per-triangle path taken
+  uint32_t seq;               // call number, stored last
+  unsigned long long pad[2];
+};
+static_assert(sizeof(SceneVoxResult) == 64, "SceneVoxResult layout");
+
 // One scene of a voxelize batch (device memory). acc, mask and bin_packed belong to the
 // scene and stay resident; every other array is this scene's slice of the calling
 // stream's voxelize scratch, valid for the duration of one call.
@@ -75,6 +86,7 @@ struct VoxScene {
   uint32_t* acc;                   // n^3 x 8 words, brick-major (8^3 bricks)
   uint32_t* mask;                  // 16 words (512 bits) per 8^3 brick: occupied voxels
   DevStats* stats;
+  SceneVoxResult* result;          // pinned host record of the scene
   float origin[3];
   float snap_scale;                // float(n * 256) / extent
   uint32_t nmesh;
@@ -94,9 +106,13 @@ struct VoxBatch {
   const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
   unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
   unsigned long long* scan_all;     // exclusive scan of packed_all
-  uint2* work_map;                  // (scene, bin) of every work unit
+  uint2* work_map;                  // (scene, bin) of every work unit (work_cap entries)
   uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn
-  uint32_t pairs_total;             // entries of pairs_all
+  uint32_t pair_cap;                // entries of pairs_all
+  uint32_t work_cap;                // entries of work_map
+  uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps
+  unsigned long long* host_hist;    // pinned: batch pairs, units, then the call number
+  uint32_t seq;                     // this call's number (never 0)
   uint32_t nscenes;
   uint32_t total_bins;
 };
@@ -307,12 +323,12 @@ hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
-hipError_t launch_publish(const VoxBatch& B, const void* stats, size_t stats_bytes, void* dst_stats,
-                          void* dst_totals, uint32_t* flag_host, uint32_t seq, hipStream_t s);
+hipError_t launch_probe(uint32_t kind, void* buf, uint32_t blocks, uint32_t iters, hipStream_t s);
+hipError_t launch_settle(const VoxBatch& B, hipStream_t s);
 hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,
                           uint32_t zero_blocks, hipStream_t s);
-hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
-                           hipStream_t s);
+hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, hipStream_t s);
+hipError_t launch_voxelize_fallback(const VoxBatch& B, uint32_t grid_blocks, hipStream_t s);
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s);
 hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s);
 // Morton keys of every pixel of a batch (entries seg[0]..seg[nscenes], padding included).

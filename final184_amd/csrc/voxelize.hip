@@ -8,9 +8,11 @@
 // Every kernel serves a batch of scenes (one scene is a batch of one): per-triangle and
 // per-bin kernels map workgroups to (scene, first item) through a block table, and the
 // scenes' bins are concatenated so that one scan and one persistent voxelize launch
-// cover the whole batch. Pipeline (one stream; after step 4 k_publish writes the per-scene
-// totals to pinned host memory and raises a flag the host waits on, and the host sizes the
-// pair list and the work map):
+// cover the whole batch. Pipeline (one stream, enqueued without waiting: the host sizes
+// the pair list and the work map before the call from the triangle count and what earlier
+// calls on the stream needed; after step 4 k_settle checks that the pairs and work units
+// fit, and when they do not, steps 5 and 6 give way to a per-triangle path (7) that needs
+// neither):
 //   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
 //                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
 //                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
@@ -22,8 +24,8 @@
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
 //                   bins with > 1 unit are listed.
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
-//                   offset and its first work unit at once; k_publish adds the per-scene
-//                   totals and hands them to the host.
+//                   offset and its first work unit at once; k_settle stores the per-scene
+//                   totals and statistics and sets the overflow word.
 //   5 k_scatter     one launch: work unit -> (scene, bin); small triangles to their kept
 //                   slots (no atomics); large triangles after the small ones of each bin;
 //                   clear the accumulator and occupancy mask of bins with several units.
@@ -36,10 +38,15 @@
 //                   flush every tile. Flush: occupancy mask + the records of occupied
 //                   voxels, by plain stores for bins of one unit (most bins: a unit is up
 //                   to kUnitItems chunks) and by 64-bit integer atomics otherwise.
+//   7 k_voxelize_fallback  only when the overflow word is set (k_scatter then cleared every
+//                   bin with pairs instead of scattering): one wavefront per triangle tests
+//                   every voxel of its box and adds its increments by 64-bit atomics. The
+//                   voxel set and the sums are the same; it is slower, and the host grows the
+//                   pair list for the calls that follow.
 //
 // All coverage decisions are exact integer arithmetic on snapped vertices, so the
 // voxel set and the integer sums do not depend on order, rounding or thread mapping.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "vgi_internal.hpp"
 
@@ -439,7 +446,7 @@ __device__ __forceinline__ void big_pass(const VoxBatch& B, const VoxScene* __re
         } else {
           const uint32_t c = ctr(P, bin);
           const uint32_t pos = pair_base(B, P, bin) + gld(P.bin_small + c) + gatomic_add(P.big_cursor + c, 1u);
-          if (pos < B.pairs_total) gst(B.pairs_all + pos, t);
+          if (pos < B.pair_cap) gst(B.pairs_all + pos, t);
         }
       }
     }
@@ -489,40 +496,48 @@ hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, in, out, (int)count, s);
+  return rocprim::exclusive_scan(tmp, *tmp_bytes, in, out, 0ull, (size_t)count, rocprim::plus<unsigned long long>(), s);
 }
 
-// Hands the per-scene results of steps 1-4 to the host: copies the DevStats words to pinned
-// host memory, writes each scene's total (units << 32 | pairs of its bins, from the scan)
-// and the batch total at [nscenes] beside them, then stores `seq` into the host flag. One
-// wavefront: the system-scope release orders every lane's stores before the flag, so a host
-// thread that sees the flag (acquire) reads all of them. The host waits on the flag instead
-// of a stream synchronize and a device-to-host copy.
-__global__ __launch_bounds__(64) void k_publish(VoxBatch B, const VoxScene* __restrict__ scenes,
-                                                const uint32_t* __restrict__ stats, uint32_t stats_words,
-                                                uint32_t* dst_stats, unsigned long long* dst_totals,
-                                                uint32_t* flag, uint32_t seq) {
-  for (uint32_t i = threadIdx.x; i < stats_words; i += 64) dst_stats[i] = gld(stats + i);
-  for (uint32_t sc = threadIdx.x; sc <= B.nscenes; sc += 64) {
-    unsigned long long t;
-    if (sc == B.nscenes) {
-      t = gld(B.scan_all + B.total_bins);
-    } else {
-      const VoxScene& P = scenes[sc];
-      const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-      t = gld(B.scan_all + P.bin_base + nbins) - gld(B.scan_all + P.bin_base);
-    }
-    dst_totals[sc] = t;
+// After the scan (one wavefront): each scene's totals (work units << 32 | pairs of its
+// bins) and statistics go to its pinned host record; the batch's exact pair count (a
+// 64-bit sum of the per-scene counts, which the packed 32-bit field cannot hold beyond
+// 2^32) and unit count are checked against the pair list and work map the host sized, and
+// the overflow word tells steps 5-7 which path to take. The batch totals and the call
+// number go to the stream's pinned history, from which the host sizes later calls.
+__global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  unsigned long long pairs = 0;
+  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) pairs += gld(&scenes[sc].stats->pairs);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  const uint32_t units = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);
+  // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is exact
+  const uint32_t ovf = (pairs > B.pair_cap || units > B.work_cap) ? 1u : 0u;
+  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
+    const VoxScene& P = scenes[sc];
+    const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
+    SceneVoxResult* r = P.result;
+    r->st.rejected = gld(&P.stats->rejected);
+    r->st.degenerate = gld(&P.stats->degenerate);
+    r->st.pairs = gld(&P.stats->pairs);
+    r->st.big_count = gld(&P.stats->big_count);
+    r->st.zero_count = gld(&P.stats->zero_count);
+    r->totals = gld(B.scan_all + P.bin_base + nbins) - gld(B.scan_all + P.bin_base);
+    r->overflow = ovf;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __hip_atomic_store(&r->seq, B.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0) {
+    gst(B.overflow, ovf);
+    B.host_hist[0] = pairs;
+    B.host_hist[1] = units;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __hip_atomic_store(B.host_hist + 2, (unsigned long long)B.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
-hipError_t launch_publish(const VoxBatch& B, const void* stats, size_t stats_bytes, void* dst_stats,
-                          void* dst_totals, uint32_t* flag_host, uint32_t seq, hipStream_t s) {
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, B, B.scenes, (const uint32_t*)stats,
-                     (uint32_t)(stats_bytes / 4), (uint32_t*)dst_stats, (unsigned long long*)dst_totals, flag_host,
-                     seq);
+hipError_t launch_settle(const VoxBatch& B, hipStream_t s) {
+  hipLaunchKernelGGL(k_settle, dim3(1), dim3(64), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 
@@ -558,7 +573,7 @@ __device__ __forceinline__ void emit_small_block(const VoxBatch& B, const VoxSce
     uint32_t bin;
     if (!bin_candidate(P, q, N, bb, j, bin)) continue;
     const uint32_t pos = pair_base(B, P, bin) + gld(P.tri_slot + (size_t)t * kSmallBins + nslot++);
-    if (pos < B.pairs_total) gst(B.pairs_all + pos, t);
+    if (pos < B.pair_cap) gst(B.pairs_all + pos, t);
   }
 }
 
@@ -583,13 +598,39 @@ __device__ __forceinline__ void zero_pass(const VoxBatch& Bt, const VoxScene* __
   }
 }
 
-// One launch for the steps after the host handoff, before k_voxelize: workgroups
-// [0, nblk_b) map work units, the next nblk_t scatter small triangles, the next nblk_big
-// place large triangles (one wavefront each), the rest clear the bins that several work
-// units add into (none on most scenes). They write disjoint outputs.
+// Overflow path: workgroup blk of the bins clears the accumulator records and occupancy
+// words of its bins that have pairs, for the per-triangle kernel to add into.
+__device__ __forceinline__ void clear_bins_block(const VoxBatch& Bt, const VoxScene* __restrict__ scenes,
+                                                 uint32_t blk_id) {
+  const uint2 blk = bin_block(Bt, blk_id);
+  const VoxScene& P = scenes[blk.x];
+  const uint32_t nb = P.nbins_axis, nbins = nb * nb * nb, B = 1u << P.bin_shift, nbr = P.n >> 3, bpb = B >> 3;
+  const uint32_t bend = min(blk.y + kBlockBins, nbins);
+  for (uint32_t bin = blk.y; bin < bend; bin++) {
+    if ((uint32_t)gld(P.bin_packed + bin) == 0u) continue;  // no pairs: the bin stays empty
+    const uint32_t bx = bin % nb, by = (bin / nb) % nb, bz = bin / (nb * nb);
+    for (uint32_t k = 0; k < bpb * bpb * bpb; k++) {
+      const uint32_t bxx = bx * bpb + k % bpb, byy = by * bpb + (k / bpb) % bpb, bzz = bz * bpb + k / (bpb * bpb);
+      const uint32_t brick = bxx + nbr * (byy + nbr * bzz);
+      uint4* dst = (uint4*)(P.acc + (size_t)brick * 512u * kAccWords);
+      for (uint32_t j = threadIdx.x; j < 1024; j += blockDim.x) gst4(dst + j, make_uint4(0, 0, 0, 0));
+      if (threadIdx.x < 4) gst4((uint4*)(P.mask + (size_t)brick * 16u) + threadIdx.x, make_uint4(0, 0, 0, 0));
+    }
+  }
+}
+
+// One launch after the scan, before k_voxelize: workgroups [0, nblk_b) map work units, the
+// next nblk_t scatter small triangles, the next nblk_big place large triangles (one
+// wavefront each), the rest clear the bins that several work units add into (none on most
+// scenes). They write disjoint outputs. When the pair list or the work map would overflow,
+// workgroups [0, nblk_b) clear every bin with pairs instead and the rest do nothing.
 __global__ __launch_bounds__(256) void k_scatter(VoxBatch B, const VoxScene* __restrict__ scenes, uint32_t nblk_b,
                                                  uint32_t nblk_t, uint32_t nblk_big) {
   const uint32_t b = blockIdx.x;
+  if (gld(B.overflow) != 0u) {
+    if (b < nblk_b) clear_bins_block(B, scenes, b);
+    return;
+  }
   if (b < nblk_b) {
     workmap_block(B, scenes, b);
   } else if (b < nblk_b + nblk_t) {
@@ -767,7 +808,7 @@ struct TileLayout {
 // 6 instead of 5 took the room's launch from 80 to 75 us.
 template <int TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 6 : 1)))
-void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork) {
+void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
   using L = TileLayout<TS>;
@@ -779,8 +820,11 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
+  if (gld(Bt.overflow) != 0u) return;  // the per-triangle kernel voxelizes this call
+  // the batch's work units, from the scan (the launch is sized from earlier calls: normally
+  // one unit per workgroup, and workgroups take further units in turn when it falls short)
+  const uint32_t nwork = (uint32_t)(gld(Bt.scan_all + Bt.total_bins) >> 32);
 
-  // work units of every scene of the batch (normally one per workgroup)
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const uint2 item = gld2(Bt.work_map + w);
     const VoxScene& P = scenes[item.x];
@@ -934,12 +978,67 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes, uint32_t nwork
   }
 }
 
-hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, uint32_t nwork,
-                           hipStream_t s) {
-  if (grid_blocks == 0 || nwork == 0) return hipSuccess;
+hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, hipStream_t s) {
+  if (grid_blocks == 0) return hipSuccess;
   static_assert(TileLayout<8>::ENTRIES == 582 && TileLayout<16>::ENTRIES == 4366, "tile layout");
-  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
-  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes, nwork);
+  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 7 overflow path: per-triangle voxelize with global atomics
+// ---------------------------------------------------------------------------
+
+// Runs only when k_settle set the overflow word (k_scatter then cleared every bin with
+// pairs). Wavefront w of the launch takes triangles w, w + nwaves, ... of every scene; its
+// lanes walk the triangle's voxel box, keep the voxels the exact closed-box test accepts
+// (the same test as the bin pass, at voxel size) and add the triangle's increments: R, G,
+// B and the count, then the three biased normal components, as four 64-bit atomics (exact
+// with fewer than 2^21 triangles per voxel: no 32-bit word's total reaches 2^32), and set
+// the voxel's occupancy bit. Integer sums do not depend on order, so the accumulator is
+// the same as the binned path's.
+__global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const VoxScene* __restrict__ scenes) {
+  if (gld(B.overflow) == 0u) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), nwaves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t sc = 0; sc < B.nscenes; sc++) {
+    const VoxScene& P = scenes[sc];
+    const uint32_t nbr = P.n >> 3;
+    for (uint32_t t = wave; t < P.ntri; t += nwaves) {
+      const TriRec R = gld_rec(P.tris + t);
+      if (!tri_ok(R)) continue;
+      int64_t q[3][3], N[3];
+      load_q(R, q, N);
+      const int nx = R.hi[0] - R.lo[0] + 1, ny = R.hi[1] - R.lo[1] + 1, nz = R.hi[2] - R.lo[2] + 1;
+      const uint32_t r = R.rgb & 0xFFu, g = (R.rgb >> 8) & 0xFFu, b = (R.rgb >> 16) & 0xFFu;
+      const unsigned long long i0 = (unsigned long long)r | ((unsigned long long)g << 32);
+      const unsigned long long i1 = (unsigned long long)b | (1ull << 32);
+      const unsigned long long i2 = (unsigned long long)(uint32_t)(R.nq[0] + kNormalBias) |
+                                    ((unsigned long long)(uint32_t)(R.nq[1] + kNormalBias) << 32);
+      const unsigned long long i3 = (unsigned long long)(uint32_t)(R.nq[2] + kNormalBias);
+      const uint32_t total = (uint32_t)nx * (uint32_t)ny * (uint32_t)nz;
+      for (uint32_t j = lane; j < total; j += 64) {
+        const int x = R.lo[0] + (int)(j % (uint32_t)nx), y = R.lo[1] + (int)((j / (uint32_t)nx) % (uint32_t)ny),
+                  z = R.lo[2] + (int)(j / ((uint32_t)nx * (uint32_t)ny));
+        const int32_t bb[3] = {x << VGI_FIXED_BITS, y << VGI_FIXED_BITS, z << VGI_FIXED_BITS};
+        if (!sat_box(q, N, bb, VGI_FIXED_BITS)) continue;
+        const uint32_t brick = ((uint32_t)x >> 3) + nbr * (((uint32_t)y >> 3) + nbr * ((uint32_t)z >> 3));
+        const uint32_t l = ((uint32_t)x & 7u) + 8u * ((uint32_t)y & 7u) + 64u * ((uint32_t)z & 7u);
+        unsigned long long* d64 = (unsigned long long*)(P.acc + ((size_t)brick * 512u + l) * kAccWords);
+        gatomic_add(d64 + 0, i0);
+        gatomic_add(d64 + 1, i1);
+        gatomic_add(d64 + 2, i2);
+        gatomic_add(d64 + 3, i3);
+        gatomic_or(P.mask + (size_t)brick * 16u + (l >> 5), 1u << (l & 31u));
+      }
+    }
+  }
+}
+
+hipError_t launch_voxelize_fallback(const VoxBatch& B, uint32_t grid_blocks, hipStream_t s) {
+  if (grid_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_voxelize_fallback, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 

@@ -192,7 +192,7 @@ def run_serving(ctx, vgi, seconds=60.0, qps=200.0, budget_gb=200.0, keys=1000, s
                      diffuse_ring_deg=60.0, specular_aperture_deg=10.0, step_factor=0.5,
                      max_distance=1.0, alpha_cutoff=0.99, normal_offset=1.0, start_offset=1.0)
     outs = _OutPool(ctx, vgi, out_buffers)
-    budget = int(budget_gb * (1 << 30))
+    budget = int(budget_gb * 1e9)  # decimal gigabytes
     lat, builds, frames_done, px_done = [], 0, 0, 0
     held = {}
     with vgi.Server(ctx, budget, vgi.PREC_FP16, slots=slots, max_batch_requests=max_batch) as srv:

@@ -70,6 +70,7 @@ class GBuffer(C.Structure):
 ORDER_TILES, ORDER_MORTON = 0, 1
 ADDRESS_AUTO, ADDRESS_WIDE = 0, 1  # cone-march texel addressing (vgi.h VGI_ADDRESS_*)
 IMAGE_RGBA32F, IMAGE_RGBA16F = 0, 1
+PROBE_FMA, PROBE_ATOMIC_RETURN, PROBE_ATOMIC_NORETURN = 0, 1, 2  # vgi.h VGI_PROBE_*
 
 
 class GBufferRef(C.Structure):
@@ -116,7 +117,8 @@ class SceneStats(C.Structure):
                 ("triangles_degenerate", C.c_uint64), ("bin_pairs", C.c_uint64),
                 ("work_items", C.c_uint64), ("device_bytes", C.c_uint64), ("n", C.c_uint32),
                 ("levels", C.c_uint32), ("precision", C.c_uint32), ("has_voxels", C.c_uint32),
-                ("has_mips", C.c_uint32), ("bin_size", C.c_uint32)]
+                ("has_mips", C.c_uint32), ("bin_size", C.c_uint32), ("voxelize_fallback", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class Timings(C.Structure):
@@ -242,6 +244,8 @@ SIGNATURES = [
     ("vgi_stream_create", C.c_int, [_P, C.POINTER(_P)]),
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
     ("vgi_stream_wait", C.c_int, [_P, _P, _P]),
+    ("vgi_stream_query", C.c_int, [_P, _P]),
+    ("vgi_probe", C.c_int, [_P, C.c_uint32, _P, C.c_size_t, C.c_uint32, C.c_uint32, _P]),
     ("vgi_workspace_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("vgi_timer_start", C.c_int, [_P, _P]),
     ("vgi_timer_stop", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
@@ -618,6 +622,18 @@ class Context:
     def stream_wait(self, waiting, signalling) -> None:
         """Later work on `waiting` waits for the work enqueued on `signalling` so far."""
         _check(lib.vgi_stream_wait(self.handle, waiting, signalling))
+
+    def stream_busy(self, stream=None) -> bool:
+        """True while work enqueued on the stream has not completed (does not block)."""
+        rc = lib.vgi_stream_query(self.handle, stream)
+        if rc == -5:
+            return True
+        _check(rc)
+        return False
+
+    def probe(self, kind: int, buf: "DeviceArray", blocks: int, iters: int, stream=None) -> None:
+        """One calibration kernel (PROBE_FMA, PROBE_ATOMIC_RETURN, PROBE_ATOMIC_NORETURN)."""
+        _check(lib.vgi_probe(self.handle, kind, buf.ptr, buf.nbytes, blocks, iters, stream))
 
     def workspace_bytes(self) -> int:
         b = C.c_uint64()

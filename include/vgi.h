@@ -8,7 +8,12 @@
  * Every entry point returns an int status (VGI_OK or a negative VGI_E_* code) and
  * never throws. vgi_last_error() returns the message of the calling thread's last
  * failure. All GPU work is asynchronous on the given stream (a hipStream_t passed as
- * void*; NULL selects the context's own stream); vgi_sync() blocks.
+ * void*; NULL selects the context's own stream): the voxelize, mips, trace and screen
+ * entries enqueue their kernels and return without waiting for the device. The calls
+ * that wait are named as such: vgi_sync, vgi_scene_get_stats (for the scene's last
+ * voxelize), the parity readbacks, the glTF upload, the autotuner and the server's
+ * drain/destroy; a call that must replace scratch grown too small first drains its own
+ * stream (rare: scratch only grows).
  *
  * Reference interfaces each entry replaces (tobyc11/Final184, path:line):
  *   - CVoxelizeRenderer::RenderList / Render   Foreground/Renderer/VoxelizeRenderer.cpp:19-30,81-127
@@ -36,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VGI_ABI_VERSION 1u
+#define VGI_ABI_VERSION 2u
 
 #define VGI_MAX_LEVELS 11u   /* 1024^3 -> 1^3 */
 #define VGI_MAX_LIGHTS 16u
@@ -178,6 +183,9 @@ typedef struct vgi_scene_stats {
   uint32_t has_voxels;
   uint32_t has_mips;
   uint32_t bin_size;
+  uint32_t voxelize_fallback; /* 1: the last voxelize took the per-triangle path (its pair
+                                 list did not fit the size planned for the call) */
+  uint32_t pad;
 } vgi_scene_stats;
 
 /* Per-phase GPU durations of the last call on a context with profiling enabled,
@@ -213,6 +221,8 @@ int vgi_sync(vgi_context* ctx, void* stream);
  * the accumulator and the mip levels; memory is sized on the first voxelize. */
 int vgi_scene_create(vgi_context* ctx, vgi_scene** out);
 int vgi_scene_release(vgi_scene* scene);
+/* Waits until the scene's last voxelize call has stored its statistics on the device
+ * (the bin scan of that call has run), then reads them. */
 int vgi_scene_get_stats(const vgi_scene* scene, vgi_scene_stats* out);
 /* Device bytes a scene of this grid and precision holds while resident: accumulator,
  * occupancy mask, bin table and mip levels (admission control). The voxelize scratch
@@ -223,7 +233,13 @@ uint64_t vgi_scene_bytes_estimate(const vgi_grid* grid, uint32_t precision, uint
 /* Clear + voxelize all meshes into the scene's accumulator (one frame's voxelize pass).
  * Replaces MegaPipeline.cpp:196 (clear) + :218-223 (VoxelizeRenderer.RenderList), i.e.
  * VoxelizeRenderer.cpp:19-30,81-127 and the VoxelGS/VoxelPS pipeline
- * (Pipelang/Internal/main.lua:83-144,242-275). Conservative, order-independent. */
+ * (Pipelang/Internal/main.lua:83-144,242-275). Conservative, order-independent.
+ * Asynchronous like the reference's recorded voxelize pass (committed with the frame,
+ * MegaPipeline.cpp:332): the (triangle, bin) pair list and the work map are sized on the
+ * host before the call (4 pairs per triangle + 64 Ki, or 5/4 of the most an earlier
+ * call on the same stream needed); a call whose pairs do not fit is detected on the
+ * device and voxelized by a per-triangle path instead, with the same result (reported by
+ * vgi_scene_stats.voxelize_fallback), and later calls on the stream are sized from it. */
 int vgi_voxelize(vgi_context* ctx, vgi_scene* scene, const vgi_mesh* meshes, uint32_t num_meshes,
                  const vgi_grid* grid, void* stream);
 
@@ -519,9 +535,24 @@ int vgi_stream_destroy(vgi_context* ctx, void* stream);
  * own mip build, which reads the accumulator, and its mip build must not overlap its
  * trace, which reads the levels). */
 int vgi_stream_wait(vgi_context* ctx, void* waiting, void* signalling);
+/* VGI_OK when every call enqueued on the stream has completed, VGI_E_NOT_READY while work
+ * is pending (hipStreamQuery; does not block). */
+int vgi_stream_query(vgi_context* ctx, void* stream);
 /* Device bytes of voxelize scratch and descriptor staging the context holds, summed over
  * the streams it has run on (each stream keeps its own, grown to its largest batch). */
 int vgi_workspace_bytes(vgi_context* ctx, uint64_t* out);
+/* ---- calibration kernels (ceilings for the counter-based rooflines) ---------- */
+/* No reference counterpart. One launch of `blocks` workgroups of 256 lanes on the stream:
+ * VGI_PROBE_FMA: each lane runs `iters` rounds of 8 independent FP32 fused multiply-adds
+ *   and stores one float (buf holds blocks * 256 floats): the VALU issue ceiling;
+ * VGI_PROBE_ATOMIC_RETURN / _NORETURN: each lane adds 1 `iters` times to the first word of
+ *   its own 64-byte line (buf holds blocks * 256 * 64 bytes), with returning atomics whose
+ *   results are summed into the line's second word, or non-returning ones: the atomic
+ *   rate without contention. */
+enum { VGI_PROBE_FMA = 0, VGI_PROBE_ATOMIC_RETURN = 1, VGI_PROBE_ATOMIC_NORETURN = 2 };
+int vgi_probe(vgi_context* ctx, uint32_t kind, void* buf, size_t buf_bytes, uint32_t blocks, uint32_t iters,
+              void* stream);
+
 /* Device-side elapsed time between two points on a stream (HIP events). */
 int vgi_timer_start(vgi_context* ctx, void* stream);
 int vgi_timer_stop(vgi_context* ctx, void* stream, float* out_ms);

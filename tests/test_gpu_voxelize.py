@@ -176,3 +176,113 @@ def test_axis_aligned_quad_closed_form(ctx):
     assert np.array_equal(acc[..., 0][occ], 100 * acc[..., 3][occ])
     # normal points along +z: sum(nz + 1024) = count * 2048
     assert np.array_equal(acc[..., 6][occ], 2048 * acc[..., 3][occ])
+
+
+def _planes(count, n, seed):
+    """`count` axis-aligned z planes across the whole [0,1]^2, two triangles each, at
+    seeded depths off the voxel boundaries, with seeded albedo: every triangle overlaps
+    about (n/8)^2 / 2 bins of 8^3, far more pairs per triangle than the pair list plans."""
+    rng = np.random.default_rng(seed)
+    zs = (np.sort(rng.choice(n, count, replace=False)) + rng.uniform(0.2, 0.8, count)) / n
+    tris = []
+    for z in zs:
+        a, b, c, d = (0.001, 0.001, z), (0.999, 0.001, z), (0.999, 0.999, z), (0.001, 0.999, z)
+        tris += [a + b + c, a + c + d]
+    pos = np.array(tris, np.float32).reshape(-1, 9)
+    alb = rng.integers(0, 256, (len(pos), 3), dtype=np.uint8)
+    return pos, alb
+
+
+def test_pair_overflow_takes_exact_fallback(ctx):
+    """160 full-width triangles at 256^3 make about 87k (triangle, bin) pairs against a pair
+    list planned for 4 per triangle + 64 Ki on a stream with no history: the device detects
+    the overflow and the per-triangle path voxelizes the call. Accumulator and FP16 levels
+    must equal the oracle's; the second call on the stream is sized from the first and takes
+    the binned path, with the same bits. A two-scene batch on a fresh stream overflows too."""
+    vgi = _vgi()
+    print("case pair-list overflow fallback", flush=True)
+    n = 256
+    grid_d = dict(origin=(0.0, 0.0, 0.0), extent=1.0, n=n, levels=8)
+    pos, alb = _planes(80, n, 5)
+    ref, _ = orc.voxelize(pos, grid_d, albedo=alb)
+    lights = [dict(type="directional", vec=(0.0, 0.0, -1.0), color=(1.0, 1.0, 1.0))]
+    ref16 = orc.inject_mips(ref, grid_d, lights, "fp16")
+    mesh = vgi.MeshData(ctx, pos.reshape(-1, 3), albedo=alb)
+    grid = vgi.make_grid(**grid_d)
+    st_a = ctx.stream_create()
+    sc = ctx.scene()
+    try:
+        for call in range(2):
+            sc.voxelize([mesh], grid, stream=st_a)
+            sc.build_mips(vgi.make_lights(lights), vgi.PREC_FP16, stream=st_a)
+            ctx.sync(st_a)
+            st = sc.stats()
+            print(f"call {call}: pairs {st.bin_pairs} fallback {st.voxelize_fallback}", flush=True)
+            assert st.voxelize_fallback == (1 if call == 0 else 0)
+            assert st.bin_pairs > 4 * len(pos) + 65536
+            acc = sc.read_voxels()
+            assert np.array_equal(acc, ref), _diff_msg(acc, ref)
+            for k in range(grid_d["levels"]):
+                assert np.array_equal(sc.read_level(k), ref16[k]), f"call {call} level {k}"
+    finally:
+        sc.release()
+        ctx.stream_destroy(st_a)
+    # a batch: the planes scene plus a soup, on a fresh stream
+    w = sg.random_soup(ntri=3000, n=64)
+    ref_w, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+    mesh_w = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    st_b = ctx.stream_create()
+    s1, s2 = ctx.scene(), ctx.scene()
+    try:
+        ctx.voxelize_batch([(s1, [mesh], grid), (s2, [mesh_w], vgi.make_grid(**w.grid))], st_b)
+        ctx.sync(st_b)
+        assert s1.stats().voxelize_fallback == 1 and s2.stats().voxelize_fallback == 1
+        assert np.array_equal(s1.read_voxels(), ref) and np.array_equal(s2.read_voxels(), ref_w)
+    finally:
+        s1.release()
+        s2.release()
+        ctx.stream_destroy(st_b)
+
+
+def test_voxelize_returns_without_waiting(ctx):
+    """vgi_voxelize enqueues its kernels and returns: queued behind about 40 cone traces on
+    the same stream, the call comes back while the stream is still busy and in a small
+    fraction of the queued time, and the accumulator it leaves is the oracle's."""
+    import time
+    vgi = _vgi()
+    print("case voxelize is asynchronous", flush=True)
+    w = sg.room(ntri=60_000, n=128, levels=7, width=960, height=540)
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    grid = vgi.make_grid(**w.grid)
+    lights = vgi.make_lights(w.lights)
+    gb = vgi.GBufferData(ctx, w.gb_pos, w.gb_nrm, w.width, w.height, w.gb_rough)
+    cones = vgi.make_cones(**w.cones)
+    od = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    osp = vgi.DeviceArray.empty(ctx, (w.num_pixels, 4), np.float32)
+    tracer, target = ctx.scene(), ctx.scene()
+    try:
+        tracer.voxelize([mesh], grid)
+        tracer.build_mips(lights, vgi.PREC_FP32)
+        target.voxelize([mesh], grid)  # allocations and scratch are in place
+        ctx.sync()
+        ctx.timer_start()
+        tracer.trace(gb, cones, od, osp)
+        one_ms = ctx.timer_stop()
+        for _ in range(40):
+            tracer.trace(gb, cones, od, osp)
+        t0 = time.perf_counter()
+        target.voxelize([mesh], grid)
+        call_ms = (time.perf_counter() - t0) * 1e3
+        busy = ctx.stream_busy()
+        print(f"trace {one_ms:.3f} ms each, 40 queued; voxelize call returned in {call_ms:.3f} ms, "
+              f"stream busy {busy}", flush=True)
+        assert busy
+        assert call_ms < 0.25 * 40 * one_ms
+        ctx.sync()
+        ref, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+        acc = target.read_voxels()
+        assert np.array_equal(acc, ref), _diff_msg(acc, ref)
+        assert target.stats().voxelize_fallback == 0
+    finally:
+        tracer.release()
+        target.release()

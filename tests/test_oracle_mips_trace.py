@@ -258,3 +258,162 @@ def test_pixel_subset_equals_full_trace():
     px = np.array([3, 100, 517, 1023])
     ds, ss, _ = orc.trace(lv, w.grid, w.gb_pos, w.gb_nrm, w.cones, pixels=px)
     assert np.array_equal(ds, d[px]) and np.array_equal(ss, s[px])
+
+
+# ---------------------------------------------------------------------------
+# cone trace on non-uniform media: closed forms that pin the gather
+# ---------------------------------------------------------------------------
+#
+# Volume: level k holds, at texel (x, y, z), opacity alpha_k (one value per level) and
+# premultiplied colour alpha_k * (a + b * c) per channel, where c is the texel centre in
+# level-0 voxel units along that channel's axis (R: x, G: y, B: z), (i + 1/2) * 2^k. A
+# trilinear gather at u (level-0 units) whose 2x2x2 footprint lies inside the level
+# returns exactly alpha_k * (a + b * u) per channel: trilinear interpolation reproduces
+# linear functions, and the centre convention fixes where. Blending two levels gives
+# (alpha mixed by the level fraction) * (a + b * u). With per-level opacities the sample's
+# opacity depends only on its level, so the compositing weights depend only on the
+# schedule, and the march result has a closed form. What fails: swapped or transposed
+# trilinear weights (another channel's fraction), a texel-centre offset other than 1/2,
+# a level blend in the wrong direction, a wrong level for a diameter, a wrong compositing
+# order, a wrong ring angle or weight for the diffuse cones.
+
+_RAMP_A = np.array([0.30, 0.20, 0.10])
+_RAMP_B = np.array([0.011, -0.007, 0.013])
+_RAMP_ALPHA = np.array([0.02, 0.05, 0.11, 0.17, 0.23])
+
+
+def _ramp_levels(n, alphas):
+    out = []
+    for k, al in enumerate(alphas):
+        nk = n >> k
+        c = (np.arange(nk, dtype=np.float64) + 0.5) * (1 << k)
+        lv = np.empty((nk, nk, nk, 4), np.float64)  # [z][y][x][ch]
+        lv[..., 0] = al * (_RAMP_A[0] + _RAMP_B[0] * c[None, None, :])
+        lv[..., 1] = al * (_RAMP_A[1] + _RAMP_B[1] * c[None, :, None])
+        lv[..., 2] = al * (_RAMP_A[2] + _RAMP_B[2] * c[:, None, None])
+        lv[..., 3] = al
+        out.append(lv)
+    return out
+
+
+def _ramp_schedule(h, tan_half, step, start, maxd, L, alphas):
+    """Per sample: distance t and the sample's opacity (level blend of alphas), from the
+    march definition: diameter max(h, 2 t tan), level max(0, log2(d / h)), the top level
+    alone at or above L - 1, t += step * d, while t <= maxd."""
+    ts, al = [], []
+    t = start * h
+    while t <= maxd:
+        d = max(h, 2.0 * t * tan_half)
+        lvl = max(0.0, math.log2(d / h))
+        if lvl >= L - 1:
+            a = alphas[L - 1]
+        else:
+            l0 = int(math.floor(lvl))
+            fr = lvl - l0
+            a = alphas[l0] * (1.0 - fr) + alphas[l0 + 1] * fr
+        ts.append(t)
+        al.append(a)
+        t = t + step * d
+    return np.array(ts), np.array(al)
+
+
+def _composite_ramp(ts, al, u0, du):
+    """Front-to-back sum over samples of (1 - A) * alpha_i * (a + b * u_i), u_i = u0 + du t_i
+    (per channel), and the final opacity A."""
+    C = np.zeros(3)
+    A = 0.0
+    for t, a in zip(ts, al):
+        u = u0 + du * t
+        C += (1.0 - A) * a * (_RAMP_A + _RAMP_B * u)
+        A += (1.0 - A) * a
+    return C, A
+
+
+def _ramp_setup():
+    n, L = 64, 5
+    grid = dict(origin=(0.0, 0.0, 0.0), extent=1.0, n=n, levels=L)
+    return n, L, grid, _ramp_levels(n, _RAMP_ALPHA)
+
+
+@pytest.mark.parametrize("aperture", [10.0, 60.0, 100.0])
+def test_linear_ramp_specular_cone_closed_form(aperture):
+    """The specular cone along +-x, +-y, +-z (eye placed on the normal, so the reflection is
+    the normal itself) through the ramp volume; apertures 10, 60 and 100 degrees put the
+    samples on levels 0-1, 0-4 and up to the top level alone."""
+    n, L, grid, lv = _ramp_setup()
+    h = 1.0 / n
+    maxd = 22 * h  # every footprint stays within [8, 56] voxels of a 64-voxel grid
+    tan_half = math.tan(math.radians(aperture / 2))
+    ts, al = _ramp_schedule(h, tan_half, 0.5, 1.0, maxd, L, _RAMP_ALPHA)
+    levels_used = {int(math.floor(max(0.0, math.log2(max(h, 2 * t * tan_half) / h)))) for t in ts}
+    if aperture == 60.0:
+        assert levels_used >= {0, 1, 2, 3, 4}
+    P = np.array([0.5, 0.5, 0.5])
+    for ax in range(3):
+        for sgn in (1.0, -1.0):
+            nv = np.zeros(3)
+            nv[ax] = sgn
+            cones = _cones(num_diffuse=0, specular=True, specular_aperture_deg=aperture, max_distance=maxd,
+                           alpha_cutoff=2.0, normal_offset=1.0, start_offset=1.0, eye=tuple(P + nv))
+            _, s, _ = orc.trace(lv, grid, P[None].astype(np.float32), nv[None].astype(np.float32), cones,
+                                f64=True)
+            o = P + nv * h  # normal offset of one voxel
+            C, A = _composite_ramp(ts, al, o / h, nv / h)
+            np.testing.assert_allclose(s[0, :3], C, rtol=1e-12, atol=1e-15)
+            np.testing.assert_allclose(s[0, 3], A, rtol=1e-12)
+
+
+@pytest.mark.parametrize("K", [6, 9, 16])
+def test_linear_ramp_diffuse_cones_closed_form(K):
+    """K diffuse cones: cone 0 along the normal with weight 1 / (1 + (K-1) cos 60deg), K-1
+    cones on the 60-degree ring with weight cos 60deg / (1 + (K-1) cos 60deg). All share one
+    schedule, so the weighted result is one march along the weighted mean direction
+    D = n (1 + (K-1) cos^2 60deg) / (1 + (K-1) cos 60deg) (the ring's tangential parts cancel):
+    the channel along the normal fixes the ring angle and the weights, the other two the
+    cancellation. Diffuse alpha is 1 - A."""
+    n, L, grid, lv = _ramp_setup()
+    h = 1.0 / n
+    maxd = 22 * h
+    tan_half = math.tan(math.radians(30.0))
+    ts, al = _ramp_schedule(h, tan_half, 0.5, 1.0, maxd, L, _RAMP_ALPHA)
+    c = 0.5  # cos 60 degrees
+    scale = (1 + (K - 1) * c * c) / (1 + (K - 1) * c)
+    P = np.array([0.5, 0.5, 0.5])
+    for ax in range(3):
+        for sgn in (1.0, -1.0):
+            nv = np.zeros(3)
+            nv[ax] = sgn
+            cones = _cones(num_diffuse=K, specular=False, max_distance=maxd, alpha_cutoff=2.0,
+                           diffuse_aperture_deg=60.0, diffuse_ring_deg=60.0)
+            d, _, _ = orc.trace(lv, grid, P[None].astype(np.float32), nv[None].astype(np.float32), cones,
+                                f64=True)
+            o = P + nv * h
+            C, A = _composite_ramp(ts, al, o / h, nv * scale / h)
+            np.testing.assert_allclose(d[0, :3], C, rtol=1e-12, atol=1e-15)
+            np.testing.assert_allclose(d[0, 3], 1.0 - A, rtol=1e-12)
+
+
+def test_linear_ramp_off_axis_cone():
+    """A specular cone along a generic direction: each channel's ramp is sampled at its own
+    coordinate of u(t), so a gather that mixes axes fails."""
+    n, L, grid, lv = _ramp_setup()
+    h = 1.0 / n
+    maxd = 20 * h
+    tan_half = math.tan(math.radians(25.0))
+    ts, al = _ramp_schedule(h, tan_half, 0.5, 1.0, maxd, L, _RAMP_ALPHA)
+    nv = np.array([0.48, -0.6, 0.64])  # unit length
+    P = np.array([0.5, 0.5, 0.5], np.float32)
+    cones = _cones(num_diffuse=0, specular=True, specular_aperture_deg=50.0, max_distance=maxd,
+                   alpha_cutoff=2.0, eye=tuple(P.astype(np.float64) + nv))
+    _, s, _ = orc.trace(lv, grid, P[None], nv[None].astype(np.float32), cones, f64=True)
+    n64 = nv.astype(np.float32).astype(np.float64)
+    n64 = n64 / np.linalg.norm(n64)
+    # reflection of the view direction about the normal, as the march forms it
+    eye = np.array(cones["eye"], np.float32).astype(np.float64)
+    v = eye - P.astype(np.float64)
+    I = -v / np.linalg.norm(v)
+    r = I - 2 * np.dot(n64, I) * n64
+    o = P.astype(np.float64) + n64 * h
+    C, A = _composite_ramp(ts, al, o / h, r / h)
+    np.testing.assert_allclose(s[0, :3], C, rtol=1e-10)
+    np.testing.assert_allclose(s[0, 3], A, rtol=1e-12)
