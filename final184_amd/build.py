@@ -23,7 +23,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 HIP_SOURCES = ["voxelize.hip", "mips.hip", "trace.hip", "screen.hip", "abi.hip", "autotune.hip", "serve.hip",
                "probe.hip"]
 
-HOST_SOURCES = ["gltf.cpp"]  # host-only C++ linked into libvgi.so
+HOST_SOURCES = ["gltf.cpp", "grid.cpp"]  # host-only C++ linked into libvgi.so
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -245,6 +245,7 @@ SIGNATURES = [
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
     ("vgi_stream_wait", C.c_int, [_P, _P, _P]),
     ("vgi_stream_query", C.c_int, [_P, _P]),
+    ("vgi_grid_from_voxel_camera", C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     ("vgi_probe", C.c_int, [_P, C.c_uint32, _P, C.c_size_t, C.c_uint32, C.c_uint32, _P]),
     ("vgi_workspace_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("vgi_timer_start", C.c_int, [_P, _P]),
@@ -278,6 +279,15 @@ def make_grid(origin, extent: float, n: int, levels: int) -> Grid:
     g.extent = float(extent)
     g.n = int(n)
     g.levels = int(levels)
+    return g
+
+
+def grid_from_voxel_camera(voxel_view, voxel_proj, n: int, levels: int) -> Grid:
+    """The grid holding the reference's voxelizer camera volume (GLSL column-major mat4s)."""
+    v = np.ascontiguousarray(voxel_view, np.float32).reshape(16)
+    p = np.ascontiguousarray(voxel_proj, np.float32).reshape(16)
+    g = Grid()
+    _check(lib.vgi_grid_from_voxel_camera(v.ctypes.data, p.ctypes.data, n, levels, C.byref(g)))
     return g
 
 

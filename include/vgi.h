@@ -92,6 +92,18 @@ typedef struct vgi_grid {
   uint32_t levels;
 } vgi_grid;
 
+/* The reference's voxelizer camera as a grid: the world-space image of the clip box
+ * [-1, 1]^2 x [0, 1] under inverse(voxel_proj * voxel_view) (GLSL column-major mat4s as
+ * the reference uploads VoxelProjection / VoxelView, MegaPipeline.cpp:241-246; the
+ * camera of App/MainBehaviour.cpp:54-64 with Camera.cpp:104-111's orthographic matrix),
+ * held in the cube with the box's centre and its longest side as edge, n voxels per axis.
+ * Needs an orthographic projection and a view rotation that maps world axes to world axes
+ * (VGI_E_INVALID otherwise). Replaces the VoxelView / VoxelProj pair as the voxelize
+ * pass's grid input; the reference's 32 x 19 x 32 box at 128^3 becomes a 32-unit cube
+ * of 0.25-unit voxels (the reference's are 0.25 x 0.148 x 0.25). Host only. */
+int vgi_grid_from_voxel_camera(const float voxel_view[16], const float voxel_proj[16], uint32_t n,
+                               uint32_t levels, vgi_grid* out);
+
 /* One drawable primitive: CTriangleMesh (vertex + index buffer, element count;
  * Foreground/Shape/TriangleMesh.h:29-89) + CBasicMaterial albedo
  * (Foreground/Material/BasicMaterial.h) + the model matrix RenderList passes

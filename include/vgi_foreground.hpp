@@ -109,6 +109,22 @@ public:
       : Ctx(ctx), Scene(scene), Grid(grid) {}
 
   void SetGrid(const vgi_grid& grid) { Grid = grid; }
+  // The voxelizer view's matrices as the reference binds them (VoxelView, VoxelProjection:
+  // GLSL column-major, MegaPipeline.cpp:241-246): the grid becomes the cube holding that
+  // orthographic volume, n voxels per axis (the reference's image is 128^3). Returns false
+  // (and keeps the previous grid) for a camera that is not orthographic and axis-aligned.
+  bool SetVoxelCamera(const float voxelView[16], const float voxelProj[16], uint32_t n = 128,
+                      uint32_t levels = 7) {
+    vgi_grid g;
+    Status = vgi_grid_from_voxel_camera(voxelView, voxelProj, n, levels, &g);
+    if (Status != VGI_OK) {
+      Error = vgi_last_error();
+      return false;
+    }
+    Grid = g;
+    return true;
+  }
+  const vgi_grid& GetGrid() const { return Grid; }
 
   // Kept for interface parity; voxelization needs no render pass here.
   void SetRenderPass(const CRenderPass& /*renderPass*/, uint32_t /*subpass*/ = 0) {}
