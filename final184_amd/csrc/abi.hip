@@ -447,6 +447,45 @@ struct VoxPlan {
   uint32_t tri_blocks, bin_blocks;
 };
 
+// The kernels of one voxelize call once its descriptors and scratch are in place: B holds
+// their device addresses, the zeroed_bytes at zb (bin counters, per-scene statistics, the
+// overflow word) are cleared first, and each scene's statistics event is recorded after
+// k_settle. Everything after the scan reads its sizes on the device: k_scatter's role
+// blocks are sized from the host's counts (triangle and bin blocks) and persistent for the
+// large triangles and the split bins, whose counts only the device knows; k_voxelize takes
+// one work unit per workgroup up to grid_blocks and loops over the rest.
+int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, size_t zeroed_bytes, void* scan_tmp,
+                            size_t scan_tmp_bytes, size_t nblk_t, size_t nblk_b, uint32_t ts, uint32_t grid_blocks,
+                            vgi_scene* const* scenes, uint32_t count, hipStream_t st, bool prof) {
+  const uint32_t cus = (uint32_t)ctx->num_cus;
+  const uint32_t persistent = cus * 4;
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
+  // bin_small, bin_big, big_cursor, the per-scene counters and the overflow word start at zero
+  HIP_TRY(launch_fill_u32(zb, 0u, zeroed_bytes / 4, st));
+  HIP_TRY(launch_setup(B, (uint32_t)nblk_t, st));
+  HIP_TRY(launch_big_count(B, persistent, st));
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
+  size_t tmp_bytes = scan_tmp_bytes;
+  HIP_TRY(scan_bins(scan_tmp, &tmp_bytes, B.packed_all, B.scan_all, B.total_bins + 1, st));
+  HIP_TRY(launch_settle(B, st));
+  for (uint32_t i = 0; i < count; i++) {
+    scenes[i]->vox_seq = B.seq;
+    HIP_TRY(hipEventRecord(scenes[i]->vox_done, st));
+  }
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
+  const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(cus / 4, B.total_bins);
+  HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, cus, zero_blocks, st));
+  if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
+  HIP_TRY(launch_voxelize(B, ts, grid_blocks, st));
+  HIP_TRY(launch_voxelize_fallback(B, cus, st));
+  if (prof) {
+    HIP_TRY(hipEventRecord(ctx->ev[4], st));
+    ctx->ev_vox = true;
+  }
+  return VGI_OK;
+}
+
 int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t count, void* stream) {
   if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
   if (count == 0) return VGI_OK;
@@ -632,41 +671,12 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.nscenes = count;
   B.total_bins = (uint32_t)total_bins;
 
-  const uint32_t persistent = (uint32_t)ctx->num_cus * 4;
-  const bool prof = ctx->profiling != 0;
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
-  // bin_small, bin_big, big_cursor and the per-scene counters start at zero
-  HIP_TRY(launch_fill_u32(zb, 0u, zeroed_need / 4, st));
-  HIP_TRY(launch_setup(B, (uint32_t)nblk_t, st));
-  HIP_TRY(launch_big_count(B, persistent, st));
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
-  HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
-  HIP_TRY(scan_bins(w->scan_tmp, &w->scan_tmp_bytes, B.packed_all, B.scan_all, (uint32_t)(total_bins + 1), st));
-  HIP_TRY(launch_settle(B, st));
-  for (uint32_t i = 0; i < count; i++) {
-    vgi_scene* s = plan[i].s;
-    s->vox_seq = w->seq;
-    HIP_TRY(hipEventRecord(s->vox_done, st));
-  }
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-
-  // Everything after the scan reads its sizes on the device: k_scatter's role blocks are
-  // sized from the host's counts (triangle and bin blocks) and persistent for the large
-  // triangles and the split bins, whose counts only the device knows; k_voxelize takes
-  // one work unit per workgroup up to the launch size, which follows the largest unit
-  // count seen on this stream, and loops over the rest.
-  const uint32_t cus = (uint32_t)ctx->num_cus;
-  const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(cus / 4, total_bins);
-  HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, cus, zero_blocks, st));
-  if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
+  std::vector<vgi_scene*> scs(count);
+  for (uint32_t i = 0; i < count; i++) scs[i] = plan[i].s;
   const uint64_t grid_want = w->hist_units ? w->hist_units + w->hist_units / 8 + 64 : (uint64_t)ctx->num_cus * 8;
   const uint32_t grid_blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid_want, work_cap));
-  HIP_TRY(launch_voxelize(B, ts, grid_blocks, st));
-  HIP_TRY(launch_voxelize_fallback(B, cus, st));
-  if (prof) {
-    HIP_TRY(hipEventRecord(ctx->ev[4], st));
-    ctx->ev_vox = true;
-  }
+  TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_need, w->scan_tmp, w->scan_tmp_bytes, nblk_t, nblk_b, ts,
+                              grid_blocks, scs.data(), count, st, ctx->profiling != 0));
   for (uint32_t i = 0; i < count; i++) {
     vgi_scene* s = plan[i].s;
     s->grid = items[i].grid;
@@ -1112,6 +1122,42 @@ const char* vgi_version(void) { return "vgi 0.2.0 (gfx950)"; }
 uint32_t vgi_abi_version(void) { return VGI_ABI_VERSION; }
 const char* vgi_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// allocation-free voxelize: one static device array per GPU
+// ---------------------------------------------------------------------------
+
+// The scratch of vgi_voxelize_noalloc. Each GPU the library runs on holds its own copy
+// (module globals are per device) until the process ends. In host code the array's name
+// is a host-side stand-in, not a device address: the per-device address comes from
+// hipGetSymbolAddress in noalloc_scratch_setup, and only that address is used.
+static __device__ __align__(256) uint8_t scratch[VGI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_dev[kMaxDevices] = {};  // scratch's device address, per device
+
+// Once per device, with that device current (vgi_create).
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(VGI_E_RANGE, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return VGI_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return fail(VGI_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  }
+  g_scratch_dev[device] = (uint8_t*)p;
+  return VGI_OK;
+}
+
+uint8_t* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+}
+}  // namespace
+
 int vgi_create(vgi_context** out, int device) {
   if (!out) return fail(VGI_E_INVALID, "out is NULL");
   *out = nullptr;
@@ -1128,6 +1174,7 @@ int vgi_create(vgi_context** out, int device) {
     return fail(VGI_E_NODEVICE, "device %d is %s; this library carries gfx950 code only", device,
                 prop.gcnArchName);
   HIP_TRY(hipSetDevice(device));
+  TRY(noalloc_scratch_setup(device));
   vgi_context* ctx = new vgi_context();
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;
@@ -1322,6 +1369,133 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* s, const vgi_mesh* meshes, uint32_
   TRY(check_grid(grid));
   vgi_voxelize_item it{s, meshes, num_meshes, *grid};
   return voxelize_batch(ctx, &it, 1, stream);
+}
+
+int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes, uint32_t num_meshes,
+                         const vgi_grid* grid, void* stream) {
+  if (!ctx || !sc) return fail(VGI_E_INVALID, "NULL context or scene");
+  if (sc->ctx != ctx) return fail(VGI_E_INVALID, "scene belongs to another context");
+  TRY(check_grid(grid));
+  if (num_meshes > VGI_NOALLOC_MAX_MESHES)
+    return fail(VGI_E_RANGE, "%u meshes exceed the allocation-free limit of %u", num_meshes, VGI_NOALLOC_MAX_MESHES);
+  uint64_t ntri = 0;
+  TRY(check_meshes(meshes, num_meshes, &ntri));
+  const vgi_tuning tn = tuning_of(ctx);
+  const uint32_t n = grid->n;
+  const uint32_t bin_shift = ilog2(std::min(tn.bin_size, n));
+  const uint32_t nba = n >> bin_shift;
+  const uint64_t nbins = (uint64_t)nba * nba * nba;
+  // the scene's resident volume must exist already: sizing it would allocate
+  if (sc->acc_bytes != (size_t)n * n * n * kAccWords * 4 || sc->bin_packed_bytes < nbins * 8)
+    return fail(VGI_E_STATE, "scene has no volume for a %u^3 grid with %u^3 bins: voxelize it once with vgi_voxelize",
+                n, 1u << bin_shift);
+  uint8_t* base = noalloc_scratch(ctx->device);
+  if (!base) return fail(VGI_E_STATE, "no scratch for device %d", ctx->device);
+
+  // ---- the call's scratch, carved from the array ----
+  const uint64_t pair_cap64 = 4ull * ntri + 65536;
+  const uint64_t work_cap64 = nbins + pair_cap64 / std::max(1u, tn.chunk_tris) + 1;
+  const size_t stats_off = align_up(3 * nbins * 4, 256);
+  const size_t ovf_off = align_up(stats_off + sizeof(DevStats), 256);
+  const size_t zeroed_bytes = ovf_off + 256;
+  size_t scan_need = 0;
+  HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(nbins + 1), pick(ctx, stream)));
+  Carve cv;
+  const size_t o_desc = cv.take(sizeof(NoAllocDesc));
+  const size_t o_hist = cv.take(4 * sizeof(unsigned long long));
+  const size_t o_zero = cv.take(zeroed_bytes);
+  const size_t o_tris = cv.take(std::max<uint64_t>(ntri, 1) * sizeof(TriRec));
+  const size_t o_slot = cv.take(std::max<uint64_t>(ntri, 1) * kSmallBins * 4);
+  const size_t o_big = cv.take(std::max<uint64_t>(ntri, 1) * 4);
+  const size_t o_zlist = cv.take(nbins * 4);
+  const size_t o_packed = cv.take((nbins + 1) * 8);
+  const size_t o_scan = cv.take((nbins + 1) * 8);
+  const size_t o_stmp = cv.take(std::max<size_t>(scan_need, 1));
+  const size_t o_pairs = cv.take(pair_cap64 * 4);
+  const size_t o_work = cv.take(work_cap64 * sizeof(uint2));
+  if (cv.off > (size_t)VGI_NOALLOC_SCRATCH_BYTES)
+    return fail(VGI_E_RANGE,
+                "voxelize of %llu triangles on a %u^3 grid needs %zu bytes of scratch; the allocation-free limit is %u",
+                (unsigned long long)ntri, n, cv.off, (unsigned)VGI_NOALLOC_SCRATCH_BYTES);
+
+  HIP_TRY(hipSetDevice(ctx->device));
+  const hipStream_t st = pick(ctx, stream);
+  sc->has_voxels = false;
+  sc->has_mips = false;
+
+  // ---- descriptors, by value into the scratch ----
+  NoAllocDesc d;
+  std::memset(&d, 0, sizeof(d));
+  NoAllocDesc* dd = (NoAllocDesc*)(base + o_desc);
+  uint32_t tbase = 0;
+  for (uint32_t k = 0; k < num_meshes; k++) {
+    const vgi_mesh& m = meshes[k];
+    MeshDev& md = d.meshes[k];
+    md.pos = m.positions;
+    md.idx = m.indices;
+    md.albedo = m.albedo_rgb8;
+    md.stride = m.position_stride ? m.position_stride : 12u;
+    md.nverts = m.num_vertices;
+    md.idx_type = m.index_type;
+    md.ntri = m.num_triangles;
+    md.tri_base = tbase;
+    md.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) | ((uint32_t)m.base_color[2] << 16);
+    std::memcpy(md.m, m.model, sizeof(md.m));
+    tbase += m.num_triangles;
+  }
+  uint32_t* zb = (uint32_t*)(base + o_zero);
+  VoxScene& S = d.scene;
+  S.meshes = dd->meshes;
+  S.tris = (TriRec*)(base + o_tris);
+  S.tri_slot = (uint32_t*)(base + o_slot);
+  S.big_list = (uint32_t*)(base + o_big);
+  S.bin_small = zb;
+  S.bin_big = zb + nbins;
+  S.big_cursor = zb + 2 * nbins;
+  S.zero_list = (uint32_t*)(base + o_zlist);
+  S.bin_packed = (unsigned long long*)sc->bin_packed;
+  S.acc = sc->acc;
+  S.mask = sc->mask;
+  S.stats = (DevStats*)(base + o_zero + stats_off);
+  S.result = sc->result;
+  for (int a = 0; a < 3; a++) S.origin[a] = grid->origin[a];
+  S.snap_scale = (float)(n * 256u) / grid->extent;
+  S.nmesh = num_meshes;
+  S.ntri = (uint32_t)ntri;
+  S.n = n;
+  S.bin_shift = bin_shift;
+  S.nbins_axis = nba;
+  S.chunk = tn.chunk_tris;
+  S.bin_base = 0;
+  const uint32_t ts = std::min(16u, 1u << bin_shift);
+  S.unit_items = (ts == 8 && bin_shift == 3) ? kUnitItems : 1u;
+  HIP_TRY(launch_stage_desc(d, dd, st));
+
+  VoxBatch B{};
+  B.scenes = &dd->scene;
+  B.tri_blocks = nullptr;
+  B.bin_blocks = nullptr;
+  B.packed_all = (unsigned long long*)(base + o_packed);
+  B.scan_all = (unsigned long long*)(base + o_scan);
+  B.work_map = (uint2*)(base + o_work);
+  B.pairs_all = (uint32_t*)(base + o_pairs);
+  B.pair_cap = (uint32_t)pair_cap64;
+  B.work_cap = (uint32_t)work_cap64;
+  B.overflow = (uint32_t*)(base + o_zero + ovf_off);
+  B.host_hist = (unsigned long long*)(base + o_hist);  // device memory: nothing reads it back
+  B.seq = sc->vox_seq + 1u ? sc->vox_seq + 1u : 1u;
+  B.nscenes = 1;
+  B.total_bins = (uint32_t)nbins;
+  const uint32_t grid_blocks = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, work_cap64);
+  TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_bytes, base + o_stmp, scan_need, ceil_div(ntri, kBlockTris),
+                              ceil_div(nbins, kBlockBins), ts, grid_blocks, &sc, 1, st, false));
+  sc->grid = *grid;
+  sc->bin_shift = bin_shift;
+  sc->nbins_axis = nba;
+  sc->chunk = tn.chunk_tris;
+  sc->tris_in = ntri;
+  sc->has_voxels = true;
+  return VGI_OK;
 }
 
 int vgi_voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t count, void* stream) {

@@ -100,6 +100,13 @@ struct VoxScene {
                                    // kUnitItems when the bin is one 8^3 tile
 };
 
+// Descriptors of an allocation-free voxelize call (one scene), passed by value to
+// k_stage_desc, which writes them into the device scratch the other kernels read.
+struct NoAllocDesc {
+  VoxScene scene;
+  MeshDev meshes[VGI_NOALLOC_MAX_MESHES];
+};
+
 struct VoxBatch {
   const VoxScene* scenes;
   const uint2* tri_blocks;          // (scene, first triangle) per setup/emit workgroup
@@ -325,6 +332,7 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
                      uint32_t count, hipStream_t s);
 hipError_t launch_probe(uint32_t kind, void* buf, uint32_t blocks, uint32_t iters, hipStream_t s);
 hipError_t launch_settle(const VoxBatch& B, hipStream_t s);
+hipError_t launch_stage_desc(const NoAllocDesc& d, NoAllocDesc* dst, hipStream_t s);
 hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,
                           uint32_t zero_blocks, hipStream_t s);
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, hipStream_t s);

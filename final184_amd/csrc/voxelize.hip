@@ -78,6 +78,20 @@ __global__ void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, s
   for (; i < count; i += stride) dst[i] = src[i];
 }
 
+// Copies an allocation-free call's descriptors from the kernel argument into the device
+// scratch (one workgroup, one word per lane in turn).
+__global__ __launch_bounds__(256) void k_stage_desc(NoAllocDesc d, NoAllocDesc* __restrict__ dst) {
+  static_assert(sizeof(NoAllocDesc) % 4 == 0, "descriptors are copied as words");
+  const uint32_t* src = (const uint32_t*)&d;
+  uint32_t* out = (uint32_t*)dst;
+  for (uint32_t i = threadIdx.x; i < sizeof(NoAllocDesc) / 4; i += blockDim.x) gst(out + i, src[i]);
+}
+
+hipError_t launch_stage_desc(const NoAllocDesc& d, NoAllocDesc* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_stage_desc, dim3(1), dim3(256), 0, s, d, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
   const size_t count = bytes / 16;
   if (count == 0) return hipSuccess;

@@ -245,6 +245,7 @@ SIGNATURES = [
     ("vgi_stream_destroy", C.c_int, [_P, _P]),
     ("vgi_stream_wait", C.c_int, [_P, _P, _P]),
     ("vgi_stream_query", C.c_int, [_P, _P]),
+    ("vgi_voxelize_noalloc", C.c_int, [_P, _P, _P, C.c_uint32, _P, _P]),
     ("vgi_grid_from_voxel_camera", C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     ("vgi_probe", C.c_int, [_P, C.c_uint32, _P, C.c_size_t, C.c_uint32, C.c_uint32, _P]),
     ("vgi_workspace_bytes", C.c_int, [_P, C.POINTER(C.c_uint64)]),
@@ -440,6 +441,15 @@ class Scene:
         for i, m in enumerate(meshes):
             arr[i] = m.desc if isinstance(m, MeshData) else m
         _check(lib.vgi_voxelize(self.ctx.handle, self.handle, arr, len(meshes), C.byref(grid), stream))
+        self.grid = grid
+
+    def voxelize_noalloc(self, meshes, grid: Grid, stream=None) -> None:
+        """vgi_voxelize_noalloc: the same pass with every per-call buffer in the library's
+        static per-GPU scratch (the scene must have been voxelized once at this grid size)."""
+        arr = (Mesh * max(1, len(meshes)))()
+        for i, m in enumerate(meshes):
+            arr[i] = m.desc if isinstance(m, MeshData) else m
+        _check(lib.vgi_voxelize_noalloc(self.ctx.handle, self.handle, arr, len(meshes), C.byref(grid), stream))
         self.grid = grid
 
     def build_mips(self, lights: Lights, precision: int = PREC_FP32, stream=None) -> None:

@@ -255,6 +255,27 @@ uint64_t vgi_scene_bytes_estimate(const vgi_grid* grid, uint32_t precision, uint
 int vgi_voxelize(vgi_context* ctx, vgi_scene* scene, const vgi_mesh* meshes, uint32_t num_meshes,
                  const vgi_grid* grid, void* stream);
 
+/* Voxelize without allocating. No counterpart in the reference: for callers that cannot
+ * allocate device memory while they run (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and the same result as
+ * vgi_voxelize for one scene, but every per-call buffer (descriptor table, triangle
+ * records, bin counters and scan, pair list, work map) is carved from one static device
+ * array of VGI_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from vgi_create
+ * until the process ends, and the descriptors reach it through a kernel argument instead
+ * of a host copy. No hipMalloc, hipFree, pinned allocation, host copy or memset is
+ * issued. Limits: at most VGI_NOALLOC_MAX_MESHES meshes; the call's scratch (about 100
+ * bytes per triangle, 16 per (triangle, bin) pair planned as 4 per triangle + 64 Ki, 44
+ * per bin) must fit the array (a scene of 1,048,576 triangles on a 512^3 grid with 8^3
+ * bins does), else VGI_E_RANGE; the scene's resident volume must already exist for this
+ * grid size (one earlier vgi_voxelize with the same n and bin size), else VGI_E_STATE.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a GPU (on two
+ * streams, or from two threads, without ordering between them) race and corrupt each
+ * other; order them on one stream. */
+#define VGI_NOALLOC_SCRATCH_BYTES (144u << 20)
+#define VGI_NOALLOC_MAX_MESHES 16u
+int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* scene, const vgi_mesh* meshes, uint32_t num_meshes,
+                         const vgi_grid* grid, void* stream);
+
 /* Inject direct light into level 0 and build the 2x2x2 box mip chain in the given
  * precision. No reference code (planned in docs/index.md:36). */
 int vgi_build_mips(vgi_context* ctx, vgi_scene* scene, const vgi_lights* lights,
