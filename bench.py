@@ -250,39 +250,45 @@ def run_screen(ctx, vgi, sg, w, mesh, gb_rough, steps):
 
 
 def trace_roofline(cn, probes, r, px):
-    """Roofline of one cone-march launch. It is bound by VALU issue: `achieved` is its
-    wave64 VALU instruction rate (SQ_INSTS_VALU per launch / its HIP-event time), `peak` the
-    rate the FMA probe reaches on this chip in this process (its SQ_INSTS_VALU / its time).
-    Beside it: HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated) against 8 TB/s, the
-    effective rate of the algorithmic bytes (60 B per pixel + 8 texels per trilinear gather;
-    served mostly from L1/L2, so not an HBM figure), and the gathers' FP32 FMAs (64 flops a
-    gather) against the 157.3 TFLOP/s vector peak."""
+    """Roofline of one cone-march launch, bound by VALU work. `achieved` is the gathers'
+    algorithmic FP32 arithmetic: 64 flops per trilinear gather (8 texels x 4 channels, one
+    multiply-add each; the kernel counts its gathers) per second of the launch's mean
+    HIP-event time, against MI355X's published 157.3 TFLOP/s FP32 vector peak (a figure
+    that counts packed FP32, which this code does not issue). `valu_issue` is what the
+    launch actually issues: SQ_INSTS_VALU per launch and per gather, its rate and rocprof's
+    VALUBusy, beside the FMA probe's rate. A wave64 FP32 FMA issues about once per CU per
+    clock and 32-bit integer VALU instructions about twice as often (a probe sweep), so an
+    instruction rate is not bounded by the FP32 probe's and is not used as the fraction.
+    Beside it: HBM traffic (FETCH_SIZE + WRITE_SIZE, calibrated) against 8 TB/s and the
+    effective rate of the algorithmic bytes (60 B per pixel + 8 texels per gather; served
+    mostly from L1/L2, so above HBM peak and not a roofline fraction)."""
     prec = r["prec"]
     texel = {"fp32": 16, "fp16": 8, "bf16": 8}[prec]
     t_s = r["trace_ms"] * 1e-3
     k = (cn or {}).get(f"trace_{prec}", {})
-    alg = px * (24 + 4 + 32) + r["trilinear"] * 8 * texel
-    d = {"bound": "valu", "achieved": None, "peak": None, "unit": "G wave64 VALU instr/s", "frac": None,
-         "traffic": k.get("hbm_bytes"), "kernel": f"k_trace<{prec}>", "time_ms": r["trace_ms"],
-         "trilinear_gathers": r["trilinear"], "l2_hit_rate": k.get("l2_hit_rate")}
+    tf = r["trilinear"] * 64 / t_s / 1e12
+    d = {"bound": "valu", "achieved": tf, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+         "frac": tf / FP32_VECTOR_PEAK_TFLOPS, "traffic": k.get("hbm_bytes"), "kernel": f"k_trace<{prec}>",
+         "time_ms": r["trace_ms"], "trilinear_gathers": r["trilinear"], "flops_per_gather": 64,
+         "l2_hit_rate": k.get("l2_hit_rate")}
     valu = k.get("SQ_INSTS_VALU")
-    fma = (cn or {}).get("probe_fma", {}).get("SQ_INSTS_VALU")
-    if valu and fma and probes.get("fma"):
-        ach = valu / t_s / 1e9
-        peak = fma / (probes["fma"] * 1e-3) / 1e9
-        d.update(achieved=ach, peak=peak, frac=ach / peak, valu_instructions=valu,
-                 valu_lane_instructions_per_gather=valu * 64 / max(r["trilinear"], 1),
-                 issue_per_cu_per_clock=ach * 1e9 / CU_CLOCKS_PER_S,
-                 probe_issue_per_cu_per_clock=peak * 1e9 / CU_CLOCKS_PER_S,
-                 salu_instructions=k.get("SQ_INSTS_SALU"))
+    pr = (cn or {}).get("probe_fma", {})
+    if valu:
+        rate = valu / t_s / 1e9
+        d["valu_issue"] = {
+            "instructions": valu, "salu_instructions": k.get("SQ_INSTS_SALU"),
+            "lane_instructions_per_gather": valu * 64 / max(r["trilinear"], 1),
+            "rate": rate, "unit": "G wave64 VALU instr/s", "per_cu_per_2p4ghz_clock": rate * 1e9 / CU_CLOCKS_PER_S,
+            "valu_busy": (k["VALUBusy"] / 100.0) if "VALUBusy" in k else None,
+            "fma_probe_rate": (pr["SQ_INSTS_VALU"] / (probes["fma"] * 1e-3) / 1e9
+                               if pr.get("SQ_INSTS_VALU") and probes.get("fma") else None),
+            "fma_probe_valu_busy": (pr["VALUBusy"] / 100.0) if "VALUBusy" in pr else None}
     if d["traffic"]:
         gbs = d["traffic"] / t_s / 1e9
         d["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    alg = px * (24 + 4 + 32) + r["trilinear"] * 8 * texel
     d["effective"] = {"algorithmic_bytes": alg, "achieved": alg / t_s / 1e9, "unit": "GB/s",
                       "note": "algorithmic bytes / time; gathers hit L1/L2, so this exceeds HBM peak"}
-    tf = r["trilinear"] * 64 / t_s / 1e12
-    d["fma_flops"] = {"achieved": tf, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                      "frac": tf / FP32_VECTOR_PEAK_TFLOPS}
     return d
 
 
@@ -495,7 +501,7 @@ COUNTER_KERNELS = [
     ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
 ]
 COUNTER_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),
-                  ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "TCC_HIT_sum", "TCC_MISS_sum",
+                  ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "VALUBusy", "TCC_HIT_sum", "TCC_MISS_sum",
                    "TCP_TCC_ATOMIC_WITH_RET_REQ_sum", "TCP_TCC_ATOMIC_WITHOUT_RET_REQ_sum"))
 
 

@@ -168,9 +168,13 @@ struct vgi_workspace {
   size_t sort_pix_bytes = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  unsigned long long* hist = nullptr;  // pinned: last call's batch pairs, units, call number
-  uint32_t seq = 0;                     // last voxelize call number on this stream (never 0)
-  uint64_t hist_pairs = 0, hist_units = 0;  // largest batch totals seen on this stream
+  // pinned: the last call's batch pairs, work units, large triangles and split bins,
+  // readable once the event `settled` (recorded behind the call's k_settle) has completed
+  unsigned long long* hist = nullptr;
+  hipEvent_t settled = nullptr;
+  bool settled_pending = false;
+  uint32_t seq = 0;  // last voxelize call number on this stream (never 0)
+  uint64_t hist_pairs = 0, hist_units = 0, hist_big = 0, hist_split = 0;  // largest seen
 
   size_t device_bytes() const {
     size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
@@ -246,6 +250,7 @@ void free_workspace(vgi_workspace* w) {
     if (s.done) (void)hipEventDestroy(s.done);
   }
   if (w->hist) (void)hipHostFree(w->hist);
+  if (w->settled) (void)hipEventDestroy(w->settled);
 }
 
 hipStream_t pick(vgi_context* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
@@ -456,7 +461,8 @@ struct VoxPlan {
 // one work unit per workgroup up to grid_blocks and loops over the rest.
 int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, size_t zeroed_bytes, void* scan_tmp,
                             size_t scan_tmp_bytes, size_t nblk_t, size_t nblk_b, uint32_t ts, uint32_t grid_blocks,
-                            vgi_scene* const* scenes, uint32_t count, hipStream_t st, bool prof) {
+                            uint32_t big_blocks, uint32_t zero_blocks, vgi_scene* const* scenes, uint32_t count,
+                            hipEvent_t settled, hipStream_t st, bool prof) {
   const uint32_t cus = (uint32_t)ctx->num_cus;
   const uint32_t persistent = cus * 4;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
@@ -473,9 +479,9 @@ int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, s
     scenes[i]->vox_seq = B.seq;
     HIP_TRY(hipEventRecord(scenes[i]->vox_done, st));
   }
+  if (settled) HIP_TRY(hipEventRecord(settled, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
-  const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(cus / 4, B.total_bins);
-  HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, cus, zero_blocks, st));
+  HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
   HIP_TRY(launch_voxelize(B, ts, grid_blocks, st));
   HIP_TRY(launch_voxelize_fallback(B, cus, st));
@@ -539,9 +545,19 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
 
   // ---- history: the largest batch totals of earlier calls on this stream that have
   // completed (read without waiting; a call still in flight is read by a later one) ----
-  if (w->hist && w->seq != 0 && __atomic_load_n(w->hist + 2, __ATOMIC_ACQUIRE) == (unsigned long long)w->seq) {
-    w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
-    w->hist_units = std::max<uint64_t>(w->hist_units, w->hist[1]);
+  if (w->settled_pending) {
+    const hipError_t q = hipEventQuery(w->settled);
+    if (q == hipSuccess) {
+      w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
+      w->hist_units = std::max<uint64_t>(w->hist_units, w->hist[1]);
+      w->hist_big = std::max<uint64_t>(w->hist_big, w->hist[2]);
+      w->hist_split = std::max<uint64_t>(w->hist_split, w->hist[3]);
+      w->settled_pending = false;
+    } else if (q != hipErrorNotReady) {
+      HIP_TRY(q);
+    } else {
+      (void)hipGetLastError();
+    }
   }
 
   // ---- scratch sized for this batch (grown; queued work on this stream is drained
@@ -582,7 +598,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
   if (!w->hist) {
     HIP_TRY(hipHostMalloc((void**)&w->hist, 4 * sizeof(unsigned long long), hipHostMallocCoherent));
-    std::memset(w->hist, 0, 4 * sizeof(unsigned long long));  // call number 0 is never used
+    std::memset(w->hist, 0, 4 * sizeof(unsigned long long));
+    HIP_TRY(hipEventCreateWithFlags(&w->settled, hipEventDisableTiming));
   }
   if (++w->seq == 0u) w->seq = 1u;
 
@@ -673,10 +690,20 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
 
   std::vector<vgi_scene*> scs(count);
   for (uint32_t i = 0; i < count; i++) scs[i] = plan[i].s;
-  const uint64_t grid_want = w->hist_units ? w->hist_units + w->hist_units / 8 + 64 : (uint64_t)ctx->num_cus * 8;
+  // launch sizes that only the device knows, from the largest this stream has seen:
+  // voxelize work units (one per workgroup), large triangles (4 per workgroup, one per
+  // wavefront) and split bins (one per workgroup); kernels loop over any excess
+  const uint64_t cus = (uint64_t)ctx->num_cus;
+  const uint64_t grid_want = w->hist_units ? w->hist_units + w->hist_units / 8 + 64 : cus * 8;
   const uint32_t grid_blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid_want, work_cap));
+  const uint32_t big_blocks =
+      (uint32_t)std::min<uint64_t>(cus * 4, w->hist_big ? (w->hist_big + w->hist_big / 4 + 3) / 4 + 1 : cus);
+  const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(
+      std::min<uint64_t>(cus * 4, total_bins), w->hist_split ? w->hist_split + w->hist_split / 4 + 1 : cus / 4);
   TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_need, w->scan_tmp, w->scan_tmp_bytes, nblk_t, nblk_b, ts,
-                              grid_blocks, scs.data(), count, st, ctx->profiling != 0));
+                              grid_blocks, big_blocks, zero_blocks, scs.data(), count, w->settled, st,
+                              ctx->profiling != 0));
+  w->settled_pending = true;
   for (uint32_t i = 0; i < count; i++) {
     vgi_scene* s = plan[i].s;
     s->grid = items[i].grid;
@@ -1344,9 +1371,8 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   if (s->vox_seq != 0) {  // the voxelize call's own statistics, once the device stored them
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(hipEventSynchronize(s->vox_done));
-    const SceneVoxResult* r = s->result;
-    if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq)
-      return fail(VGI_E_HIP, "voxelize statistics of the scene's last call are missing");
+    const SceneVoxResult* r = s->result;  // written by k_settle before the event
+    if (r->seq != s->vox_seq) return fail(VGI_E_HIP, "voxelize statistics of the scene's last call are missing");
     out->triangles_rejected = r->st.rejected;
     out->triangles_degenerate = r->st.degenerate;
     out->bin_pairs = r->st.pairs;
@@ -1487,8 +1513,10 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.nscenes = 1;
   B.total_bins = (uint32_t)nbins;
   const uint32_t grid_blocks = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, work_cap64);
+  const uint32_t cus = (uint32_t)ctx->num_cus;
   TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_bytes, base + o_stmp, scan_need, ceil_div(ntri, kBlockTris),
-                              ceil_div(nbins, kBlockBins), ts, grid_blocks, &sc, 1, st, false));
+                              ceil_div(nbins, kBlockBins), ts, grid_blocks, cus,
+                              (uint32_t)std::min<uint64_t>(cus / 4, nbins), &sc, 1, nullptr, st, false));
   sc->grid = *grid;
   sc->bin_shift = bin_shift;
   sc->nbins_axis = nba;

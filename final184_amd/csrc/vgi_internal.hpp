@@ -64,7 +64,7 @@ struct SceneVoxResult {
   DevStats st;
   unsigned long long totals;  // work units << 32 | pairs of the scene's bins
   uint32_t overflow;          // the call's pair list did not fit: per-triangle path taken
-  uint32_t seq;               // call number, stored last
+  uint32_t seq;               // call number
   unsigned long long pad[2];
 };
 static_assert(sizeof(SceneVoxResult) == 64, "SceneVoxResult layout");
@@ -118,7 +118,7 @@ struct VoxBatch {
   uint32_t pair_cap;                // entries of pairs_all
   uint32_t work_cap;                // entries of work_map
   uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps
-  unsigned long long* host_hist;    // pinned: batch pairs, units, then the call number
+  unsigned long long* host_hist;    // pinned: batch pairs, units, large triangles, split bins
   uint32_t seq;                     // this call's number (never 0)
   uint32_t nscenes;
   uint32_t total_bins;

@@ -517,36 +517,51 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
 // bins) and statistics go to its pinned host record; the batch's exact pair count (a
 // 64-bit sum of the per-scene counts, which the packed 32-bit field cannot hold beyond
 // 2^32) and unit count are checked against the pair list and work map the host sized, and
-// the overflow word tells steps 5-7 which path to take. The batch totals and the call
-// number go to the stream's pinned history, from which the host sizes later calls.
+// the overflow word tells steps 5-7 which path to take. The batch totals (pairs, units,
+// large triangles, split bins) go to the stream's pinned history, from which the host
+// sizes later calls. Plain stores: the host reads the records only after an event
+// recorded behind this kernel has completed, which makes them visible; no cache
+// writeback is needed here.
 __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __restrict__ scenes) {
-  unsigned long long pairs = 0;
-  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) pairs += gld(&scenes[sc].stats->pairs);
+  unsigned long long pairs = 0, big = 0, split = 0;
+  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
+    pairs += gld(&scenes[sc].stats->pairs);
+    big += gld(&scenes[sc].stats->big_count);
+    split += gld(&scenes[sc].stats->zero_count);
+  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  for (int o = 32; o > 0; o >>= 1) {
+    pairs += __shfl_xor(pairs, o);
+    big += __shfl_xor(big, o);
+    split += __shfl_xor(split, o);
+  }
   const uint32_t units = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);
   // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is exact
   const uint32_t ovf = (pairs > B.pair_cap || units > B.work_cap) ? 1u : 0u;
   for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
     const VoxScene& P = scenes[sc];
     const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
-    SceneVoxResult* r = P.result;
-    r->st.rejected = gld(&P.stats->rejected);
-    r->st.degenerate = gld(&P.stats->degenerate);
-    r->st.pairs = gld(&P.stats->pairs);
-    r->st.big_count = gld(&P.stats->big_count);
-    r->st.zero_count = gld(&P.stats->zero_count);
-    r->totals = gld(B.scan_all + P.bin_base + nbins) - gld(B.scan_all + P.bin_base);
-    r->overflow = ovf;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-    __hip_atomic_store(&r->seq, B.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    SceneVoxResult r;
+    r.st.rejected = gld(&P.stats->rejected);
+    r.st.degenerate = gld(&P.stats->degenerate);
+    r.st.pairs = gld(&P.stats->pairs);
+    r.st.big_count = gld(&P.stats->big_count);
+    r.st.zero_count = gld(&P.stats->zero_count);
+    r.totals = gld(B.scan_all + P.bin_base + nbins) - gld(B.scan_all + P.bin_base);
+    r.overflow = ovf;
+    r.seq = B.seq;
+    r.pad[0] = r.pad[1] = 0;
+    const uint4* src = (const uint4*)&r;
+    uint4* dst = (uint4*)P.result;
+#pragma unroll
+    for (int q = 0; q < 4; q++) dst[q] = src[q];
   }
   if (threadIdx.x == 0) {
     gst(B.overflow, ovf);
     B.host_hist[0] = pairs;
     B.host_hist[1] = units;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-    __hip_atomic_store(B.host_hist + 2, (unsigned long long)B.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    B.host_hist[2] = big;
+    B.host_hist[3] = split;
   }
 }
 
@@ -1000,18 +1015,23 @@ hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_block
   return hipGetLastError();
 }
 
+
 // ---------------------------------------------------------------------------
 // 7 overflow path: per-triangle voxelize with global atomics
 // ---------------------------------------------------------------------------
 
 // Runs only when k_settle set the overflow word (k_scatter then cleared every bin with
-// pairs). Wavefront w of the launch takes triangles w, w + nwaves, ... of every scene; its
+// pairs). A kernel of its own: inlined into k_voxelize it would spill that kernel's
+// registers, called from it the call would cut its occupancy. Wavefront w of the launch
+// takes triangles w, w + nwaves, ... of every scene; its
 // lanes walk the triangle's voxel box, keep the voxels the exact closed-box test accepts
 // (the same test as the bin pass, at voxel size) and add the triangle's increments: R, G,
 // B and the count, then the three biased normal components, as four 64-bit atomics (exact
 // with fewer than 2^21 triangles per voxel: no 32-bit word's total reaches 2^32), and set
 // the voxel's occupancy bit. Integer sums do not depend on order, so the accumulator is
 // the same as the binned path's.
+// Not inlined into k_voxelize, whose register budget (6 workgroups per CU) it would
+// otherwise share.
 __global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const VoxScene* __restrict__ scenes) {
   if (gld(B.overflow) == 0u) return;
   const uint32_t lane = threadIdx.x & 63u;
