@@ -263,7 +263,7 @@ def trace_roofline(cn, probes, r, px):
     effective rate of the algorithmic bytes (60 B per pixel + 8 texels per gather; served
     mostly from L1/L2, so above HBM peak and not a roofline fraction)."""
     prec = r["prec"]
-    texel = {"fp32": 16, "fp16": 8, "bf16": 8}[prec]
+    texel = {"fp32": 16, "fp16": 8, "bf16": 8, "fp16x2": 8}[prec]  # data bytes per texel a gather reads
     t_s = r["trace_ms"] * 1e-3
     k = (cn or {}).get(f"trace_{prec}", {})
     tf = r["trilinear"] * 64 / t_s / 1e12
@@ -440,8 +440,8 @@ def run_probes(ctx, vgi, reps=1):
 
 def traffic_probe(args) -> int:
     """Run under rocprofv3 --pmc by measure_traffic: the calibration kernels (one 1 GiB
-    k_copy, the FMA probe, the two atomic probes), then per mip precision (FP16, then
-    FP32) one headline frame: voxelize, mips, and the trace three times."""
+    k_copy, the FMA probe, the two atomic probes), then per mip precision (FP16, FP32, FP16
+    x-pairs) one headline frame: voxelize, mips, and the trace three times."""
     from final184_amd import scenegen as sg
     from final184_amd import vgi
 
@@ -468,7 +468,7 @@ def traffic_probe(args) -> int:
     scene.voxelize([mesh], grid)  # sizes the stream's voxelize scratch
     ctx.sync()
     cones = vgi.make_cones(**w.cones)
-    for prec in (vgi.PREC_FP16, vgi.PREC_FP32):
+    for prec in (vgi.PREC_FP16, vgi.PREC_FP32, vgi.PREC_FP16X2):
         scene.voxelize([mesh], grid)
         scene.build_mips(vgi.make_lights(w.lights), prec)
         for _ in range(3):
@@ -493,9 +493,10 @@ def _counter_rows(outdir):
 # take the launches of that precision; the voxelize-front-end keys take the last frame
 # (FP32), whose front end is identical to the FP16 frame's.
 COUNTER_KERNELS = [
-    ("trace_fp16", r"k_trace<1\b"), ("trace_fp32", r"k_trace<0\b"),
+    ("trace_fp16", r"k_trace<1\b"), ("trace_fp32", r"k_trace<0\b"), ("trace_fp16x2", r"k_trace<3\b"),
     ("mips_fp16_first", r"k_mips<1, true>"), ("mips_fp16_rest", r"k_mips<1, false>"),
     ("mips_fp32_first", r"k_mips<0, true>"), ("mips_fp32_rest", r"k_mips<0, false>"),
+    ("mips_fp16x2_first", r"k_mips<3, true>"), ("mips_fp16x2_rest", r"k_mips<3, false>"),
     ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8>"),
     ("probe_fma", r"k_probe_fma"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
     ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
@@ -543,7 +544,7 @@ def measure_traffic(args):
         for key, rx in COUNTER_KERNELS:
             vals = [float(x["Counter_Value"]) for x in rows if re.search(rx, x["Kernel_Name"])]
             if key in ("setup", "voxelize"):
-                vals = vals[-1:]  # the FP32 frame's front end
+                vals = vals[-1:]  # the last frame's front end (the same in every frame)
             if vals:
                 out.setdefault(key, {})[counter] = float(np.mean(vals))
     calib = {}
@@ -560,7 +561,7 @@ def measure_traffic(args):
             v["l2_hit_rate"] = v["TCC_HIT_sum"] / max(v["TCC_HIT_sum"] + v["TCC_MISS_sum"], 1.0)
         if "FETCH_SIZE_bytes" in v and "WRITE_SIZE_bytes" in v:
             v["hbm_bytes"] = v["FETCH_SIZE_bytes"] + v["WRITE_SIZE_bytes"]
-    for prec in ("fp16", "fp32"):  # a mip build is its two launches
+    for prec in ("fp16", "fp32", "fp16x2"):  # a mip build is its two launches
         a, b = out.get(f"mips_{prec}_first", {}), out.get(f"mips_{prec}_rest", {})
         if "hbm_bytes" in a:
             out[f"mips_{prec}"] = {"hbm_bytes": a["hbm_bytes"] + b.get("hbm_bytes", 0.0),
@@ -595,7 +596,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="skip the two-stream overlapped-frames measurement")
-    ap.add_argument("--precisions", default="fp32,fp16")
+    ap.add_argument("--precisions", default="fp32,fp16,fp16x2")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched run")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 10k-triangle soup and the 1M-triangle 512^3 / 4K precision study")
@@ -673,7 +674,9 @@ def main() -> int:
             f"{r['trilinear'] / w.num_pixels:.1f}")
         results.append(r)
     by = {r["prec"]: r for r in results}
-    head = by.get("fp16", results[-1])
+    # headline: FP16 mips in the cone march's fastest layout (x-pairs), the compact FP16
+    # form beside it; the plain FP32 form is the baseline
+    head = by.get("fp16x2", by.get("fp16", results[-1]))
     base = by.get("fp32", results[0])
     pipe_ms = None
     if not args.no_overlap:
@@ -716,6 +719,8 @@ def main() -> int:
                       for r in results},
         "fp32_frame_mpixels_s": px / (base["frame_ms"] * 1e-3) / 1e6,
         "fp16_vs_fp32_frame_speedup": base["frame_ms"] / head["frame_ms"],
+        "fp16_vs_fp32_trace_speedup": base["trace_ms"] / head["trace_ms"],
+        "frame_mpixels_s": {r["prec"]: px / (r["frame_ms"] * 1e-3) / 1e6 for r in results},
         "overlapped_frames": None if pipe_ms is None else {
             "streams": 2, "frame_ms": pipe_ms, "mpixels_s": px / (pipe_ms * 1e-3) / 1e6,
             "note": "voxelize of frame k+1 on a second stream while frame k is traced"},
@@ -739,14 +744,15 @@ def main() -> int:
         # every pixel is checked against FP64
         ws = sg.random_soup()
         log(f"config {ws.name}")
-        soup = (ws, run_config(ctx, vgi, ws, ("fp32", "fp16", "bf16"), args.steps))
+        soup = (ws, run_config(ctx, vgi, ws, ("fp32", "fp16", "bf16", "fp16x2"), args.steps))
         # precision study: room at 1,048,576 triangles, 512^3 / 9 levels, 3840x2160, 16+1 cones;
         # a seeded 4096-pixel subset is checked against FP64
         t0 = time.perf_counter()
         wp = sg.room(ntri=1_048_576, n=512, levels=9, width=3840, height=2160, num_diffuse=16)
         log(f"config {wp.name} generated in {time.perf_counter() - t0:.1f}s")
         sub = np.random.default_rng(4096).choice(wp.num_pixels, 4096, replace=False)
-        study = (wp, sub, run_config(ctx, vgi, wp, ("fp32", "fp16", "bf16"), max(2, args.steps // 4), keep=sub))
+        study = (wp, sub, run_config(ctx, vgi, wp, ("fp32", "fp16", "bf16", "fp16x2"), max(2, args.steps // 4),
+                                     keep=sub))
         for name, res in (("soup", soup[1]), ("precision study", study[2])):
             log(f"{name}: " + ", ".join(f"{p} frame {r['frame_ms']:.3f} ms (trace {r['trace_ms']:.3f})"
                                         for p, r in res.items()))

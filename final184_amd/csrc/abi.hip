@@ -770,7 +770,7 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   if (!ctx) return fail(VGI_E_INVALID, "ctx is NULL");
   if (count == 0) return VGI_OK;
   if (!items) return fail(VGI_E_INVALID, "items is NULL");
-  if (precision > VGI_PREC_BF16) return fail(VGI_E_INVALID, "precision %u", precision);
+  if (precision > VGI_PREC_FP16X2) return fail(VGI_E_INVALID, "precision %u", precision);
   std::vector<const vgi_scene*> seen(count);
   for (uint32_t i = 0; i < count; i++) {
     const vgi_mips_item& it = items[i];
@@ -918,6 +918,9 @@ void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tunin
     P.level[k] = s->level[k];
     P.level_bytes[k] = s->level_bytes[k];
     P.nk[k] = g.n >> k;
+    P.lv[k].ptr = s->level[k];
+    P.lv[k].bytes32 = (uint32_t)std::min<uint64_t>(s->level_bytes[k], 0xFFFFFFFFull);
+    P.lv[k].p1 = (g.n >> k) + 2;
   }
   P.levels = g.levels;
   P.n = g.n;
@@ -1006,7 +1009,8 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   const vgi_tuning tn = tuning_of(ctx);
   const bool morton = tn.pixel_order == VGI_ORDER_MORTON;
   // one launch per storage precision
-  std::vector<uint32_t> group[3];
+  constexpr uint32_t kPrecisions = VGI_PREC_FP16X2 + 1;
+  std::vector<uint32_t> group[kPrecisions];
   for (uint32_t i = 0; i < count; i++) group[items[i].scene->prec].push_back(i);
   struct Launch {
     uint32_t prec, nscenes;
@@ -1024,7 +1028,7 @@ int trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t count, v
   std::vector<Launch> launches;
   Carve cv;
   size_t max_entries = 0;
-  for (uint32_t p = 0; p < 3; p++) {
+  for (uint32_t p = 0; p < kPrecisions; p++) {
     if (group[p].empty()) continue;
     Launch L{p, (uint32_t)group[p].size(), 0, 0, 0, 0, 0, 0, 0, tn.address_mode == VGI_ADDRESS_AUTO};
     for (uint32_t i : group[p]) {
@@ -1819,6 +1823,19 @@ int vgi_scene_read_level(vgi_context* ctx, const vgi_scene* s, uint32_t level, f
         float* o = host_out + ((size_t)x + (size_t)nk * ((size_t)y + (size_t)nk * z)) * 4;
         if (s->prec == VGI_PREC_FP32) {
           std::memcpy(o, raw.data() + pi * 16, 16);
+        } else if (s->prec == VGI_PREC_FP16X2) {
+          // texel from its entry's low halves; the entry before holds it in its high halves
+          uint16_t h[8], g[8];
+          std::memcpy(h, raw.data() + pi * 16, 16);
+          std::memcpy(g, raw.data() + (pi - 1) * 16, 16);
+          for (int c = 0; c < 4; c++) {
+            if (g[2 * c + 1] != h[2 * c])
+              return fail(VGI_E_STATE, "level %u texel (%u, %u, %u): x-pair copies differ", level, x, y, z);
+            o[c] = host_half(h[2 * c]);
+          }
+          if (x + 1 == nk)  // the last texel's entry pairs it with the zero border
+            for (int c = 0; c < 4; c++)
+              if (h[2 * c + 1] != 0) return fail(VGI_E_STATE, "level %u: border of row (%u, %u) not zero", level, y, z);
         } else {
           uint16_t h[4];
           std::memcpy(h, raw.data() + pi * 8, 8);

@@ -15,7 +15,8 @@
 // then z) and every level is rounded to its storage precision before it is reduced, so
 // the result is a pure function of the accumulator.
 //
-// Level k is stored padded: (n_k + 2)^3 texels with a zero border, x fastest. The levels
+// Level k is stored padded: (n_k + 2)^3 texels with a zero border, x fastest (FP16 x-pairs:
+// (n_k + 2)^3 entries of two texels, below). The levels
 // are zero-filled when allocated, and the first launch stores a 16^3 tile (its level-0
 // texels and its blocks of the four levels above) only while the tile holds occupied
 // voxels, plus once, as zeros, when it has lost them; a per-tile word records which. A
@@ -93,6 +94,50 @@ template <> struct Texel<VGI_PREC_BF16> {
     return make_float4(bf16_bits_to_f32(f32_to_bf16_bits(v.x)), bf16_bits_to_f32(f32_to_bf16_bits(v.y)),
                        bf16_bits_to_f32(f32_to_bf16_bits(v.z)), f16_bits_to_f32(f32_to_f16_bits(v.w)));
   }
+};
+
+// FP16 texels in x-pairs: entry i of a padded level holds texel i in its low halves and
+// texel i + 1 in its high halves, channel by channel (dwords R_i R_i+1, G.., B.., A..), so a
+// texel is written twice: as the low halves of its own entry and the high halves of the
+// entry before it (the border entries' other halves stay zero). Values and rounding are
+// VGI_PREC_FP16's; Packed (the LDS form) is the plain FP16 texel.
+template <> struct Texel<VGI_PREC_FP16X2> {
+  using Packed = uint2;
+  __device__ static Packed pack(float4 v) { return Texel<VGI_PREC_FP16>::pack(v); }
+  __device__ static float4 unpack(Packed u) { return Texel<VGI_PREC_FP16>::unpack(u); }
+  __device__ static float4 load(const void* base, size_t i) {
+    const uint4 e = gld4((const uint4*)base + i);
+    return unpack(make_uint2((e.x & 0xFFFFu) | (e.y << 16), (e.z & 0xFFFFu) | (e.w << 16)));
+  }
+  // the halves of texel u: (R, G, B, A) as 16-bit words
+  __device__ static void halves(Packed u, uint16_t (&h)[4]) {
+    h[0] = (uint16_t)(u.x & 0xFFFFu);
+    h[1] = (uint16_t)(u.x >> 16);
+    h[2] = (uint16_t)(u.y & 0xFFFFu);
+    h[3] = (uint16_t)(u.y >> 16);
+  }
+  // one whole entry: texel lo in the low halves, texel hi in the high halves
+  __device__ static void store_entry(void* base, size_t i, Packed lo, Packed hi) {
+    uint16_t a[4], b[4];
+    halves(lo, a);
+    halves(hi, b);
+    gst4((uint4*)base + i, make_uint4((uint32_t)a[0] | ((uint32_t)b[0] << 16), (uint32_t)a[1] | ((uint32_t)b[1] << 16),
+                                      (uint32_t)a[2] | ((uint32_t)b[2] << 16), (uint32_t)a[3] | ((uint32_t)b[3] << 16)));
+  }
+  // one half of an entry: the texel's halves at word offset `hi` (0: low, 1: high) of each dword
+  __device__ static void store_half(void* base, size_t i, Packed u, uint32_t hi) {
+    uint16_t h[4];
+    halves(u, h);
+    uint16_t* e = (uint16_t*)((uint4*)base + i);
+#pragma unroll
+    for (int c = 0; c < 4; c++) gst(e + 2 * c + hi, h[c]);
+  }
+  __device__ static void store_packed(void* base, size_t i, Packed u) {
+    store_half(base, i, u, 0u);      // its own entry
+    store_half(base, i - 1, u, 1u);  // the entry before it (i >= 1: padded indices)
+  }
+  __device__ static void store(void* base, size_t i, float4 v) { store_packed(base, i, pack(v)); }
+  __device__ static float4 round(float4 v) { return Texel<VGI_PREC_FP16>::round(v); }
 };
 
 __device__ __forceinline__ size_t padded_index(uint32_t x, uint32_t y, uint32_t z, uint32_t nk) {
@@ -271,9 +316,22 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       s_val[lx + (ly << ssh) + (lz << (2 * ssh))] = TX::pack(c);
     }
     __syncthreads();
-    for (uint32_t v = g; v < NV; v += 512) {
-      const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
-      TX::store_packed(P.level[0], padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns), s_val[v]);
+    if constexpr (PREC == VGI_PREC_FP16X2) {
+      // whole entries from the tile's own texels; the entry of the tile's last x gets its
+      // low halves here and its high halves from the next tile, the entry before the
+      // tile's first x its high halves from here
+      for (uint32_t v = g; v < NV; v += 512) {
+        const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
+        const size_t pi = padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns);
+        if (lx + 1u < S) TX::store_entry(P.level[0], pi, s_val[v], s_val[v + 1]);
+        else TX::store_half(P.level[0], pi, s_val[v], 0u);
+        if (lx == 0u) TX::store_half(P.level[0], pi - 1, s_val[v], 1u);
+      }
+    } else {
+      for (uint32_t v = g; v < NV; v += 512) {
+        const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
+        TX::store_packed(P.level[0], padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns), s_val[v]);
+      }
     }
     if (grp) {
       float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -348,6 +406,10 @@ hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t
     case VGI_PREC_BF16:
       if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
       else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+      break;
+    case VGI_PREC_FP16X2:
+      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16X2, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16X2, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
       break;
     default:
       return hipErrorInvalidValue;

@@ -277,7 +277,7 @@ int vgi_server_create(vgi_context* ctx, const vgi_server_config* cfg, vgi_server
   *out = nullptr;
   if (cfg->slots < 1 || cfg->slots > 16) return vgi::set_error(VGI_E_INVALID, "slots %u not in [1,16]", cfg->slots);
   if (cfg->max_batch_requests < 1) return vgi::set_error(VGI_E_INVALID, "max_batch_requests must be >= 1");
-  if (cfg->precision > VGI_PREC_BF16) return vgi::set_error(VGI_E_INVALID, "precision %u", cfg->precision);
+  if (cfg->precision > VGI_PREC_FP16X2) return vgi::set_error(VGI_E_INVALID, "precision %u", cfg->precision);
   if (cfg->memory_budget == 0) return vgi::set_error(VGI_E_INVALID, "memory_budget is 0");
   auto S = std::make_unique<vgi_server>();
   S->ctx = ctx;

@@ -72,7 +72,21 @@ template <> struct Acc<VGI_PREC_BF16> {  // RGB bf16, opacity fp16
   }
 };
 
-template <int PREC> constexpr uint32_t kTexelBytes = (PREC == VGI_PREC_FP32) ? 16u : 8u;
+// FP16 x-pairs: one dword holds a channel of texels x and x + 1, and one packed dot
+// product (FP16 products, FP32 sum) interpolates it along x with the weight pair
+// (w_x0 * w_row, w_x1 * w_row): four per row of the 2x2x2 footprint, where the other forms
+// need eight multiply-adds.
+struct AccPair {
+  __device__ static void add(float4& a, half2_t w, uint32_t rr, uint32_t gg, uint32_t bb, uint32_t aa) {
+    a.x = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, rr), w, a.x, false);
+    a.y = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, gg), w, a.y, false);
+    a.z = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, bb), w, a.z, false);
+    a.w = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, aa), w, a.w, false);
+  }
+};
+
+template <int PREC>
+constexpr uint32_t kTexelBytes = (PREC == VGI_PREC_FP32 || PREC == VGI_PREC_FP16X2) ? 16u : 8u;
 
 // Whether a zero-opacity texel of level l0 + 1 proves its whole subtree empty, so that the
 // fine gather at level l0 may be skipped. Occupied level-0 voxels have opacity 1 and a
@@ -95,6 +109,15 @@ struct Foot {
   float w00, w10, w01, w11;  // xy weights
   float wz0, wz1;            // z weights x level weight
 };
+
+// FP16 weight pairs of the four rows (y0 z0, y1 z0, y0 z1, y1 z1) of a footprint, as
+// packed half products: (ex, fx) * (wy * wz) with wy * wz rounded to FP16 once.
+struct PairW {
+  half2_t r[4];
+};
+template <bool UNIFORM_LEVEL>
+__device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, float ux, float uy, float uz, float wl,
+                                                    PairW& pw);
 
 // floor(x) as an integer in one instruction (the compiler emits floor, then convert); x is
 // finite and within the padded grid here (positions outside [0, n]^3 end the march first)
@@ -140,6 +163,26 @@ __device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, flo
   return f;
 }
 
+template <bool UNIFORM_LEVEL>
+__device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, float ux, float uy, float uz, float wl,
+                                                    PairW& pw) {
+  const float sc = UNIFORM_LEVEL ? __builtin_bit_cast(float, (127u - k) << 23) : __builtin_ldexpf(1.0f, -(int)k);
+  const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
+  const int ix = cvt_floor(gx), iy = cvt_floor(gy), iz = cvt_floor(gz);
+  const float fx = __builtin_amdgcn_fractf(gx), fy = __builtin_amdgcn_fractf(gy),
+              fz = __builtin_amdgcn_fractf(gz);
+  const float wz0 = (1.0f - fz) * wl, wz1 = fz * wl;
+  const half2_t xw = {(_Float16)(1.0f - fx), (_Float16)fx};
+  const half2_t r0 = {(_Float16)((1.0f - fy) * wz0), (_Float16)(fy * wz0)};  // rows y0 z0, y1 z0
+  const half2_t r1 = {(_Float16)((1.0f - fy) * wz1), (_Float16)(fy * wz1)};  // rows y0 z1, y1 z1
+  pw.r[0] = xw * (half2_t){r0.x, r0.x};
+  pw.r[1] = xw * (half2_t){r0.y, r0.y};
+  pw.r[2] = xw * (half2_t){r1.x, r1.x};
+  pw.r[3] = xw * (half2_t){r1.y, r1.y};
+  const uint32_t row = mad_i24<UNIFORM_LEVEL>(iz, p1, p1 + (uint32_t)(iy + 1));
+  return __umul24(p1, row) + (uint32_t)(ix + 1);
+}
+
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
 // FP32: when every active lane's footprint is the same 2x2x2 texel block (usual at coarse
 // levels, where a texel spans many pixels' cone positions), the block is read once with
@@ -154,48 +197,78 @@ template <int PREC>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
                                                float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
-  const uint32_t p1 = P.nk[k] + 2;
+  const LevelDesc L = P.lv[k];  // one scalar load: the level is wave-uniform
+  const uint32_t p1 = L.p1;
   const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
-  const Foot f = footprint<true>(k, p1, ux, uy, uz, wl);
-  const uint32_t off = f.idx * TB;  // levels on this path are < 2^32 bytes
-  const int soff[4] = {0, rowb, slab, rowb + slab};
-  const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
-  if constexpr (PREC == VGI_PREC_FP32) {
+  if constexpr (PREC == VGI_PREC_FP16X2) {
+    PairW pw;
+    const uint32_t off = footprint_pairs<true>(k, p1, ux, uy, uz, wl, pw) * TB;
+    const int soff[4] = {0, rowb, slab, rowb + slab};
+    // When every active lane's footprint is the same 2x2x2 block (usual at coarse levels),
+    // its four rows are read once with scalar loads and broadcast, sparing the
+    // texture-address path four 16-byte vector loads.
     const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
     if (__all(off == off0)) {
-      const char* base = (const char*)P.level[k] + off0;
+      const char* base = (const char*)L.ptr + off0;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const float wz = q < 2 ? f.wz0 : f.wz1;
-        const u32x8_t v = *(const VGI_CONSTANT u32x8_t*)(base + soff[q]);
-        const uint32_t a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
-        const uint32_t b0 = v[4], b1 = v[5], b2 = v[6], b3 = v[7];
-        Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
-        Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
-        abits |= a3 | b3;
+        const vgi_u32x4 e = *(const VGI_CONSTANT vgi_u32x4*)(base + soff[q]);
+        AccPair::add(acc, pw.r[q], e[0], e[1], e[2], e[3]);
+        abits |= e[3];
       }
       return;
     }
-  }
-  const __amdgpu_buffer_rsrc_t r =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(P.level[k]), 0, (int)(uint32_t)P.level_bytes[k], 0x00020000);  // unsigned field
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(L.ptr), 0, (int)L.bytes32, 0x00020000);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const float wz = q < 2 ? f.wz0 : f.wz1;
+    for (int q = 0; q < 4; q++) {
+      const auto e = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
+      const uint32_t rr = e[0], gg = e[1], bb = e[2], aa = e[3];
+      AccPair::add(acc, pw.r[q], rr, gg, bb, aa);
+      abits |= aa;
+    }
+  } else {
+    const Foot f = footprint<true>(k, p1, ux, uy, uz, wl);
+    const uint32_t off = f.idx * TB;  // levels on this path are < 2^32 bytes
+    const int soff[4] = {0, rowb, slab, rowb + slab};
+    const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
     if constexpr (PREC == VGI_PREC_FP32) {
-      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
-      const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, soff[q], 0);
-      const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
-      const uint32_t b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
-      Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
-      Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
-      abits |= a3 | b3;
-    } else {
-      const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
-      const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
-      Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
-      Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
-      abits |= (a1 | a3) & 0xFFFF0000u;  // opacity is the high half
+      const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
+      if (__all(off == off0)) {
+        const char* base = (const char*)L.ptr + off0;
+  #pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const float wz = q < 2 ? f.wz0 : f.wz1;
+          const u32x8_t v = *(const VGI_CONSTANT u32x8_t*)(base + soff[q]);
+          const uint32_t a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+          const uint32_t b0 = v[4], b1 = v[5], b2 = v[6], b3 = v[7];
+          Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
+          Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
+          abits |= a3 | b3;
+        }
+        return;
+      }
+    }
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(L.ptr), 0, (int)L.bytes32, 0x00020000);  // unsigned field
+  #pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float wz = q < 2 ? f.wz0 : f.wz1;
+      if constexpr (PREC == VGI_PREC_FP32) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, soff[q], 0);
+        const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        const uint32_t b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a0, a1, a2, a3);
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b0, b1, b2, b3);
+        abits |= a3 | b3;
+      } else {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
+        const uint32_t a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
+        Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
+        abits |= (a1 | a3) & 0xFFFF0000u;  // opacity is the high half
+      }
     }
   }
 }
@@ -207,24 +280,36 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
-  const Foot f = footprint<false>(k, p1, ux, uy, uz, wl);
-  const char* base = (const char*)P.level[k] + (size_t)f.idx * TB;
-  const size_t offs[4] = {0, rowb, slab, rowb + slab};
-  const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+  if constexpr (PREC == VGI_PREC_FP16X2) {
+    PairW pw;
+    const char* base = (const char*)P.level[k] + (size_t)footprint_pairs<false>(k, p1, ux, uy, uz, wl, pw) * TB;
+    const size_t offs[4] = {0, rowb, slab, rowb + slab};
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const float wz = q < 2 ? f.wz0 : f.wz1;
-    if constexpr (PREC == VGI_PREC_FP32) {
-      const uint4 a = gld4(base + offs[q]);
-      const uint4 b = gld4(base + offs[q] + 16);
-      Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a.x, a.y, a.z, a.w);
-      Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
-      abits |= a.w | b.w;
-    } else {
-      const uint4 a = gld4(base + offs[q]);  // texels x and x+1 (8-byte aligned)
-      Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
-      Acc<PREC>::add(acc, wxy[q][1] * wz, a.z, a.w);
-      abits |= (a.y | a.w) & 0xFFFF0000u;
+    for (int q = 0; q < 4; q++) {
+      const uint4 e = gld4(base + offs[q]);
+      AccPair::add(acc, pw.r[q], e.x, e.y, e.z, e.w);
+      abits |= e.w;
+    }
+  } else {
+    const Foot f = footprint<false>(k, p1, ux, uy, uz, wl);
+    const char* base = (const char*)P.level[k] + (size_t)f.idx * TB;
+    const size_t offs[4] = {0, rowb, slab, rowb + slab};
+    const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+  #pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float wz = q < 2 ? f.wz0 : f.wz1;
+      if constexpr (PREC == VGI_PREC_FP32) {
+        const uint4 a = gld4(base + offs[q]);
+        const uint4 b = gld4(base + offs[q] + 16);
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][0] * wz, a.x, a.y, a.z, a.w);
+        Acc<VGI_PREC_FP32>::add(acc, wxy[q][1] * wz, b.x, b.y, b.z, b.w);
+        abits |= a.w | b.w;
+      } else {
+        const uint4 a = gld4(base + offs[q]);  // texels x and x+1 (8-byte aligned)
+        Acc<PREC>::add(acc, wxy[q][0] * wz, a.x, a.y);
+        Acc<PREC>::add(acc, wxy[q][1] * wz, a.z, a.w);
+        abits |= (a.y | a.w) & 0xFFFF0000u;
+      }
     }
   }
 }
@@ -482,6 +567,10 @@ hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s) {
     case VGI_PREC_BF16:
       if (sorted) launch_trace_prec<VGI_PREC_BF16, true>(B, buffer_path, grid, block, s);
       else launch_trace_prec<VGI_PREC_BF16, false>(B, buffer_path, grid, block, s);
+      break;
+    case VGI_PREC_FP16X2:
+      if (sorted) launch_trace_prec<VGI_PREC_FP16X2, true>(B, buffer_path, grid, block, s);
+      else launch_trace_prec<VGI_PREC_FP16X2, false>(B, buffer_path, grid, block, s);
       break;
     default:
       return hipErrorInvalidValue;

@@ -173,7 +173,18 @@ struct SchedEntry {
 constexpr uint32_t kSchedTop = 0x80000000u;
 static_assert(sizeof(SchedEntry) == 16, "schedule entries are read as 16-byte scalar loads");
 
+// One mip level as the cone march addresses it, read with one 16-byte scalar load: base
+// pointer, bytes (as a buffer record size, levels under 4 GiB) and the padded row pitch
+// in texels.
+struct __align__(16) LevelDesc {
+  const void* ptr;
+  uint32_t bytes32;
+  uint32_t p1;  // n_k + 2
+};
+static_assert(sizeof(LevelDesc) == 16, "LevelDesc is one 16-byte load");
+
 struct TraceParams {
+  LevelDesc lv[VGI_MAX_LEVELS];
   const void* level[VGI_MAX_LEVELS];
   uint64_t level_bytes[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
@@ -219,7 +230,7 @@ constexpr uint32_t kNoPixel = 0xFFFFFFFFu;
 constexpr uint32_t kSortBlock = 256;  // sorted pixels per workgroup
 
 // Texel bytes per precision.
-inline uint32_t texel_bytes(uint32_t prec) { return prec == VGI_PREC_FP32 ? 16u : 8u; }
+inline uint32_t texel_bytes(uint32_t prec) { return (prec == VGI_PREC_FP32 || prec == VGI_PREC_FP16X2) ? 16u : 8u; }
 
 // ---- global-memory access ----
 // Kernels take their descriptor tables as __restrict__ arguments, so the compiler may

@@ -20,8 +20,8 @@ ERRORS = {
     -1: "VGI_E_INVALID", -2: "VGI_E_NOMEM", -3: "VGI_E_HIP", -4: "VGI_E_BUDGET",
     -5: "VGI_E_NOT_READY", -6: "VGI_E_RANGE", -7: "VGI_E_NODEVICE", -8: "VGI_E_STATE",
 }
-PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
-PREC_NAMES = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16}
+PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP16X2 = 0, 1, 2, 3
+PREC_NAMES = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp16x2": PREC_FP16X2}
 INDEX_NONE, INDEX_U16, INDEX_U32 = 0, 1, 2
 LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
 MAX_LEVELS, MAX_LIGHTS, MAX_DIFFUSE_CONES = 11, 16, 32

@@ -65,9 +65,13 @@ typedef enum vgi_status {
 } vgi_status;
 
 typedef enum vgi_precision {
-  VGI_PREC_FP32 = 0, /* radiance RGBA fp32, 16 B/texel */
-  VGI_PREC_FP16 = 1, /* radiance RGB fp16 + opacity fp16, 8 B/texel */
-  VGI_PREC_BF16 = 2  /* radiance RGB bf16 + opacity fp16, 8 B/texel */
+  VGI_PREC_FP32 = 0,  /* radiance RGBA fp32, 16 B/texel */
+  VGI_PREC_FP16 = 1,  /* radiance RGB fp16 + opacity fp16, 8 B/texel */
+  VGI_PREC_BF16 = 2,  /* radiance RGB bf16 + opacity fp16, 8 B/texel */
+  VGI_PREC_FP16X2 = 3 /* the VGI_PREC_FP16 texels (same values), each stored beside its +x
+                         neighbour as channel pairs (R_x R_x+1, G_x G_x+1, B.., A..: 16 B/texel)
+                         so that the cone march interpolates along x with packed dot
+                         products (FP16 products, FP32 accumulation) */
 } vgi_precision;
 
 typedef enum vgi_index_type {

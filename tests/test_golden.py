@@ -41,8 +41,9 @@ def test_oracle_reproduces_fixture(golden, soup):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16", "fp16x2"])
 def test_gpu_matches_fixture(ctx, golden, soup, prec):
+    """fp16x2 stores the fp16 texels in x-pairs: its levels read back as the fp16 hashes."""
     from final184_amd import vgi
     print(f"case golden soup {prec}", flush=True)
     mesh = vgi.MeshData(ctx, soup.positions.reshape(-1, 3), albedo=soup.albedo)
@@ -51,7 +52,8 @@ def test_gpu_matches_fixture(ctx, golden, soup, prec):
     assert sha(scene.read_voxels()) == str(golden["acc_sha256"])
     scene.build_mips(vgi.make_lights(soup.lights), vgi.PREC_NAMES[prec])
     for k in range(soup.grid["levels"]):
-        assert sha(scene.read_level(k)) == str(golden[f"{prec}_level_sha256"][k]), f"level {k}"
+        key = "fp16" if prec == "fp16x2" else prec
+        assert sha(scene.read_level(k)) == str(golden[f"{key}_level_sha256"][k]), f"level {k}"
     gb = vgi.GBufferData(ctx, soup.gb_pos, soup.gb_nrm, soup.width, soup.height)
     od = vgi.DeviceArray.empty(ctx, (soup.num_pixels, 4), np.float32)
     osp = vgi.DeviceArray.empty(ctx, (soup.num_pixels, 4), np.float32)

@@ -93,7 +93,7 @@ def test_batch_equals_alone(ctx, bin_size, tile, order):
     print(f"case batch_equals_alone bin={bin_size} tile={tile} order={order}", flush=True)
     try:
         ws = _workloads()
-        precs = [vgi.PREC_FP16, vgi.PREC_FP32, vgi.PREC_FP16, vgi.PREC_BF16, vgi.PREC_FP16, vgi.PREC_FP32]
+        precs = [vgi.PREC_FP16, vgi.PREC_FP32, vgi.PREC_FP16X2, vgi.PREC_BF16, vgi.PREC_FP16, vgi.PREC_FP16X2]
         alone = [Job(ctx, w) for w in ws]
         batch = [Job(ctx, w) for w in ws]
         _run_alone(ctx, alone, precs)

@@ -1,9 +1,9 @@
 """GPU parity at the measured workloads' full sizes.
 
 - Room, 262,144 triangles, 256^3 grid with 8 levels, 1920x1080, 9 + 1 cones (the headline
-  frame): accumulator bit-exact vs the oracle, FP32 and FP16 levels bit-exact, a seeded
-  4096-pixel subset of the traced frame within 1e-3 relative L2 of the oracle's FP64 run
-  for FP32 and FP16 mips.
+  frame): accumulator bit-exact vs the oracle, FP32, FP16 and FP16 x-pair levels bit-exact,
+  a seeded 4096-pixel subset of the traced frame within 1e-3 relative L2 of the oracle's
+  FP64 run for each.
 - Room, 1,048,576 triangles, 512^3 grid with 9 levels, 3840x2160, 16 + 1 cones (the
   precision study): accumulator and FP16 levels bit-exact, the seeded subset within 1e-3
   of FP64 for FP32 and FP16 mips, BF16 within 1e-2 (its error is printed).
@@ -116,9 +116,9 @@ def test_room_1080p_voxels_and_levels(ctx, room_1080p):
     r = Run(ctx, w)
     try:
         _check_voxels(r.scene, fx)
-        for prec in ("fp32", "fp16"):
+        for prec in ("fp32", "fp16", "fp16x2"):
             r.frame(prec)
-            _check_levels(r.scene, fx["acc"], w, prec)
+            _check_levels(r.scene, fx["acc"], w, "fp16" if prec == "fp16x2" else prec)
     finally:
         r.release()
 
@@ -129,7 +129,7 @@ def test_room_1080p_trace_vs_fp64(ctx, room_1080p):
     print(f"case {w.name}: trace subset vs fp64", flush=True)
     r = Run(ctx, w)
     try:
-        for prec in ("fp32", "fp16"):
+        for prec in ("fp32", "fp16", "fp16x2"):
             d, s = r.frame(prec)
             err = rel_l2(_irr(d[sub], s[sub]), fx["ref64"])
             print(f"{prec} rel L2 vs FP64 = {err:.3e}", flush=True)
@@ -158,7 +158,7 @@ def test_room_4k_trace_vs_fp64(ctx, room_4k):
     print(f"case {w.name}: trace subset vs fp64", flush=True)
     r = Run(ctx, w)
     try:
-        for prec, bound in (("fp32", 1e-3), ("fp16", 1e-3), ("bf16", 1e-2)):
+        for prec, bound in (("fp32", 1e-3), ("fp16", 1e-3), ("bf16", 1e-2), ("fp16x2", 1e-3)):
             d, s = r.frame(prec)
             err = rel_l2(_irr(d[sub], s[sub]), fx["ref64"])
             print(f"{prec} rel L2 vs FP64 = {err:.3e}", flush=True)
