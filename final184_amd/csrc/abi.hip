@@ -816,11 +816,16 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   const bool table = count > 1;
   size_t nblocks = 0;
   for (const auto& v : stages) nblocks += v.size();
+  size_t total_lights = 0;
+  for (uint32_t i = 0; i < count; i++) total_lights += items[i].lights ? items[i].lights->count : 0;
   Carve cv;
   const size_t o_scenes = cv.take((size_t)count * sizeof(MipScene));
   const size_t o_blocks = cv.take(table ? nblocks * sizeof(MipBlock) : 0);
-  auto fill = [&](uint8_t* h, const uint8_t*) {
+  const size_t o_lights = cv.take(std::max<size_t>(total_lights, 1) * sizeof(LightDev));
+  auto fill = [&](uint8_t* h, const uint8_t* d) {
   MipScene* hs = (MipScene*)(h + o_scenes);
+  LightDev* hl = (LightDev*)(h + o_lights);
+  size_t lbase = 0;
   for (uint32_t i = 0; i < count; i++) {
     const vgi_scene* s = items[i].scene;
     const vgi_grid& g = s->grid;
@@ -842,14 +847,17 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     P.h = g.extent / (float)g.n;
     const vgi_lights* L = items[i].lights;
     P.nlights = L ? L->count : 0;
+    P.lights = (const LightDev*)(d + o_lights) + lbase;
     for (uint32_t l = 0; l < P.nlights; l++) {
-      P.lights[l].type = L->light[l].type;
+      LightDev& D = hl[lbase + l];
+      D.type = L->light[l].type;
       for (int a = 0; a < 3; a++) {
-        P.lights[l].vec[a] = L->light[l].vec[a];
-        P.lights[l].color[a] = L->light[l].color[a];
+        D.vec[a] = L->light[l].vec[a];
+        D.color[a] = L->light[l].color[a];
       }
-      P.lights[l].min_distance = L->light[l].min_distance;
+      D.min_distance = L->light[l].min_distance;
     }
+    lbase += P.nlights;
   }
   MipBlock* hb = (MipBlock*)(h + o_blocks);
   size_t boff = 0;

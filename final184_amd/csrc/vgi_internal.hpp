@@ -146,7 +146,7 @@ struct MipScene {
   uint32_t nlights;
   float origin[3];
   float h;
-  LightDev lights[VGI_MAX_LIGHTS];
+  const LightDev* lights;  // nlights entries, in the call's descriptor upload
 };
 
 // Mip workgroup: scene, tile of its source level, source level, levels produced.

@@ -24,7 +24,7 @@ PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP16X2 = 0, 1, 2, 3
 PREC_NAMES = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp16x2": PREC_FP16X2}
 INDEX_NONE, INDEX_U16, INDEX_U32 = 0, 1, 2
 LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
-MAX_LEVELS, MAX_LIGHTS, MAX_DIFFUSE_CONES = 11, 16, 32
+MAX_LEVELS, MAX_LIGHTS, MAX_DIFFUSE_CONES = 11, 128, 32
 
 
 class VgiError(RuntimeError):

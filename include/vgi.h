@@ -44,7 +44,7 @@ extern "C" {
 #define VGI_ABI_VERSION 2u
 
 #define VGI_MAX_LEVELS 11u   /* 1024^3 -> 1^3 */
-#define VGI_MAX_LIGHTS 16u
+#define VGI_MAX_LIGHTS 128u  /* the reference's LightLists holds 100 (MegaPipeline.cpp:101-105) */
 #define VGI_MAX_DIFFUSE_CONES 32u
 /* Snapped vertex coordinates must satisfy |v| <= VGI_GUARD_FIXED in 1/256-voxel units. */
 #define VGI_GUARD_FIXED 262144

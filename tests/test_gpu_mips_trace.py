@@ -344,3 +344,28 @@ def test_changed_and_repeated_descriptors(ctx, soup):
     d3, s3, _ = orc.trace(orc.inject_mips(soup["acc"], w.grid, w.lights, "fp32"), w.grid, w.gb_pos, w.gb_nrm, cb)
     assert rel_l2(_irr(outs[1][0], outs[1][1]), _irr(d3, s3)) <= 1e-5
     f.release()
+
+
+def test_hundred_lights_bit_exact(ctx, soup):
+    """The reference's light list holds 100 lights (MegaPipeline.cpp:101-105): 99 seeded point
+    lights and one directional light inject level 0 and build every level bit for bit as
+    the oracle does (lights summed in list order)."""
+    vgi = _vgi()
+    w = soup["w"]
+    print("case 100 lights", flush=True)
+    rng = np.random.default_rng(100)
+    lights = [dict(type="point", vec=tuple(float(v) for v in rng.uniform(0.0, 1.0, 3)),
+                   color=tuple(float(v) for v in rng.uniform(0.05, 0.4, 3)), min_distance=0.05)
+              for _ in range(99)]
+    lights.append(w.lights[0])
+    mesh = vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)
+    sc = ctx.scene()
+    sc.voxelize([mesh], vgi.make_grid(**w.grid))
+    ml = vgi.make_lights(lights)
+    assert ml.count == 100
+    for prec in ("fp32", "fp16x2"):
+        sc.build_mips(ml, vgi.PREC_NAMES[prec])
+        ref = orc.inject_mips(soup["acc"], w.grid, lights, ORC_PREC[prec])
+        for k in range(w.grid["levels"]):
+            assert np.array_equal(sc.read_level(k), ref[k]), f"{prec} level {k}"
+    sc.release()
