@@ -282,6 +282,9 @@ def trace_roofline(cn, probes, r, px):
             "valu_busy": (k["VALUBusy"] / 100.0) if "VALUBusy" in k else None,
             "fma_probe_rate": (pr["SQ_INSTS_VALU"] / (probes["fma"] * 1e-3) / 1e9
                                if pr.get("SQ_INSTS_VALU") and probes.get("fma") else None),
+            "int_probe_rate": ((cn or {})["probe_int"]["SQ_INSTS_VALU"] / (probes["int"] * 1e-3) / 1e9
+                               if (cn or {}).get("probe_int", {}).get("SQ_INSTS_VALU") and probes.get("int")
+                               else None),
             "fma_probe_valu_busy": (pr["VALUBusy"] / 100.0) if "VALUBusy" in pr else None}
     if d["traffic"]:
         gbs = d["traffic"] / t_s / 1e9
@@ -421,6 +424,7 @@ def run_probes(ctx, vgi, reps=1):
     abuf = vgi.DeviceArray(ctx, PROBE_BLOCKS * 256 * 64)
     try:
         for name, kind, buf, iters in (("fma", vgi.PROBE_FMA, fbuf, PROBE_FMA_ITERS),
+                                       ("int", vgi.PROBE_INT, fbuf, PROBE_FMA_ITERS),
                                        ("atomic_ret", vgi.PROBE_ATOMIC_RETURN, abuf, PROBE_ATOMIC_ITERS),
                                        ("atomic_noret", vgi.PROBE_ATOMIC_NORETURN, abuf, PROBE_ATOMIC_ITERS)):
             abuf.zero()
@@ -498,7 +502,7 @@ COUNTER_KERNELS = [
     ("mips_fp32_first", r"k_mips<0, true>"), ("mips_fp32_rest", r"k_mips<0, false>"),
     ("mips_fp16x2_first", r"k_mips<3, true>"), ("mips_fp16x2_rest", r"k_mips<3, false>"),
     ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8>"),
-    ("probe_fma", r"k_probe_fma"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
+    ("probe_fma", r"k_probe_fma"), ("probe_int", r"k_probe_int"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
     ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
 ]
 COUNTER_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),

@@ -1918,9 +1918,10 @@ int vgi_stream_create(vgi_context* ctx, void** out) {
 int vgi_probe(vgi_context* ctx, uint32_t kind, void* buf, size_t buf_bytes, uint32_t blocks, uint32_t iters,
               void* stream) {
   if (!ctx || !buf) return fail(VGI_E_INVALID, "NULL argument");
-  if (kind > VGI_PROBE_ATOMIC_NORETURN) return fail(VGI_E_INVALID, "probe kind %u", kind);
+  if (kind > VGI_PROBE_INT) return fail(VGI_E_INVALID, "probe kind %u", kind);
   if (blocks == 0 || blocks > (1u << 20)) return fail(VGI_E_INVALID, "probe blocks %u", blocks);
-  const size_t need = (size_t)blocks * 256u * (kind == VGI_PROBE_FMA ? 4u : 64u);
+  const bool lines = kind == VGI_PROBE_ATOMIC_RETURN || kind == VGI_PROBE_ATOMIC_NORETURN;
+  const size_t need = (size_t)blocks * 256u * (lines ? 64u : 4u);
   if (buf_bytes < need) return fail(VGI_E_INVALID, "probe buffer of %zu bytes, %zu needed", buf_bytes, need);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(launch_probe(kind, buf, blocks, iters, pick(ctx, stream)));

@@ -5,7 +5,8 @@
 // are measured, not assumed: a kernel of independent FP32 fused multiply-add chains gives
 // the VALU issue rate a kernel can reach, and kernels of 32-bit atomic adds, each lane on
 // its own 64-byte line (no two lanes contend), give the rate of returning and of
-// non-returning atomics. HBM bytes are calibrated with the plain copy kernel k_copy
+// non-returning atomics; a kernel of independent 32-bit integer chains gives the integer
+// VALU issue rate. HBM bytes are calibrated with the plain copy kernel k_copy
 // (vgi_memcpy_d2d). No reference counterpart: the reference has no measurement code.
 #include "vgi_internal.hpp"
 
@@ -29,6 +30,26 @@ __global__ __launch_bounds__(256) void k_probe_fma(float* __restrict__ out, uint
     a7 = fmaf(a7, m, c);
   }
   gst(out + i, ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7)));
+}
+
+// The same shape with 32-bit integer operations (exclusive-or with a value the loop
+// changes, so that no two rounds cancel and the compiler cannot fold them): the issue rate
+// of the integer index and mask work that a gather kernel mixes with its arithmetic.
+__global__ __launch_bounds__(256) void k_probe_int(uint32_t* __restrict__ out, uint32_t iters, uint32_t m) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a0 = i, a1 = i + 1, a2 = i + 2, a3 = i + 3, a4 = i + 4, a5 = i + 5, a6 = i + 6, a7 = i + 7;
+  for (uint32_t k = 0; k < iters; k++) {
+    a0 ^= m;
+    a1 ^= m;
+    a2 ^= m;
+    a3 ^= m;
+    a4 ^= m;
+    a5 ^= m;
+    a6 ^= m;
+    a7 ^= m;
+    m = m * 3u + 1u;  // keeps the loop from collapsing (even iteration counts would cancel)
+  }
+  gst(out + i, ((a0 ^ a1) ^ (a2 ^ a3)) ^ ((a4 ^ a5) ^ (a6 ^ a7)));
 }
 
 // Each lane adds 1 to the first word of its own 64-byte line `iters` times. RETURN: the
@@ -56,6 +77,9 @@ hipError_t launch_probe(uint32_t kind, void* buf, uint32_t blocks, uint32_t iter
       break;
     case VGI_PROBE_ATOMIC_NORETURN:
       hipLaunchKernelGGL(k_probe_atomic<false>, dim3(blocks), dim3(256), 0, s, (uint32_t*)buf, iters);
+      break;
+    case VGI_PROBE_INT:
+      hipLaunchKernelGGL(k_probe_int, dim3(blocks), dim3(256), 0, s, (uint32_t*)buf, iters, 0x5A5A5A5Bu);
       break;
     default:
       return hipErrorInvalidValue;

@@ -70,7 +70,7 @@ class GBuffer(C.Structure):
 ORDER_TILES, ORDER_MORTON = 0, 1
 ADDRESS_AUTO, ADDRESS_WIDE = 0, 1  # cone-march texel addressing (vgi.h VGI_ADDRESS_*)
 IMAGE_RGBA32F, IMAGE_RGBA16F = 0, 1
-PROBE_FMA, PROBE_ATOMIC_RETURN, PROBE_ATOMIC_NORETURN = 0, 1, 2  # vgi.h VGI_PROBE_*
+PROBE_FMA, PROBE_ATOMIC_RETURN, PROBE_ATOMIC_NORETURN, PROBE_INT = 0, 1, 2, 3  # vgi.h VGI_PROBE_*
 
 
 class GBufferRef(C.Structure):

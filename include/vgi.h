@@ -585,8 +585,10 @@ int vgi_workspace_bytes(vgi_context* ctx, uint64_t* out);
  * VGI_PROBE_ATOMIC_RETURN / _NORETURN: each lane adds 1 `iters` times to the first word of
  *   its own 64-byte line (buf holds blocks * 256 * 64 bytes), with returning atomics whose
  *   results are summed into the line's second word, or non-returning ones: the atomic
- *   rate without contention. */
-enum { VGI_PROBE_FMA = 0, VGI_PROBE_ATOMIC_RETURN = 1, VGI_PROBE_ATOMIC_NORETURN = 2 };
+ *   rate without contention;
+ * VGI_PROBE_INT: as VGI_PROBE_FMA with 8 independent 32-bit integer (exclusive-or)
+ *   chains and one store per lane: the integer VALU issue rate. */
+enum { VGI_PROBE_FMA = 0, VGI_PROBE_ATOMIC_RETURN = 1, VGI_PROBE_ATOMIC_NORETURN = 2, VGI_PROBE_INT = 3 };
 int vgi_probe(vgi_context* ctx, uint32_t kind, void* buf, size_t buf_bytes, uint32_t blocks, uint32_t iters,
               void* stream);
 
