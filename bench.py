@@ -82,6 +82,32 @@ def run_precision(ctx, vgi, w, mesh, gb, prec, steps, warmup):
     return out
 
 
+def run_shadowed(ctx, vgi, w, mesh, prec, reps=10):
+    """The mip build (inject + levels) with the workload's lights casting shadows against
+    the same build with plain lights: mean HIP-event time of each over `reps` builds of one
+    voxelized scene (the visibility walks run inside the injecting launch)."""
+    grid = vgi.make_grid(**w.grid)
+    plain = vgi.make_lights(w.lights)
+    shadowed = vgi.make_lights([dict(l, shadow=True) for l in w.lights])
+    p = vgi.PREC_NAMES[prec]
+    scene = ctx.scene()
+    out = {}
+    try:
+        scene.voxelize([mesh], grid)
+        for name, lights in (("plain", plain), ("shadowed", shadowed)):
+            scene.build_mips(lights, p)  # warm-up
+            ms = []
+            ctx.set_profiling(True)
+            for _ in range(reps):
+                scene.build_mips(lights, p)
+                ms.append(ctx.timings().mips_ms)
+            ctx.set_profiling(False)
+            out[name] = float(np.mean(ms))
+    finally:
+        scene.release()
+    return out
+
+
 def run_pipelined(ctx, vgi, w, mesh, gb, prec, steps, warmup):
     """Frames overlapped on two streams: the voxelize of frame k+1 (stream B) runs while
     frame k is traced (stream A). Ordering by vgi_stream_wait: a frame's mip build waits for
@@ -498,9 +524,9 @@ def _counter_rows(outdir):
 # (FP32), whose front end is identical to the FP16 frame's.
 COUNTER_KERNELS = [
     ("trace_fp16", r"k_trace<1\b"), ("trace_fp32", r"k_trace<0\b"), ("trace_fp16x2", r"k_trace<3\b"),
-    ("mips_fp16_first", r"k_mips<1, true>"), ("mips_fp16_rest", r"k_mips<1, false>"),
-    ("mips_fp32_first", r"k_mips<0, true>"), ("mips_fp32_rest", r"k_mips<0, false>"),
-    ("mips_fp16x2_first", r"k_mips<3, true>"), ("mips_fp16x2_rest", r"k_mips<3, false>"),
+    ("mips_fp16_first", r"k_mips<1, true, false>"), ("mips_fp16_rest", r"k_mips<1, false, false>"),
+    ("mips_fp32_first", r"k_mips<0, true, false>"), ("mips_fp32_rest", r"k_mips<0, false, false>"),
+    ("mips_fp16x2_first", r"k_mips<3, true, false>"), ("mips_fp16x2_rest", r"k_mips<3, false, false>"),
     ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8>"),
     ("probe_fma", r"k_probe_fma"), ("probe_int", r"k_probe_int"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
     ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
@@ -687,6 +713,9 @@ def main() -> int:
         pipe_ms = run_pipelined(ctx, vgi, w, mesh, gb, head["prec"], max(args.steps, 20), args.warmup)
         log(f"{head['prec']}: two-stream overlapped frames {pipe_ms:.3f} ms per frame "
             f"(sequential {head['frame_ms']:.3f} ms)")
+    shadow_ms = run_shadowed(ctx, vgi, w, mesh, head["prec"])
+    log(f"{head['prec']}: mip build with shadowed lights {shadow_ms['shadowed']:.3f} ms "
+        f"(plain {shadow_ms['plain']:.3f} ms)")
     # calibration kernels timed in this process (their counters came from the probe runs)
     probes = run_probes(ctx, vgi, reps=5)
     log(f"probes (ms): {probes}")
@@ -728,6 +757,10 @@ def main() -> int:
         "overlapped_frames": None if pipe_ms is None else {
             "streams": 2, "frame_ms": pipe_ms, "mpixels_s": px / (pipe_ms * 1e-3) / 1e6,
             "note": "voxelize of frame k+1 on a second stream while frame k is traced"},
+        "shadowed_mips": {"lights": f"the workload's {len(w.lights)} lights casting shadows",
+                          "mip_storage": head["prec"], "mips_ms": shadow_ms["shadowed"],
+                          "plain_mips_ms": shadow_ms["plain"],
+                          "note": "visibility walks through the level-0 occupancy inside the injecting launch"},
         "trilinear_per_pixel": head["trilinear"] / px,
         "tuning": tuned.as_dict() if tuned is not None else
                   {"bin_size": args.bin, "schedule": args.order if args.order == "morton" else args.tile},

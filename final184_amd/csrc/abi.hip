@@ -781,8 +781,8 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
       if (it.lights->count > VGI_MAX_LIGHTS)
         return fail(VGI_E_RANGE, "item %u: %u lights exceed %u", i, it.lights->count, VGI_MAX_LIGHTS);
       for (uint32_t l = 0; l < it.lights->count; l++)
-        if (it.lights->light[l].type > VGI_LIGHT_POINT)
-          return fail(VGI_E_INVALID, "item %u: light %u: type %u", i, l, it.lights->light[l].type);
+        if ((it.lights->light[l].type & ~VGI_LIGHT_SHADOW) > VGI_LIGHT_POINT)
+          return fail(VGI_E_INVALID, "item %u: light %u: type %#x", i, l, it.lights->light[l].type);
     }
     seen[i] = it.scene;
   }
@@ -817,7 +817,13 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   size_t nblocks = 0;
   for (const auto& v : stages) nblocks += v.size();
   size_t total_lights = 0;
-  for (uint32_t i = 0; i < count; i++) total_lights += items[i].lights ? items[i].lights->count : 0;
+  bool shadow = false;  // the injecting launch walks toward shadow-casting lights
+  for (uint32_t i = 0; i < count; i++)
+    if (items[i].lights) {
+      total_lights += items[i].lights->count;
+      for (uint32_t l = 0; l < items[i].lights->count; l++)
+        shadow = shadow || (items[i].lights->light[l].type & VGI_LIGHT_SHADOW) != 0;
+    }
   Carve cv;
   const size_t o_scenes = cv.take((size_t)count * sizeof(MipScene));
   const size_t o_blocks = cv.take(table ? nblocks * sizeof(MipBlock) : 0);
@@ -848,15 +854,7 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     const vgi_lights* L = items[i].lights;
     P.nlights = L ? L->count : 0;
     P.lights = (const LightDev*)(d + o_lights) + lbase;
-    for (uint32_t l = 0; l < P.nlights; l++) {
-      LightDev& D = hl[lbase + l];
-      D.type = L->light[l].type;
-      for (int a = 0; a < 3; a++) {
-        D.vec[a] = L->light[l].vec[a];
-        D.color[a] = L->light[l].color[a];
-      }
-      D.min_distance = L->light[l].min_distance;
-    }
+    for (uint32_t l = 0; l < P.nlights; l++) hl[lbase + l] = light_dev(L->light[l], P.origin, P.h);
     lbase += P.nlights;
   }
   MipBlock* hb = (MipBlock*)(h + o_blocks);
@@ -879,6 +877,7 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     B.lut = ctx->d_lut;
     B.src = stages[j][0].src;  // one scene: every block of the stage shares src and nout
     B.nout = stages[j][0].nout;
+    B.shadow = shadow;
     HIP_TRY(launch_mips(B, precision, j == 0, (uint32_t)stages[j].size(), st));
     off += stages[j].size();
   }
@@ -1148,7 +1147,8 @@ int vgi::check_request_input(const vgi_mesh* meshes, uint32_t num_meshes, const 
   if (lights) {
     if (lights->count > VGI_MAX_LIGHTS) return fail(VGI_E_RANGE, "%u lights exceed %u", lights->count, VGI_MAX_LIGHTS);
     for (uint32_t l = 0; l < lights->count; l++)
-      if (lights->light[l].type > VGI_LIGHT_POINT) return fail(VGI_E_INVALID, "light %u: type %u", l, lights->light[l].type);
+      if ((lights->light[l].type & ~VGI_LIGHT_SHADOW) > VGI_LIGHT_POINT)
+        return fail(VGI_E_INVALID, "light %u: type %#x", l, lights->light[l].type);
   }
   if (num_frames > 0 && !frames) return fail(VGI_E_INVALID, "frames is NULL");
   for (uint32_t f = 0; f < num_frames; f++) TRY(check_frame(frames[f].gbuffer, frames[f].cones, frames[f].out_diffuse, f));

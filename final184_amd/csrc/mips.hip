@@ -22,6 +22,8 @@
 // voxels, plus once, as zeros, when it has lost them; a per-tile word records which. A
 // surface leaves most tiles empty (70% at 256^3 for the room), so most of the level
 // bytes are not rewritten each build, and every texel still equals the full computation.
+#include <cmath>
+
 #include "vgi_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -182,7 +184,73 @@ __device__ __forceinline__ uint32_t group_occupancy(const MipScene& P, uint32_t 
   return pairs != 0 ? bits : 0u;  // a select, not a branch, so the mask loads stay with the first
 }
 
-// Radiance of an occupied level-0 voxel (x, y, z) from its accumulator record.
+// Occupancy of voxel (x, y, z), all three < n, with the bin entry and the mask word loaded
+// together (every brick's mask words exist; those of an empty bin are discarded).
+__device__ __forceinline__ bool occupied_any(const MipScene& P, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
+  const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
+  const uint32_t nbr = P.n >> 3;
+  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
+  const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
+  const uint32_t pairs = (uint32_t)gld(P.bin_packed + bin);
+  const uint32_t word = gld(P.mask + (size_t)brick * 16u + (local >> 5));
+  return pairs != 0u && ((word >> (local & 31u)) & 1u) != 0u;
+}
+
+__device__ __forceinline__ int32_t sign_i32(int32_t v) { return (v > 0) - (v < 0); }
+__device__ __forceinline__ uint32_t abs_u32(int32_t v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
+
+// Visibility of shadow-casting light L from occupied voxel (x, y, z) whose summed normal
+// (unbiased integer sums) is (n0, n1, n2): a walk through the level-0 occupancy toward the
+// light in 16.16 fixed point. Start: the voxel centre moved one voxel along the dominant
+// axis of the normal sum (ties to the lowest axis). Each sample advances exactly one voxel
+// along the direction's dominant axis, so a walk leaves the grid within n + 1 samples; a
+// point light's walk also ends when it reaches the light's distance along that axis. The
+// voxel itself is passed over. Integer steps and IEEE single-precision subtraction,
+// division and rounding only, so every build of the walk takes the same decisions.
+__device__ bool light_visible(const MipScene& P, const LightDev& L, uint32_t x, uint32_t y, uint32_t z,
+                              int32_t n0, int32_t n1, int32_t n2) {
+  const uint32_t m0 = abs_u32(n0), m1 = abs_u32(n1), m2 = abs_u32(n2);
+  int32_t o0 = 0, o1 = 0, o2 = 0;
+  if (m0 >= m1 && m0 >= m2) o0 = sign_i32(n0);
+  else if (m1 >= m2) o1 = sign_i32(n1);
+  else o2 = sign_i32(n2);
+  const int32_t p0 = (int32_t)x + o0, p1 = (int32_t)y + o1, p2 = (int32_t)z + o2;
+  int32_t s0, s1, s2;
+  uint32_t steps = P.n + 2u;
+  if (L.type == VGI_LIGHT_DIRECTIONAL) {
+    s0 = L.step[0];
+    s1 = L.step[1];
+    s2 = L.step[2];
+  } else {
+    const float l0 = L.pv[0] - ((float)p0 + 0.5f);
+    const float l1 = L.pv[1] - ((float)p1 + 0.5f);
+    const float l2 = L.pv[2] - ((float)p2 + 0.5f);
+    const float a0 = fabsf(l0), a1 = fabsf(l1), a2 = fabsf(l2);
+    const float la = (a0 >= a1 && a0 >= a2) ? a0 : (a1 >= a2 ? a1 : a2);
+    if (!(la > 0.0f)) return true;  // the light sits at the start
+    s0 = (int32_t)__builtin_rintf((l0 / la) * 65536.0f);
+    s1 = (int32_t)__builtin_rintf((l1 / la) * 65536.0f);
+    s2 = (int32_t)__builtin_rintf((l2 / la) * 65536.0f);
+    const float c = __builtin_ceilf(la);
+    if (c < (float)steps) steps = (uint32_t)c;
+  }
+  int32_t q0 = p0 * 65536 + 32768, q1 = p1 * 65536 + 32768, q2 = p2 * 65536 + 32768;
+  for (uint32_t m = 0; m < steps; m++) {
+    const int32_t c0 = q0 >> 16, c1 = q1 >> 16, c2 = q2 >> 16;  // floor
+    if ((uint32_t)c0 >= P.n || (uint32_t)c1 >= P.n || (uint32_t)c2 >= P.n) return true;
+    const bool self = c0 == (int32_t)x && c1 == (int32_t)y && c2 == (int32_t)z;
+    if (!self && occupied_any(P, (uint32_t)c0, (uint32_t)c1, (uint32_t)c2)) return false;
+    q0 += s0;
+    q1 += s1;
+    q2 += s2;
+  }
+  return true;
+}
+
+// Radiance of an occupied level-0 voxel (x, y, z) from its accumulator record. SHADOW:
+// lights flagged to cast shadows are cut where their visibility walk is blocked.
+template <bool SHADOW>
 __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t nbr = P.n >> 3;
   const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
@@ -206,7 +274,10 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
     const LightDev& L = P.lights[l];
     if (L.type == VGI_LIGHT_DIRECTIONAL) {
       const float cosv = (nx * -L.vec[0] + ny * -L.vec[1]) + nz * -L.vec[2];
-      if (cosv > 0.f) {
+      bool lit = cosv > 0.f;
+      if constexpr (SHADOW)
+        if (lit && L.shadow) lit = light_visible(P, L, x, y, z, (int32_t)s1.x - bias, (int32_t)s1.y - bias, (int32_t)s1.z - bias);
+      if (lit) {
         er = er + L.color[0] * cosv;
         eg = eg + L.color[1] * cosv;
         eb = eb + L.color[2] * cosv;
@@ -216,7 +287,10 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
       const float d2 = (dx * dx + dy * dy) + dz * dz;
       if (d2 > 0.f) {
         const float cosv = ((nx * dx + ny * dy) + nz * dz) / __builtin_sqrtf(d2);
-        if (cosv > 0.f) {
+        bool lit = cosv > 0.f;
+        if constexpr (SHADOW)
+          if (lit && L.shadow) lit = light_visible(P, L, x, y, z, (int32_t)s1.x - bias, (int32_t)s1.y - bias, (int32_t)s1.z - bias);
+        if (lit) {
           const float md = L.min_distance;
           const float att = fmaxf(d2, md * md);
           const float f = cosv / att;
@@ -230,7 +304,7 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
   return make_float4(lr * er, lg * eg, lb * eb, 1.0f);
 }
 
-template <int PREC, bool SRC_ACC>
+template <int PREC, bool SRC_ACC, bool SHADOW>
 __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scenes,
                                                const MipBlock* __restrict__ blocks,
                                                const float* __restrict__ lut, uint32_t src1, uint32_t nout1) {
@@ -262,7 +336,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       // single-level chain: resolve level 0 only, one voxel per thread
       for (uint32_t v = g; v < S * S * S; v += 512) {
         const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
-        const float4 c = occupied(P, x, y, z) ? TX::round(inject(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 c = occupied(P, x, y, z) ? TX::round(inject<SHADOW>(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
         TX::store(P.level[0], padded_index(x, y, z, ns), c);
       }
       return;
@@ -312,7 +386,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       const uint32_t lx = 2 * (gi & (G - 1u)) + (bit & 1u);
       const uint32_t ly = 2 * ((gi >> gsh) & (G - 1u)) + ((bit >> 1) & 1u);
       const uint32_t lz = 2 * (gi >> (2 * gsh)) + (bit >> 2);
-      const float4 c = TX::round(inject(P, s_lut, tx * S + lx, ty * S + ly, tz * S + lz));
+      const float4 c = TX::round(inject<SHADOW>(P, s_lut, tx * S + lx, ty * S + ly, tz * S + lz));
       s_val[lx + (ly << ssh) + (lz << (2 * ssh))] = TX::pack(c);
     }
     __syncthreads();
@@ -391,30 +465,53 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
   }
 }
 
+template <int PREC>
+static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, dim3 b, hipStream_t s) {
+  if (!from_acc)
+    hipLaunchKernelGGL((k_mips<PREC, false, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+  else if (B.shadow)
+    hipLaunchKernelGGL((k_mips<PREC, true, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+  else
+    hipLaunchKernelGGL((k_mips<PREC, true, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+}
+
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
   const dim3 g(blocks), b(512);
   switch (prec) {
-    case VGI_PREC_FP32:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP32, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      break;
-    case VGI_PREC_FP16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      break;
-    case VGI_PREC_BF16:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_BF16, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      break;
-    case VGI_PREC_FP16X2:
-      if (from_acc) hipLaunchKernelGGL((k_mips<VGI_PREC_FP16X2, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      else hipLaunchKernelGGL((k_mips<VGI_PREC_FP16X2, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case VGI_PREC_FP32: launch_mips_prec<VGI_PREC_FP32>(B, from_acc, g, b, s); break;
+    case VGI_PREC_FP16: launch_mips_prec<VGI_PREC_FP16>(B, from_acc, g, b, s); break;
+    case VGI_PREC_BF16: launch_mips_prec<VGI_PREC_BF16>(B, from_acc, g, b, s); break;
+    case VGI_PREC_FP16X2: launch_mips_prec<VGI_PREC_FP16X2>(B, from_acc, g, b, s); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+LightDev light_dev(const vgi_light& L, const float origin[3], float h) {
+  LightDev D{};
+  D.type = L.type & ~VGI_LIGHT_SHADOW;
+  for (int a = 0; a < 3; a++) {
+    D.vec[a] = L.vec[a];
+    D.color[a] = L.color[a];
+  }
+  D.min_distance = L.min_distance;
+  D.shadow = (L.type & VGI_LIGHT_SHADOW) ? 1u : 0u;
+  if (!D.shadow) return D;
+  if (D.type == VGI_LIGHT_DIRECTIONAL) {
+    // the step toward the light is the same from every voxel: -vec / |vec_a| in 16.16
+    const float l[3] = {-L.vec[0], -L.vec[1], -L.vec[2]};
+    const float a0 = std::fabs(l[0]), a1 = std::fabs(l[1]), a2 = std::fabs(l[2]);
+    const float la = (a0 >= a1 && a0 >= a2) ? a0 : (a1 >= a2 ? a1 : a2);
+    if (!(la > 0.0f)) {
+      D.shadow = 0;  // no direction to walk: visible from everywhere
+      return D;
+    }
+    for (int a = 0; a < 3; a++) D.step[a] = (int32_t)std::nearbyint((l[a] / la) * 65536.0f);
+  } else {
+    for (int a = 0; a < 3; a++) D.pv[a] = (L.vec[a] - origin[a]) / h;
+  }
+  return D;
 }
 
 }  // namespace vgi

@@ -125,11 +125,20 @@ struct VoxBatch {
 };
 
 struct LightDev {
-  uint32_t type;
+  uint32_t type;  // VGI_LIGHT_DIRECTIONAL or VGI_LIGHT_POINT
   float vec[3];
   float color[3];
   float min_distance;
+  uint32_t shadow;  // 1: cut where the visibility walk toward the light meets an occupied voxel
+  int32_t step[3];  // directional, shadow: the walk's step toward the light, 16.16 fixed point
+  float pv[3];      // point, shadow: position in voxel units of level 0, (vec - origin) / h
+  uint32_t pad;
 };
+static_assert(sizeof(LightDev) == 64, "one light is 64 bytes of the descriptor upload");
+
+// The per-scene light record of a vgi_light (host side): kind, shadow flag, and the
+// visibility walk's precomputed step (directional) or voxel-unit position (point).
+LightDev light_dev(const vgi_light& L, const float origin[3], float h);
 
 // One scene of a mip batch (device memory).
 struct MipScene {
@@ -160,6 +169,7 @@ struct MipBatch {
                            // scene 0 with the launch's src and nout
   const float* lut;        // 256 entries: (i/255)^2.2
   uint32_t src, nout;
+  bool shadow;             // some light of the batch casts shadows (the injecting launch walks)
 };
 
 // One scene of a trace batch (device memory).

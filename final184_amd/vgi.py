@@ -24,6 +24,7 @@ PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP16X2 = 0, 1, 2, 3
 PREC_NAMES = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp16x2": PREC_FP16X2}
 INDEX_NONE, INDEX_U16, INDEX_U32 = 0, 1, 2
 LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1
+LIGHT_SHADOW = 0x100  # vgi.h VGI_LIGHT_SHADOW: OR'd into the type of a light that casts shadows
 MAX_LEVELS, MAX_LIGHTS, MAX_DIFFUSE_CONES = 11, 128, 32
 
 
@@ -293,14 +294,15 @@ def grid_from_voxel_camera(voxel_view, voxel_proj, n: int, levels: int) -> Grid:
 
 
 def make_lights(lights) -> Lights:
-    """lights: iterable of dicts {type: 'directional'|'point', vec, color, min_distance}."""
+    """lights: iterable of dicts {type: 'directional'|'point', vec, color, min_distance, shadow}."""
     L = Lights()
     items = list(lights)
     if len(items) > MAX_LIGHTS:
         raise ValueError("too many lights")
     L.count = len(items)
     for i, d in enumerate(items):
-        L.light[i].type = LIGHT_DIRECTIONAL if d["type"] == "directional" else LIGHT_POINT
+        L.light[i].type = ((LIGHT_DIRECTIONAL if d["type"] == "directional" else LIGHT_POINT)
+                           | (LIGHT_SHADOW if d.get("shadow") else 0))
         L.light[i].vec[:] = [float(v) for v in d["vec"]]
         L.light[i].color[:] = [float(v) for v in d["color"]]
         L.light[i].min_distance = float(d.get("min_distance", 0.0))

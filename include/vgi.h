@@ -82,6 +82,21 @@ typedef enum vgi_index_type {
 
 typedef enum vgi_light_type { VGI_LIGHT_DIRECTIONAL = 0, VGI_LIGHT_POINT = 1 } vgi_light_type;
 
+/* OR'd into vgi_light.type: the light casts shadows. Its contribution to an occupied voxel
+ * is cut when a walk through the voxel occupancy toward the light meets an occupied voxel
+ * first (the voxel-space counterpart of the reference's shadow-map test of each lit hit,
+ * Shader/Lighting/indirect.frag:160-165). The walk, in voxel units of level 0:
+ *  - start: the voxel centre moved one voxel along the dominant axis of the voxel's summed
+ *    normal (ties to the lowest axis; no move when the sum is zero);
+ *  - direction l toward the light: -vec (directional), or the light's position in voxel
+ *    units minus the start (point); dominant axis a of l (ties to the lowest axis);
+ *  - samples at start + m * l / |l_a|, m = 0, 1, ..., in 16.16 fixed point (the step rounded
+ *    to nearest), each naming the voxel that contains it; the voxel itself is passed over;
+ *  - lit when a sample leaves the grid, or (point) when m reaches ceil(|l_a|), the light's
+ *    distance along a; unlit at the first occupied sample.
+ * Every sample advances one voxel along a, so a walk takes at most n + 1 samples. */
+#define VGI_LIGHT_SHADOW 0x100u
+
 typedef struct vgi_context vgi_context;
 typedef struct vgi_scene vgi_scene;
 
@@ -125,7 +140,7 @@ typedef struct vgi_mesh {
 } vgi_mesh;
 
 typedef struct vgi_light {
-  uint32_t type;      /* vgi_light_type */
+  uint32_t type;      /* vgi_light_type, optionally | VGI_LIGHT_SHADOW */
   float vec[3];       /* directional: unit direction the light travels; point: position */
   float color[3];     /* RGB radiant intensity (colour x intensity) */
   float min_distance; /* point: distance clamp for 1/d^2 falloff (world units) */

@@ -49,6 +49,7 @@ def _load():
         "orc_bf16_to_float": (C.c_float, [C.c_uint16]),
         "orc_inject_mips_f32": (C.c_int, [P, P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, C.c_int, P]),
         "orc_inject_mips_f64": (C.c_int, [P, P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P]),
+        "orc_light_visibility": (C.c_int, [P, P, C.c_float, C.c_uint32, P, C.c_uint32, P]),
         "orc_trace_f32": (C.c_int, [P, P, C.c_float, C.c_uint32, C.c_uint32, P, P, P, P, C.c_uint64, P, P, P, P]),
         "orc_trace_f64": (C.c_int, [P, P, C.c_float, C.c_uint32, C.c_uint32, P, P, P, P, C.c_uint64, P, P, P, P]),
         "orc_cone_schedule_f64": (C.c_int, [C.c_double] * 5 + [P, C.c_int]),
@@ -128,10 +129,13 @@ def tri_box_overlap(q, b, delta) -> bool:
     return bool(lib.orc_tri_box_overlap(_p(q), _p(b), int(delta)))
 
 
+LIGHT_SHADOW = 0x100  # orc_light.type flag: the light is cut where the visibility walk meets a voxel
+
+
 def _lights(lights):
     arr = (OrcLight * max(1, len(lights)))()
     for i, d in enumerate(lights):
-        arr[i].type = 0 if d["type"] == "directional" else 1
+        arr[i].type = (0 if d["type"] == "directional" else 1) | (LIGHT_SHADOW if d.get("shadow") else 0)
         arr[i].vec[:] = [float(v) for v in d["vec"]]
         arr[i].color[:] = [float(v) for v in d["color"]]
         arr[i].min_distance = float(d.get("min_distance", 0.0))
@@ -163,6 +167,18 @@ def inject_mips(acc, grid, lights, prec="fp32", f64=False):
         levels.append(out[off:off + nk ** 3 * 4].reshape(nk, nk, nk, 4))
         off += nk ** 3 * 4
     return levels
+
+
+def light_visibility(acc, grid, lights):
+    """(n, n, n) [z][y][x] uint32: bit l set where light l is visible from an occupied voxel."""
+    n = int(grid["n"])
+    acc = np.ascontiguousarray(acc, np.uint32)
+    origin = np.array(grid["origin"], np.float32)
+    out = np.empty(n ** 3, np.uint32)
+    rc = lib.orc_light_visibility(_p(acc), _p(origin), float(grid["extent"]), n, _lights(lights),
+                                  len(lights), _p(out))
+    assert rc == 0
+    return out.reshape(n, n, n)
 
 
 def _cones(c):

@@ -23,7 +23,10 @@
 //      light at the voxel centre (Lambert with the voxel's mean normal; directional
 //      lights and point lights with 1/max(d^2, r_min^2) falloff); the reference
 //      lights voxels at lookup with lambert(-sun, voxel normal) x pow(colour, 2.2)
-//      (Shader/Lighting/indirect.frag:157-169). Then the 2x2x2 box mip chain
+//      (Shader/Lighting/indirect.frag:157-169). A light whose type carries the shadow
+//      flag (0x100) is cut to zero at voxels from which a walk toward it meets an occupied
+//      voxel first: the voxel-space counterpart of the reference's shadow-map test of
+//      every lit hit (indirect.frag:160-165), restated in light_visible below. Then the 2x2x2 box mip chain
 //      (no reference code; Crassin et al. 2011), children summed x fastest then y then
 //      z, times 1/8, each level rounded to its storage precision before the next.
 //  (3) orc_trace_* — front-to-back cone march with quadrilinear (trilinear x level)
@@ -357,9 +360,77 @@ template <typename T> static T gamma_lut_value(unsigned i) {
   return (T)std::pow((double)i / 255.0, 2.2);
 }
 
+// Shadow flag in orc_light.type; the low byte is the light's kind.
+static const uint32_t kLightShadow = 0x100u;
+
+// Visibility of light L from occupied voxel (i, j, k) of the level-0 grid (acc: dense
+// accumulator, occupied = nonzero count). Integers and single-precision IEEE operations
+// only, so the decision is the same at every arithmetic width of the radiance:
+//  * start: the voxel centre moved one voxel along the dominant axis of the voxel's
+//    integer normal sum ns (ties to the lowest axis; no move when ns is zero), in voxel
+//    units p0 = (i, j, k) + o + 0.5;
+//  * direction toward the light: -vec for a directional light; Pv - p0 for a point light,
+//    Pv = (vec - origin) / h in float (h = extent / n);
+//  * dominant axis a of the direction l (ties to the lowest axis); step s_x = l_x / |l_a|
+//    (so s_a = +-1), in 16.16 fixed point rint(s_x * 65536);
+//  * samples q_m = p0 + m s for m = 0, 1, ... in fixed point, each naming the voxel
+//    floor(q_m); the walk ends lit when a sample leaves the grid and, for a point light,
+//    when m reaches ceil(|l_a|) (the sample has reached the light along a); a sample
+//    naming the voxel itself is passed over; the first occupied one ends it unlit.
+// Every step moves exactly one voxel along a, so a walk takes at most n + 1 samples.
+static bool light_visible(const uint32_t* acc, uint32_t n, const float origin[3], float h,
+                          const orc_light& L, uint32_t i, uint32_t j, uint32_t k, const int32_t ns[3]) {
+  const int32_t v[3] = {(int32_t)i, (int32_t)j, (int32_t)k};
+  int32_t o[3] = {0, 0, 0};
+  {
+    const int64_t m0 = std::llabs((long long)ns[0]), m1 = std::llabs((long long)ns[1]),
+                  m2 = std::llabs((long long)ns[2]);
+    const int na = (m0 >= m1 && m0 >= m2) ? 0 : (m1 >= m2 ? 1 : 2);
+    o[na] = ns[na] > 0 ? 1 : (ns[na] < 0 ? -1 : 0);
+  }
+  float l[3];
+  for (int x = 0; x < 3; x++) {
+    if ((L.type & 0xffu) == 0) {
+      l[x] = -L.vec[x];
+    } else {
+      const float pv = (L.vec[x] - origin[x]) / h;
+      l[x] = pv - ((float)(v[x] + o[x]) + 0.5f);
+    }
+  }
+  const float m0 = std::fabs(l[0]), m1 = std::fabs(l[1]), m2 = std::fabs(l[2]);
+  const int a = (m0 >= m1 && m0 >= m2) ? 0 : (m1 >= m2 ? 1 : 2);
+  const float la = std::fabs(l[a]);
+  if (!(la > 0.0f)) return true;
+  int32_t s[3];
+  for (int x = 0; x < 3; x++) s[x] = (int32_t)std::nearbyint((l[x] / la) * 65536.0f);
+  int64_t steps = (int64_t)n + 2;  // leaves the grid before this
+  if ((L.type & 0xffu) != 0) {
+    const float c = std::ceil(la);
+    if (c < (float)steps) steps = (int64_t)c;
+  }
+  int32_t q[3];
+  for (int x = 0; x < 3; x++) q[x] = (v[x] + o[x]) * 65536 + 32768;
+  for (int64_t m = 0; m < steps; m++) {
+    int32_t c[3];
+    bool out = false;
+    for (int x = 0; x < 3; x++) {
+      c[x] = q[x] >> 16;  // floor (arithmetic shift)
+      out = out || c[x] < 0 || c[x] >= (int32_t)n;
+    }
+    if (out) return true;
+    if (!(c[0] == v[0] && c[1] == v[1] && c[2] == v[2])) {
+      const size_t lin = (size_t)c[0] + (size_t)n * ((size_t)c[1] + (size_t)n * (size_t)c[2]);
+      if (acc[lin * 8 + 3] != 0) return false;
+    }
+    for (int x = 0; x < 3; x++) q[x] += s[x];
+  }
+  return true;
+}
+
 template <typename T>
 static void inject_voxel(const uint32_t* v, uint32_t i, uint32_t j, uint32_t k, const T origin[3],
-                         T h, const orc_light* lights, uint32_t nl, const T* lut, T out[4]) {
+                         T h, const orc_light* lights, uint32_t nl, const T* lut, T out[4],
+                         const uint32_t* acc, uint32_t n, const float originf[3], float hf) {
   uint32_t c = v[3];
   if (c == 0) { out[0] = out[1] = out[2] = out[3] = (T)0; return; }
   T lin[3];
@@ -370,12 +441,15 @@ static void inject_voxel(const uint32_t* v, uint32_t i, uint32_t j, uint32_t k, 
   T pos[3];
   uint32_t idx[3] = {i, j, k};
   for (int a = 0; a < 3; a++) pos[a] = origin[a] + ((T)idx[a] + (T)0.5) * h;
+  int32_t ns[3];
+  for (int a = 0; a < 3; a++) ns[a] = (int32_t)v[4 + a] - (int32_t)(1024u * c);
   T E[3] = {0, 0, 0};
   for (uint32_t l = 0; l < nl; l++) {
     const orc_light& L = lights[l];
-    if (L.type == 0) {
+    const bool shadow = (L.type & kLightShadow) != 0;
+    if ((L.type & 0xffu) == 0) {
       T cosv = ((nrm[0] * -(T)L.vec[0] + nrm[1] * -(T)L.vec[1]) + nrm[2] * -(T)L.vec[2]);
-      if (cosv > (T)0)
+      if (cosv > (T)0 && (!shadow || light_visible(acc, n, originf, hf, L, i, j, k, ns)))
         for (int ch = 0; ch < 3; ch++) E[ch] = E[ch] + (T)L.color[ch] * cosv;
     } else {
       T d[3];
@@ -383,7 +457,7 @@ static void inject_voxel(const uint32_t* v, uint32_t i, uint32_t j, uint32_t k, 
       T d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
       if (d2 > (T)0) {
         T cosv = ((nrm[0] * d[0] + nrm[1] * d[1]) + nrm[2] * d[2]) / std::sqrt(d2);
-        if (cosv > (T)0) {
+        if (cosv > (T)0 && (!shadow || light_visible(acc, n, originf, hf, L, i, j, k, ns))) {
           T md = (T)L.min_distance;
           T att = std::max(d2, md * md);
           T f = cosv / att;
@@ -405,6 +479,7 @@ static int inject_mips(const uint32_t* acc, const float* originf, float extentf,
   for (unsigned i = 0; i < 256; i++) lut[i] = gamma_lut_value<T>(i);
   T origin[3] = {(T)originf[0], (T)originf[1], (T)originf[2]};
   T h = (T)extentf / (T)n;
+  const float hf = extentf / (float)n;  // the visibility walk's voxel size, single precision
   // level 0
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < (int64_t)n; k++)
@@ -412,7 +487,7 @@ static int inject_mips(const uint32_t* acc, const float* originf, float extentf,
       for (uint32_t i = 0; i < n; i++) {
         size_t lin = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * (size_t)k);
         T t[4];
-        inject_voxel<T>(acc + lin * 8, i, j, (uint32_t)k, origin, h, lights, nl, lut, t);
+        inject_voxel<T>(acc + lin * 8, i, j, (uint32_t)k, origin, h, lights, nl, lut, t, acc, n, originf, hf);
         for (int c = 0; c < 4; c++) out[lin * 4 + c] = store_round<T>(t[c], prec, c == 3);
       }
   // levels 1..L-1
@@ -437,6 +512,31 @@ static int inject_mips(const uint32_t* acc, const float* originf, float extentf,
         }
     src = dst;
   }
+  return 0;
+}
+
+// Visibility walks (light_visible) of up to 32 lights from every occupied voxel: bit l of
+// out[voxel] is set when light l is visible from it (whatever its shadow flag); 0 for
+// unoccupied voxels. For tests of the walk itself.
+OAPI int orc_light_visibility(const uint32_t* acc, const float* origin, float extent, uint32_t n,
+                              const orc_light* lights, uint32_t nl, uint32_t* out) {
+  if (nl > 32 || n == 0) return -1;
+  const float hf = extent / (float)n;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)n; k++)
+    for (uint32_t j = 0; j < n; j++)
+      for (uint32_t i = 0; i < n; i++) {
+        const size_t lin = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * (size_t)k);
+        const uint32_t* v = acc + lin * 8;
+        uint32_t bits = 0;
+        if (v[3] != 0) {
+          int32_t ns[3];
+          for (int a = 0; a < 3; a++) ns[a] = (int32_t)v[4 + a] - (int32_t)(1024u * v[3]);
+          for (uint32_t l = 0; l < nl; l++)
+            if (light_visible(acc, n, origin, hf, lights[l], i, j, (uint32_t)k, ns)) bits |= 1u << l;
+        }
+        out[lin] = bits;
+      }
   return 0;
 }
 

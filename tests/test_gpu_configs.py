@@ -123,6 +123,28 @@ def test_room_1080p_voxels_and_levels(ctx, room_1080p):
         r.release()
 
 
+def test_room_1080p_shadowed_levels(ctx, room_1080p):
+    """The headline room with its four point lights casting shadows (visibility walks through
+    the 256^3 occupancy): FP16 levels bit-exact vs the oracle."""
+    fx = room_1080p
+    w = fx["w"]
+    lights = [dict(l, shadow=True) for l in w.lights]
+    print(f"case {w.name}: shadowed point lights, fp16 levels", flush=True)
+    vgi = _vgi()
+    r = Run(ctx, w)
+    try:
+        r.scene.build_mips(vgi.make_lights(lights), vgi.PREC_FP16)
+        ref = orc.inject_mips(fx["acc"], w.grid, lights, "fp16")
+        for k in range(w.grid["levels"]):
+            got = r.scene.read_level(k)
+            bad = (got != ref[k]).any(axis=-1)
+            assert not bad.any(), (f"level {k}: {bad.sum()} texels differ, first at "
+                                   f"{np.argwhere(bad)[:3].tolist()}")
+            del got
+    finally:
+        r.release()
+
+
 def test_room_1080p_trace_vs_fp64(ctx, room_1080p):
     fx = room_1080p
     w, sub = fx["w"], fx["sub"]
