@@ -184,19 +184,6 @@ __device__ __forceinline__ uint32_t group_occupancy(const MipScene& P, uint32_t 
   return pairs != 0 ? bits : 0u;  // a select, not a branch, so the mask loads stay with the first
 }
 
-// Occupancy of voxel (x, y, z), all three < n, with the bin entry and the mask word loaded
-// together (every brick's mask words exist; those of an empty bin are discarded).
-__device__ __forceinline__ bool occupied_any(const MipScene& P, uint32_t x, uint32_t y, uint32_t z) {
-  const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
-  const uint32_t bin = (x >> bs) + nb * ((y >> bs) + nb * (z >> bs));
-  const uint32_t nbr = P.n >> 3;
-  const uint32_t brick = (x >> 3) + nbr * ((y >> 3) + nbr * (z >> 3));
-  const uint32_t local = (x & 7u) + 8u * ((y & 7u) + 8u * (z & 7u));
-  const uint32_t pairs = (uint32_t)gld(P.bin_packed + bin);
-  const uint32_t word = gld(P.mask + (size_t)brick * 16u + (local >> 5));
-  return pairs != 0u && ((word >> (local & 31u)) & 1u) != 0u;
-}
-
 __device__ __forceinline__ int32_t sign_i32(int32_t v) { return (v > 0) - (v < 0); }
 __device__ __forceinline__ uint32_t abs_u32(int32_t v) { return v < 0 ? 0u - (uint32_t)v : (uint32_t)v; }
 
@@ -236,11 +223,24 @@ __device__ bool light_visible(const MipScene& P, const LightDev& L, uint32_t x, 
     if (c < (float)steps) steps = (uint32_t)c;
   }
   int32_t q0 = p0 * 65536 + 32768, q1 = p1 * 65536 + 32768, q2 = p2 * 65536 + 32768;
+  const uint32_t bs = P.bin_shift, nb = P.nbins_axis, nbr = P.n >> 3;
+  // One sample per iteration: issuing the loads of 2, 4 or 8 samples together before
+  // testing the first measured no faster (0.27, 0.31, 0.43 ms against 0.27 ms for the
+  // room's four shadowed lights), the extra samples' work being wasted when a walk ends.
   for (uint32_t m = 0; m < steps; m++) {
     const int32_t c0 = q0 >> 16, c1 = q1 >> 16, c2 = q2 >> 16;  // floor
     if ((uint32_t)c0 >= P.n || (uint32_t)c1 >= P.n || (uint32_t)c2 >= P.n) return true;
-    const bool self = c0 == (int32_t)x && c1 == (int32_t)y && c2 == (int32_t)z;
-    if (!self && occupied_any(P, (uint32_t)c0, (uint32_t)c1, (uint32_t)c2)) return false;
+    const uint32_t u0 = (uint32_t)c0, u1 = (uint32_t)c1, u2 = (uint32_t)c2;
+    const uint32_t bin = (u0 >> bs) + nb * ((u1 >> bs) + nb * (u2 >> bs));
+    const uint32_t brick = (u0 >> 3) + nbr * ((u1 >> 3) + nbr * (u2 >> 3));
+    const uint32_t local = (u0 & 7u) + 8u * ((u1 & 7u) + 8u * (u2 & 7u));
+    // both loads in flight together (the mask word of an empty bin is stale and unused);
+    // byte offsets below 2^32 (bins * 8 B, bricks * 64 B for n <= 1024), a bin entry's pair
+    // count is its low word
+    const uint32_t pairs = gld_at((const uint32_t*)P.bin_packed, bin * 8u);
+    const uint32_t word = gld_at(P.mask, brick * 64u + ((local >> 5) << 2));
+    const bool self = u0 == x && u1 == y && u2 == z;
+    if (!self && pairs != 0u && ((word >> (local & 31u)) & 1u) != 0u) return false;
     q0 += s0;
     q1 += s1;
     q2 += s2;
@@ -272,33 +272,28 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
   float er = 0.f, eg = 0.f, eb = 0.f;
   for (uint32_t l = 0; l < P.nlights; l++) {
     const LightDev& L = P.lights[l];
+    float f = 0.f;  // the light's factor: Lambert term, with the falloff for a point light
+    bool lit;
     if (L.type == VGI_LIGHT_DIRECTIONAL) {
-      const float cosv = (nx * -L.vec[0] + ny * -L.vec[1]) + nz * -L.vec[2];
-      bool lit = cosv > 0.f;
-      if constexpr (SHADOW)
-        if (lit && L.shadow) lit = light_visible(P, L, x, y, z, (int32_t)s1.x - bias, (int32_t)s1.y - bias, (int32_t)s1.z - bias);
-      if (lit) {
-        er = er + L.color[0] * cosv;
-        eg = eg + L.color[1] * cosv;
-        eb = eb + L.color[2] * cosv;
-      }
+      f = (nx * -L.vec[0] + ny * -L.vec[1]) + nz * -L.vec[2];
+      lit = f > 0.f;
     } else {
       const float dx = L.vec[0] - px, dy = L.vec[1] - py, dz = L.vec[2] - pz;
       const float d2 = (dx * dx + dy * dy) + dz * dz;
+      lit = false;
       if (d2 > 0.f) {
         const float cosv = ((nx * dx + ny * dy) + nz * dz) / __builtin_sqrtf(d2);
-        bool lit = cosv > 0.f;
-        if constexpr (SHADOW)
-          if (lit && L.shadow) lit = light_visible(P, L, x, y, z, (int32_t)s1.x - bias, (int32_t)s1.y - bias, (int32_t)s1.z - bias);
-        if (lit) {
-          const float md = L.min_distance;
-          const float att = fmaxf(d2, md * md);
-          const float f = cosv / att;
-          er = er + L.color[0] * f;
-          eg = eg + L.color[1] * f;
-          eb = eb + L.color[2] * f;
-        }
+        lit = cosv > 0.f;
+        const float md = L.min_distance;
+        f = cosv / fmaxf(d2, md * md);
       }
+    }
+    if constexpr (SHADOW)  // one walk site for both kinds of light
+      if (lit && L.shadow) lit = light_visible(P, L, x, y, z, (int32_t)s1.x - bias, (int32_t)s1.y - bias, (int32_t)s1.z - bias);
+    if (lit) {
+      er = er + L.color[0] * f;
+      eg = eg + L.color[1] * f;
+      eb = eb + L.color[2] * f;
     }
   }
   return make_float4(lr * er, lg * eg, lb * eb, 1.0f);

@@ -256,6 +256,12 @@ template <class T>
 __device__ __forceinline__ T gld(const T* p) {  // scalar types
   return *(const VGI_GLOBAL T*)p;
 }
+// base + a 32-bit byte offset: with a wave-uniform base the load takes the base in SGPRs
+// and one VGPR of offset (global_load ... v_off, s[base]) instead of a 64-bit address
+template <class T>
+__device__ __forceinline__ T gld_at(const T* base, uint32_t byte_off) {  // scalar types
+  return *(const VGI_GLOBAL T*)((const VGI_GLOBAL char*)base + byte_off);
+}
 template <class T>
 __device__ __forceinline__ void gst(T* p, T v) {  // scalar types
   *(VGI_GLOBAL T*)p = v;
