@@ -143,6 +143,8 @@ struct vgi_workspace {
   std::vector<uint8_t> staging;     // candidate descriptor bytes for the reuse check
   uint32_t next = 0;
   void* tris = nullptr;       // TriRec per triangle
+  void* tritex = nullptr;     // TriTex per triangle (batches with a textured mesh)
+  size_t tritex_bytes = 0;
   size_t tris_bytes = 0;
   void* tri_slot = nullptr;   // kSmallBins words per triangle
   size_t tri_slot_bytes = 0;
@@ -177,7 +179,7 @@ struct vgi_workspace {
   uint64_t hist_pairs = 0, hist_units = 0, hist_big = 0, hist_split = 0;  // largest seen
 
   size_t device_bytes() const {
-    size_t b = tris_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
+    size_t b = tris_bytes + tritex_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
                scan_bytes + work_bytes + pairs_bytes + scan_tmp_bytes + sort_keys_bytes + sort_pix_bytes +
                sort_tmp_bytes;
     for (const Slot& s : slot) b += s.cap;
@@ -240,8 +242,8 @@ size_t scene_device_bytes(const vgi_scene* s) {
 }
 
 void free_workspace(vgi_workspace* w) {
-  void* dev[] = {w->tris, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all, w->scan_all,
-                 w->work_map, w->pairs, w->scan_tmp, w->sort_keys, w->sort_pix, w->sort_tmp};
+  void* dev[] = {w->tris, w->tritex, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all,
+                 w->scan_all, w->work_map, w->pairs, w->scan_tmp, w->sort_keys, w->sort_pix, w->sort_tmp};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (Slot& s : w->slot) {
@@ -356,6 +358,34 @@ int check_grid(const vgi_grid* g) {
   return VGI_OK;
 }
 
+bool mesh_textured(const vgi_mesh& m) { return m.texcoords != nullptr && m.texture_rgba8 != nullptr; }
+
+// The device form of a mesh; tri_base = its first triangle within its scene.
+MeshDev mesh_dev(const vgi_mesh& m, uint32_t tri_base) {
+  MeshDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.pos = m.positions;
+  d.idx = m.indices;
+  d.albedo = m.albedo_rgb8;
+  d.stride = m.position_stride ? m.position_stride : 12u;
+  d.nverts = m.num_vertices;
+  d.idx_type = m.index_type;
+  d.ntri = m.num_triangles;
+  d.tri_base = tri_base;
+  d.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) | ((uint32_t)m.base_color[2] << 16);
+  std::memcpy(d.m, m.model, sizeof(d.m));
+  if (mesh_textured(m)) {
+    d.uv = m.texcoords;
+    d.tex = (const uint32_t*)m.texture_rgba8;
+    d.uv_stride = m.texcoord_stride ? m.texcoord_stride : 8u;
+    d.tw = m.texture_width;
+    d.th = m.texture_height;
+    d.textured = 1;
+    std::memcpy(d.factor, m.base_color_factor, sizeof(d.factor));
+  }
+  return d;
+}
+
 int check_meshes(const vgi_mesh* meshes, uint32_t num_meshes, uint64_t* ntri_out) {
   if (num_meshes > 0 && !meshes) return fail(VGI_E_INVALID, "meshes is NULL");
   uint64_t ntri = 0;
@@ -371,6 +401,18 @@ int check_meshes(const vgi_mesh* meshes, uint32_t num_meshes, uint64_t* ntri_out
       return fail(VGI_E_INVALID, "mesh %u: non-indexed mesh needs 3 vertices per triangle", i);
     for (int k = 0; k < 12; k++)
       if (!std::isfinite(m.model[k])) return fail(VGI_E_INVALID, "mesh %u: model matrix not finite", i);
+    if (mesh_textured(m)) {
+      const uint32_t us = m.texcoord_stride ? m.texcoord_stride : 8u;
+      if (us < 8 || (us & 3)) return fail(VGI_E_INVALID, "mesh %u: texcoord_stride %u", i, us);
+      if (m.texture_width == 0 || m.texture_height == 0 || m.texture_width > 16384 || m.texture_height > 16384)
+        return fail(VGI_E_INVALID, "mesh %u: texture %ux%u (1..16384 per side)", i, m.texture_width,
+                    m.texture_height);
+      if (((uintptr_t)m.texture_rgba8 & 3) || ((uintptr_t)m.texcoords & 3))
+        return fail(VGI_E_INVALID, "mesh %u: texture and texcoords must be 4-byte aligned", i);
+      for (int k = 0; k < 4; k++)
+        if (!std::isfinite(m.base_color_factor[k]))
+          return fail(VGI_E_INVALID, "mesh %u: base_color_factor not finite", i);
+    }
     ntri += m.num_triangles;
   }
   if (ntri >= (1ull << 31)) return fail(VGI_E_RANGE, "%llu triangles exceed 2^31", (unsigned long long)ntri);
@@ -500,6 +542,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   std::vector<VoxPlan> plan(count);
   std::vector<const vgi_scene*> seen(count);
   size_t total_bins = 0, total_tri = 0, total_mesh = 0, nblk_t = 0, nblk_b = 0;
+  bool textured = false;  // some mesh is textured: per-triangle texcoord records, sampling kernels
   uint32_t ts = 16;
   for (uint32_t i = 0; i < count; i++) {
     const vgi_voxelize_item& it = items[i];
@@ -522,6 +565,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     total_bins += p.nbins;
     total_tri += p.ntri;
     total_mesh += it.num_meshes;
+    for (uint32_t k = 0; k < it.num_meshes; k++)
+      textured = textured || (it.meshes[k].num_triangles > 0 && mesh_textured(it.meshes[k]));
     nblk_t += p.tri_blocks;
     nblk_b += p.bin_blocks;
     ts = std::min(ts, 1u << p.bin_shift);
@@ -583,12 +628,14 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(total_bins + 1), st));
   const size_t tneed = std::max<size_t>(total_tri, 1);
   const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
+                    (textured && w->tritex_bytes < tneed * sizeof(TriTex)) ||
                     w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need ||
                     w->pairs_bytes < (size_t)pair_cap * 4 || w->work_bytes < (size_t)work_cap * sizeof(uint2);
   if (grow) HIP_TRY(hipStreamSynchronize(st));
   TRY(grow_bytes(&w->pairs, &w->pairs_bytes, (size_t)pair_cap * 4, "bin pairs"));
   TRY(grow_bytes(&w->work_map, &w->work_bytes, (size_t)work_cap * sizeof(uint2), "work map"));
   TRY(grow_bytes(&w->tris, &w->tris_bytes, tneed * sizeof(TriRec), "triangle records"));
+  if (textured) TRY(grow_bytes(&w->tritex, &w->tritex_bytes, tneed * sizeof(TriTex), "texcoord records"));
   TRY(grow_bytes(&w->tri_slot, &w->tri_slot_bytes, tneed * kSmallBins * 4, "triangle slots"));
   TRY(grow_bytes(&w->big_list, &w->big_list_bytes, tneed * 4, "large-triangle list"));
   TRY(grow_bytes(&w->zeroed, &w->zeroed_bytes, zeroed_need, "bin counters"));
@@ -623,25 +670,14 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     const vgi_voxelize_item& it = *p.it;
     uint32_t base = 0;
     for (uint32_t k = 0; k < it.num_meshes; k++) {
-      const vgi_mesh& m = it.meshes[k];
-      MeshDev& d = hm[p.mesh_off + k];
-      std::memset(&d, 0, sizeof(d));
-      d.pos = m.positions;
-      d.idx = m.indices;
-      d.albedo = m.albedo_rgb8;
-      d.stride = m.position_stride ? m.position_stride : 12u;
-      d.nverts = m.num_vertices;
-      d.idx_type = m.index_type;
-      d.ntri = m.num_triangles;
-      d.tri_base = base;
-      d.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) | ((uint32_t)m.base_color[2] << 16);
-      std::memcpy(d.m, m.model, sizeof(d.m));
-      base += m.num_triangles;
+      hm[p.mesh_off + k] = mesh_dev(it.meshes[k], base);
+      base += it.meshes[k].num_triangles;
     }
     VoxScene& S = hs[i];
     std::memset(&S, 0, sizeof(S));
     S.meshes = (const MeshDev*)(d + o_meshes) + p.mesh_off;
     S.tris = (TriRec*)w->tris + p.tri_off;
+    S.tritex = textured ? (TriTex*)w->tritex + p.tri_off : nullptr;
     S.tri_slot = (uint32_t*)w->tri_slot + p.tri_off * kSmallBins;
     S.big_list = (uint32_t*)w->big_list + p.tri_off;
     S.bin_small = zb + p.bin_base;
@@ -687,6 +723,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.seq = w->seq;
   B.nscenes = count;
   B.total_bins = (uint32_t)total_bins;
+  B.textured = textured ? 1u : 0u;
 
   std::vector<vgi_scene*> scs(count);
   for (uint32_t i = 0; i < count; i++) scs[i] = plan[i].s;
@@ -1442,7 +1479,11 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   const size_t o_desc = cv.take(sizeof(NoAllocDesc));
   const size_t o_hist = cv.take(4 * sizeof(unsigned long long));
   const size_t o_zero = cv.take(zeroed_bytes);
+  bool textured = false;
+  for (uint32_t k = 0; k < num_meshes; k++)
+    textured = textured || (meshes[k].num_triangles > 0 && mesh_textured(meshes[k]));
   const size_t o_tris = cv.take(std::max<uint64_t>(ntri, 1) * sizeof(TriRec));
+  const size_t o_tritex = cv.take(textured ? ntri * sizeof(TriTex) : 0);
   const size_t o_slot = cv.take(std::max<uint64_t>(ntri, 1) * kSmallBins * 4);
   const size_t o_big = cv.take(std::max<uint64_t>(ntri, 1) * 4);
   const size_t o_zlist = cv.take(nbins * 4);
@@ -1467,24 +1508,14 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   NoAllocDesc* dd = (NoAllocDesc*)(base + o_desc);
   uint32_t tbase = 0;
   for (uint32_t k = 0; k < num_meshes; k++) {
-    const vgi_mesh& m = meshes[k];
-    MeshDev& md = d.meshes[k];
-    md.pos = m.positions;
-    md.idx = m.indices;
-    md.albedo = m.albedo_rgb8;
-    md.stride = m.position_stride ? m.position_stride : 12u;
-    md.nverts = m.num_vertices;
-    md.idx_type = m.index_type;
-    md.ntri = m.num_triangles;
-    md.tri_base = tbase;
-    md.base_rgb = (uint32_t)m.base_color[0] | ((uint32_t)m.base_color[1] << 8) | ((uint32_t)m.base_color[2] << 16);
-    std::memcpy(md.m, m.model, sizeof(md.m));
-    tbase += m.num_triangles;
+    d.meshes[k] = mesh_dev(meshes[k], tbase);
+    tbase += meshes[k].num_triangles;
   }
   uint32_t* zb = (uint32_t*)(base + o_zero);
   VoxScene& S = d.scene;
   S.meshes = dd->meshes;
   S.tris = (TriRec*)(base + o_tris);
+  S.tritex = textured ? (TriTex*)(base + o_tritex) : nullptr;
   S.tri_slot = (uint32_t*)(base + o_slot);
   S.big_list = (uint32_t*)(base + o_big);
   S.bin_small = zb;
@@ -1524,6 +1555,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.seq = sc->vox_seq + 1u ? sc->vox_seq + 1u : 1u;
   B.nscenes = 1;
   B.total_bins = (uint32_t)nbins;
+  B.textured = textured ? 1u : 0u;
   const uint32_t grid_blocks = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, work_cap64);
   const uint32_t cus = (uint32_t)ctx->num_cus;
   TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_bytes, base + o_stmp, scan_need, ceil_div(ntri, kBlockTris),

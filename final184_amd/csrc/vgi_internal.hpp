@@ -34,7 +34,22 @@ struct MeshDev {
   uint32_t tri_base;  // first triangle of this mesh within its scene
   uint32_t base_rgb;  // r | g << 8 | b << 16
   float m[12];
+  const float* uv;      // textured meshes: uv per vertex (uv_stride bytes apart), else null
+  const uint32_t* tex;  // RGBA8 texels, one word each, row-major tw x th
+  uint32_t uv_stride;
+  uint32_t tw, th;
+  uint32_t textured;    // 1: albedo from the texture (uv and tex set)
+  float factor[4];      // RGBA multiplier of the sample
 };
+static_assert(sizeof(MeshDev) == 144, "MeshDev layout");
+
+// Texture coordinates of a triangle of a textured batch (written by the setup kernel).
+struct __align__(16) TriTex {
+  float uv[6];        // uv of vertices 0, 1, 2
+  uint32_t mesh;      // index of the triangle's mesh in its scene
+  uint32_t textured;  // 1: its mesh is textured
+};
+static_assert(sizeof(TriTex) == 32, "TriTex layout");
 
 // Triangle record written by the setup kernel (64 bytes).
 struct __align__(16) TriRec {
@@ -75,6 +90,7 @@ static_assert(sizeof(SceneVoxResult) == 64, "SceneVoxResult layout");
 struct VoxScene {
   const MeshDev* meshes;
   TriRec* tris;
+  TriTex* tritex;                  // per triangle when the batch has a textured mesh, else null
   uint32_t* tri_slot;              // kSmallBins slots per triangle (small triangles)
   uint32_t* big_list;              // ids of large triangles
   // per-bin counters (nbins each), bin b at a hashed index (voxelize.hip: ctr)
@@ -122,6 +138,7 @@ struct VoxBatch {
   uint32_t seq;                     // this call's number (never 0)
   uint32_t nscenes;
   uint32_t total_bins;
+  uint32_t textured;                // some mesh of the batch is textured (scenes carry tritex)
 };
 
 struct LightDev {

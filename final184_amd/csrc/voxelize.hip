@@ -277,6 +277,21 @@ __device__ void make_record(const VoxScene& P, uint32_t t, TriRec& R) {
       else qi[i][r] = (int32_t)__builtin_rintf(v);
     }
   }
+  if (P.tritex) {  // a textured batch: the triangle's texcoords (zero when untextured)
+    TriTex T{};
+    T.mesh = mi;
+    T.textured = M.textured;
+    if (M.textured) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        if (vid[i] >= M.nverts) continue;  // rejected (status 1): never sampled
+        const float* u = (const float*)((const char*)M.uv + (size_t)vid[i] * M.uv_stride);
+        T.uv[2 * i] = gld(u);
+        T.uv[2 * i + 1] = gld(u + 1);
+      }
+    }
+    gst_rec(P.tritex + t, T);
+  }
 #pragma unroll
   for (int i = 0; i < 9; i++) R.q[i] = qi[i / 3][i % 3];
   R.nq[0] = R.nq[1] = R.nq[2] = 0;
@@ -691,6 +706,98 @@ hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, u
 // 7 brick voxelize
 // ---------------------------------------------------------------------------
 
+// A textured triangle's texture inputs, beside its coefficients in LDS.
+struct TexCoef {
+  float uv[6];
+  const uint32_t* tex;
+  uint32_t tw, th;
+  float factor[4];
+  uint32_t textured;
+  uint32_t pad;
+};
+
+// Albedo of a covered voxel from a textured triangle (BasicMaterial's texture path,
+// Pipelang/Internal/main.lua:188-205). e0, e1, e2: the barycentric numerators of vertices
+// 0, 1, 2 at the voxel centre projected along the dominant axis (exact integers, >= 0
+// inside the triangle); negative ones are clamped to zero. Single-precision steps in a
+// fixed order (no contraction): uv = ((w0 uv0 + w1 uv1) + w2 uv2) / (w0 + w1 + w2), a
+// non-finite uv taken as (0, 0); wrap addressing; bilinear between texel centres at level
+// 0 (x = u W - 1/2), texel values c * (1/255), a + t (b - a) along x, then y; times the
+// factor. Returns false for a discarded fragment (alpha < 0.05), else RGB8 in rgb
+// (r | g << 8 | b << 16), each (uint)(clamp(c, 0, 1) * 255 + 0.5).
+__device__ __forceinline__ bool tex_albedo(const TexCoef& T, int64_t e0, int64_t e1, int64_t e2, uint32_t& rgb) {
+  e0 = imax64(e0, 0);
+  e1 = imax64(e1, 0);
+  e2 = imax64(e2, 0);
+  const float w0 = (float)e0, w1 = (float)e1, w2 = (float)e2, ws = (float)(e0 + e1 + e2);
+  float tu = ((w0 * T.uv[0] + w1 * T.uv[2]) + w2 * T.uv[4]) / ws;
+  float tv = ((w0 * T.uv[1] + w1 * T.uv[3]) + w2 * T.uv[5]) / ws;
+  if (!__builtin_isfinite(tu) || !__builtin_isfinite(tv)) tu = tv = 0.0f;
+  const float fu = tu - floorf(tu), fv = tv - floorf(tv);
+  const float x = fu * (float)T.tw - 0.5f, y = fv * (float)T.th - 0.5f;
+  const float x0 = floorf(x), y0 = floorf(y);
+  const float gx = x - x0, gy = y - y0;
+  int ix0 = (int)x0, iy0 = (int)y0, ix1 = ix0 + 1, iy1 = iy0 + 1;  // in [-1, W], [-1, H]
+  if (ix0 < 0) ix0 += (int)T.tw;
+  if (iy0 < 0) iy0 += (int)T.th;
+  if (ix1 >= (int)T.tw) ix1 -= (int)T.tw;
+  if (iy1 >= (int)T.th) iy1 -= (int)T.th;
+  const uint32_t r0 = (uint32_t)iy0 * T.tw, r1 = (uint32_t)iy1 * T.tw;  // < 2^28 texels
+  const uint32_t t00 = gld(T.tex + r0 + (uint32_t)ix0), t10 = gld(T.tex + r0 + (uint32_t)ix1);
+  const uint32_t t01 = gld(T.tex + r1 + (uint32_t)ix0), t11 = gld(T.tex + r1 + (uint32_t)ix1);
+  const float k255 = 1.0f / 255.0f;
+  float col[4];
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) {
+    const float a = (float)((t00 >> (8 * ch)) & 0xFFu) * k255, b = (float)((t10 >> (8 * ch)) & 0xFFu) * k255;
+    const float c = (float)((t01 >> (8 * ch)) & 0xFFu) * k255, d = (float)((t11 >> (8 * ch)) & 0xFFu) * k255;
+    const float top = a + gx * (b - a), bot = c + gx * (d - c);
+    col[ch] = (top + gy * (bot - top)) * T.factor[ch];
+  }
+  if (col[3] < 0.05f) return false;
+  uint32_t out = 0;
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) out |= (uint32_t)(fminf(fmaxf(col[ch], 0.0f), 1.0f) * 255.0f + 0.5f) << (8 * ch);
+  rgb = out;
+  return true;
+}
+
+__device__ __forceinline__ void load_texcoef(const VoxScene& P, uint32_t t, TexCoef& C) {
+  const TriTex X = gld_rec(P.tritex + t);
+  C.textured = X.textured;
+  if (!X.textured) return;
+#pragma unroll
+  for (int i = 0; i < 6; i++) C.uv[i] = X.uv[i];
+  const MeshDev* M = P.meshes + X.mesh;
+  C.tex = (const uint32_t*)gld((const unsigned long long*)&M->tex);
+  C.tw = gld(&M->tw);
+  C.th = gld(&M->th);
+#pragma unroll
+  for (int i = 0; i < 4; i++) C.factor[i] = gld(&M->factor[i]);
+}
+
+// The same numerators from the snapped vertices (the per-triangle path):
+// sign(N_u) * cross(q_j - q_i, c - q_i) for the edge (i, j) opposite vertex k.
+__device__ __forceinline__ void voxel_bary(const int64_t (&q)[3][3], const int64_t (&N)[3], int u, int x, int y,
+                                           int z, int64_t (&e)[3]) {
+  const int A = (u + 1) % 3, B = (u + 2) % 3;
+  const int v[3] = {x, y, z};
+  const int64_t cA = (int64_t)v[A] * kFixedOne + kFixedOne / 2, cB = (int64_t)v[B] * kFixedOne + kFixedOne / 2;
+  const int64_t sg = N[u] < 0 ? -1 : 1;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int i = (k + 1) % 3, j = (k + 2) % 3;
+    const int64_t dA = q[j][A] - q[i][A], dB = q[j][B] - q[i][B];
+    const int64_t pA = cA - q[i][A], pB = cB - q[i][B];
+    e[k] = sg * (dA * pB - dB * pA);
+  }
+}
+
+__device__ __forceinline__ unsigned long long inc0_of(uint32_t rgb) {
+  return (unsigned long long)(rgb & 0xFFu) | ((unsigned long long)((rgb >> 8) & 0xFFu) << 16) |
+         ((unsigned long long)((rgb >> 16) & 0xFFu) << 32) | (1ull << 48);
+}
+
 // Per-triangle coefficients in LDS. Edge function of projection u (up axis), edge i:
 //   E = eA * v_A + eB * v_B + eC >= 0   (v = integer voxel coordinates)
 // plane: s = N256 . v;  overlap iff s + pmin <= 0 <= s + pmax.
@@ -706,6 +813,22 @@ struct TriCoef {
   uint32_t axis;
   unsigned long long inc0, inc1;
 };
+
+// Barycentric numerators of vertices 0, 1, 2 at the centre of column (va, vb) of the
+// triangle's dominant projection u: edge i (vertex i to i + 1, opposite vertex i + 2)
+// evaluated without the box-corner offset of the coverage test,
+// eA va + eB vb + eC - max(0, eA) - max(0, eB) + (eA + eB) / 2 (eA, eB are even).
+__device__ __forceinline__ void column_bary(const TriCoef& C, int u, int va, int vb, int64_t (&e)[3]) {
+  int64_t f[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int64_t ea = C.eA[u * 3 + i], eb = C.eB[u * 3 + i];
+    f[i] = ea * va + eb * vb + C.eC[u * 3 + i] - imax64(0, ea) - imax64(0, eb) + ((ea + eb) >> 1);
+  }
+  e[0] = f[1];
+  e[1] = f[2];
+  e[2] = f[0];
+}
 
 __device__ __forceinline__ int sel3(int a, int b, int c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
 
@@ -835,8 +958,11 @@ struct TileLayout {
 // spills one 64-bit value per thread to scratch, reloaded at the flush. The kernel is
 // latency-bound (dependent loads and barriers per chunk), so resident workgroups count:
 // 6 instead of 5 took the room's launch from 80 to 75 us.
-template <int TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 ? 6 : 1)))
+// TEX: the batch has a textured mesh; each triangle's texture inputs sit beside its
+// coefficients and a textured triangle samples its albedo per column (kept apart so that
+// untextured batches run the kernel without the sampler's registers and LDS).
+template <int TS, bool TEX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS == 8 && !TEX ? 6 : 1)))
 void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   constexpr int TV = TS * TS * TS;
   constexpr int NBK = TS / 8;  // 8^3 accumulator bricks per tile edge
@@ -844,6 +970,7 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   __shared__ unsigned long long s_acc0[L::ENTRIES];
   __shared__ unsigned long long s_acc1[L::ENTRIES];
   __shared__ TriCoef s_tri[kMaxChunk];
+  __shared__ TexCoef s_tex[TEX ? kMaxChunk : 1];
   __shared__ uint32_t s_pref[kMaxChunk + 1];
   static_assert(kMaxChunk == 64, "a chunk's triangles are the 64 lanes of each of the 4 wavefronts");
 
@@ -884,6 +1011,8 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
         const uint32_t t = gld(Bt.pairs_all + first + (tid & 63u));
         const TriRec R = gld_rec(P.tris + t);
         make_coef(R, s_tri[tid & 63u], tid >> 6);
+        if constexpr (TEX)
+          if ((tid >> 6) == 3u) load_texcoef(P, t, s_tex[tid & 63u]);
       }
 
       for (uint32_t st = 0; st < sub * sub * sub; st++) {
@@ -938,6 +1067,17 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
           const int va = C.clo[A] + (int)(local % (uint32_t)wa);
           const int vb = C.clo[Bx] + (int)(local / (uint32_t)wa);
           if (!edge_ok(C, u, va, vb)) continue;
+          unsigned long long inc0 = C.inc0;
+          if constexpr (TEX) {
+            const TexCoef& X = s_tex[lo];
+            if (X.textured) {  // one sample per column: every voxel of it projects to its centre
+              int64_t e[3];
+              column_bary(C, u, va, vb, e);
+              uint32_t rgb;
+              if (!tex_albedo(X, e[0], e[1], e[2], rgb)) continue;  // discarded
+              inc0 = inc0_of(rgb);
+            }
+          }
           const float zc = C.zq0 - C.ra * ((float)(va * kFixedOne + kFixedOne / 2) - C.q0a) -
                            C.rb * ((float)(vb * kFixedOne + kFixedOne / 2) - C.q0b);
           const float wd = 0.5f * (float)kFixedOne * (fabsf(C.ra) + fabsf(C.rb));
@@ -955,7 +1095,7 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
             if (!edge_ok(C, u1, sel3(vx, vy, vz, (u1 + 1) % 3), sel3(vx, vy, vz, (u1 + 2) % 3))) continue;
             if (!edge_ok(C, u2, sel3(vx, vy, vz, (u2 + 1) % 3), sel3(vx, vy, vz, (u2 + 2) % 3))) continue;
             const int pi = (vx - tx0) + L::PY * (vy - ty0) + L::PZ * (vz - tz0);
-            atomicAdd(&s_acc0[pi], C.inc0);
+            atomicAdd(&s_acc0[pi], inc0);
             atomicAdd(&s_acc1[pi], C.inc1);
           }
         }
@@ -1010,8 +1150,13 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, hipStream_t s) {
   if (grid_blocks == 0) return hipSuccess;
   static_assert(TileLayout<8>::ENTRIES == 582 && TileLayout<16>::ENTRIES == 4366, "tile layout");
-  if (tile == 8) hipLaunchKernelGGL(k_voxelize<8>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
-  else hipLaunchKernelGGL(k_voxelize<16>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  if (B.textured) {
+    if (tile == 8) hipLaunchKernelGGL((k_voxelize<8, true>), dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+    else hipLaunchKernelGGL((k_voxelize<16, true>), dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  } else {
+    if (tile == 8) hipLaunchKernelGGL((k_voxelize<8, false>), dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+    else hipLaunchKernelGGL((k_voxelize<16, false>), dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  }
   return hipGetLastError();
 }
 
@@ -1032,6 +1177,7 @@ hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_block
 // the same as the binned path's.
 // Not inlined into k_voxelize, whose register budget (6 workgroups per CU) it would
 // otherwise share.
+template <bool TEX>
 __global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const VoxScene* __restrict__ scenes) {
   if (gld(B.overflow) == 0u) return;
   const uint32_t lane = threadIdx.x & 63u;
@@ -1046,8 +1192,11 @@ __global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const Vox
       load_q(R, q, N);
       const int nx = R.hi[0] - R.lo[0] + 1, ny = R.hi[1] - R.lo[1] + 1, nz = R.hi[2] - R.lo[2] + 1;
       const uint32_t r = R.rgb & 0xFFu, g = (R.rgb >> 8) & 0xFFu, b = (R.rgb >> 16) & 0xFFu;
-      const unsigned long long i0 = (unsigned long long)r | ((unsigned long long)g << 32);
-      const unsigned long long i1 = (unsigned long long)b | (1ull << 32);
+      unsigned long long i0 = (unsigned long long)r | ((unsigned long long)g << 32);
+      unsigned long long i1 = (unsigned long long)b | (1ull << 32);
+      TexCoef X;
+      X.textured = 0;
+      if constexpr (TEX) load_texcoef(P, t, X);
       const unsigned long long i2 = (unsigned long long)(uint32_t)(R.nq[0] + kNormalBias) |
                                     ((unsigned long long)(uint32_t)(R.nq[1] + kNormalBias) << 32);
       const unsigned long long i3 = (unsigned long long)(uint32_t)(R.nq[2] + kNormalBias);
@@ -1057,6 +1206,16 @@ __global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const Vox
                   z = R.lo[2] + (int)(j / ((uint32_t)nx * (uint32_t)ny));
         const int32_t bb[3] = {x << VGI_FIXED_BITS, y << VGI_FIXED_BITS, z << VGI_FIXED_BITS};
         if (!sat_box(q, N, bb, VGI_FIXED_BITS)) continue;
+        if constexpr (TEX) {
+          if (X.textured) {
+            int64_t e[3];
+            voxel_bary(q, N, R.axis, x, y, z, e);
+            uint32_t rgb;
+            if (!tex_albedo(X, e[0], e[1], e[2], rgb)) continue;  // discarded: nothing, not even the bit
+            i0 = (unsigned long long)(rgb & 0xFFu) | ((unsigned long long)((rgb >> 8) & 0xFFu) << 32);
+            i1 = (unsigned long long)((rgb >> 16) & 0xFFu) | (1ull << 32);
+          }
+        }
         const uint32_t brick = ((uint32_t)x >> 3) + nbr * (((uint32_t)y >> 3) + nbr * ((uint32_t)z >> 3));
         const uint32_t l = ((uint32_t)x & 7u) + 8u * ((uint32_t)y & 7u) + 64u * ((uint32_t)z & 7u);
         unsigned long long* d64 = (unsigned long long*)(P.acc + ((size_t)brick * 512u + l) * kAccWords);
@@ -1072,7 +1231,8 @@ __global__ __launch_bounds__(256) void k_voxelize_fallback(VoxBatch B, const Vox
 
 hipError_t launch_voxelize_fallback(const VoxBatch& B, uint32_t grid_blocks, hipStream_t s) {
   if (grid_blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_voxelize_fallback, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  if (B.textured) hipLaunchKernelGGL(k_voxelize_fallback<true>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
+  else hipLaunchKernelGGL(k_voxelize_fallback<false>, dim3(grid_blocks), dim3(256), 0, s, B, B.scenes);
   return hipGetLastError();
 }
 

@@ -43,7 +43,13 @@ class Mesh(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("position_stride", C.c_uint32),
                 ("num_vertices", C.c_uint32), ("indices", C.c_void_p), ("index_type", C.c_uint32),
                 ("num_triangles", C.c_uint32), ("albedo_rgb8", C.c_void_p),
-                ("base_color", C.c_uint8 * 4), ("model", C.c_float * 12)]
+                ("base_color", C.c_uint8 * 4), ("model", C.c_float * 12),
+                ("texcoords", C.c_void_p), ("texcoord_stride", C.c_uint32), ("texture_width", C.c_uint32),
+                ("texture_height", C.c_uint32), ("texture_reserved", C.c_uint32),
+                ("texture_rgba8", C.c_void_p), ("base_color_factor", C.c_float * 4)]
+
+
+assert C.sizeof(Mesh) == 144, "vgi_mesh layout"
 
 
 class Light(C.Structure):
@@ -379,11 +385,14 @@ class DeviceArray:
 
 
 class MeshData:
-    """Device-resident mesh (positions + optional indices/albedo) and its vgi_mesh view."""
+    """Device-resident mesh (positions + optional indices/albedo, or texcoords + an RGBA8
+    texture) and its vgi_mesh view."""
 
     def __init__(self, ctx: "Context", positions: np.ndarray, indices: np.ndarray | None = None,
                  albedo: np.ndarray | None = None, base_color=(255, 255, 255),
-                 model=IDENTITY_3X4, index_type: int | None = None, position_stride: int = 0):
+                 model=IDENTITY_3X4, index_type: int | None = None, position_stride: int = 0,
+                 texcoords: np.ndarray | None = None, texture: np.ndarray | None = None,
+                 base_color_factor=(1.0, 1.0, 1.0, 1.0)):
         pos = np.ascontiguousarray(positions, dtype=np.float32)
         self.pos = DeviceArray.from_numpy(ctx, pos)
         self.num_vertices = pos.size * 4 // (position_stride or 12)
@@ -410,6 +419,18 @@ class MeshData:
         m.albedo_rgb8 = self.albedo.ptr if self.albedo is not None else None
         m.base_color[:] = [int(base_color[0]), int(base_color[1]), int(base_color[2]), 0]
         m.model[:] = [float(v) for v in model]
+        self.uv = self.tex = None
+        if texcoords is not None and texture is not None:
+            tex = np.ascontiguousarray(texture, np.uint8)
+            if tex.ndim != 3 or tex.shape[2] != 4:
+                raise ValueError("texture must be (height, width, 4) RGBA8")
+            self.uv = DeviceArray.from_numpy(ctx, np.ascontiguousarray(texcoords, np.float32))
+            self.tex = DeviceArray.from_numpy(ctx, tex)
+            m.texcoords = self.uv.ptr
+            m.texcoord_stride = 0
+            m.texture_height, m.texture_width = int(tex.shape[0]), int(tex.shape[1])
+            m.texture_rgba8 = self.tex.ptr
+        m.base_color_factor[:] = [float(v) for v in base_color_factor]
         self.desc = m
 
 

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VGI_ABI_VERSION 2u
+#define VGI_ABI_VERSION 3u
 
 #define VGI_MAX_LEVELS 11u   /* 1024^3 -> 1^3 */
 #define VGI_MAX_LIGHTS 128u  /* the reference's LightLists holds 100 (MegaPipeline.cpp:101-105) */
@@ -137,6 +137,23 @@ typedef struct vgi_mesh {
   const uint8_t* albedo_rgb8; /* 3 bytes per triangle, or NULL to use base_color */
   uint8_t base_color[4];      /* RGB8 (+ unused) used when albedo_rgb8 is NULL */
   float model[12];            /* row-major 3x4: world = M * (x, y, z, 1) */
+  /* Texture (BasicMaterial with UseTextures, Pipelang/Internal/main.lua:188-205), used when
+   * texcoords and texture_rgba8 are both set; albedo_rgb8 and base_color are then ignored.
+   * A voxel covered by a triangle takes the albedo texture(uv) * base_color_factor, sampled
+   * at the voxel centre projected along the triangle's dominant axis: barycentric weights
+   * from the exact snapped coordinates (a centre outside the triangle, which conservative
+   * coverage admits, has its negative weights clamped to zero), uv interpolated in single
+   * precision, wrap addressing, bilinear filtering between texel centres at level 0. A
+   * voxel whose sampled alpha is below 0.05 receives nothing from the triangle (the
+   * reference's discard): no colour, normal or count, so alone it stays unoccupied.
+   * RGB is then rounded to 8 bits for the integer sums. */
+  const float* texcoords;       /* uv float32 x 2 per vertex (TEXCOORD_0) */
+  uint32_t texcoord_stride;     /* bytes between vertices' uv; 0 means 8 */
+  uint32_t texture_width;       /* 1 .. 16384 texels */
+  uint32_t texture_height;      /* 1 .. 16384 texels */
+  uint32_t texture_reserved;    /* 0 */
+  const uint8_t* texture_rgba8; /* RGBA8 texels, row-major, row 0 at v = 0; 4-byte aligned */
+  float base_color_factor[4];   /* RGBA multiplier of the sample */
 } vgi_mesh;
 
 typedef struct vgi_light {

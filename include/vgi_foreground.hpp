@@ -64,8 +64,16 @@ public:
   // Number of indices (or vertices when non-indexed) drawn, as DrawIndexed(ElementCount).
   void SetElementCount(uint32_t count) { ElementCount = count; }
   uint32_t GetElementCount() const { return ElementCount; }
+  // TEXCOORD_0 (float32 x2 per vertex, device memory) with the byte stride of its binding
+  // (the importer's TexCoord0 attribute, glTFSceneImporter.cpp:262-263).
+  void SetTexCoords(const float* device_uv, uint32_t stride = 8) {
+    TexCoords = device_uv;
+    TexCoordStride = stride;
+  }
 
   const float* Positions = nullptr;
+  const float* TexCoords = nullptr;
+  uint32_t TexCoordStride = 8;
   uint32_t VertexCount = 0;
   uint32_t Stride = 12;
   const void* Indices = nullptr;
@@ -75,18 +83,35 @@ public:
 
 class CBasicMaterial {
 public:
-  // BaseColorFactor (main.lua:169-177); RGB in [0, 1], stored as RGB8.
-  void SetAlbedo(float r, float g, float b, float /*a*/ = 1.0f) {
+  // BaseColorFactor (main.lua:169-177); RGB in [0, 1], stored as RGB8 for untextured
+  // draws and kept as floats to multiply a texture sample.
+  void SetAlbedo(float r, float g, float b, float a = 1.0f) {
     auto q = [](float v) { return (uint8_t)(v <= 0.f ? 0 : v >= 1.f ? 255 : (int)(v * 255.f + 0.5f)); };
     Albedo[0] = q(r);
     Albedo[1] = q(g);
     Albedo[2] = q(b);
+    Factor[0] = r;
+    Factor[1] = g;
+    Factor[2] = b;
+    Factor[3] = a;
   }
   // Optional per-triangle RGB8 albedo in device memory (replaces the texture lookup).
   void SetAlbedoPerTriangle(const uint8_t* device_rgb8) { PerTriangle = device_rgb8; }
+  // The base colour texture (SetAlbedoImage, BasicMaterial.h; UseTextures when set,
+  // BasicMaterial.cpp:26): RGBA8 texels in device memory, row-major width x height. A
+  // primitive whose shape has texcoords then voxelizes with texture(uv) x the albedo
+  // factor and the alpha < 0.05 discard (main.lua:188-205).
+  void SetAlbedoImage(const uint8_t* device_rgba8, uint32_t width, uint32_t height) {
+    Image = device_rgba8;
+    ImageWidth = width;
+    ImageHeight = height;
+  }
 
   uint8_t Albedo[3] = {255, 255, 255};
+  float Factor[4] = {1.f, 1.f, 1.f, 1.f};
   const uint8_t* PerTriangle = nullptr;
+  const uint8_t* Image = nullptr;
+  uint32_t ImageWidth = 0, ImageHeight = 0;
 };
 
 class CPrimitive : public std::enable_shared_from_this<CPrimitive> {
@@ -172,6 +197,14 @@ public:
       m.base_color[0] = mat->Albedo[0];
       m.base_color[1] = mat->Albedo[1];
       m.base_color[2] = mat->Albedo[2];
+      if (mat->Image && shape->TexCoords) {
+        m.texcoords = shape->TexCoords;
+        m.texcoord_stride = shape->TexCoordStride;
+        m.texture_rgba8 = mat->Image;
+        m.texture_width = mat->ImageWidth;
+        m.texture_height = mat->ImageHeight;
+        for (int k = 0; k < 4; k++) m.base_color_factor[k] = mat->Factor[k];
+      }
     } else {
       m.base_color[0] = m.base_color[1] = m.base_color[2] = 255;
     }

@@ -43,6 +43,8 @@ def _load():
         "orc_tri_box_overlap": (C.c_int, [P, P, C.c_int64]),
         "orc_triangle_setup": (C.c_int, [P, C.c_uint32, P, C.c_uint32, P, P, C.c_float, C.c_uint32, P, P]),
         "orc_voxelize": (C.c_int, [P, C.c_uint32, P, C.c_uint32, P, P, P, P, C.c_float, C.c_uint32, P, P]),
+        "orc_voxelize_textured": (C.c_int, [P, C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, C.c_uint32, P, P, P,
+                                            C.c_float, C.c_uint32, P, P]),
         "orc_float_to_half": (C.c_uint16, [C.c_float]),
         "orc_half_to_float": (C.c_float, [C.c_uint16]),
         "orc_float_to_bf16": (C.c_uint16, [C.c_float]),
@@ -107,6 +109,32 @@ def voxelize(positions, grid, albedo=None, base_rgb=(255, 255, 255), indices=Non
     st = OrcVoxStats()
     rc = lib.orc_voxelize(_p(pos), nv, _p(idx), ntri, _p(alb), _p(base), _p(m), _p(origin),
                           float(grid["extent"]), n, _p(acc), C.byref(st))
+    assert rc == 0
+    return acc, st
+
+
+def voxelize_textured(positions, grid, uv, texels, factor=(1.0, 1.0, 1.0, 1.0), indices=None, model=None,
+                      num_vertices=None, acc=None, stats=None):
+    """Adds a textured mesh into acc ((n, n, n, 8) uint32 [z][y][x][word], zeros when None).
+    uv: (num_vertices, 2) float32; texels: (h, w, 4) uint8 RGBA, row 0 first."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    nv = pos.size // 3 if num_vertices is None else int(num_vertices)
+    idx = None if indices is None else np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    ntri = (idx.size // 3) if idx is not None else nv // 3
+    uvs = np.ascontiguousarray(uv, np.float32).reshape(-1)
+    assert uvs.size >= 2 * nv
+    tx = np.ascontiguousarray(texels, np.uint8)
+    assert tx.ndim == 3 and tx.shape[2] == 4
+    fac = np.array(list(factor), np.float32)
+    m = IDENTITY_3X4 if model is None else np.ascontiguousarray(model, np.float32).reshape(12)
+    origin = np.array(grid["origin"], np.float32)
+    n = int(grid["n"])
+    if acc is None:
+        acc = np.zeros((n, n, n, 8), np.uint32)
+    assert acc.dtype == np.uint32 and acc.flags.c_contiguous and acc.shape == (n, n, n, 8)
+    st = stats if stats is not None else OrcVoxStats()
+    rc = lib.orc_voxelize_textured(_p(pos), nv, _p(idx), ntri, _p(uvs), _p(tx), tx.shape[1], tx.shape[0],
+                                   _p(fac), _p(m), _p(origin), float(grid["extent"]), n, _p(acc), C.byref(st))
     assert rc == 0
     return acc, st
 

@@ -18,7 +18,10 @@
 //      Akenine-Moller 2001 in the form of Schwarz & Seidel 2010 (plane test + three
 //      axis-plane projections, each with three edge functions), evaluated on vertices
 //      snapped to 1/256 voxel with 128-bit integer products: brute force over the
-//      triangle's voxel bounding box, no pruning.
+//      triangle's voxel bounding box, no pruning. A textured mesh's albedo is sampled per
+//      covered voxel and a voxel whose sampled alpha is below 0.05 gets nothing from the
+//      triangle (BasicMaterial's texture path and discard, main.lua:188-205; restated in
+//      texture_albedo below).
 //  (2) orc_inject_mips_* — radiance at each occupied voxel = linear albedo x direct
 //      light at the voxel centre (Lambert with the voxel's mean normal; directional
 //      lights and point lights with 1/max(d^2, r_min^2) falloff); the reference
@@ -236,15 +239,83 @@ OAPI int orc_triangle_setup(const float* positions, uint32_t num_vertices, const
   return 0;
 }
 
-// acc: n^3 * 8 uint32, zeroed by this call. indices may be NULL (non-indexed).
-// albedo: 3 bytes per triangle (or NULL -> base_rgb). model: row-major 3x4.
-OAPI int orc_voxelize(const float* positions, uint32_t num_vertices, const uint32_t* indices,
-                 uint32_t num_triangles, const uint8_t* albedo, const uint8_t* base_rgb,
-                 const float* model, const float* origin, float extent, uint32_t n,
-                 uint32_t* acc, orc_vox_stats* stats) {
-  if (n == 0 || n > 1024 || (n & (n - 1))) return -1;
-  size_t nvox = (size_t)n * n * n;
-  std::memset(acc, 0, nvox * 8 * sizeof(uint32_t));
+// A textured material (the reference's BasicMaterial with UseTextures,
+// Pipelang/Internal/main.lua:188-205: BaseColor = texture(uv) * BaseColorFactor, discard
+// when BaseColor.a < 0.05). uv: 2 floats per vertex; texels: RGBA8, row-major w x h.
+struct orc_texture {
+  const float* uv;
+  const uint8_t* texels;
+  uint32_t w, h;
+  float factor[4];
+};
+
+// Albedo of one covered voxel from a textured triangle. The voxel centre is projected
+// along the triangle's dominant axis u (VoxelGS's rule, as its fragment is rasterized in
+// that projection) onto the plane of the other two axes (A = u+1, B = u+2 mod 3), where
+// the barycentric numerator of vertex k is sign(N_u) * cross(q_j - q_i, c - q_i) for the
+// edge (i, j) opposite k, in the exact snapped units (c = 256 * voxel + 128). Negative
+// numerators (centre outside the triangle: conservative coverage) are clamped to 0, so
+// the weights are those of the nearest point's region. Then, in single precision:
+// uv = ((w0 uv0 + w1 uv1) + w2 uv2) / (w0 + w1 + w2); a non-finite uv is taken as (0, 0);
+// wrap addressing (u - floor(u)); bilinear filtering at level 0 between texel centres
+// (x = u W - 1/2), texel values c * (1/255), lerp(a, b, t) = a + t (b - a), first along x
+// then y; times the factor; discarded when alpha < 0.05; RGB8 = (uint)(clamp(c, 0, 1) *
+// 255 + 0.5). Returns false for a discarded fragment.
+static bool texture_albedo(const orc_texture& T, const uint32_t vid[3], const int64_t q[3][3],
+                           const int64_t N[3], int64_t vx, int64_t vy, int64_t vz, uint32_t rgb[3]) {
+  const int64_t ax = std::llabs((long long)N[0]), ay = std::llabs((long long)N[1]), az = std::llabs((long long)N[2]);
+  const int u = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  const int A = (u + 1) % 3, B = (u + 2) % 3;
+  const int64_t v[3] = {vx, vy, vz};
+  const int64_t cA = v[A] * 256 + 128, cB = v[B] * 256 + 128;
+  const int64_t sg = N[u] < 0 ? -1 : 1;
+  int64_t e[3];
+  for (int k = 0; k < 3; k++) {
+    const int i = (k + 1) % 3, j = (k + 2) % 3;  // the edge opposite vertex k
+    const int64_t dA = q[j][A] - q[i][A], dB = q[j][B] - q[i][B];
+    const int64_t pA = cA - q[i][A], pB = cB - q[i][B];
+    e[k] = std::max<int64_t>(0, sg * (dA * pB - dB * pA));
+  }
+  const float w0 = (float)e[0], w1 = (float)e[1], w2 = (float)e[2], ws = (float)(e[0] + e[1] + e[2]);
+  const float* uv0 = T.uv + (size_t)vid[0] * 2;
+  const float* uv1 = T.uv + (size_t)vid[1] * 2;
+  const float* uv2 = T.uv + (size_t)vid[2] * 2;
+  float tu = ((w0 * uv0[0] + w1 * uv1[0]) + w2 * uv2[0]) / ws;
+  float tv = ((w0 * uv0[1] + w1 * uv1[1]) + w2 * uv2[1]) / ws;
+  if (!std::isfinite(tu) || !std::isfinite(tv)) tu = tv = 0.0f;
+  const float fu = tu - std::floor(tu), fv = tv - std::floor(tv);
+  const float x = fu * (float)T.w - 0.5f, y = fv * (float)T.h - 0.5f;
+  const float x0 = std::floor(x), y0 = std::floor(y);
+  const float gx = x - x0, gy = y - y0;
+  int64_t ix0 = (int64_t)x0, iy0 = (int64_t)y0, ix1 = ix0 + 1, iy1 = iy0 + 1;
+  if (ix0 < 0) ix0 += T.w;
+  if (iy0 < 0) iy0 += T.h;
+  if (ix1 >= (int64_t)T.w) ix1 -= T.w;
+  if (iy1 >= (int64_t)T.h) iy1 -= T.h;
+  const uint8_t* t00 = T.texels + ((size_t)iy0 * T.w + (size_t)ix0) * 4;
+  const uint8_t* t10 = T.texels + ((size_t)iy0 * T.w + (size_t)ix1) * 4;
+  const uint8_t* t01 = T.texels + ((size_t)iy1 * T.w + (size_t)ix0) * 4;
+  const uint8_t* t11 = T.texels + ((size_t)iy1 * T.w + (size_t)ix1) * 4;
+  const float k255 = 1.0f / 255.0f;
+  float col[4];
+  for (int ch = 0; ch < 4; ch++) {
+    const float a = (float)t00[ch] * k255, b = (float)t10[ch] * k255;
+    const float c = (float)t01[ch] * k255, d = (float)t11[ch] * k255;
+    const float top = a + gx * (b - a), bot = c + gx * (d - c);
+    col[ch] = (top + gy * (bot - top)) * T.factor[ch];
+  }
+  if (col[3] < 0.05f) return false;
+  for (int ch = 0; ch < 3; ch++)
+    rgb[ch] = (uint32_t)(std::fmin(std::fmax(col[ch], 0.0f), 1.0f) * 255.0f + 0.5f);
+  return true;
+}
+
+// Adds one mesh's triangles into acc (n^3 * 8 uint32, not cleared). indices may be NULL
+// (non-indexed). albedo: 3 bytes per triangle (or NULL -> base_rgb), unless tex is given.
+static void voxelize_into(const float* positions, uint32_t num_vertices, const uint32_t* indices,
+                          uint32_t num_triangles, const uint8_t* albedo, const uint8_t* base_rgb,
+                          const orc_texture* tex, const float* model, const float* origin, float extent,
+                          uint32_t n, uint32_t* acc, orc_vox_stats* stats) {
   uint64_t rejected = 0, degenerate = 0, hits = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : rejected, degenerate, hits)
   for (int64_t t = 0; t < (int64_t)num_triangles; t++) {
@@ -257,6 +328,19 @@ OAPI int orc_voxelize(const float* positions, uint32_t num_vertices, const uint3
     int64_t q[3][3];
     for (int i = 0; i < 3; i++)
       for (int a = 0; a < 3; a++) q[i][a] = q9[i * 3 + a];
+    int64_t N[3];
+    {
+      int64_t e0[3], e1[3];
+      for (int a = 0; a < 3; a++) {
+        e0[a] = q[1][a] - q[0][a];
+        e1[a] = q[2][a] - q[0][a];
+      }
+      N[0] = e0[1] * e1[2] - e0[2] * e1[1];
+      N[1] = e0[2] * e1[0] - e0[0] * e1[2];
+      N[2] = e0[0] * e1[1] - e0[1] * e1[0];
+    }
+    uint32_t vid[3];
+    for (int i = 0; i < 3; i++) vid[i] = indices ? indices[t * 3 + i] : (uint32_t)t * 3 + i;
     const uint8_t* c = albedo ? albedo + (size_t)t * 3 : base_rgb;
     int64_t lo[3], hi[3];
     bool empty = false;
@@ -274,11 +358,13 @@ OAPI int orc_voxelize(const float* positions, uint32_t num_vertices, const uint3
         for (int64_t x = lo[0]; x <= hi[0]; x++) {
           int64_t b[3] = {x * 256, y * 256, z * 256};
           if (!tri_box_overlap(q, b, 256)) continue;
+          uint32_t rgb[3] = {c[0], c[1], c[2]};
+          if (tex && !texture_albedo(*tex, vid, q, N, x, y, z, rgb)) continue;  // discarded
           hits++;
           uint32_t* v = acc + ((size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z)) * 8;
-          __atomic_fetch_add(&v[0], (uint32_t)c[0], __ATOMIC_RELAXED);
-          __atomic_fetch_add(&v[1], (uint32_t)c[1], __ATOMIC_RELAXED);
-          __atomic_fetch_add(&v[2], (uint32_t)c[2], __ATOMIC_RELAXED);
+          __atomic_fetch_add(&v[0], rgb[0], __ATOMIC_RELAXED);
+          __atomic_fetch_add(&v[1], rgb[1], __ATOMIC_RELAXED);
+          __atomic_fetch_add(&v[2], rgb[2], __ATOMIC_RELAXED);
           __atomic_fetch_add(&v[3], 1u, __ATOMIC_RELAXED);
           __atomic_fetch_add(&v[4], (uint32_t)(nq[0] + 1024), __ATOMIC_RELAXED);
           __atomic_fetch_add(&v[5], (uint32_t)(nq[1] + 1024), __ATOMIC_RELAXED);
@@ -286,11 +372,38 @@ OAPI int orc_voxelize(const float* positions, uint32_t num_vertices, const uint3
         }
   }
   if (stats) {
-    stats->triangles_in = num_triangles;
-    stats->rejected = rejected;
-    stats->degenerate = degenerate;
-    stats->voxel_hits = hits;
+    stats->triangles_in += num_triangles;
+    stats->rejected += rejected;
+    stats->degenerate += degenerate;
+    stats->voxel_hits += hits;
   }
+}
+
+// acc: n^3 * 8 uint32, zeroed by this call. indices may be NULL (non-indexed).
+// albedo: 3 bytes per triangle (or NULL -> base_rgb). model: row-major 3x4.
+OAPI int orc_voxelize(const float* positions, uint32_t num_vertices, const uint32_t* indices,
+                 uint32_t num_triangles, const uint8_t* albedo, const uint8_t* base_rgb,
+                 const float* model, const float* origin, float extent, uint32_t n,
+                 uint32_t* acc, orc_vox_stats* stats) {
+  if (n == 0 || n > 1024 || (n & (n - 1))) return -1;
+  std::memset(acc, 0, (size_t)n * n * n * 8 * sizeof(uint32_t));
+  if (stats) *stats = orc_vox_stats{0, 0, 0, 0};
+  voxelize_into(positions, num_vertices, indices, num_triangles, albedo, base_rgb, nullptr, model, origin,
+                extent, n, acc, stats);
+  return 0;
+}
+
+// The same for a textured mesh, added into acc (not cleared, so meshes accumulate).
+// uv: 2 floats per vertex; texels: RGBA8 w x h; factor: RGBA.
+OAPI int orc_voxelize_textured(const float* positions, uint32_t num_vertices, const uint32_t* indices,
+                               uint32_t num_triangles, const float* uv, const uint8_t* texels, uint32_t w,
+                               uint32_t h, const float* factor, const float* model, const float* origin,
+                               float extent, uint32_t n, uint32_t* acc, orc_vox_stats* stats) {
+  if (n == 0 || n > 1024 || (n & (n - 1)) || !uv || !texels || w == 0 || h == 0) return -1;
+  orc_texture T{uv, texels, w, h, {factor[0], factor[1], factor[2], factor[3]}};
+  const uint8_t white[3] = {255, 255, 255};
+  voxelize_into(positions, num_vertices, indices, num_triangles, nullptr, white, &T, model, origin, extent, n,
+                acc, stats);
   return 0;
 }
 

@@ -16,6 +16,9 @@ extern "C" {
 struct orc_vox_stats { uint64_t a, b, c, d; };
 int orc_voxelize(const float*, uint32_t, const uint32_t*, uint32_t, const uint8_t*, const uint8_t*,
                  const float*, const float*, float, uint32_t, uint32_t*, orc_vox_stats*);
+int orc_voxelize_textured(const float*, uint32_t, const uint32_t*, uint32_t, const float*, const uint8_t*, uint32_t,
+                          uint32_t, const float*, const float*, const float*, float, uint32_t, uint32_t*,
+                          orc_vox_stats*);
 void orc_gbuffer_decode_f32(const float*, const uint16_t*, const float*, const float*, uint32_t, uint32_t,
                             float*, float*, float*);
 void orc_temporal(const float*, const float*, const float*, const float*, const float*, const float*, uint32_t,
@@ -215,11 +218,31 @@ static int gpu() {
   meshB->SetElementCount(450);
   auto matB = std::make_shared<CBasicMaterial>();
   matB->SetAlbedoPerTriangle((const uint8_t*)dev_copy(c, ab.data(), ab.size()));
+  // primitive C: 100 non-indexed triangles with TEXCOORD_0 and a 24 x 20 RGBA8 base colour
+  // texture (a third of its texels below the 0.05 alpha discard) times the albedo factor
+  std::vector<float> vc(900), uvc(600);
+  for (auto& v : vc) v = U(rng) * 0.8f;
+  for (auto& v : uvc) v = U(rng) * 3.0f;
+  const uint32_t TW = 24, TH = 20;
+  std::vector<uint8_t> texc(TW * TH * 4);
+  for (size_t i = 0; i < texc.size(); i++) texc[i] = (uint8_t)(rng() & 255);
+  for (size_t i = 3; i < texc.size(); i += 4)
+    if (rng() % 3 == 0) texc[i] = (uint8_t)(rng() % 20);
+  auto meshC = std::make_shared<CTriangleMesh>();
+  meshC->SetPositions((const float*)dev_copy(c, vc.data(), vc.size() * 4), 300);
+  meshC->SetElementCount(300);
+  meshC->SetTexCoords((const float*)dev_copy(c, uvc.data(), uvc.size() * 4));
+  auto matC = std::make_shared<CBasicMaterial>();
+  matC->SetAlbedo(0.9f, 0.8f, 1.0f, 1.0f);
+  matC->SetAlbedoImage((const uint8_t*)dev_copy(c, texc.data(), texc.size()), TW, TH);
   auto pA = std::make_shared<CPrimitive>(), pB = std::make_shared<CPrimitive>();
+  auto pC = std::make_shared<CPrimitive>();
   pA->SetShape(meshA);
   pA->SetMaterial(matA);
   pB->SetShape(meshB);
   pB->SetMaterial(matB);
+  pC->SetShape(meshC);
+  pC->SetMaterial(matC);
   tc::Matrix3x4 mA, mB;
   const float cs = std::cos(0.4f), sn = std::sin(0.4f);
   float ma[12] = {cs, -sn, 0, 0.1f, sn, cs, 0, 0, 0, 0, 1, -0.05f};
@@ -229,18 +252,20 @@ static int gpu() {
 
   CVoxelizeRenderer vr(c, s, g);
   CRenderContext ctx;
-  vr.RenderList(ctx, {mA, mB}, {pA.get(), pB.get()});
+  vr.RenderList(ctx, {mA, mB, mB}, {pA.get(), pB.get(), pC.get()});
   CHECK(vr.LastStatus() == VGI_OK);
-  std::vector<uint32_t> got((size_t)n * n * n * 8), ra(got.size()), rb(got.size());
+  std::vector<uint32_t> got((size_t)n * n * n * 8), ra(got.size()), rb(got.size()), rc(got.size(), 0u);
   CHECK(vgi_scene_read_voxels(c, s, got.data()) == VGI_OK);
   std::vector<uint32_t> ia32(ia.begin(), ia.end());
   const uint8_t baseA[3] = {matA->Albedo[0], matA->Albedo[1], matA->Albedo[2]};
   orc_vox_stats st;
   orc_voxelize(va.data(), 200, ia32.data(), 300, nullptr, baseA, ma, g.origin, g.extent, n, ra.data(), &st);
   orc_voxelize(vb.data(), 450, nullptr, 150, ab.data(), nullptr, mb, g.origin, g.extent, n, rb.data(), &st);
+  orc_voxelize_textured(vc.data(), 300, nullptr, 100, uvc.data(), texc.data(), TW, TH, matC->Factor, mb, g.origin,
+                        g.extent, n, rc.data(), &st);
   size_t bad = 0, occ = 0;
   for (size_t i = 0; i < got.size(); i++) {
-    bad += got[i] != ra[i] + rb[i];
+    bad += got[i] != ra[i] + rb[i] + rc[i];
     if (i % 8 == 3 && got[i]) occ++;
   }
   std::printf("occupied voxels %zu, mismatched words %zu\n", occ, bad);

@@ -79,7 +79,7 @@ def test_struct_layouts_match_c(tmp_path):
 
 
 def test_version_and_abi():
-    assert vgi.lib.vgi_abi_version() == 2
+    assert vgi.lib.vgi_abi_version() == 3
     assert b"gfx950" in vgi.lib.vgi_version()
 
 
