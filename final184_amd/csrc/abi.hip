@@ -1601,7 +1601,14 @@ int vgi_trace_batch(vgi_context* ctx, const vgi_trace_item* items, uint32_t coun
 struct vgi_gltf {
   vgi::GltfScene scene;
   vgi_context* ctx = nullptr;               // device copies belong to this context's device
-  std::vector<void*> dev_pos, dev_idx;      // per geometry
+  std::vector<void*> dev_pos, dev_idx, dev_uv;  // per geometry (dev_uv null without texcoords)
+  struct Texels {
+    std::vector<uint8_t> rgba;              // decoded by the caller, RGBA8 row-major
+    uint32_t w = 0, h = 0;
+    void* dev = nullptr;                    // device copy (on ctx's device)
+    bool dirty = false;                     // set since the last upload
+  };
+  std::vector<Texels> texels;               // per image
 };
 
 namespace {
@@ -1611,6 +1618,7 @@ int gltf_finish(int rc, const std::string& err, vgi::GltfScene& scene, vgi_gltf*
   auto* g = new (std::nothrow) vgi_gltf;
   if (!g) return fail(VGI_E_NOMEM, "out of host memory");
   g->scene = std::move(scene);
+  g->texels.resize(g->scene.images.size());
   *out = g;
   return VGI_OK;
 }
@@ -1621,8 +1629,16 @@ void gltf_free_device(vgi_gltf* g) {
   (void)hipSetDevice(g->ctx->device);
   for (void* p : g->dev_pos) (void)hipFree(p);
   for (void* p : g->dev_idx) (void)hipFree(p);
+  for (void* p : g->dev_uv)
+    if (p) (void)hipFree(p);
+  for (auto& t : g->texels) {
+    if (t.dev) (void)hipFree(t.dev);
+    t.dev = nullptr;
+    t.dirty = !t.rgba.empty();
+  }
   g->dev_pos.clear();
   g->dev_idx.clear();
+  g->dev_uv.clear();
   g->ctx = nullptr;
 }
 
@@ -1663,6 +1679,7 @@ int vgi_gltf_get_info(const vgi_gltf* g, vgi_gltf_info* out) {
   out->num_primitives = (uint32_t)g->scene.instances.size();
   out->num_geometries = (uint32_t)g->scene.geometry.size();
   out->skipped_primitives = g->scene.skipped_primitives;
+  out->num_images = (uint32_t)g->scene.images.size();
   for (const auto& in : g->scene.instances) out->num_triangles += g->scene.geometry[in.geometry].indices.size() / 3;
   return VGI_OK;
 }
@@ -1683,6 +1700,37 @@ int vgi_gltf_get_primitive(const vgi_gltf* g, uint32_t index, vgi_gltf_primitive
   out->mesh = ge.mesh;
   out->primitive = ge.primitive;
   out->material = in.material;
+  out->texcoords = ge.texcoords.empty() ? nullptr : ge.texcoords.data();
+  out->image = in.image;
+  out->reserved = 0;
+  std::memcpy(out->base_color_factor, in.factor, sizeof(in.factor));
+  return VGI_OK;
+}
+
+int vgi_gltf_get_image(vgi_gltf* g, uint32_t index, vgi_gltf_image* out) {
+  if (!g || !out) return fail(VGI_E_INVALID, "NULL argument");
+  if (index >= g->scene.images.size()) return fail(VGI_E_INVALID, "image %u >= %zu", index, g->scene.images.size());
+  vgi::GltfImage& im = g->scene.images[index];
+  std::string err;
+  const int rc = vgi::gltf_load_image(g->scene, im, err);
+  if (rc != VGI_OK) return fail(rc, "glTF: %s", err.c_str());
+  out->data = im.data.data();
+  out->bytes = im.data.size();
+  out->mime_type = im.mime_type.c_str();
+  out->uri = im.uri.c_str();
+  return VGI_OK;
+}
+
+int vgi_gltf_set_image_rgba8(vgi_gltf* g, uint32_t index, const uint8_t* rgba8, uint32_t width, uint32_t height) {
+  if (!g || !rgba8) return fail(VGI_E_INVALID, "NULL argument");
+  if (index >= g->texels.size()) return fail(VGI_E_INVALID, "image %u >= %zu", index, g->texels.size());
+  if (width == 0 || height == 0 || width > 16384 || height > 16384)
+    return fail(VGI_E_INVALID, "image %ux%u (1..16384 per side)", width, height);
+  auto& t = g->texels[index];
+  t.rgba.assign(rgba8, rgba8 + (size_t)width * height * 4);
+  t.w = width;
+  t.h = height;
+  t.dirty = true;
   return VGI_OK;
 }
 
@@ -1697,21 +1745,39 @@ int vgi_gltf_upload(vgi_context* ctx, vgi_gltf* g, vgi_mesh* meshes, uint32_t ca
   if (!g->ctx) {
     g->dev_pos.assign(sc.geometry.size(), nullptr);
     g->dev_idx.assign(sc.geometry.size(), nullptr);
+    g->dev_uv.assign(sc.geometry.size(), nullptr);
     g->ctx = ctx;
     for (size_t i = 0; i < sc.geometry.size(); i++) {
       const auto& ge = sc.geometry[i];
       const size_t pb = std::max<size_t>(ge.positions.size() * 4, 4), ib = std::max<size_t>(ge.indices.size() * 4, 4);
       hipError_t e = hipMalloc(&g->dev_pos[i], pb);
       if (e == hipSuccess) e = hipMalloc(&g->dev_idx[i], ib);
+      if (e == hipSuccess && !ge.texcoords.empty()) e = hipMalloc(&g->dev_uv[i], ge.texcoords.size() * 4);
       if (e == hipSuccess && !ge.positions.empty())
         e = hipMemcpy(g->dev_pos[i], ge.positions.data(), ge.positions.size() * 4, hipMemcpyHostToDevice);
       if (e == hipSuccess && !ge.indices.empty())
         e = hipMemcpy(g->dev_idx[i], ge.indices.data(), ge.indices.size() * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess && !ge.texcoords.empty())
+        e = hipMemcpy(g->dev_uv[i], ge.texcoords.data(), ge.texcoords.size() * 4, hipMemcpyHostToDevice);
       if (e != hipSuccess) {
         gltf_free_device(g);
         return fail(e == hipErrorOutOfMemory ? VGI_E_NOMEM : VGI_E_HIP, "glTF upload: %s", hipGetErrorString(e));
       }
     }
+  }
+  for (auto& t : g->texels) {  // decoded images set since the last upload
+    if (!t.dirty) continue;
+    if (t.dev) (void)hipFree(t.dev);
+    t.dev = nullptr;
+    hipError_t e = hipMalloc(&t.dev, t.rgba.size());
+    if (e == hipSuccess) e = hipMemcpy(t.dev, t.rgba.data(), t.rgba.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (t.dev) (void)hipFree(t.dev);
+      t.dev = nullptr;
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? VGI_E_NOMEM : VGI_E_HIP, "glTF image upload: %s", hipGetErrorString(e));
+    }
+    t.dirty = false;
   }
   for (size_t i = 0; i < sc.instances.size(); i++) {
     const auto& in = sc.instances[i];
@@ -1726,6 +1792,15 @@ int vgi_gltf_upload(vgi_context* ctx, vgi_gltf* g, vgi_mesh* meshes, uint32_t ca
     m.albedo_rgb8 = nullptr;
     std::memcpy(m.base_color, in.base_color, 4);
     std::memcpy(m.model, in.model, sizeof(m.model));
+    if (in.image >= 0 && g->dev_uv[in.geometry] && g->texels[(size_t)in.image].dev) {
+      const auto& t = g->texels[(size_t)in.image];
+      m.texcoords = (const float*)g->dev_uv[in.geometry];
+      m.texcoord_stride = 8;
+      m.texture_rgba8 = (const uint8_t*)t.dev;
+      m.texture_width = t.w;
+      m.texture_height = t.h;
+      std::memcpy(m.base_color_factor, in.factor, sizeof(m.base_color_factor));
+    }
     meshes[i] = m;
   }
   return VGI_OK;

@@ -3,6 +3,7 @@
 // Host code only; see gltf.hpp for what is read and what it replaces.
 #include "gltf.hpp"
 
+#include <algorithm>
 #include <charconv>
 #include <cmath>
 #include <cstring>
@@ -417,6 +418,7 @@ private:
         for (int r = 0; r < 3; r++)
           for (int c = 0; c < 4; c++) inst.model[r * 4 + c] = (float)world[c * 4 + r];
         base_color(mat, inst.base_color);
+        texture(mat, inst);
         out_.instances.push_back(inst);
       }
     }
@@ -439,6 +441,32 @@ private:
       }
     }
     for (int k = 0; k < 4; k++) rgba[k] = (uint8_t)std::lround(std::fmin(std::fmax(f[k], 0.0), 1.0) * 255.0);
+  }
+
+  // The material's base colour texture and the factor it is multiplied by. The reference's
+  // importer (GetMaterial, glTFSceneImporter.cpp:197-201) binds GetImage(texture index):
+  // it indexes the images array with the texture's index (GetImage, :110-131), not with
+  // the texture's "source", and leaves the factor white. Mode 0 does the same; mode 1
+  // follows the texture to its source image and uses baseColorFactor. Either way the
+  // texture is sampled with TEXCOORD_0 (the reference's shaders have no other set).
+  void texture(int64_t mat, GltfInstance& inst) {
+    for (int k = 0; k < 4; k++) inst.factor[k] = 1.0f;
+    inst.image = -1;
+    if (mat < 0) return;
+    const Json& m = array_at(root_.get("materials"), mat, "material");
+    const Json* pbr = m.get("pbrMetallicRoughness");
+    if (!pbr) return;
+    if (mode_ == 1)
+      if (const Json* bc = pbr->get("baseColorFactor"))
+        for (int k = 0; k < 4 && k < (int)bc->size(); k++) inst.factor[k] = (float)bc->arr[k].num;
+    const Json* bt = pbr->get("baseColorTexture");
+    if (!bt) return;
+    const int64_t ti = get_int(*bt, "index", -1, true);
+    const Json& t = array_at(root_.get("textures"), ti, "texture");
+    const int64_t src = mode_ == 0 ? ti : get_int(t, "source", -1);
+    if (src < 0) return;
+    array_at(root_.get("images"), src, "image");  // in range
+    inst.image = (int32_t)src;
   }
 
   // the bytes an accessor covers: element e starts at base + e * stride
@@ -518,6 +546,29 @@ private:
       for (size_t e = 0; e < pv.count; e++)
         if (pv.base) std::memcpy(&g.positions[e * 3], pv.base + e * pv.stride, 12);
         else g.positions[e * 3] = g.positions[e * 3 + 1] = g.positions[e * 3 + 2] = 0.f;
+      // TEXCOORD_0: float, or normalized unsigned byte / short (c / 255, c / 65535)
+      const int64_t ta = get_int(*attrs, "TEXCOORD_0", -1);
+      if (ta >= 0) {
+        const View tv = accessor(ta, "VEC2", 2);
+        if (tv.count != pv.count) raise(VGI_E_INVALID, "TEXCOORD_0 and POSITION counts differ");
+        if (tv.comp != 5126 && tv.comp != 5121 && tv.comp != 5123)
+          raise(VGI_E_INVALID, "TEXCOORD_0 is not float, unsigned byte or unsigned short");
+        g.texcoords.assign(tv.count * 2, 0.f);
+        for (size_t e = 0; e < tv.count && tv.base; e++) {
+          const uint8_t* q = tv.base + e * tv.stride;
+          for (int c = 0; c < 2; c++) {
+            if (tv.comp == 5126) {
+              std::memcpy(&g.texcoords[e * 2 + c], q + 4 * c, 4);
+            } else if (tv.comp == 5121) {
+              g.texcoords[e * 2 + c] = (float)q[c] / 255.0f;
+            } else {
+              uint16_t h;
+              std::memcpy(&h, q + 2 * c, 2);
+              g.texcoords[e * 2 + c] = (float)h / 65535.0f;
+            }
+          }
+        }
+      }
       // vertex order: the index accessor, or 0..count-1
       std::vector<uint32_t> order;
       const int64_t ia = get_int(pr, "indices", -1);
@@ -567,6 +618,25 @@ private:
 };
 
 }  // namespace
+
+int gltf_load_image(const GltfScene& scene, GltfImage& img, std::string& err) {
+  if (img.loaded) return VGI_OK;
+  if (scene.base_dir.empty()) {
+    err = "external image " + img.uri + " without a base directory";
+    return VGI_E_INVALID;
+  }
+  try {
+    img.data = read_file(scene.base_dir + "/" + img.uri);
+  } catch (const Fail& f) {
+    err = f.msg;
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    err = "out of host memory";
+    return VGI_E_NOMEM;
+  }
+  img.loaded = true;
+  return VGI_OK;
+}
 
 int gltf_parse(const uint8_t* data, size_t bytes, const std::string& base_dir, uint32_t albedo_mode,
                GltfScene& out, std::string& err) {
@@ -634,6 +704,45 @@ int gltf_parse(const uint8_t* data, size_t bytes, const std::string& base_dir, u
     }
     Reader r(root, buffers, albedo_mode, out);
     r.run();
+    // images as stored: a bufferView's bytes, a base64 data URI, or a relative file read
+    // on request (the reference decodes them with stb_image inside tinygltf; decoding is
+    // the caller's here)
+    out.base_dir = base_dir;
+    if (const Json* il = root.get("images")) {
+      for (size_t i = 0; i < il->size(); i++) {
+        const Json& im = il->arr[i];
+        GltfImage g;
+        if (const Json* mt = im.get("mimeType"))
+          if (mt->type == Json::kStr) g.mime_type = mt->str;
+        const Json* uri = im.get("uri");
+        const int64_t bvi = get_int(im, "bufferView", -1);
+        if (uri && uri->type == Json::kStr && uri->str.compare(0, 5, "data:") == 0) {
+          const size_t comma = uri->str.find(',');
+          const size_t b64 = uri->str.find(";base64");
+          if (comma == std::string::npos || b64 == std::string::npos || b64 > comma ||
+              !base64(uri->str, comma + 1, g.data))
+            raise(VGI_E_INVALID, "image " + std::to_string(i) + ": unsupported data URI");
+          if (g.mime_type.empty()) g.mime_type = uri->str.substr(5, std::min(b64, uri->str.find(';')) - 5);
+          g.loaded = true;
+        } else if (uri && uri->type == Json::kStr) {
+          g.uri = percent_decode(uri->str);
+          if (g.uri.find("://") != std::string::npos || (!g.uri.empty() && g.uri[0] == '/'))
+            raise(VGI_E_INVALID, "image uri " + g.uri + " is not a relative path");
+        } else if (bvi >= 0) {
+          const Json& bv = array_at(root.get("bufferViews"), bvi, "bufferView");
+          const int64_t bi = get_int(bv, "buffer", 0, true);
+          if (bi < 0 || (size_t)bi >= buffers.size()) raise(VGI_E_INVALID, "bufferView buffer out of range");
+          const int64_t bo = get_int(bv, "byteOffset", 0), bl = get_int(bv, "byteLength", 0, true);
+          if (bo < 0 || bl < 0 || (uint64_t)bo + (uint64_t)bl > buffers[(size_t)bi].size())
+            raise(VGI_E_INVALID, "bufferView outside its buffer");
+          g.data.assign(buffers[(size_t)bi].begin() + bo, buffers[(size_t)bi].begin() + bo + bl);
+          g.loaded = true;
+        } else {
+          raise(VGI_E_INVALID, "image " + std::to_string(i) + " has neither uri nor bufferView");
+        }
+        out.images.push_back(std::move(g));
+      }
+    }
   } catch (const Fail& f) {
     err = f.msg;
     out = GltfScene{};

@@ -99,13 +99,19 @@ GLTF_ALBEDO_REFERENCE, GLTF_ALBEDO_FACTOR = 0, 1
 
 class GltfInfo(C.Structure):
     _fields_ = [("num_primitives", C.c_uint32), ("num_geometries", C.c_uint32),
-                ("skipped_primitives", C.c_uint32), ("pad", C.c_uint32), ("num_triangles", C.c_uint64)]
+                ("skipped_primitives", C.c_uint32), ("num_images", C.c_uint32), ("num_triangles", C.c_uint64)]
 
 
 class GltfPrimitive(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("num_vertices", C.c_uint32),
                 ("num_triangles", C.c_uint32), ("model", C.c_float * 12), ("base_color", C.c_uint8 * 4),
-                ("node", C.c_int32), ("mesh", C.c_int32), ("primitive", C.c_int32), ("material", C.c_int32)]
+                ("node", C.c_int32), ("mesh", C.c_int32), ("primitive", C.c_int32), ("material", C.c_int32),
+                ("texcoords", C.c_void_p), ("image", C.c_int32), ("reserved", C.c_uint32),
+                ("base_color_factor", C.c_float * 4)]
+
+
+class GltfImage(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("bytes", C.c_size_t), ("mime_type", C.c_char_p), ("uri", C.c_char_p)]
 
 
 class BlurParams(C.Structure):
@@ -227,6 +233,8 @@ SIGNATURES = [
     ("vgi_gltf_load_memory", C.c_int, [_P, C.c_size_t, C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     ("vgi_gltf_get_info", C.c_int, [_P, C.POINTER(GltfInfo)]),
     ("vgi_gltf_get_primitive", C.c_int, [_P, C.c_uint32, C.POINTER(GltfPrimitive)]),
+    ("vgi_gltf_get_image", C.c_int, [_P, C.c_uint32, C.POINTER(GltfImage)]),
+    ("vgi_gltf_set_image_rgba8", C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32]),
     ("vgi_gltf_upload", C.c_int, [_P, _P, C.POINTER(Mesh), C.c_uint32, C.POINTER(C.c_uint32)]),
     ("vgi_gltf_release", C.c_int, [_P]),
     ("vgi_autotune_batch", C.c_int, [_P, C.POINTER(VoxelizeItem), C.POINTER(MipsItem),
@@ -578,9 +586,26 @@ class Gltf:
             if p.num_vertices else np.zeros(0, np.float32)
         idx = np.ctypeslib.as_array(C.cast(p.indices, C.POINTER(C.c_uint32)), (p.num_triangles * 3,)).copy() \
             if p.num_triangles else np.zeros(0, np.uint32)
+        uv = np.ctypeslib.as_array(C.cast(p.texcoords, C.POINTER(C.c_float)), (p.num_vertices * 2,)).copy() \
+            if p.texcoords and p.num_vertices else None
         return dict(positions=pos.reshape(-1, 3), indices=idx.reshape(-1, 3), model=np.array(p.model[:], np.float32),
                     base_color=tuple(p.base_color[:]), node=p.node, mesh=p.mesh, primitive=p.primitive,
-                    material=p.material)
+                    material=p.material, texcoords=None if uv is None else uv.reshape(-1, 2), image=p.image,
+                    base_color_factor=tuple(p.base_color_factor[:]))
+
+    def image(self, index: int) -> dict:
+        """The image's encoded bytes as stored in the file, with its mime type and uri."""
+        im = GltfImage()
+        _check(lib.vgi_gltf_get_image(self.handle, index, C.byref(im)))
+        data = C.string_at(im.data, im.bytes) if im.bytes else b""
+        return dict(data=data, mime_type=(im.mime_type or b"").decode(), uri=(im.uri or b"").decode())
+
+    def set_image_rgba8(self, index: int, rgba: np.ndarray) -> None:
+        """Decoded texels (height, width, 4) uint8 for image `index`."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("texels must be (height, width, 4) RGBA8")
+        _check(lib.vgi_gltf_set_image_rgba8(self.handle, index, a.ctypes.data, a.shape[1], a.shape[0]))
 
     def primitives(self) -> list:
         return [self.primitive(i) for i in range(self.info().num_primitives)]

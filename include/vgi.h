@@ -546,7 +546,7 @@ typedef struct vgi_gltf_info {
   uint32_t num_primitives;     /* placed primitives (node x mesh primitive) */
   uint32_t num_geometries;     /* distinct mesh primitives */
   uint32_t skipped_primitives; /* points / lines */
-  uint32_t pad;
+  uint32_t num_images;         /* the file's images array */
   uint64_t num_triangles;      /* over placed primitives */
 } vgi_gltf_info;
 
@@ -561,7 +561,23 @@ typedef struct vgi_gltf_primitive {
   int32_t mesh;
   int32_t primitive;
   int32_t material;         /* -1: none */
+  const float* texcoords;   /* host memory owned by the handle: uv per vertex (TEXCOORD_0), or NULL */
+  int32_t image;            /* image of the material's baseColorTexture, -1: none; the reference
+                               mode indexes images with the texture's index, as the reference's
+                               importer does (glTFSceneImporter.cpp:110-131,201), the factor
+                               mode with the texture's source */
+  uint32_t reserved;
+  float base_color_factor[4]; /* multiplier of that texture: white (reference mode) or baseColorFactor */
 } vgi_gltf_primitive;
+
+/* An image as the file stores it (PNG, JPEG, ...): decoding is the caller's, as the
+ * reference decodes with stb_image inside tinygltf. */
+typedef struct vgi_gltf_image {
+  const uint8_t* data;   /* encoded bytes, owned by the handle */
+  size_t bytes;
+  const char* mime_type; /* "" when the file does not say */
+  const char* uri;       /* relative file name, "" when embedded */
+} vgi_gltf_image;
 
 int vgi_gltf_load_file(const char* path, uint32_t albedo_mode, vgi_gltf** out);
 /* base_dir: directory for relative buffer URIs (NULL: only embedded buffers). */
@@ -569,10 +585,20 @@ int vgi_gltf_load_memory(const void* data, size_t bytes, const char* base_dir, u
                          vgi_gltf** out);
 int vgi_gltf_get_info(const vgi_gltf* g, vgi_gltf_info* out);
 int vgi_gltf_get_primitive(const vgi_gltf* g, uint32_t index, vgi_gltf_primitive* out);
-/* Copies every geometry to device memory of ctx's device (kept by the handle until
- * vgi_gltf_release; a second call with the same ctx reuses it) and writes one vgi_mesh
- * per placed primitive (u32 indices, the node's model matrix, base colour) into
- * meshes[0..capacity); *count receives the number of primitives. Synchronous. */
+/* The image's encoded bytes (an external file is read on the first request). With
+ * vgi_gltf_set_image_rgba8 this replaces GetImage (glTFSceneImporter.cpp:110-131), whose
+ * decoded RGBA8 image the importer uploads as the material's albedo image. */
+int vgi_gltf_get_image(vgi_gltf* g, uint32_t index, vgi_gltf_image* out);
+/* The image's decoded texels, RGBA8 row-major width x height (copied; 1..16384 per side).
+ * A primitive whose material's baseColorTexture is this image and whose geometry has
+ * TEXCOORD_0 is then uploaded as a textured mesh (texture x base_color_factor, with the
+ * alpha discard); primitives whose image has no texels keep their base colour. */
+int vgi_gltf_set_image_rgba8(vgi_gltf* g, uint32_t index, const uint8_t* rgba8, uint32_t width, uint32_t height);
+/* Copies every geometry (and texcoords and decoded images) to device memory of ctx's
+ * device (kept by the handle until vgi_gltf_release; a second call with the same ctx
+ * reuses it, uploading images set since) and writes one vgi_mesh per placed primitive
+ * (u32 indices, the node's model matrix, base colour or texture) into meshes[0..capacity);
+ * *count receives the number of primitives. Synchronous. */
 int vgi_gltf_upload(vgi_context* ctx, vgi_gltf* g, vgi_mesh* meshes, uint32_t capacity, uint32_t* count);
 int vgi_gltf_release(vgi_gltf* g);
 

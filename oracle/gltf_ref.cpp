@@ -18,6 +18,13 @@
 //     fills (it stores glTF's "baseColorFactor"), so the albedo is (1, 1, 1, 1).
 //   CVoxelizeRenderer::Render (Foreground/Renderer/VoxelizeRenderer.cpp:103) draws only
 //     primitives whose material is a CBasicMaterial, i.e. that have a material.
+//   GetMaterial (:197-201) binds GetImage(values["baseColorTexture"].TextureIndex()), and
+//     GetImage (:110-131) indexes the images array with that texture index (albedo mode 0
+//     here); mode 1 follows the texture to its "source" image and takes
+//     values["baseColorFactor"]. TEXCOORD_0 is the vertex attribute the importer binds as
+//     TexCoord0 (:262-263), read through its binding like POSITION; a normalized unsigned
+//     byte or short is converted as the GPU converts UNORM (c / 255, c / 65535). Images are
+//     decoded by tinygltf (stb_image), as the importer receives them.
 // Vertex assembly of lists, strips and fans follows the Vulkan specification (the
 // reference draws through Vulkan): strip triangle i = (i, i+1+(i&1), i+2-(i&1)), fan
 // triangle i = (i+1, i+2, 0).
@@ -32,14 +39,18 @@ namespace {
 
 struct Prim {
   std::vector<float> pos;
+  std::vector<float> uv;  // TEXCOORD_0, empty when absent
   std::vector<uint32_t> idx;
   float model[12];
   uint8_t rgba[4];
   int material;
+  int image = -1;
+  float factor[4] = {1, 1, 1, 1};
 };
 
 struct Scene {
   std::vector<Prim> prims;
+  tinygltf::Model model;  // kept for its decoded images
 };
 
 // 3x4 affine transforms as 4x4 row-major doubles
@@ -166,6 +177,40 @@ struct Visitor {
             for (int k = 0; k < 4; k++) f[k] = it->second.number_array[k];
         }
         for (int k = 0; k < 4; k++) pr.rgba[k] = (uint8_t)std::lround(std::fmin(std::fmax(f[k], 0.0), 1.0) * 255.0);
+        const auto ta = p.attributes.find("TEXCOORD_0");
+        if (ta != p.attributes.end()) {
+          const tinygltf::Accessor& tacc = model.accessors[ta->second];
+          const tinygltf::BufferView& tbv = model.bufferViews[tacc.bufferView];
+          const size_t ts = tbv.byteStride ? tbv.byteStride : (size_t)tacc.ByteStride(tbv);
+          const unsigned char* tb = model.buffers[tbv.buffer].data.data() + tbv.byteOffset + tacc.byteOffset;
+          pr.uv.resize(tacc.count * 2);
+          for (size_t e = 0; e < tacc.count; e++)
+            for (int c = 0; c < 2; c++) {
+              const unsigned char* q = tb + e * ts;
+              if (tacc.componentType == TINYGLTF_COMPONENT_TYPE_FLOAT) {
+                std::memcpy(&pr.uv[e * 2 + c], q + 4 * c, 4);
+              } else if (tacc.componentType == TINYGLTF_COMPONENT_TYPE_UNSIGNED_BYTE) {
+                pr.uv[e * 2 + c] = (float)q[c] / 255.0f;
+              } else {
+                uint16_t h;
+                std::memcpy(&h, q + 2 * c, 2);
+                pr.uv[e * 2 + c] = (float)h / 65535.0f;
+              }
+            }
+        }
+        if (p.material >= 0) {
+          const tinygltf::Material& m = model.materials[p.material];
+          const auto bt = m.values.find("baseColorTexture");
+          if (bt != m.values.end()) {
+            const int ti = bt->second.TextureIndex();
+            pr.image = albedo_mode == 0 ? ti : model.textures[ti].source;
+          }
+          if (albedo_mode == 1) {
+            const auto it = m.values.find("baseColorFactor");
+            if (it != m.values.end() && it->second.number_array.size() == 4)
+              for (int k = 0; k < 4; k++) pr.factor[k] = (float)it->second.number_array[k];
+          }
+        }
         out.prims.push_back(std::move(pr));
       }
     }
@@ -189,12 +234,36 @@ void* gref_load(const char* path, int albedo_mode, char* err, int errlen) {
     return nullptr;
   }
   auto* s = new Scene;
-  if (!model.scenes.empty()) {
-    const int si = model.defaultScene >= 0 ? model.defaultScene : 0;
-    Visitor v{model, albedo_mode, *s};
-    for (int n : model.scenes[si].nodes) v.visit(n, identity());
+  s->model = std::move(model);
+  if (!s->model.scenes.empty()) {
+    const int si = s->model.defaultScene >= 0 ? s->model.defaultScene : 0;
+    Visitor v{s->model, albedo_mode, *s};
+    for (int n : s->model.scenes[si].nodes) v.visit(n, identity());
   }
   return s;
+}
+
+// TEXCOORD_0 (NULL, 0 when absent), the base colour image and its factor.
+int gref_get_texture(void* h, int i, const float** uv, int* nuv, int* image, float* factor) {
+  const Prim& p = static_cast<Scene*>(h)->prims.at((size_t)i);
+  *uv = p.uv.empty() ? nullptr : p.uv.data();
+  *nuv = (int)(p.uv.size() / 2);
+  *image = p.image;
+  std::memcpy(factor, p.factor, sizeof(p.factor));
+  return 0;
+}
+
+// The image as tinygltf decoded it: width, height, components, bits, pixel bytes.
+int gref_image(void* h, int i, int* w, int* hh, int* comp, int* bits, const unsigned char** px) {
+  const tinygltf::Model& m = static_cast<Scene*>(h)->model;
+  if (i < 0 || (size_t)i >= m.images.size()) return -1;
+  const tinygltf::Image& im = m.images[(size_t)i];
+  *w = im.width;
+  *hh = im.height;
+  *comp = im.component;
+  *bits = im.bits;
+  *px = im.image.data();
+  return 0;
 }
 
 int gref_count(void* h) { return (int)static_cast<Scene*>(h)->prims.size(); }

@@ -13,8 +13,41 @@ import base64
 import json
 import math
 import struct
+import zlib
 
 import numpy as np
+
+
+def png_rgba(texels: np.ndarray) -> bytes:
+    """A PNG of RGBA8 texels (h, w, 4): filter 0 on every row, one zlib stream."""
+    t = np.ascontiguousarray(texels, np.uint8)
+    h, w, _ = t.shape
+    raw = b"".join(b"\x00" + t[y].tobytes() for y in range(h))
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+
+
+def png_decode_rgba(data: bytes) -> np.ndarray:
+    """Decodes the PNGs png_rgba writes (8-bit RGBA, filter 0 rows) back to (h, w, 4)."""
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    off, idat = 8, b""
+    w = h = 0
+    while off < len(data):
+        n, tag = struct.unpack(">I4s", data[off:off + 8])
+        body = data[off + 8:off + 8 + n]
+        if tag == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert depth == 8 and ctype == 6
+        elif tag == b"IDAT":
+            idat += body
+        off += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    assert (raw[:, 0] == 0).all()
+    return raw[:, 1:].reshape(h, w, 4).copy()
 
 
 class Writer:
@@ -61,6 +94,46 @@ class Writer:
         comp = {np.uint8: 5121, np.uint16: 5123, np.uint32: 5125}[dtype]
         v = self._view(a.tobytes(), target=34963)
         return self._accessor(v, comp, a.size, "SCALAR")
+
+    def texcoords(self, uv: np.ndarray, normalized_u16: bool = False) -> int:
+        """TEXCOORD_0: float VEC2, or normalized unsigned short (rounded from uv in [0, 1])."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        if normalized_u16:
+            q = np.round(np.clip(uv, 0, 1) * 65535).astype(np.uint16)
+            v = self._view(q.tobytes(), target=34962)
+            a = self._accessor(v, 5123, len(uv), "VEC2")
+            self.doc["accessors"][a]["normalized"] = True
+            return a
+        v = self._view(uv.tobytes(), target=34962)
+        return self._accessor(v, 5126, len(uv), "VEC2")
+
+    def image(self, texels: np.ndarray, how: str = "bufferView", file_dir: str | None = None,
+              name: str | None = None) -> int:
+        """An RGBA8 PNG image: in a bufferView, as a data URI, or as a file beside the scene."""
+        data = png_rgba(texels)
+        im = {}
+        if how == "bufferView":
+            im = {"bufferView": self._view(data), "mimeType": "image/png"}
+        elif how == "datauri":
+            im = {"uri": "data:image/png;base64," + base64.b64encode(data).decode()}
+        else:
+            fname = name or f"image{len(self.doc.setdefault('images', []))}.png"
+            with open(file_dir + "/" + fname, "wb") as f:
+                f.write(data)
+            im = {"uri": fname.replace(" ", "%20")}
+        self.doc.setdefault("images", []).append(im)
+        return len(self.doc["images"]) - 1
+
+    def texture(self, source: int) -> int:
+        self.doc.setdefault("samplers", [{"magFilter": 9729, "minFilter": 9987, "wrapS": 10497, "wrapT": 10497}])
+        self.doc.setdefault("textures", []).append({"sampler": 0, "source": source})
+        return len(self.doc["textures"]) - 1
+
+    def textured_material(self, rgba, texture: int) -> int:
+        self.doc["materials"].append({"pbrMetallicRoughness": {"baseColorFactor": [float(c) for c in rgba],
+                                                               "baseColorTexture": {"index": texture},
+                                                               "metallicFactor": 0.0}})
+        return len(self.doc["materials"]) - 1
 
     def material(self, rgba) -> int:
         self.doc["materials"].append({"pbrMetallicRoughness": {"baseColorFactor": [float(c) for c in rgba],
@@ -187,3 +260,41 @@ def kitchen_sink(seed: int = 5):
         dict(world=world(W4, p2, list2), rgba_factor=blue, material=m_blue, model=W4),
     ]
     return w, expect
+
+
+def textured_scene(seed: int = 9, file_dir: str | None = None, file_name: str = "tex_two.png"):
+    """Three textured primitives and one plain one. Images: 0 in a bufferView, 1 as a data
+    URI, 2 as a file beside the scene when file_dir is given (else a bufferView), named
+    file_name (percent-encoded in the uri). Texture 0
+    names image 1 and texture 1 names image 0, so a reader that takes the texture's index
+    as the image index (the reference's importer) and one that follows "source" differ;
+    texture 2 names image 2. Returns the writer, the texels of each image and, per placed
+    primitive, what a reader must give: uv per vertex, texture index and material."""
+    rng = np.random.default_rng(seed)
+    w = Writer()
+    texels = [rng.integers(0, 256, (h, wd, 4), dtype=np.uint8) for h, wd in ((8, 16), (12, 12), (5, 7))]
+    for t in texels:
+        t[..., 3] = np.where(rng.random(t.shape[:2]) < 0.3, rng.integers(0, 20, t.shape[:2]), 255)
+    i0 = w.image(texels[0], "bufferView")
+    i1 = w.image(texels[1], "datauri")
+    i2 = w.image(texels[2], "file" if file_dir else "bufferView", file_dir=file_dir, name=file_name)
+    t0, t1, t2 = w.texture(i1), w.texture(i0), w.texture(i2)
+    mats = [w.textured_material((0.9, 0.8, 1.0, 1.0), t0), w.textured_material((1.0, 0.5, 0.25, 0.8), t1),
+            w.textured_material((1.0, 1.0, 1.0, 1.0), t2), w.material((0.3, 0.6, 0.9, 1.0))]
+    prims, expect = [], []
+    for k, mat in enumerate(mats):
+        nv = 30
+        pos = rng.uniform(-0.6, 0.6, (nv, 3)).astype(np.float32)
+        uv = rng.uniform(-1.0, 2.0, (nv, 2)).astype(np.float32)
+        pr = {"attributes": {"POSITION": w.positions(pos)}, "indices": w.indices(rng.integers(0, nv, 60), np.uint16),
+              "material": mat}
+        u16 = k == 2
+        if u16:
+            uv = np.round(np.clip(uv, 0, 1) * 65535).astype(np.uint16).astype(np.float32) / np.float32(65535.0)
+        if k < 3:
+            pr["attributes"]["TEXCOORD_0"] = w.texcoords(uv, normalized_u16=u16)
+        prims.append(pr)
+        expect.append(dict(uv=uv if k < 3 else None, texture=[t0, t1, t2, -1][k], material=mat))
+    mesh = w.mesh(prims)
+    w.doc["scenes"][0]["nodes"] = [w.node(mesh=mesh, translation=[0.05, -0.02, 0.01])]
+    return w, texels, expect

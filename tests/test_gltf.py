@@ -277,3 +277,129 @@ def test_mutated_glb_never_crashes(files):
         except vgi.VgiError:
             bad += 1
     assert ok + bad == 400 and bad > 0
+
+
+# ---------------------------------------------------------------------------
+# texture coordinates, base colour textures and images
+# ---------------------------------------------------------------------------
+
+def _ref_textures(L, path, mode):
+    L.gref_get_texture.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), C.c_void_p]
+    L.gref_image.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int), C.POINTER(C.c_void_p)]
+    err = C.create_string_buffer(512)
+    h = L.gref_load(path.encode(), mode, err, 512)
+    assert h, err.value
+    prims, images = [], []
+    for i in range(L.gref_count(h)):
+        uvp, nuv, img = C.c_void_p(), C.c_int(), C.c_int()
+        fac = np.zeros(4, np.float32)
+        L.gref_get_texture(h, i, C.byref(uvp), C.byref(nuv), C.byref(img), fac.ctypes.data)
+        uv = np.ctypeslib.as_array(C.cast(uvp, C.POINTER(C.c_float)), (nuv.value * 2,)).copy().reshape(-1, 2) \
+            if nuv.value else None
+        prims.append(dict(texcoords=uv, image=img.value, base_color_factor=tuple(fac)))
+    k = 0
+    while True:
+        w_, h_, comp, bits, px = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_void_p()
+        if L.gref_image(h, k, C.byref(w_), C.byref(h_), C.byref(comp), C.byref(bits), C.byref(px)) != 0:
+            break
+        assert comp.value == 4 and bits.value == 8
+        images.append(np.ctypeslib.as_array(C.cast(px, C.POINTER(C.c_uint8)),
+                                            (h_.value * w_.value * 4,)).copy().reshape(h_.value, w_.value, 4))
+        k += 1
+    L.gref_free(h)
+    return prims, images
+
+
+@pytest.fixture(scope="module")
+def textured(tmp_path_factory):
+    d = tmp_path_factory.mktemp("gltf_tex")
+    w, texels, expect = gw.textured_scene(file_dir=str(d))
+    paths = {"gltf": str(d / "tex.gltf"), "glb": str(d / "tex.glb")}
+    w.write_gltf(paths["gltf"])
+    w.write_glb(paths["glb"])
+    return paths, texels, expect, str(d)
+
+
+@pytest.mark.parametrize("kind", ["gltf", "glb"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_texcoords_and_textures_closed_form(textured, kind, mode):
+    """TEXCOORD_0 as written (float, and normalized u16 as c / 65535); the base colour
+    texture's image: the texture's index in the reference mode (the reference's importer
+    indexes images with it), the texture's source in the factor mode; the factor white in
+    the reference mode; images as stored (bufferView, data URI, a file beside the scene),
+    decoding to the texels written."""
+    paths, texels, expect, _ = textured
+    g = vgi.Gltf(paths[kind], albedo_mode=mode)
+    assert g.info().num_images == 3
+    prims = g.primitives()
+    assert len(prims) == len(expect)
+    source = {0: 1, 1: 0, 2: 2}  # texture -> image, as written
+    for p, e in zip(prims, expect):
+        if e["uv"] is None:
+            assert p["texcoords"] is None and p["image"] == -1
+            continue
+        assert np.array_equal(p["texcoords"], e["uv"])
+        assert p["image"] == (e["texture"] if mode == 0 else source[e["texture"]])
+        if mode == 0:
+            assert p["base_color_factor"] == (1.0, 1.0, 1.0, 1.0)
+    im = [g.image(k) for k in range(3)]
+    assert im[0]["mime_type"] == "image/png" and im[0]["uri"] == ""
+    assert im[1]["mime_type"] == "image/png" and im[1]["uri"] == ""
+    assert im[2]["uri"] == "tex_two.png"
+    for k in range(3):
+        assert np.array_equal(gw.png_decode_rgba(im[k]["data"]), texels[k])
+
+
+@pytest.mark.parametrize("kind", ["gltf", "glb"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_textures_match_reference_parser(textured, kind, mode):
+    L = _ref()
+    paths, texels, _, _ = textured
+    ref, ref_images = _ref_textures(L, paths[kind], mode)
+    g = vgi.Gltf(paths[kind], albedo_mode=mode)
+    got = g.primitives()
+    assert len(got) == len(ref)
+    for a, b in zip(got, ref):
+        if b["texcoords"] is None:
+            assert a["texcoords"] is None
+        else:
+            assert np.array_equal(a["texcoords"], b["texcoords"])
+        assert a["image"] == b["image"]
+        assert np.allclose(a["base_color_factor"], b["base_color_factor"], rtol=0, atol=0)
+    # the bytes the library hands over decode to the pixels tinygltf decoded
+    assert len(ref_images) == 3
+    for k in range(3):
+        assert np.array_equal(ref_images[k], texels[k])
+        assert np.array_equal(gw.png_decode_rgba(g.image(k)["data"]), ref_images[k])
+
+
+def test_percent_encoded_image_uri(tmp_path):
+    """An image file whose name has a space is named "tex%20two.png" in the uri, which the
+    reader decodes (glTF URIs are RFC 3986 references)."""
+    w, texels, _ = gw.textured_scene(file_dir=str(tmp_path), file_name="tex two.png")
+    path = str(tmp_path / "s.gltf")
+    w.write_gltf(path)
+    g = vgi.Gltf(path)
+    assert g.image(2)["uri"] == "tex two.png"
+    assert np.array_equal(gw.png_decode_rgba(g.image(2)["data"]), texels[2])
+
+
+def test_external_image_is_read_on_request(textured, tmp_path):
+    paths, texels, _, d = textured
+    with open(paths["gltf"], "rb") as f:
+        js = f.read()
+    g = vgi.Gltf(data=js, base_dir=d)
+    assert np.array_equal(gw.png_decode_rgba(g.image(2)["data"]), texels[2])
+    # the same document elsewhere: the file is not there
+    other = tmp_path / "elsewhere"
+    other.mkdir()
+    import shutil
+    shutil.copy(paths["gltf"].replace(".gltf", ".bin"), str(other / "tex.bin"))
+    g2 = vgi.Gltf(data=js, base_dir=str(other))
+    assert g2.image(0)["data"]  # embedded images need no file
+    with pytest.raises(vgi.VgiError):
+        g2.image(2)
+    with pytest.raises(vgi.VgiError):
+        g2.set_image_rgba8(3, np.zeros((2, 2, 4), np.uint8))
