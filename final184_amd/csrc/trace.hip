@@ -83,6 +83,28 @@ struct AccPair {
     a.z = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, bb), w, a.z, false);
     a.w = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, aa), w, a.w, false);
   }
+  // The first row of a gather, as the three-operand dot product whose destination is not
+  // its addend: the compiler selects the two-operand form, whose destination is the
+  // accumulator, and so first moves 0 into each accumulator of a sum that starts at zero
+  // (INIT), or copies the running sum when the scalar-load and vector-load paths of a
+  // gather keep it in different registers. SGPR: the texel words are wave-uniform scalars.
+  template <bool SGPR, bool INIT>
+  __device__ static void first(float4& a, half2_t w, uint32_t rr, uint32_t gg, uint32_t bb, uint32_t aa) {
+    const uint32_t wv = __builtin_bit_cast(uint32_t, w);
+    const uint32_t t[4] = {rr, gg, bb, aa};
+    float* o[4] = {&a.x, &a.y, &a.z, &a.w};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if constexpr (INIT) {
+        if constexpr (SGPR) __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "s"(t[c]), "v"(wv));
+        else __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "v"(t[c]), "v"(wv));
+      } else {
+        const float in = *o[c];
+        if constexpr (SGPR) __asm__("v_dot2_f32_f16 %0, %1, %2, %3" : "=v"(*o[c]) : "s"(t[c]), "v"(wv), "v"(in));
+        else __asm__("v_dot2_f32_f16 %0, %1, %2, %3" : "=v"(*o[c]) : "v"(t[c]), "v"(wv), "v"(in));
+      }
+    }
+  }
 };
 
 template <int PREC>
@@ -193,7 +215,8 @@ __device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, flo
 typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
 #define VGI_CONSTANT __attribute__((address_space(4)))
 
-template <int PREC>
+// INIT: acc is zero on entry (the sample's first gather)
+template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
                                                float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
@@ -213,7 +236,8 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const vgi_u32x4 e = *(const VGI_CONSTANT vgi_u32x4*)(base + soff[q]);
-        AccPair::add(acc, pw.r[q], e[0], e[1], e[2], e[3]);
+        if (q == 0) AccPair::first<true, INIT>(acc, pw.r[q], e[0], e[1], e[2], e[3]);
+        else AccPair::add(acc, pw.r[q], e[0], e[1], e[2], e[3]);
         abits |= e[3];
       }
       return;
@@ -224,7 +248,8 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
     for (int q = 0; q < 4; q++) {
       const auto e = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
       const uint32_t rr = e[0], gg = e[1], bb = e[2], aa = e[3];
-      AccPair::add(acc, pw.r[q], rr, gg, bb, aa);
+      if (q == 0) AccPair::first<false, INIT>(acc, pw.r[q], rr, gg, bb, aa);
+      else AccPair::add(acc, pw.r[q], rr, gg, bb, aa);
       abits |= aa;
     }
   } else {
@@ -274,7 +299,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 }
 
 // ---- per-lane level: 64-bit global loads ----
-template <int PREC>
+template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, float ux, float uy,
                                             float uz, float wl, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
@@ -287,7 +312,8 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint4 e = gld4(base + offs[q]);
-      AccPair::add(acc, pw.r[q], e.x, e.y, e.z, e.w);
+      if (q == 0) AccPair::first<false, INIT>(acc, pw.r[q], e.x, e.y, e.z, e.w);
+      else AccPair::add(acc, pw.r[q], e.x, e.y, e.z, e.w);
       abits |= e.w;
     }
   } else {
@@ -369,11 +395,11 @@ __device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const S
       float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
       uint32_t abits = 0;
       if (e.code & kSchedTop) {
-        gather_uniform<PREC>(P, e.code & ~kSchedTop, ux, uy, uz, 1.0f, s, abits);
+        gather_uniform<PREC, true>(P, e.code & ~kSchedTop, ux, uy, uz, 1.0f, s, abits);
         count += 1;
       } else {
-        gather_uniform<PREC>(P, e.code + 1, ux, uy, uz, e.fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(e.code)) gather_uniform<PREC>(P, e.code, ux, uy, uz, e.gr, s, abits);
+        gather_uniform<PREC, true>(P, e.code + 1, ux, uy, uz, e.fr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(e.code)) gather_uniform<PREC, false>(P, e.code, ux, uy, uz, e.gr, s, abits);
         count += 2;
       }
       composite(s, cr, cg, cb, a);
@@ -412,8 +438,8 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t abits = 0;
     if (lvl >= top) {
-      if constexpr (BUF) gather_uniform<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // uniform level
-      else gather_lane<PREC>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
+      if constexpr (BUF) gather_uniform<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // uniform level
+      else gather_lane<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
       count += 1;
     } else {
       const float fl = floorf(lvl);
@@ -423,11 +449,11 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
       // level's descriptor and pitches are scalars
       const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
       if (BUF && __all(l0 == lu)) {
-        gather_uniform<PREC>(P, lu + 1, ux, uy, uz, fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(lu)) gather_uniform<PREC>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
+        gather_uniform<PREC, true>(P, lu + 1, ux, uy, uz, fr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(lu)) gather_uniform<PREC, false>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
       } else {
-        gather_lane<PREC>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(l0)) gather_lane<PREC>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+        gather_lane<PREC, true>(P, l0 + 1, ux, uy, uz, fr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(l0)) gather_lane<PREC, false>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
       }
       count += 2;
     }
