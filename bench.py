@@ -108,6 +108,40 @@ def run_shadowed(ctx, vgi, w, mesh, prec, reps=10):
     return out
 
 
+def run_textured(ctx, vgi, w, reps=10):
+    """The voxelize front end with the workload's triangles as one textured mesh (uv from
+    each vertex's position, a seeded 1024 x 1024 RGBA8 texture with a tenth of its texels
+    below the alpha discard) against the same triangles with per-triangle albedo: mean
+    HIP-event time of each over `reps` calls."""
+    rng = np.random.default_rng(5)
+    pos = w.positions.reshape(-1, 3)
+    uv = np.ascontiguousarray(pos[:, :2] * 4.0 + pos[:, 2:3], np.float32)
+    tex = rng.integers(0, 256, (1024, 1024, 4), dtype=np.uint8)
+    tex[..., 3] = np.where(rng.random((1024, 1024)) < 0.1, 0, 255).astype(np.uint8)
+    grid = vgi.make_grid(**w.grid)
+    meshes = {"plain": vgi.MeshData(ctx, pos, albedo=w.albedo),
+              "textured": vgi.MeshData(ctx, pos, texcoords=uv, texture=tex)}
+    scene = ctx.scene()
+    out = {}
+    try:
+        for name, m in meshes.items():
+            scene.voxelize([m], grid)  # warm-up (sizes the stream's workspace)
+            ms = []
+            ctx.set_profiling(True)
+            for _ in range(reps):
+                scene.voxelize([m], grid)
+                ms.append(ctx.timings().voxelize_total_ms)
+            ctx.set_profiling(False)
+            out[name] = float(np.mean(ms))
+    finally:
+        scene.release()
+        for m in meshes.values():
+            for a in (m.pos, m.albedo, m.uv, m.tex):
+                if a is not None:
+                    a.free()
+    return out
+
+
 def run_pipelined(ctx, vgi, w, mesh, gb, prec, steps, warmup):
     """Frames overlapped on two streams: the voxelize of frame k+1 (stream B) runs while
     frame k is traced (stream A). Ordering by vgi_stream_wait: a frame's mip build waits for
@@ -713,6 +747,8 @@ def main() -> int:
         pipe_ms = run_pipelined(ctx, vgi, w, mesh, gb, head["prec"], max(args.steps, 20), args.warmup)
         log(f"{head['prec']}: two-stream overlapped frames {pipe_ms:.3f} ms per frame "
             f"(sequential {head['frame_ms']:.3f} ms)")
+    tex_ms = run_textured(ctx, vgi, w)
+    log(f"voxelize with a textured mesh {tex_ms['textured']:.3f} ms (per-triangle albedo {tex_ms['plain']:.3f} ms)")
     shadow_ms = run_shadowed(ctx, vgi, w, mesh, head["prec"])
     log(f"{head['prec']}: mip build with shadowed lights {shadow_ms['shadowed']:.3f} ms "
         f"(plain {shadow_ms['plain']:.3f} ms)")
@@ -757,6 +793,9 @@ def main() -> int:
         "overlapped_frames": None if pipe_ms is None else {
             "streams": 2, "frame_ms": pipe_ms, "mpixels_s": px / (pipe_ms * 1e-3) / 1e6,
             "note": "voxelize of frame k+1 on a second stream while frame k is traced"},
+        "textured_voxelize": {"texture": "1024x1024 RGBA8, 10% of texels below the alpha discard",
+                              "voxelize_ms": tex_ms["textured"], "plain_voxelize_ms": tex_ms["plain"],
+                              "note": "texture(uv) x factor sampled per voxel column, alpha < 0.05 discarded"},
         "shadowed_mips": {"lights": f"the workload's {len(w.lights)} lights casting shadows",
                           "mip_storage": head["prec"], "mips_ms": shadow_ms["shadowed"],
                           "plain_mips_ms": shadow_ms["plain"],
