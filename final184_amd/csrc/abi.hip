@@ -220,7 +220,7 @@ struct vgi_scene {
   uint32_t prec = 0;
   void* level[VGI_MAX_LEVELS] = {};
   size_t level_bytes[VGI_MAX_LEVELS] = {};
-  uint32_t* live = nullptr;  // one word per 16^3 tile of level 0 (the mip kernel's skip state)
+  uint32_t* live = nullptr;  // per 16^3 tile of level 0: a word, then 8 row words (the mip kernel's skip state)
   size_t live_bytes = 0;
   uint32_t alloc_levels = 0, alloc_n = 0, alloc_prec = 0;
   bool has_voxels = false, has_mips = false;
@@ -773,10 +773,11 @@ int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
   s->live_bytes = 0;
   s->alloc_n = s->alloc_levels = 0;
   {
-    // The levels start all zero and a tile's texels are stored only while the tile or its
-    // previous build held occupied voxels, so every texel of the levels stays exact.
+    // The levels start all zero and a tile's texels (a level-0 x-row's) are stored only
+    // while the tile (the row) or its previous build held occupied voxels, so every texel
+    // of the levels stays exact: one word per tile, then eight row words per tile.
     const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u);
-    const size_t bytes = tpa * tpa * tpa * sizeof(uint32_t);
+    const size_t bytes = tpa * tpa * tpa * 9 * sizeof(uint32_t);
     hipError_t e = hipMalloc((void**)&s->live, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -1405,7 +1406,7 @@ uint64_t vgi_scene_bytes_estimate(const vgi_grid* g, uint32_t precision, uint64_
   const uint64_t n = g->n;
   const uint64_t tpa = n / std::min<uint64_t>(n, 16);  // 16^3 tiles per axis (mip tile state)
   uint64_t b = n * n * n * kAccWords * 4 + (n / 8) * (n / 8) * (n / 8) * 64 + (n / 8) * (n / 8) * (n / 8) * 8 +
-               tpa * tpa * tpa * 4;
+               tpa * tpa * tpa * 9 * 4;
   for (uint32_t k = 0; k < g->levels; k++) {
     const uint64_t p = (n >> k) + 2;
     b += p * p * p * texel_bytes(precision);

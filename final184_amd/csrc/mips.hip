@@ -19,9 +19,11 @@
 // (n_k + 2)^3 entries of two texels, below). The levels
 // are zero-filled when allocated, and the first launch stores a 16^3 tile (its level-0
 // texels and its blocks of the four levels above) only while the tile holds occupied
-// voxels, plus once, as zeros, when it has lost them; a per-tile word records which. A
-// surface leaves most tiles empty (70% at 256^3 for the room), so most of the level
-// bytes are not rewritten each build, and every texel still equals the full computation.
+// voxels, plus once, as zeros, when it has lost them; a per-tile word records which, and
+// for the 16-byte texel forms a level-0 x-row of a stored tile is stored only while it
+// holds voxels or held them at the last build (one bit per row). A surface leaves most
+// tiles empty (70% at 256^3 for the room), so most level bytes are not rewritten each
+// build, and every texel still equals the full computation.
 #include <cmath>
 
 #include "vgi_internal.hpp"
@@ -342,18 +344,34 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     // each 2x2x2 group summed from LDS.
     __shared__ typename TX::Packed s_val[4096];
     __shared__ uint32_t s_nocc, s_was;
+    __shared__ uint32_t s_rows_now[8], s_rows_prev[8];  // one bit per x-row (ly + S lz) of the tile
     // the tile's state from the previous build, read now so that its latency overlaps the
     // occupancy pass: 1 while its stored texels (level 0 and its blocks of the levels
-    // produced here) are not all zero
+    // produced here) are not all zero; and per level-0 x-row, 1 while the row's stored
+    // texels are not all zero (the rows' words follow the tiles' words)
+    // (rows are skipped for the 16-byte texel forms, FP32 and FP16 x-pairs, whose level-0
+    // stores cost most; for 8-byte texels the row bookkeeping cost more than it saved)
+    constexpr bool ROWS = PREC == VGI_PREC_FP32 || PREC == VGI_PREC_FP16X2;
+    const uint32_t ntiles = tpa * tpa * tpa, nrw = (S * S + 31u) >> 5;
     const uint32_t was_v = g == 0 ? gld(P.live + tile) : 0u;
+    const uint32_t rows_v = ROWS && g < nrw ? gld(P.live + ntiles + tile * 8u + g) : 0u;
     uint16_t* s_item = (uint16_t*)s_a;  // occupied voxel list (group * 8 + bit), before s_a is used
     static_assert(sizeof(s_a) >= 4096 * sizeof(uint16_t), "item list fits in s_a");
     const uint32_t ssh = gsh + 1u, NV = S * S * S;
     const uint32_t lane = g & 63u;
     for (uint32_t v = g; v < NV; v += 512) s_val[v] = TX::pack(make_float4(0.f, 0.f, 0.f, 0.f));
     if (g == 0) s_nocc = 0u;
+    if (g < 8) s_rows_now[g] = 0u;
     __syncthreads();
     const uint32_t occ = grp ? group_occupancy(P, tx * S + 2 * gx, ty * S + 2 * gy, tz * S + 2 * gz) : 0u;
+    if constexpr (ROWS) {
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++)  // x-rows (2 gy + dy, 2 gz + dz) of the group that hold voxels
+        if ((occ >> (2 * q)) & 3u) {
+          const uint32_t row = (2 * gy + (q & 1u)) + S * (2 * gz + (q >> 1));
+          atomicOr(&s_rows_now[row >> 5], 1u << (row & 31u));
+        }
+    }
     const uint32_t cnt = (uint32_t)__builtin_popcount(occ);
     uint32_t incl = cnt;  // inclusive scan of the counts over the wavefront
 #pragma unroll
@@ -368,6 +386,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     for (uint32_t m = occ; m != 0u; m &= m - 1u) s_item[k++] = (uint16_t)(g * 8u + (uint32_t)__builtin_ctz(m));
     if (g < 256) s_lut[g] = lut_v;
     if (g == 0) s_was = was_v;
+    if (g < 8) s_rows_prev[g] = rows_v;
     __syncthreads();
     const uint32_t nocc = s_nocc;
     // A tile without occupied voxels whose stored texels are already zero (never written
@@ -376,6 +395,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     const uint32_t was = s_was;
     if (nocc == 0u && was == 0u) return;  // uniform: both values come from LDS
     if (g == 0 && (nocc != 0u) != (was != 0u)) gst(P.live + tile, nocc != 0u ? 1u : 0u);
+    if (ROWS && g < nrw && s_rows_now[g] != s_rows_prev[g]) gst(P.live + ntiles + tile * 8u + g, s_rows_now[g]);
     for (uint32_t it = g; it < nocc; it += 512) {
       const uint32_t e = s_item[it], gi = e >> 3, bit = e & 7u;
       const uint32_t lx = 2 * (gi & (G - 1u)) + (bit & 1u);
@@ -390,6 +410,8 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       // low halves here and its high halves from the next tile, the entry before the
       // tile's first x its high halves from here
       for (uint32_t v = g; v < NV; v += 512) {
+        const uint32_t row = v >> ssh;  // rows that hold voxels now or held them at the last build
+        if (!(((s_rows_now[row >> 5] | s_rows_prev[row >> 5]) >> (row & 31u)) & 1u)) continue;
         const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
         const size_t pi = padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns);
         if (lx + 1u < S) TX::store_entry(P.level[0], pi, s_val[v], s_val[v + 1]);
@@ -398,6 +420,8 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       }
     } else {
       for (uint32_t v = g; v < NV; v += 512) {
+        const uint32_t row = v >> ssh;
+        if (ROWS && !(((s_rows_now[row >> 5] | s_rows_prev[row >> 5]) >> (row & 31u)) & 1u)) continue;
         const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
         TX::store_packed(P.level[0], padded_index(tx * S + lx, ty * S + ly, tz * S + lz, ns), s_val[v]);
       }

@@ -162,7 +162,9 @@ struct MipScene {
   const uint32_t* acc;
   const uint32_t* mask;
   const unsigned long long* bin_packed;
-  uint32_t* live;  // per 16^3 tile of level 0: 1 while the tile's stored texels are not all zero
+  uint32_t* live;  // per 16^3 tile of level 0: 1 while the tile's stored texels are not all zero;
+                   // after the tiles' words, 8 words per tile: bit (ly + 16 lz) while that
+                   // x-row's stored level-0 texels are not all zero
   void* level[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t n;

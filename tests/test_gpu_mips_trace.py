@@ -105,7 +105,8 @@ def test_mips_room_levels(ctx):
 def test_mips_after_geometry_changes(ctx):
     """Rebuilding one scene's mips after its geometry changed: 16^3 tiles that lose their
     voxels must read as zeros, tiles that gain voxels as the oracle's texels, at every
-    level, through a change of precision too."""
+    level, through changes of precision too; a different soup then puts voxels in other
+    level-0 rows of the same tiles (rows that held voxels must read as zeros)."""
     vgi = _vgi()
     w = sg.room(ntri=20_000, n=64, levels=6, width=16, height=16)
     pos, alb = w.positions, w.albedo
@@ -115,14 +116,20 @@ def test_mips_after_geometry_changes(ctx):
     grid = vgi.make_grid(**w.grid)
     lights = vgi.make_lights(w.lights)
     sc = ctx.scene()
-    for step, (sel, prec) in enumerate([(None, "fp16"), (half, "fp16"), (None, "fp16"), (half, "fp32")]):
-        p, a = (pos, alb) if sel is None else (pos[sel], alb[sel])
+    soup = sg.random_soup(seed=77, ntri=4000, n=64)  # other rows of the same tiles
+    steps = [(None, "fp16"), (half, "fp16"), (None, "fp16"), (half, "fp32"), ("soup", "fp16x2"), (None, "fp16x2"),
+             ("soup", "fp16x2")]
+    for step, (sel, prec) in enumerate(steps):
+        if isinstance(sel, str):
+            p, a = soup.positions, soup.albedo
+        else:
+            p, a = (pos, alb) if sel is None else (pos[sel], alb[sel])
         print(f"case room 64^3 step {step}: {len(p)} triangles, mips {prec}", flush=True)
         mesh = vgi.MeshData(ctx, p.reshape(-1, 3), albedo=a)
         sc.voxelize([mesh], grid)
         sc.build_mips(lights, vgi.PREC_NAMES[prec])
         acc, _ = orc.voxelize(p, w.grid, albedo=a)
-        ref = orc.inject_mips(acc, w.grid, w.lights, prec)
+        ref = orc.inject_mips(acc, w.grid, w.lights, ORC_PREC[prec])
         for k in range(w.grid["levels"]):
             got = sc.read_level(k)
             bad = (got != ref[k]).any(axis=-1)
