@@ -1015,6 +1015,17 @@ void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tunin
   P.out_s = (float4*)it.out_specular;
 }
 
+// (-w, w) and (w, 0) as packed halves, each rounded to nearest even
+static HalfW half_weight(float w) {
+  auto bits = [](float f) {
+    const _Float16 h = (_Float16)f;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return (uint32_t)b;
+  };
+  return HalfW{bits(-w) | (bits(w) << 16), bits(w)};
+}
+
 // March schedule of a cone whose sample distances do not depend on the pixel (a fixed
 // aperture): the per-pixel recurrence of the cone march evaluated once, in FP32 -- diameter
 // d = max(h, 2 t tan), level max(0, log2(d / h)), next t = t + step d -- while
@@ -1040,6 +1051,8 @@ void march_schedule(const TraceParams& P, float tan_half, std::vector<SchedEntry
       e.gr = 1.0f - e.fr;
       e.code = (uint32_t)fl;
     }
+    e.hfr = half_weight(e.fr);
+    e.hgr = half_weight(e.gr);
     out.push_back(e);
     const float adv = P.step * diam;
     t = t + adv;

@@ -125,21 +125,12 @@ __device__ __forceinline__ bool skip_fine_ok(uint32_t l0) {
 }
 
 // Trilinear footprint of level k at level-0 voxel coordinates u (texel centres at i+0.5):
-// index of texel (x0, y0, z0) in the padded level, and the 8 weights scaled by wl.
-struct Foot {
-  uint32_t idx;
-  float w00, w10, w01, w11;  // xy weights
-  float wz0, wz1;            // z weights x level weight
+// texel (ix, iy, iz) of the level (unpadded) and the fractions toward the next texel on
+// each axis.
+struct Cell {
+  int ix, iy, iz;
+  float fx, fy, fz;
 };
-
-// FP16 weight pairs of the four rows (y0 z0, y1 z0, y0 z1, y1 z1) of a footprint, as
-// packed half products: (ex, fx) * (wy * wz) with wy * wz rounded to FP16 once.
-struct PairW {
-  half2_t r[4];
-};
-template <bool UNIFORM_LEVEL>
-__device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, float ux, float uy, float uz, float wl,
-                                                    PairW& pw);
 
 // floor(x) as an integer in one instruction (the compiler emits floor, then convert); x is
 // finite and within the padded grid here (positions outside [0, n]^3 end the march first)
@@ -160,50 +151,92 @@ __device__ __forceinline__ uint32_t mad_i24(int a, uint32_t b, uint32_t c) {
 }
 
 template <bool UNIFORM_LEVEL>
-__device__ __forceinline__ Foot footprint(uint32_t k, uint32_t p1, float ux, float uy, float uz,
-                                          float wl) {
+__device__ __forceinline__ Cell cell_of(uint32_t k, float ux, float uy, float uz) {
   // 2^-k: for a wave-uniform level built from its bit pattern (k <= 126) by scalar
   // instructions, per lane by one ldexp
   const float sc = UNIFORM_LEVEL ? __builtin_bit_cast(float, (127u - k) << 23) : __builtin_ldexpf(1.0f, -(int)k);
   const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
   // integer part by a flooring convert, fraction by v_fract (x - floor(x))
-  const int ix = cvt_floor(gx), iy = cvt_floor(gy), iz = cvt_floor(gz);
-  const float fx = __builtin_amdgcn_fractf(gx), fy = __builtin_amdgcn_fractf(gy),
-              fz = __builtin_amdgcn_fractf(gz);
-  const float ex = 1.0f - fx, ey = 1.0f - fy;
+  return Cell{cvt_floor(gx), cvt_floor(gy), cvt_floor(gz), __builtin_amdgcn_fractf(gx),
+              __builtin_amdgcn_fractf(gy), __builtin_amdgcn_fractf(gz)};
+}
+
+// Index of texel (ix + 1, iy + 1, iz + 1) (the +1: padded border) of a padded level of row
+// pitch p1, per lane: iz >= -1 and every factor is < 2^11, so 24-bit multiplies are exact;
+// row = (iz + 1) p1 + iy + 1, written as iz p1 + (p1 + iy + 1) for one multiply-add.
+__device__ __forceinline__ uint32_t padded_texel(const Cell& c, uint32_t p1) {
+  const uint32_t row = mad_i24<false>(c.iz, p1, p1 + (uint32_t)(c.iy + 1));
+  return __umul24(p1, row) + (uint32_t)(c.ix + 1);  // < 1026^3 < 2^31
+}
+
+// Byte offset of the same texel with wave-uniform pitches (TB bytes a texel, rowb a row,
+// slab a slice): ix TB + iy rowb + iz slab + (TB + rowb + slab), one shift-add and two
+// 24-bit multiply-adds. ix, iy, iz >= -1, and on a level under 4 GiB slab < 2^23 (slab =
+// TB p1^2 >= 2^23 needs p1 >= 724 for 16-byte texels and p1 >= 1024 for 8-byte ones, and
+// then TB p1^3 >= 4 GiB), so every factor fits 24 signed bits; the terms add modulo 2^32
+// to the offset, which is below 2^32.
+template <uint32_t TB>
+__device__ __forceinline__ uint32_t texel_offset(const Cell& c, uint32_t rowb, uint32_t slab, uint32_t base) {
+  static_assert(TB == 8 || TB == 16, "texel size");
+  uint32_t o;  // ix TB + base as one shift-add (the compiler splits it around the scalar sum)
+  __asm__("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(o) : "v"(c.ix), "i"(TB == 16 ? 4 : 3), "s"(base));
+  return mad_i24<true>(c.iz, slab, mad_i24<true>(c.iy, rowb, o));
+}
+
+// The 8 weights of a footprint, scaled by the level weight wl: xy weights, z weights.
+struct Foot {
+  float w00, w10, w01, w11;
+  float wz0, wz1;
+};
+__device__ __forceinline__ Foot foot_weights(const Cell& c, float wl) {
+  const float ex = 1.0f - c.fx, ey = 1.0f - c.fy;
   Foot f;
-  // +1: padded border; iz >= -1 and every factor is < 2^11, so 24-bit multiplies are exact.
-  // row = (iz + 1) p1 + iy + 1, written as iz p1 + (p1 + iy + 1) for one multiply-add
-  const uint32_t row = mad_i24<UNIFORM_LEVEL>(iz, p1, p1 + (uint32_t)(iy + 1));
-  f.idx = __umul24(p1, row) + (uint32_t)(ix + 1);  // < 1026^3 < 2^31
   f.w00 = ex * ey;
-  f.w10 = fx * ey;
-  f.w01 = ex * fy;
-  f.w11 = fx * fy;
-  f.wz0 = (1.0f - fz) * wl;
-  f.wz1 = fz * wl;
+  f.w10 = c.fx * ey;
+  f.w01 = ex * c.fy;
+  f.w11 = c.fx * c.fy;
+  f.wz0 = (1.0f - c.fz) * wl;
+  f.wz1 = c.fz * wl;
   return f;
 }
 
-template <bool UNIFORM_LEVEL>
-__device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, float ux, float uy, float uz, float wl,
-                                                    PairW& pw) {
-  const float sc = UNIFORM_LEVEL ? __builtin_bit_cast(float, (127u - k) << 23) : __builtin_ldexpf(1.0f, -(int)k);
-  const float gx = ux * sc - 0.5f, gy = uy * sc - 0.5f, gz = uz * sc - 0.5f;
-  const int ix = cvt_floor(gx), iy = cvt_floor(gy), iz = cvt_floor(gz);
-  const float fx = __builtin_amdgcn_fractf(gx), fy = __builtin_amdgcn_fractf(gy),
-              fz = __builtin_amdgcn_fractf(gz);
-  const float wz0 = (1.0f - fz) * wl, wz1 = fz * wl;
-  const half2_t xw = {(_Float16)(1.0f - fx), (_Float16)fx};
-  const half2_t r0 = {(_Float16)((1.0f - fy) * wz0), (_Float16)(fy * wz0)};  // rows y0 z0, y1 z0
-  const half2_t r1 = {(_Float16)((1.0f - fy) * wz1), (_Float16)(fy * wz1)};  // rows y0 z1, y1 z1
-  pw.r[0] = xw * (half2_t){r0.x, r0.x};
-  pw.r[1] = xw * (half2_t){r0.y, r0.y};
-  pw.r[2] = xw * (half2_t){r1.x, r1.x};
-  pw.r[3] = xw * (half2_t){r1.y, r1.y};
-  const uint32_t row = mad_i24<UNIFORM_LEVEL>(iz, p1, p1 + (uint32_t)(iy + 1));
-  return __umul24(p1, row) + (uint32_t)(ix + 1);
+// FP16 weight pairs of the four rows (y0 z0, y1 z0, y0 z1, y1 z1) of a footprint, in
+// packed-half arithmetic: the fractions rounded to FP16, the x pair (w (1 - fx), w fx)
+// from the level weight's constants (HalfW), the y and z pairs (1 - f, f), the four row
+// products wy wz, and each row's pair as x pair times row product. Eleven instructions for
+// the 16 weights.
+struct PairW {
+  half2_t r[4];
+};
+// (a, b) rounded to FP16 (to nearest even) in one instruction; written out, since the
+// compiler converts the halves one at a time when they are used apart
+__device__ __forceinline__ half2_t cvt_pk_half(float a, float b) {
+  half2_t r;
+  __asm__("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
 }
+__device__ __forceinline__ void pair_weights(const Cell& c, HalfW w, PairW& pw) {
+  const half2_t np1 = {(_Float16)-1.0f, (_Float16)1.0f}, one0 = {(_Float16)1.0f, (_Float16)0.0f};
+  const half2_t fxy = cvt_pk_half(c.fx, c.fy);
+  const half2_t fz = cvt_pk_half(c.fz, c.fz);
+  const half2_t xw = (half2_t){fxy.x, fxy.x} * __builtin_bit_cast(half2_t, w.np) + __builtin_bit_cast(half2_t, w.w0);
+  const half2_t yw = (half2_t){fxy.y, fxy.y} * np1 + one0;
+  const half2_t zw = fz * np1 + one0;
+  const half2_t r01 = yw * (half2_t){zw.x, zw.x};  // rows y0 z0, y1 z0
+  const half2_t r23 = yw * (half2_t){zw.y, zw.y};  // rows y0 z1, y1 z1
+  pw.r[0] = xw * (half2_t){r01.x, r01.x};
+  pw.r[1] = xw * (half2_t){r01.y, r01.y};
+  pw.r[2] = xw * (half2_t){r23.x, r23.x};
+  pw.r[3] = xw * (half2_t){r23.y, r23.y};
+}
+
+// (-w, w) and (w, 0) as packed halves, per lane (the schedule holds them for fixed cones)
+__device__ __forceinline__ HalfW half_weight(float w) {
+  const half2_t np = {(_Float16)(-w), (_Float16)w};
+  const half2_t w0 = {(_Float16)w, (_Float16)0.0f};
+  return HalfW{__builtin_bit_cast(uint32_t, np), __builtin_bit_cast(uint32_t, w0)};
+}
+constexpr HalfW kHalfOne = {0x3C00BC00u, 0x00003C00u};  // w = 1: (-1, 1), (1, 0)
 
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
 // FP32: when every active lane's footprint is the same 2x2x2 texel block (usual at coarse
@@ -215,18 +248,20 @@ __device__ __forceinline__ uint32_t footprint_pairs(uint32_t k, uint32_t p1, flo
 typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
 #define VGI_CONSTANT __attribute__((address_space(4)))
 
-// INIT: acc is zero on entry (the sample's first gather)
+// INIT: acc is zero on entry (the sample's first gather). The level weight is wl (FP32
+// forms) or hw (the FP16 x-pair form).
 template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
-                                               float uz, float wl, float4& acc, uint32_t& abits) {
+                                               float uz, float wl, HalfW hw, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const LevelDesc L = P.lv[k];  // one scalar load: the level is wave-uniform
-  const uint32_t p1 = L.p1;
-  const int rowb = (int)(p1 * TB), slab = (int)(p1 * p1 * TB);
+  const uint32_t rowb = L.p1 * TB, slab = L.p1 * rowb;
+  const Cell c = cell_of<true>(k, ux, uy, uz);
+  const uint32_t off = texel_offset<TB>(c, rowb, slab, TB + rowb + slab);  // levels here are < 2^32 bytes
+  const int soff[4] = {0, (int)rowb, (int)slab, (int)(rowb + slab)};
   if constexpr (PREC == VGI_PREC_FP16X2) {
     PairW pw;
-    const uint32_t off = footprint_pairs<true>(k, p1, ux, uy, uz, wl, pw) * TB;
-    const int soff[4] = {0, rowb, slab, rowb + slab};
+    pair_weights(c, hw, pw);
     // When every active lane's footprint is the same 2x2x2 block (usual at coarse levels),
     // its four rows are read once with scalar loads and broadcast, sparing the
     // texture-address path four 16-byte vector loads.
@@ -253,9 +288,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
       abits |= aa;
     }
   } else {
-    const Foot f = footprint<true>(k, p1, ux, uy, uz, wl);
-    const uint32_t off = f.idx * TB;  // levels on this path are < 2^32 bytes
-    const int soff[4] = {0, rowb, slab, rowb + slab};
+    const Foot f = foot_weights(c, wl);
     const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
     if constexpr (PREC == VGI_PREC_FP32) {
       const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
@@ -301,14 +334,16 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 // ---- per-lane level: 64-bit global loads ----
 template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, float ux, float uy,
-                                            float uz, float wl, float4& acc, uint32_t& abits) {
+                                            float uz, float wl, HalfW hw, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
+  const Cell c = cell_of<false>(k, ux, uy, uz);
+  const char* base = (const char*)P.level[k] + (size_t)padded_texel(c, p1) * TB;
+  const size_t offs[4] = {0, rowb, slab, rowb + slab};
   if constexpr (PREC == VGI_PREC_FP16X2) {
     PairW pw;
-    const char* base = (const char*)P.level[k] + (size_t)footprint_pairs<false>(k, p1, ux, uy, uz, wl, pw) * TB;
-    const size_t offs[4] = {0, rowb, slab, rowb + slab};
+    pair_weights(c, hw, pw);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint4 e = gld4(base + offs[q]);
@@ -317,9 +352,7 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
       abits |= e.w;
     }
   } else {
-    const Foot f = footprint<false>(k, p1, ux, uy, uz, wl);
-    const char* base = (const char*)P.level[k] + (size_t)f.idx * TB;
-    const size_t offs[4] = {0, rowb, slab, rowb + slab};
+    const Foot f = foot_weights(c, wl);
     const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
   #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -381,31 +414,29 @@ __device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const S
   const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
   const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
   const float cutoff = P.cutoff;
-  // Wave-uniform loop control: the sample index is the same for every lane (the entry is
-  // read with scalar loads), lanes whose march has ended idle under the exec mask, and the
-  // loop ends when none is left.
-  bool alive = true;
-  for (uint32_t it = 0; it < nsched; it++) {
+  // The sample index is the same for every lane still marching (the entry is read with
+  // scalar loads); a lane whose march ends leaves the loop, and the loop ends when none is
+  // left.
+  const uint32_t n = __builtin_amdgcn_readfirstlane(nsched);
+  for (uint32_t it = 0; it < n; it++) {
     const SchedEntry e = sld_rec(sched + it);  // scalar load: the index is wave-uniform
     const float ux = fmaf(udx, e.t, uox);
     const float uy = fmaf(udy, e.t, uoy);
     const float uz = fmaf(udz, e.t, uoz);
-    alive = alive && inside_grid(ux, uy, uz, nf);
-    if (alive) {
-      float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-      uint32_t abits = 0;
-      if (e.code & kSchedTop) {
-        gather_uniform<PREC, true>(P, e.code & ~kSchedTop, ux, uy, uz, 1.0f, s, abits);
-        count += 1;
-      } else {
-        gather_uniform<PREC, true>(P, e.code + 1, ux, uy, uz, e.fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(e.code)) gather_uniform<PREC, false>(P, e.code, ux, uy, uz, e.gr, s, abits);
-        count += 2;
-      }
-      composite(s, cr, cg, cb, a);
-      alive = a < cutoff;
+    if (!inside_grid(ux, uy, uz, nf)) break;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t abits = 0;
+    if (e.code & kSchedTop) {
+      gather_uniform<PREC, true>(P, e.code & ~kSchedTop, ux, uy, uz, 1.0f, kHalfOne, s, abits);
+      count += 1;
+    } else {
+      gather_uniform<PREC, true>(P, e.code + 1, ux, uy, uz, e.fr, e.hfr, s, abits);
+      if (abits || !skip_fine_ok<PREC>(e.code))
+        gather_uniform<PREC, false>(P, e.code, ux, uy, uz, e.gr, e.hgr, s, abits);
+      count += 2;
     }
-    if (!__any(alive)) break;
+    composite(s, cr, cg, cb, a);
+    if (a >= cutoff) break;
   }
   return make_float4(cr, cg, cb, a);
 }
@@ -438,22 +469,23 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t abits = 0;
     if (lvl >= top) {
-      if constexpr (BUF) gather_uniform<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);  // uniform level
-      else gather_lane<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, s, abits);
+      if constexpr (BUF) gather_uniform<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, kHalfOne, s, abits);
+      else gather_lane<PREC, true>(P, P.levels - 1, ux, uy, uz, 1.0f, kHalfOne, s, abits);
       count += 1;
     } else {
       const float fl = floorf(lvl);
-      const float fr = lvl - fl;
+      const float fr = lvl - fl, gr = 1.0f - fr;
       const uint32_t l0 = (uint32_t)fl;
+      const HalfW hfr = half_weight(fr), hgr = half_weight(gr);  // used by the FP16 x-pair form
       // per-lane schedules usually still agree on the level across the wavefront: then the
       // level's descriptor and pitches are scalars
       const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);
       if (BUF && __all(l0 == lu)) {
-        gather_uniform<PREC, true>(P, lu + 1, ux, uy, uz, fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(lu)) gather_uniform<PREC, false>(P, lu, ux, uy, uz, 1.0f - fr, s, abits);
+        gather_uniform<PREC, true>(P, lu + 1, ux, uy, uz, fr, hfr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(lu)) gather_uniform<PREC, false>(P, lu, ux, uy, uz, gr, hgr, s, abits);
       } else {
-        gather_lane<PREC, true>(P, l0 + 1, ux, uy, uz, fr, s, abits);
-        if (abits || !skip_fine_ok<PREC>(l0)) gather_lane<PREC, false>(P, l0, ux, uy, uz, 1.0f - fr, s, abits);
+        gather_lane<PREC, true>(P, l0 + 1, ux, uy, uz, fr, hfr, s, abits);
+        if (abits || !skip_fine_ok<PREC>(l0)) gather_lane<PREC, false>(P, l0, ux, uy, uz, gr, hgr, s, abits);
       }
       count += 2;
     }

@@ -30,10 +30,12 @@ def _irr(d, s):
 
 # the oracle's storage precision for each GPU form: fp16x2 holds the fp16 values in x-pairs
 ORC_PREC = {"fp32": "fp32", "fp16": "fp16", "bf16": "bf16", "fp16x2": "fp16"}
-# agreement with the oracle's FP32 arithmetic on the same levels: the x-pair form rounds its
-# trilinear weights to FP16 for its packed dot products (about 1e-4 relative), the others
-# differ only in FP32 operation order
-SAME_LEVELS_BOUND = {"fp32": 1e-5, "fp16": 1e-5, "bf16": 1e-5, "fp16x2": 3e-4}
+# agreement with the oracle's FP32 arithmetic on the same levels: the x-pair form rounds the
+# footprint fractions to FP16 and forms its 16 trilinear weights in packed-half arithmetic
+# for its packed dot products (3.5e-4 relative on the 10k-triangle soup; the rounding of
+# each weight is about 2^-11 relative, and a gather sums 16 of them), the others differ only
+# in FP32 operation order
+SAME_LEVELS_BOUND = {"fp32": 1e-5, "fp16": 1e-5, "bf16": 1e-5, "fp16x2": 5e-4}
 
 
 class Frame:
