@@ -646,7 +646,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   if (!w->hist) {
     HIP_TRY(hipHostMalloc((void**)&w->hist, 4 * sizeof(unsigned long long), hipHostMallocCoherent));
     std::memset(w->hist, 0, 4 * sizeof(unsigned long long));
-    HIP_TRY(hipEventCreateWithFlags(&w->settled, hipEventDisableTiming));
+    // no system-scope fence: k_settle writes the history with system-scope stores
+    HIP_TRY(hipEventCreateWithFlags(&w->settled, hipEventDisableTiming | hipEventDisableSystemFence));
   }
   if (++w->seq == 0u) w->seq = 1u;
 
@@ -1386,7 +1387,7 @@ int vgi_scene_create(vgi_context* ctx, vgi_scene** out) {
   vgi_scene* s = new vgi_scene();
   s->ctx = ctx;
   if (hipHostMalloc((void**)&s->result, sizeof(SceneVoxResult), hipHostMallocCoherent) != hipSuccess ||
-      hipEventCreateWithFlags(&s->vox_done, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&s->vox_done, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
     (void)hipGetLastError();
     if (s->result) (void)hipHostFree(s->result);
     delete s;

@@ -534,9 +534,16 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
 // 2^32) and unit count are checked against the pair list and work map the host sized, and
 // the overflow word tells steps 5-7 which path to take. The batch totals (pairs, units,
 // large triangles, split bins) go to the stream's pinned history, from which the host
-// sizes later calls. Plain stores: the host reads the records only after an event
-// recorded behind this kernel has completed, which makes them visible; no cache
-// writeback is needed here.
+// sizes later calls. Both are coherent host memory, written with system-scope stores,
+// which are complete in host memory before the kernel ends; the host reads them after an
+// event recorded behind this kernel has completed. That event needs no system-scope
+// fence (an L2 writeback of everything the voxelize kernels left dirty, about 10 us on
+// the room), and is created without one.
+// a store to coherent host memory performed at system scope (written through the caches)
+__device__ __forceinline__ void sys_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __restrict__ scenes) {
   unsigned long long pairs = 0, big = 0, split = 0;
   for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
@@ -566,17 +573,17 @@ __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __res
     r.overflow = ovf;
     r.seq = B.seq;
     r.pad[0] = r.pad[1] = 0;
-    const uint4* src = (const uint4*)&r;
-    uint4* dst = (uint4*)P.result;
+    const unsigned long long* src = (const unsigned long long*)&r;
+    unsigned long long* dst = (unsigned long long*)P.result;
 #pragma unroll
-    for (int q = 0; q < 4; q++) dst[q] = src[q];
+    for (int q = 0; q < 8; q++) sys_store(dst + q, src[q]);
   }
   if (threadIdx.x == 0) {
     gst(B.overflow, ovf);
-    B.host_hist[0] = pairs;
-    B.host_hist[1] = units;
-    B.host_hist[2] = big;
-    B.host_hist[3] = split;
+    sys_store(B.host_hist + 0, pairs);
+    sys_store(B.host_hist + 1, (unsigned long long)units);
+    sys_store(B.host_hist + 2, big);
+    sys_store(B.host_hist + 3, split);
   }
 }
 
