@@ -356,10 +356,12 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __restrict__ scenes) {
   __shared__ uint32_t s_key[kSlotTable], s_cnt[kSlotTable], s_base[kSlotTable];
+  __shared__ uint32_t s_big_n, s_big_base;  // the workgroup's large triangles and their list run
   for (uint32_t e = threadIdx.x; e < kSlotTable; e += blockDim.x) {
     s_key[e] = kNoKey;
     s_cnt[e] = 0u;
   }
+  if (threadIdx.x == 0) s_big_n = 0u;
   const uint2 blk = tri_block(B);  // a workgroup never spans two scenes
   const VoxScene& P = scenes[blk.x];
   const uint32_t t = blk.y + threadIdx.x;
@@ -381,18 +383,21 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
       if (small) load_q(R, q, N);
     }
   }
-  {  // large triangles: one list append per wavefront (they are often neighbours in a mesh)
+  __syncthreads();  // table and large-triangle count cleared
+  // large triangles: a rank within the workgroup (one LDS atomic per wavefront); the
+  // workgroup's run of the list is reserved with one global atomic beside the bin
+  // reservations below, and the ids are written after it
+  uint32_t big_rank = 0;
+  {
     const uint64_t bm = __ballot(big);
     if (bm) {
       const uint32_t lane = __lane_id();
       const int leader = __ffsll((unsigned long long)bm) - 1;
       uint32_t base = 0;
-      if (lane == (uint32_t)leader) base = gatomic_add(&P.stats->big_count, (uint32_t)__popcll(bm));
-      base = __shfl(base, leader);
-      if (big) gst(P.big_list + base + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull)), t);
+      if (lane == (uint32_t)leader) base = atomicAdd(&s_big_n, (uint32_t)__popcll(bm));
+      big_rank = __shfl(base, leader) + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
     }
   }
-  __syncthreads();  // table cleared
   // rank each (triangle, bin) pair within the workgroup
   uint32_t ent[kSmallBins], rank[kSmallBins];  // table entry (or kNoKey: own atomic) and rank
   uint32_t hits = 0;                           // bit j: candidate j overlaps its bin
@@ -417,7 +422,8 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
     }
   }
   __syncthreads();
-  {  // reserve every listed bin's run: all four returning atomics in flight together
+  {  // reserve every listed bin's run (all four returning atomics in flight together) and
+     // the large triangles' run of the list
     static_assert(kSlotTable == 4 * kBlockTris, "four table entries per thread");
     uint32_t key[4], base[4];
 #pragma unroll
@@ -426,11 +432,14 @@ __global__ __launch_bounds__(256) void k_setup(VoxBatch B, const VoxScene* __res
       key[r] = s_key[e];
       base[r] = key[r] != kNoKey ? gatomic_add(P.bin_small + ctr(P, key[r]), s_cnt[e]) : 0u;
     }
+    const uint32_t nbig = s_big_n;
+    if (threadIdx.x == 0 && nbig != 0u) s_big_base = gatomic_add(&P.stats->big_count, nbig);
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (key[r] != kNoKey) s_base[threadIdx.x + (uint32_t)r * kBlockTris] = base[r];
   }
   __syncthreads();
+  if (big) gst(P.big_list + s_big_base + big_rank, t);
   uint32_t nslot = 0;
 #pragma unroll
   for (int j = 0; j < kSmallBins; j++) {
