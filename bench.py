@@ -379,7 +379,7 @@ def front_end_rooflines(cn, probes, head):
                        "frac": gbs / HBM_PEAK_GBS}
     v = cn.get("voxelize", {})
     if "SQ_INSTS_VALU" in v:
-        out["voxelize"] = {"kernel": "k_voxelize<8>", "valu_instructions": v["SQ_INSTS_VALU"],
+        out["voxelize"] = {"kernel": "k_voxelize<8, false>", "valu_instructions": v["SQ_INSTS_VALU"],
                            "hbm_bytes": v.get("hbm_bytes"), "l2_hit_rate": v.get("l2_hit_rate")}
     return out
 
@@ -561,7 +561,7 @@ COUNTER_KERNELS = [
     ("mips_fp16_first", r"k_mips<1, true, false>"), ("mips_fp16_rest", r"k_mips<1, false, false>"),
     ("mips_fp32_first", r"k_mips<0, true, false>"), ("mips_fp32_rest", r"k_mips<0, false, false>"),
     ("mips_fp16x2_first", r"k_mips<3, true, false>"), ("mips_fp16x2_rest", r"k_mips<3, false, false>"),
-    ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8>"),
+    ("setup", r"k_setup\b"), ("voxelize", r"k_voxelize<8, false>"),
     ("probe_fma", r"k_probe_fma"), ("probe_int", r"k_probe_int"), ("probe_atomic_ret", r"k_probe_atomic<true>"),
     ("probe_atomic_noret", r"k_probe_atomic<false>"), ("copy", r"k_copy\b"),
 ]
