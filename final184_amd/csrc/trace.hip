@@ -385,12 +385,21 @@ __device__ __forceinline__ void composite(float4 s, float& cr, float& cg, float&
   a = fmaf(w, s.w, a);
 }
 
-// inside [0, n]^3: one minimum3/maximum3 pair instead of six compares; these propagate NaN
-// (fminf/fmaxf would drop it), so a NaN in any coordinate ends the march
+// inside [0, n]^3, as one unsigned maximum3 and one compare of the bit patterns: for
+// x >= +0 the patterns order as the values do, a negative x or a NaN has a pattern above
+// n's (sign bit, or an all-ones exponent), so it ends the march. -0.0, which the float
+// test would accept, never occurs: the cone origin is canonicalized (+0.0 added), and a
+// sample fma(ud, t, uo) with uo != -0.0 is never -0.0.
 __device__ __forceinline__ bool inside_grid(float ux, float uy, float uz, float nf) {
-  const float umin = __builtin_elementwise_minimum(__builtin_elementwise_minimum(ux, uy), uz);
-  const float umax = __builtin_elementwise_maximum(__builtin_elementwise_maximum(ux, uy), uz);
-  return umin >= 0.f && umax <= nf;
+  const uint32_t m = __builtin_elementwise_max(
+      __builtin_elementwise_max(__builtin_bit_cast(uint32_t, ux), __builtin_bit_cast(uint32_t, uy)),
+      __builtin_bit_cast(uint32_t, uz));
+  return m <= __builtin_bit_cast(uint32_t, nf);
+}
+
+// cone origin in level-0 voxel units, with -0.0 made +0.0 (inside_grid relies on it)
+__device__ __forceinline__ float cone_origin(float o, float origin, float inv_h) {
+  return (o - origin) * inv_h + 0.0f;
 }
 
 // One fixed-aperture cone: its samples (distance, levels, level weights) come from the
@@ -410,9 +419,9 @@ __device__ __forceinline__ float4 cone_march_fixed(const TraceParams& P, const S
   float cr = 0.f, cg = 0.f, cb = 0.f, a = 0.f;
   const float nf = (float)P.n;
   // cone origin and direction in level-0 voxel units: u(t) = uo + ud t
-  const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
-  const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
-  const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
+  const float uox = cone_origin(ox, P.origin[0], P.inv_h), udx = dx * P.inv_h;
+  const float uoy = cone_origin(oy, P.origin[1], P.inv_h), udy = dy * P.inv_h;
+  const float uoz = cone_origin(oz, P.origin[2], P.inv_h), udz = dz * P.inv_h;
   const float cutoff = P.cutoff;
   // The sample index is the same for every lane still marching (the entry is read with
   // scalar loads); a lane whose march ends leaves the loop, and the loop ends when none is
@@ -452,9 +461,9 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
   float t = P.start_off * P.h;
   const float nf = (float)P.n;
   const float top = (float)(P.levels - 1);
-  const float uox = (ox - P.origin[0]) * P.inv_h, udx = dx * P.inv_h;
-  const float uoy = (oy - P.origin[1]) * P.inv_h, udy = dy * P.inv_h;
-  const float uoz = (oz - P.origin[2]) * P.inv_h, udz = dz * P.inv_h;
+  const float uox = cone_origin(ox, P.origin[0], P.inv_h), udx = dx * P.inv_h;
+  const float uoy = cone_origin(oy, P.origin[1], P.inv_h), udy = dy * P.inv_h;
+  const float uoz = cone_origin(oz, P.origin[2], P.inv_h), udz = dz * P.inv_h;
   // loop invariants, read once (otherwise reloaded from the descriptor every sample)
   const float h = P.h, inv_h = P.inv_h, step = P.step, cutoff = P.cutoff, max_dist = P.max_dist;
   // the step cap bounds the loop even when t stops growing in float (zero aperture,

@@ -276,6 +276,51 @@ def test_nan_position_component_ends_march(ctx):
     assert rel_l2(_irr(d[good], sp[good]), _irr(d32[good], s32[good])) <= 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp16x2"])
+def test_cone_along_the_grid_edge_from_negative_zero(ctx, prec):
+    """A pixel on the grid's corner edge with x = z = -0.0, normal (-0, 1, -0) and the eye
+    straight above: the specular cone's direction is (-0, 1, -0), so its x and z stay -0.0
+    for the whole march. The oracle's bounds test (u >= 0) keeps -0.0 inside, and the
+    cone must reach the ceiling above the edge as there; the march's bit-pattern bounds
+    test is exact because the cone origin is made +0.0 first."""
+    vgi = _vgi()
+    grid_d = dict(origin=(0.0, 0.0, 0.0), extent=1.0, n=32, levels=4)
+    y = 0.6
+    ceil = np.array([[-0.1, y, -0.1], [0.4, y, -0.1], [0.4, y, 0.4], [-0.1, y, -0.1], [0.4, y, 0.4],
+                     [-0.1, y, 0.4]], np.float32)
+    alb = np.array([[200, 150, 100], [200, 150, 100]], np.uint8)
+    lights = [dict(type="directional", vec=(0.3, 1.0, 0.2), color=(1.0, 1.0, 1.0))]  # lights the underside
+    gp = np.tile(np.array([[-0.0, 0.1, -0.0]], np.float32), (64, 1))
+    gn = np.tile(np.array([[-0.0, 1.0, -0.0]], np.float32), (64, 1))
+    assert np.signbit(gp[0, 0]) and np.signbit(gn[0, 2])
+    cones = dict(num_diffuse=0, specular=True, diffuse_aperture_deg=60.0, diffuse_ring_deg=60.0,
+                 specular_aperture_deg=10.0, step_factor=0.5, max_distance=1.0, alpha_cutoff=0.99,
+                 normal_offset=1.0, start_offset=1.0, eye=(0.0, 0.9, 0.0))
+    print(f"case cone along the grid edge from -0.0, {prec}", flush=True)
+    mesh = vgi.MeshData(ctx, ceil, albedo=alb)
+    sc = ctx.scene()
+    try:
+        sc.voxelize([mesh], vgi.make_grid(**grid_d))
+        sc.build_mips(vgi.make_lights(lights), vgi.PREC_NAMES[prec])
+        gb = vgi.GBufferData(ctx, gp, gn, 8, 8)
+        od = vgi.DeviceArray.empty(ctx, (64, 4), np.float32)
+        osp = vgi.DeviceArray.empty(ctx, (64, 4), np.float32)
+        sc.trace(gb, vgi.make_cones(**cones), od, osp)
+        s = osp.numpy()
+    finally:
+        sc.release()
+    acc, _ = orc.voxelize(ceil.reshape(1, -1).reshape(2, 9), grid_d, albedo=alb)
+    ref = orc.inject_mips(acc, grid_d, lights, ORC_PREC[prec])
+    _, s_ref, _ = orc.trace(ref, grid_d, gp[:1], gn[:1], cones)
+    print(f"specular gpu {s[0]} oracle {s_ref[0]}", flush=True)
+    assert s_ref[0, 3] > 0.1 and (s_ref[0, :3] > 0).all()  # the oracle's cone reaches the lit ceiling
+    # one pixel: the x-pair form's weight rounding does not average out over pixels here
+    # (an early stop would read zero colour and a fraction of the opacity)
+    tol = {"fp32": 1e-5, "fp16x2": 3e-3}[prec]
+    np.testing.assert_allclose(s[:, 3], s_ref[0, 3], rtol=tol)
+    np.testing.assert_allclose(s[:, :3], np.tile(s_ref[0, :3], (64, 1)), rtol=tol, atol=1e-6)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16x2"])
 def test_wide_addressing_matches(ctx, soup, prec):
     """The 64-bit-address form of the cone march (what levels of 4 GiB and more use) gives
