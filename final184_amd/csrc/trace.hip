@@ -83,26 +83,19 @@ struct AccPair {
     a.z = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, bb), w, a.z, false);
     a.w = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, aa), w, a.w, false);
   }
-  // The first row of a gather, as the three-operand dot product whose destination is not
-  // its addend: the compiler selects the two-operand form, whose destination is the
-  // accumulator, and so first moves 0 into each accumulator of a sum that starts at zero
-  // (INIT), or copies the running sum when the scalar-load and vector-load paths of a
-  // gather keep it in different registers. SGPR: the texel words are wave-uniform scalars.
-  template <bool SGPR, bool INIT>
+  // The first row of a sample's first gather, as the three-operand dot product with a zero
+  // addend: the compiler would select the two-operand form, whose destination is the
+  // accumulator, and first move 0 into each accumulator. SGPR: the texel words are
+  // wave-uniform scalars.
+  template <bool SGPR>
   __device__ static void first(float4& a, half2_t w, uint32_t rr, uint32_t gg, uint32_t bb, uint32_t aa) {
     const uint32_t wv = __builtin_bit_cast(uint32_t, w);
     const uint32_t t[4] = {rr, gg, bb, aa};
     float* o[4] = {&a.x, &a.y, &a.z, &a.w};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      if constexpr (INIT) {
-        if constexpr (SGPR) __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "s"(t[c]), "v"(wv));
-        else __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "v"(t[c]), "v"(wv));
-      } else {
-        const float in = *o[c];
-        if constexpr (SGPR) __asm__("v_dot2_f32_f16 %0, %1, %2, %3" : "=v"(*o[c]) : "s"(t[c]), "v"(wv), "v"(in));
-        else __asm__("v_dot2_f32_f16 %0, %1, %2, %3" : "=v"(*o[c]) : "v"(t[c]), "v"(wv), "v"(in));
-      }
+      if constexpr (SGPR) __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "s"(t[c]), "v"(wv));
+      else __asm__("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(*o[c]) : "v"(t[c]), "v"(wv));
     }
   }
 };
@@ -264,14 +257,16 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
     pair_weights(c, hw, pw);
     // When every active lane's footprint is the same 2x2x2 block (usual at coarse levels),
     // its four rows are read once with scalar loads and broadcast, sparing the
-    // texture-address path four 16-byte vector loads.
+    // texture-address path four 16-byte vector loads. Only for a sample's first gather:
+    // the second one accumulates in place into the first's sums, which the two-path
+    // form would copy back after it (four moves, beside the vote's two instructions).
     const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
-    if (__all(off == off0)) {
+    if (INIT && __all(off == off0)) {
       const char* base = (const char*)L.ptr + off0;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const vgi_u32x4 e = *(const VGI_CONSTANT vgi_u32x4*)(base + soff[q]);
-        if (q == 0) AccPair::first<true, INIT>(acc, pw.r[q], e[0], e[1], e[2], e[3]);
+        if (q == 0) AccPair::first<true>(acc, pw.r[q], e[0], e[1], e[2], e[3]);
         else AccPair::add(acc, pw.r[q], e[0], e[1], e[2], e[3]);
         abits |= e[3];
       }
@@ -283,7 +278,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
     for (int q = 0; q < 4; q++) {
       const auto e = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff[q], 0);
       const uint32_t rr = e[0], gg = e[1], bb = e[2], aa = e[3];
-      if (q == 0) AccPair::first<false, INIT>(acc, pw.r[q], rr, gg, bb, aa);
+      if (q == 0 && INIT) AccPair::first<false>(acc, pw.r[q], rr, gg, bb, aa);
       else AccPair::add(acc, pw.r[q], rr, gg, bb, aa);
       abits |= aa;
     }
@@ -347,7 +342,7 @@ __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, fl
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint4 e = gld4(base + offs[q]);
-      if (q == 0) AccPair::first<false, INIT>(acc, pw.r[q], e.x, e.y, e.z, e.w);
+      if (q == 0 && INIT) AccPair::first<false>(acc, pw.r[q], e.x, e.y, e.z, e.w);
       else AccPair::add(acc, pw.r[q], e.x, e.y, e.z, e.w);
       abits |= e.w;
     }
