@@ -1016,15 +1016,12 @@ void fill_trace_params(TraceParams& P, const vgi_trace_item& it, const vgi_tunin
   P.out_s = (float4*)it.out_specular;
 }
 
-// (-w, w) and (w, 0) as packed halves, each rounded to nearest even
-static HalfW half_weight(float w) {
-  auto bits = [](float f) {
-    const _Float16 h = (_Float16)f;
-    uint16_t b;
-    std::memcpy(&b, &h, 2);
-    return (uint32_t)b;
-  };
-  return HalfW{bits(-w) | (bits(w) << 16), bits(w)};
+// (w, 0) as packed halves, w rounded to nearest even
+static uint32_t half_weight(float w) {
+  const _Float16 h = (_Float16)w;
+  uint16_t b;
+  std::memcpy(&b, &h, 2);
+  return (uint32_t)b;
 }
 
 // March schedule of a cone whose sample distances do not depend on the pixel (a fixed
@@ -1054,6 +1051,7 @@ void march_schedule(const TraceParams& P, float tan_half, std::vector<SchedEntry
     }
     e.hfr = half_weight(e.fr);
     e.hgr = half_weight(e.gr);
+    e.pad[0] = e.pad[1] = 0u;
     out.push_back(e);
     const float adv = P.step * diam;
     t = t + adv;

@@ -194,10 +194,11 @@ __device__ __forceinline__ Foot foot_weights(const Cell& c, float wl) {
 }
 
 // FP16 weight pairs of the four rows (y0 z0, y1 z0, y0 z1, y1 z1) of a footprint, in
-// packed-half arithmetic: the fractions rounded to FP16, the x pair (w (1 - fx), w fx)
-// from the level weight's constants (HalfW), the y and z pairs (1 - f, f), the four row
-// products wy wz, and each row's pair as x pair times row product. Eleven instructions for
-// the 16 weights.
+// packed-half arithmetic: the fractions rounded to FP16, the x pair (w (1 - fx), w fx) =
+// (-fx, fx) (w, w) + (w, 0) from the level weight w given as the packed halves (w, 0)
+// (one multiply-add that reads that scalar register twice: a sign modifier and operand
+// selection), the y and z pairs (1 - f, f), the four row products wy wz, and each row's
+// pair as x pair times row product. Eleven instructions for the 16 weights.
 struct PairW {
   half2_t r[4];
 };
@@ -208,11 +209,12 @@ __device__ __forceinline__ half2_t cvt_pk_half(float a, float b) {
   __asm__("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
   return r;
 }
-__device__ __forceinline__ void pair_weights(const Cell& c, HalfW w, PairW& pw) {
+__device__ __forceinline__ void pair_weights(const Cell& c, uint32_t hw, PairW& pw) {
   const half2_t np1 = {(_Float16)-1.0f, (_Float16)1.0f}, one0 = {(_Float16)1.0f, (_Float16)0.0f};
   const half2_t fxy = cvt_pk_half(c.fx, c.fy);
   const half2_t fz = cvt_pk_half(c.fz, c.fz);
-  const half2_t xw = (half2_t){fxy.x, fxy.x} * __builtin_bit_cast(half2_t, w.np) + __builtin_bit_cast(half2_t, w.w0);
+  const half2_t w0 = __builtin_bit_cast(half2_t, hw);
+  const half2_t xw = (half2_t){-fxy.x, fxy.x} * (half2_t){w0.x, w0.x} + w0;
   const half2_t yw = (half2_t){fxy.y, fxy.y} * np1 + one0;
   const half2_t zw = fz * np1 + one0;
   const half2_t r01 = yw * (half2_t){zw.x, zw.x};  // rows y0 z0, y1 z0
@@ -223,13 +225,12 @@ __device__ __forceinline__ void pair_weights(const Cell& c, HalfW w, PairW& pw) 
   pw.r[3] = xw * (half2_t){r23.y, r23.y};
 }
 
-// (-w, w) and (w, 0) as packed halves, per lane (the schedule holds them for fixed cones)
-__device__ __forceinline__ HalfW half_weight(float w) {
-  const half2_t np = {(_Float16)(-w), (_Float16)w};
+// (w, 0) as packed halves, per lane (the schedule holds them for fixed cones)
+__device__ __forceinline__ uint32_t half_weight(float w) {
   const half2_t w0 = {(_Float16)w, (_Float16)0.0f};
-  return HalfW{__builtin_bit_cast(uint32_t, np), __builtin_bit_cast(uint32_t, w0)};
+  return __builtin_bit_cast(uint32_t, w0);
 }
-constexpr HalfW kHalfOne = {0x3C00BC00u, 0x00003C00u};  // w = 1: (-1, 1), (1, 0)
+constexpr uint32_t kHalfOne = 0x00003C00u;  // w = 1: (1, 0)
 
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
 // FP32: when every active lane's footprint is the same 2x2x2 texel block (usual at coarse
@@ -245,7 +246,7 @@ typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
 // forms) or hw (the FP16 x-pair form).
 template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k, float ux, float uy,
-                                               float uz, float wl, HalfW hw, float4& acc, uint32_t& abits) {
+                                               float uz, float wl, uint32_t hw, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const LevelDesc L = P.lv[k];  // one scalar load: the level is wave-uniform
   const uint32_t rowb = L.p1 * TB, slab = L.p1 * rowb;
@@ -329,7 +330,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 // ---- per-lane level: 64-bit global loads ----
 template <int PREC, bool INIT>
 __device__ __forceinline__ void gather_lane(const TraceParams& P, uint32_t k, float ux, float uy,
-                                            float uz, float wl, HalfW hw, float4& acc, uint32_t& abits) {
+                                            float uz, float wl, uint32_t hw, float4& acc, uint32_t& abits) {
   constexpr uint32_t TB = kTexelBytes<PREC>;
   const uint32_t p1 = P.nk[k] + 2;
   const size_t rowb = (size_t)p1 * TB, slab = (size_t)p1 * p1 * TB;
@@ -480,7 +481,7 @@ __device__ __forceinline__ float4 cone_march_lane(const TraceParams& P, float ox
       const float fl = floorf(lvl);
       const float fr = lvl - fl, gr = 1.0f - fr;
       const uint32_t l0 = (uint32_t)fl;
-      const HalfW hfr = half_weight(fr), hgr = half_weight(gr);  // used by the FP16 x-pair form
+      const uint32_t hfr = half_weight(fr), hgr = half_weight(gr);  // used by the FP16 x-pair form
       // per-lane schedules usually still agree on the level across the wavefront: then the
       // level's descriptor and pitches are scalars
       const uint32_t lu = __builtin_amdgcn_readfirstlane(l0);

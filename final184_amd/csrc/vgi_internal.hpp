@@ -192,21 +192,15 @@ struct MipBatch {
 };
 
 // One scene of a trace batch (device memory).
-// Level weight w of a gather as the packed-half pair constants of its x weights:
-// (-w, w) and (w, 0), low half first, so that one packed multiply-add of the x fraction
-// gives (w (1 - fx), w fx).
-struct HalfW {
-  uint32_t np, w0;
-};
-
 // One sample of a fixed-aperture cone's march (the same for every pixel): distance t, the
 // level fraction fr and 1 - fr, and the lower level l0, or kSchedTop | top level for a
-// sample taken at the top level alone (fr 1, gr 0); then fr and gr as packed-half weight
-// constants for the FP16 x-pair gathers.
+// sample taken at the top level alone (fr 1, gr 0); then fr and gr as the packed halves
+// (w, 0), low half first, that the FP16 x-pair gathers scale their x weights with.
 struct SchedEntry {
   float t, fr, gr;
   uint32_t code;
-  HalfW hfr, hgr;
+  uint32_t hfr, hgr;
+  uint32_t pad[2];
 };
 constexpr uint32_t kSchedTop = 0x80000000u;
 static_assert(sizeof(SchedEntry) == 32, "schedule entries are read as 32-byte scalar loads");
