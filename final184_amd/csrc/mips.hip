@@ -252,6 +252,18 @@ __device__ bool light_visible(const MipScene& P, const LightDev& L, uint32_t x, 
 
 // Radiance of an occupied level-0 voxel (x, y, z) from its accumulator record. SHADOW:
 // lights flagged to cast shadows are cut where their visibility walk is blocked.
+// (s + c / 2) / c, the rounded mean of a colour channel (s <= 255 c, 1 <= c < 2^24),
+// from the correctly rounded reciprocal rc of c: the product's relative error is below
+// 2^-21 on a quotient below 256, so its integer part is within one of the exact quotient,
+// and one remainder test corrects it. Equal to the integer division for every such input,
+// at a fraction of a 32-bit division's instructions, with rc shared by the three channels.
+__device__ __forceinline__ uint32_t rounded_mean(uint32_t s, uint32_t c, float rc) {
+  const uint32_t x = s + (c >> 1);
+  const uint32_t q = (uint32_t)((float)x * rc);
+  const int32_t r = (int32_t)(x - __umul24(q, c));
+  return r < 0 ? q - 1u : (r >= (int32_t)c ? q + 1u : q);
+}
+
 template <bool SHADOW>
 __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32_t y, uint32_t z) {
   const uint32_t nbr = P.n >> 3;
@@ -260,10 +272,11 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
   const uint32_t* a = P.acc + ((size_t)brick * 512u + local) * kAccWords;
   const uint4 s0 = gld4(a), s1 = gld4(a + 4);
   const uint32_t c = s0.w;
-  const float lr = lut[(s0.x + c / 2) / c];
-  const float lg = lut[(s0.y + c / 2) / c];
-  const float lb = lut[(s0.z + c / 2) / c];
-  const float inv = 1.0f / ((float)c * 1024.0f);
+  const float rc = 1.0f / (float)c;
+  const float lr = lut[rounded_mean(s0.x, c, rc)];
+  const float lg = lut[rounded_mean(s0.y, c, rc)];
+  const float lb = lut[rounded_mean(s0.z, c, rc)];
+  const float inv = rc * 0x1p-10f;  // 1 / (1024 c): scaling by 2^-10 commutes with the rounding
   const int32_t bias = (int32_t)(1024u * c);
   const float nx = (float)((int32_t)s1.x - bias) * inv;
   const float ny = (float)((int32_t)s1.y - bias) * inv;
