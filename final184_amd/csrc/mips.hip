@@ -314,11 +314,24 @@ __device__ float4 inject(const MipScene& P, const float* lut, uint32_t x, uint32
   return make_float4(lr * er, lg * eg, lb * eb, 1.0f);
 }
 
+// Threads per 16^3 tile. The FP32 injecting launch without shadows runs 1024: its 64 KB LDS
+// tile holds it to two workgroups per CU either way, and a dense tile's inject (up to 4096
+// voxels, a few hundred instructions each with four point lights), which sets the launch's
+// length, then takes half the rounds (FP32 inject+mips 68.6 -> 59.6 us on the room). The
+// 8-byte forms fit three 512-thread workgroups per CU, which serve the many empty tiles
+// better (FP16 x-pair 67.5 us either way). With visibility walks 1024 threads were slower
+// (FP32 0.32 -> 0.43 ms, FP16 x-pair 0.28 -> 0.45 ms on the room's four shadowed lights).
+// The reducing launch runs one thread per 2x2x2 group of a 16^3 block.
+constexpr uint32_t mips_threads(int prec, bool src_acc, bool shadow) {
+  return src_acc && !shadow && prec == VGI_PREC_FP32 ? 1024u : 512u;
+}
+
 template <int PREC, bool SRC_ACC, bool SHADOW>
-__global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scenes,
+__global__ __launch_bounds__(1024) void k_mips(const MipScene* __restrict__ scenes,
                                                const MipBlock* __restrict__ blocks,
                                                const float* __restrict__ lut, uint32_t src1, uint32_t nout1) {
   using TX = Texel<PREC>;
+  constexpr uint32_t kThreads = mips_threads(PREC, SRC_ACC, SHADOW);
   __shared__ float4 s_a[512];
   __shared__ float4 s_b[64];
   const MipBlock blk = blocks ? blocks[blockIdx.x] : MipBlock{0u, blockIdx.x, src1, nout1};
@@ -344,7 +357,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       if (g < 256) s_lut[g] = lut_v;
       __syncthreads();
       // single-level chain: resolve level 0 only, one voxel per thread
-      for (uint32_t v = g; v < S * S * S; v += 512) {
+      for (uint32_t v = g; v < S * S * S; v += kThreads) {
         const uint32_t x = tx * S + v % S, y = ty * S + (v / S) % S, z = tz * S + v / (S * S);
         const float4 c = occupied(P, x, y, z) ? TX::round(inject<SHADOW>(P, s_lut, x, y, z)) : make_float4(0.f, 0.f, 0.f, 0.f);
         TX::store(P.level[0], padded_index(x, y, z, ns), c);
@@ -372,7 +385,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     static_assert(sizeof(s_a) >= 4096 * sizeof(uint16_t), "item list fits in s_a");
     const uint32_t ssh = gsh + 1u, NV = S * S * S;
     const uint32_t lane = g & 63u;
-    for (uint32_t v = g; v < NV; v += 512) s_val[v] = TX::pack(make_float4(0.f, 0.f, 0.f, 0.f));
+    for (uint32_t v = g; v < NV; v += kThreads) s_val[v] = TX::pack(make_float4(0.f, 0.f, 0.f, 0.f));
     if (g == 0) s_nocc = 0u;
     if (g < 8) s_rows_now[g] = 0u;
     __syncthreads();
@@ -409,7 +422,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
     if (nocc == 0u && was == 0u) return;  // uniform: both values come from LDS
     if (g == 0 && (nocc != 0u) != (was != 0u)) gst(P.live + tile, nocc != 0u ? 1u : 0u);
     if (ROWS && g < nrw && s_rows_now[g] != s_rows_prev[g]) gst(P.live + ntiles + tile * 8u + g, s_rows_now[g]);
-    for (uint32_t it = g; it < nocc; it += 512) {
+    for (uint32_t it = g; it < nocc; it += kThreads) {
       const uint32_t e = s_item[it], gi = e >> 3, bit = e & 7u;
       const uint32_t lx = 2 * (gi & (G - 1u)) + (bit & 1u);
       const uint32_t ly = 2 * ((gi >> gsh) & (G - 1u)) + ((bit >> 1) & 1u);
@@ -422,7 +435,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
       // whole entries from the tile's own texels; the entry of the tile's last x gets its
       // low halves here and its high halves from the next tile, the entry before the
       // tile's first x its high halves from here
-      for (uint32_t v = g; v < NV; v += 512) {
+      for (uint32_t v = g; v < NV; v += kThreads) {
         const uint32_t row = v >> ssh;  // rows that hold voxels now or held them at the last build
         if (!(((s_rows_now[row >> 5] | s_rows_prev[row >> 5]) >> (row & 31u)) & 1u)) continue;
         const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
@@ -432,7 +445,7 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
         if (lx == 0u) TX::store_half(P.level[0], pi - 1, s_val[v], 1u);
       }
     } else {
-      for (uint32_t v = g; v < NV; v += 512) {
+      for (uint32_t v = g; v < NV; v += kThreads) {
         const uint32_t row = v >> ssh;
         if (ROWS && !(((s_rows_now[row >> 5] | s_rows_prev[row >> 5]) >> (row & 31u)) & 1u)) continue;
         const uint32_t lx = v & (S - 1u), ly = (v >> ssh) & (S - 1u), lz = v >> (2 * ssh);
@@ -498,7 +511,8 @@ __global__ __launch_bounds__(512) void k_mips(const MipScene* __restrict__ scene
 }
 
 template <int PREC>
-static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, dim3 b, hipStream_t s) {
+static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, hipStream_t s) {
+  const dim3 b(mips_threads(PREC, from_acc, from_acc && B.shadow));
   if (!from_acc)
     hipLaunchKernelGGL((k_mips<PREC, false, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
   else if (B.shadow)
@@ -509,12 +523,12 @@ static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, dim3 b, h
 
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s) {
   if (blocks == 0) return hipSuccess;
-  const dim3 g(blocks), b(512);
+  const dim3 g(blocks);
   switch (prec) {
-    case VGI_PREC_FP32: launch_mips_prec<VGI_PREC_FP32>(B, from_acc, g, b, s); break;
-    case VGI_PREC_FP16: launch_mips_prec<VGI_PREC_FP16>(B, from_acc, g, b, s); break;
-    case VGI_PREC_BF16: launch_mips_prec<VGI_PREC_BF16>(B, from_acc, g, b, s); break;
-    case VGI_PREC_FP16X2: launch_mips_prec<VGI_PREC_FP16X2>(B, from_acc, g, b, s); break;
+    case VGI_PREC_FP32: launch_mips_prec<VGI_PREC_FP32>(B, from_acc, g, s); break;
+    case VGI_PREC_FP16: launch_mips_prec<VGI_PREC_FP16>(B, from_acc, g, s); break;
+    case VGI_PREC_BF16: launch_mips_prec<VGI_PREC_BF16>(B, from_acc, g, s); break;
+    case VGI_PREC_FP16X2: launch_mips_prec<VGI_PREC_FP16X2>(B, from_acc, g, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
