@@ -630,10 +630,10 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
                     (textured && w->tritex_bytes < tneed * sizeof(TriTex)) ||
                     w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need ||
-                    w->pairs_bytes < (size_t)pair_cap * 4 || w->work_bytes < (size_t)work_cap * sizeof(uint2);
+                    w->pairs_bytes < (size_t)pair_cap * 4 || w->work_bytes < (size_t)work_cap * kWorkEntryBytes;
   if (grow) HIP_TRY(hipStreamSynchronize(st));
   TRY(grow_bytes(&w->pairs, &w->pairs_bytes, (size_t)pair_cap * 4, "bin pairs"));
-  TRY(grow_bytes(&w->work_map, &w->work_bytes, (size_t)work_cap * sizeof(uint2), "work map"));
+  TRY(grow_bytes(&w->work_map, &w->work_bytes, (size_t)work_cap * kWorkEntryBytes, "work map"));
   TRY(grow_bytes(&w->tris, &w->tris_bytes, tneed * sizeof(TriRec), "triangle records"));
   if (textured) TRY(grow_bytes(&w->tritex, &w->tritex_bytes, tneed * sizeof(TriTex), "texcoord records"));
   TRY(grow_bytes(&w->tri_slot, &w->tri_slot_bytes, tneed * kSmallBins * 4, "triangle slots"));
@@ -716,6 +716,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.packed_all = (unsigned long long*)w->packed_all;
   B.scan_all = (unsigned long long*)w->scan_all;
   B.work_map = (uint2*)w->work_map;
+  B.unit_order = (uint32_t*)(B.work_map + work_cap);
   B.pairs_all = (uint32_t*)w->pairs;
   B.pair_cap = pair_cap;
   B.work_cap = work_cap;
@@ -1504,7 +1505,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   const size_t o_scan = cv.take((nbins + 1) * 8);
   const size_t o_stmp = cv.take(std::max<size_t>(scan_need, 1));
   const size_t o_pairs = cv.take(pair_cap64 * 4);
-  const size_t o_work = cv.take(work_cap64 * sizeof(uint2));
+  const size_t o_work = cv.take(work_cap64 * kWorkEntryBytes);
   if (cv.off > (size_t)VGI_NOALLOC_SCRATCH_BYTES)
     return fail(VGI_E_RANGE,
                 "voxelize of %llu triangles on a %u^3 grid needs %zu bytes of scratch; the allocation-free limit is %u",
@@ -1560,6 +1561,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.packed_all = (unsigned long long*)(base + o_packed);
   B.scan_all = (unsigned long long*)(base + o_scan);
   B.work_map = (uint2*)(base + o_work);
+  B.unit_order = (uint32_t*)(B.work_map + work_cap64);
   B.pairs_all = (uint32_t*)(base + o_pairs);
   B.pair_cap = (uint32_t)pair_cap64;
   B.work_cap = (uint32_t)work_cap64;

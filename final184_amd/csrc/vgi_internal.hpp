@@ -21,6 +21,7 @@ constexpr int kSmallBins = 8;          // a triangle whose bin box has <= 8 bins
 constexpr uint32_t kBlockTris = 256;   // triangles per setup / emit workgroup
 constexpr uint32_t kBlockBins = 256;   // bins per bins / workmap workgroup
 constexpr uint32_t kUnitItems = 16;    // chunks one voxelize workgroup takes in turn (8^3 bins)
+constexpr size_t kWorkEntryBytes = 12;  // per work unit: its (scene, bin) and its place in the order
 
 // Per-mesh descriptor, device resident.
 struct MeshDev {
@@ -130,10 +131,13 @@ struct VoxBatch {
   unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
   unsigned long long* scan_all;     // exclusive scan of packed_all
   uint2* work_map;                  // (scene, bin) of every work unit (work_cap entries)
+  uint32_t* unit_order;             // the order k_voxelize takes the units in (work_cap entries)
   uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn
   uint32_t pair_cap;                // entries of pairs_all
   uint32_t work_cap;                // entries of work_map
-  uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps
+  uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps;
+                                    // the two words after it (cleared with it) are the unit
+                                    // order's front and back cursors
   unsigned long long* host_hist;    // pinned: batch pairs, units, large triangles, split bins
   uint32_t seq;                     // this call's number (never 0)
   uint32_t nscenes;

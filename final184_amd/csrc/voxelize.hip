@@ -26,10 +26,12 @@
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
 //                   offset and its first work unit at once; k_settle stores the per-scene
 //                   totals and statistics and sets the overflow word.
-//   5 k_scatter     one launch: work unit -> (scene, bin); small triangles to their kept
+//   5 k_scatter     one launch: work unit -> (scene, bin) and the units' order (those of
+//                   two or more chunks first); small triangles to their kept
 //                   slots (no atomics); large triangles after the small ones of each bin;
 //                   clear the accumulator and occupancy mask of bins with several units.
-//   6 k_voxelize    one workgroup per work unit of the batch; per work item (chunk) of it:
+//   6 k_voxelize    one workgroup per work unit of the batch, in that order; per work item
+//                   (chunk) of it:
 //                   triangles' edge/plane coefficients to LDS, columns of the dominant-axis
 //                   projection spread over the workgroup, each column tests its 1-4
 //                   candidate voxels with the exact separating-axis test and adds packed
@@ -601,16 +603,59 @@ hipError_t launch_settle(const VoxBatch& B, hipStream_t s) {
   return hipGetLastError();
 }
 
-// work unit -> (scene, bin), from the scanned unit offsets; workgroup blk of the bins
+// work unit -> (scene, bin), from the scanned unit offsets; workgroup blk of the bins. Also
+// the order in which k_voxelize takes the units: those of two or more chunks first, from
+// the front of the order, then the single-chunk ones from the back. A unit's time grows
+// with its chunks (on the room a 4-chunk unit runs about 27 us against 5-8 us for one
+// chunk), and taken in bin order the long ones that start last set the launch's end.
+// Within each class the order is that of the cursors' atomics; every unit's result is the
+// same in any order.
 __device__ __forceinline__ void workmap_block(const VoxBatch& B, const VoxScene* __restrict__ scenes, uint32_t blk_id) {
   const uint2 blk = bin_block(B, blk_id);
   const VoxScene& P = scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   const uint32_t b = blk.y + threadIdx.x;
-  if (b >= nbins) return;
-  const uint32_t items = (uint32_t)(gld(P.bin_packed + b) >> 32);
-  const uint32_t first = first_item(B, P, b);
-  for (uint32_t i = 0; i < items; i++) gst2(B.work_map + first + i, make_uint2(blk.x, b));
+  uint32_t units = 0, first = 0, heavy = 0;
+  if (b < nbins) {
+    const unsigned long long packed = gld(P.bin_packed + b);
+    units = (uint32_t)(packed >> 32);
+    if (units) {
+      first = first_item(B, P, b);
+      // units 0 .. units-2 hold unit_items chunks each, the last one the rest
+      const uint32_t items = ((uint32_t)packed + P.chunk - 1) / P.chunk;
+      const uint32_t last = items - (units - 1) * P.unit_items;
+      heavy = (P.unit_items >= 2u ? units - 1u : 0u) + (last >= 2u ? 1u : 0u);
+      for (uint32_t i = 0; i < units; i++) gst2(B.work_map + first + i, make_uint2(blk.x, b));
+    }
+  }
+  // places in the order: one cursor atomic per wavefront and class
+  const uint32_t light = units - heavy;
+  const uint32_t lane = __lane_id();
+  uint32_t hs = heavy, ls = light;  // inclusive scans over the wavefront
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t h = __shfl_up(hs, o), l = __shfl_up(ls, o);
+    if ((int)lane >= o) {
+      hs += h;
+      ls += l;
+    }
+  }
+  uint32_t hbase = 0, lbase = 0;
+  if (lane == 63u) {
+    if (hs) hbase = gatomic_add(B.overflow + 1, hs);
+    if (ls) lbase = gatomic_add(B.overflow + 2, ls);
+  }
+  hbase = __shfl(hbase, 63) + hs - heavy;
+  lbase = __shfl(lbase, 63) + ls - light;
+  if (units == 0u) return;
+  const uint32_t nwork = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);  // the batch's units
+  // heavy units (the first `heavy` of the bin, or all but the last) from the front, light
+  // ones from the back: positions hbase.. and nwork-1-lbase, nwork-2-lbase, ..
+  for (uint32_t i = 0; i < units; i++) {
+    const bool h = i < heavy;  // units 0 .. heavy-1 are the heavy ones (the last may be light)
+    const uint32_t pos = h ? hbase + i : nwork - 1u - (lbase + (i - heavy));
+    gst(B.unit_order + pos, first + i);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -998,14 +1043,15 @@ void k_voxelize(VoxBatch Bt, const VoxScene* __restrict__ scenes) {
   const uint32_t nwork = (uint32_t)(gld(Bt.scan_all + Bt.total_bins) >> 32);
 
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = gld2(Bt.work_map + w);
+    const uint32_t u = gld(Bt.unit_order + w);  // the longest units first
+    const uint2 item = gld2(Bt.work_map + u);
     const VoxScene& P = scenes[item.x];
     const uint32_t bin = item.y;
     const uint32_t nb = P.nbins_axis, B = 1u << P.bin_shift, nbr = P.n >> 3;
     const uint32_t sub = B / TS;  // sub-tiles per bin edge
     const unsigned long long packed = gld(P.bin_packed + bin);
     const uint32_t cnt_bin = (uint32_t)packed, units = (uint32_t)(packed >> 32);
-    const uint32_t unit = w - first_item(Bt, P, bin);
+    const uint32_t unit = u - first_item(Bt, P, bin);
     const uint32_t base = pair_base(Bt, P, bin);
     const uint32_t items_bin = (cnt_bin + P.chunk - 1) / P.chunk;
     const uint32_t it0 = unit * P.unit_items, it1 = min(it0 + P.unit_items, items_bin);

@@ -274,7 +274,7 @@ int vgi_scene_release(vgi_scene* scene);
 int vgi_scene_get_stats(const vgi_scene* scene, vgi_scene_stats* out);
 /* Device bytes a scene of this grid and precision holds while resident: accumulator,
  * occupancy mask, bin table and mip levels (admission control). The voxelize scratch
- * (about 112 bytes per triangle, 4 per bin pair and 40 per bin) belongs to the stream
+ * (about 112 bytes per triangle, 4 per bin pair and 44 per bin) belongs to the stream
  * the call runs on and is reported by vgi_workspace_bytes. num_triangles is unused. */
 uint64_t vgi_scene_bytes_estimate(const vgi_grid* grid, uint32_t precision, uint64_t num_triangles);
 
@@ -300,7 +300,7 @@ int vgi_voxelize(vgi_context* ctx, vgi_scene* scene, const vgi_mesh* meshes, uin
  * until the process ends, and the descriptors reach it through a kernel argument instead
  * of a host copy. No hipMalloc, hipFree, pinned allocation, host copy or memset is
  * issued. Limits: at most VGI_NOALLOC_MAX_MESHES meshes; the call's scratch (about 100
- * bytes per triangle, 16 per (triangle, bin) pair planned as 4 per triangle + 64 Ki, 44
+ * bytes per triangle, 16 per (triangle, bin) pair planned as 4 per triangle + 64 Ki, 48
  * per bin) must fit the array (a scene of 1,048,576 triangles on a 512^3 grid with 8^3
  * bins does), else VGI_E_RANGE; the scene's resident volume must already exist for this
  * grid size (one earlier vgi_voxelize with the same n and bin size), else VGI_E_STATE.
