@@ -10,10 +10,10 @@
 // step t += step_factor * d, stop at max_distance, alpha >= cutoff or the grid boundary.
 //
 // Schedule: a workgroup is one screen tile (8x8, 16x16 or 32x8 pixels; each wavefront an
-// 8x8 block) of one scene of the batch; tiles are remapped so that each XCD gets a
-// contiguous band (neighbouring tiles share that XCD's L2), and every lane of a wavefront
-// marches the same cone index at the same time, so directions stay coherent across the
-// wavefront.
+// 8x8 block) of one scene of the batch; tiles are dealt to the XCDs in bands of one tile
+// row (neighbouring tiles share that XCD's L2, and every XCD works across the whole
+// screen), and every lane of a wavefront marches the same cone index at the same time,
+// so directions stay coherent across the wavefront.
 //
 // Gather: a quadrilinear sample is 16 weighted texels, sum_i w_i c_i with the level blend
 // folded into the weights, accumulated in FP32 (FP16 texels feed v_fma_mix directly; no
@@ -506,13 +506,25 @@ __global__ __launch_bounds__(256) void k_trace(TraceBatch Bt, const TraceParams*
                                                const uint2* __restrict__ blocks,
                                                const unsigned long long* __restrict__ skeys,
                                                const uint32_t* __restrict__ spix) {
-  // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous band of the
-  // batch's work list (tiles of one scene are consecutive, row-major; sorted runs are
-  // consecutive in Morton order).
+  // XCD-aware order: block b runs on XCD b % 8. The batch's work list (tiles of one scene
+  // are consecutive, row-major; sorted runs are consecutive in Morton order) is dealt to
+  // the XCDs in bands of Bt.band consecutive items, XCD x taking bands x, x + 8, ...:
+  // neighbouring items share that XCD's L2, and every XCD draws from the whole list, so
+  // they finish together (one contiguous eighth each left the room's frame with XCD
+  // workloads 20% apart). The items after the last whole round of bands are split into
+  // eight contiguous parts.
   const uint32_t b = blockIdx.x;
-  const uint32_t xcd = b & 7u, slot = b >> 3;
-  const uint32_t full = Bt.nblocks >> 3, rem = Bt.nblocks & 7u;
-  const uint32_t idx = xcd * full + (xcd < rem ? xcd : rem) + slot;
+  const uint32_t xcd = b & 7u;
+  const uint32_t per = Bt.band * 8u, rounds = Bt.band ? Bt.nblocks / per : 0u;
+  uint32_t idx;
+  if (b < rounds * per) {
+    const uint32_t slot = b >> 3;
+    idx = ((slot / Bt.band) * 8u + xcd) * Bt.band + slot % Bt.band;
+  } else {
+    const uint32_t r = b - rounds * per, n = Bt.nblocks - rounds * per;  // r % 8 == xcd
+    const uint32_t full = n >> 3, rem = n & 7u;
+    idx = rounds * per + xcd * full + (xcd < rem ? xcd : rem) + (r >> 3);
+  }
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t scene;
   size_t p;

@@ -260,6 +260,7 @@ struct TraceBatch {
   uint32_t tile_w, tile_h;
   uint32_t prec;
   uint32_t nscenes;
+  uint32_t band;  // work items (tiles or sorted runs) an XCD takes in a row before the next XCD's
 };
 
 constexpr uint32_t kNoPixel = 0xFFFFFFFFu;
