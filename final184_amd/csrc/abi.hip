@@ -777,9 +777,10 @@ int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
   {
     // The levels start all zero and a tile's texels (a level-0 x-row's) are stored only
     // while the tile (the row) or its previous build held occupied voxels, so every texel
-    // of the levels stays exact: one word per tile, then eight row words per tile.
+    // of the levels stays exact: one word per tile, then eight row words per tile; then
+    // the injecting launch's tile list (a word per tile) and its two counts.
     const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u);
-    const size_t bytes = tpa * tpa * tpa * 9 * sizeof(uint32_t);
+    const size_t bytes = tpa * tpa * tpa * 10 * sizeof(uint32_t) + 256;
     hipError_t e = hipMalloc((void**)&s->live, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -881,6 +882,10 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     P.mask = s->mask;
     P.bin_packed = (const unsigned long long*)s->bin_packed;
     P.live = s->live;
+    {
+      const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u);
+      P.order = s->live + 9 * tpa * tpa * tpa;
+    }
     for (uint32_t k = 0; k < g.levels; k++) {
       P.level[k] = s->level[k];
       P.nk[k] = g.n >> k;
@@ -918,6 +923,16 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     B.src = stages[j][0].src;  // one scene: every block of the stage shares src and nout
     B.nout = stages[j][0].nout;
     B.shadow = shadow;
+    // (not for a one-level chain: its launch writes every texel of every tile and keeps no
+    // tile words, so every tile is visited; not with visibility walks, whose cost does not
+    // follow the pair count: the room's shadowed builds were 3-5% slower in list order)
+    B.ordered = j == 0 && !table && stages[0][0].nout > 0 && !shadow;
+    if (B.ordered) {  // the injecting launch's tile list, dense tiles first
+      const vgi_grid& g = items[0].scene->grid;
+      const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u), ntl = tpa * tpa * tpa;
+      HIP_TRY(launch_fill_u32(items[0].scene->live + 10 * ntl, 0u, 2, st));
+      HIP_TRY(launch_tiles(B, (uint32_t)ntl, st));
+    }
     HIP_TRY(launch_mips(B, precision, j == 0, (uint32_t)stages[j].size(), st));
     off += stages[j].size();
   }
@@ -1430,7 +1445,7 @@ uint64_t vgi_scene_bytes_estimate(const vgi_grid* g, uint32_t precision, uint64_
   const uint64_t n = g->n;
   const uint64_t tpa = n / std::min<uint64_t>(n, 16);  // 16^3 tiles per axis (mip tile state)
   uint64_t b = n * n * n * kAccWords * 4 + (n / 8) * (n / 8) * (n / 8) * 64 + (n / 8) * (n / 8) * (n / 8) * 8 +
-               tpa * tpa * tpa * 9 * 4;
+               tpa * tpa * tpa * 10 * 4 + 256;
   for (uint32_t k = 0; k < g->levels; k++) {
     const uint64_t p = (n >> k) + 2;
     b += p * p * p * texel_bytes(precision);

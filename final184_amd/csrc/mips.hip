@@ -329,7 +329,8 @@ constexpr uint32_t mips_threads(int prec, bool src_acc, bool shadow) {
 template <int PREC, bool SRC_ACC, bool SHADOW>
 __global__ __launch_bounds__(1024) void k_mips(const MipScene* __restrict__ scenes,
                                                const MipBlock* __restrict__ blocks,
-                                               const float* __restrict__ lut, uint32_t src1, uint32_t nout1) {
+                                               const float* __restrict__ lut, uint32_t src1, uint32_t nout1,
+                                               uint32_t ordered) {
   using TX = Texel<PREC>;
   constexpr uint32_t kThreads = mips_threads(PREC, SRC_ACC, SHADOW);
   __shared__ float4 s_a[512];
@@ -340,7 +341,13 @@ __global__ __launch_bounds__(1024) void k_mips(const MipScene* __restrict__ scen
   const uint32_t ns = P.nk[src];
   const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge (a power of two)
   const uint32_t tpa = ns / S;             // tiles per axis
-  const uint32_t tile = blk.tile;
+  uint32_t tile = blk.tile;
+  if (SRC_ACC && ordered) {  // block b takes the b-th tile of k_tiles' list; unlisted tiles keep their zeros
+    const uint32_t ntl = tpa * tpa * tpa, b = blockIdx.x;
+    const uint32_t nd = gld(P.order + ntl), nl = gld(P.order + ntl + 1);
+    if (b >= nd + nl) return;  // uniform
+    tile = gld(P.order + (b < nd ? b : ntl - 1u - (b - nd)));
+  }
   const uint32_t tx = tile % tpa, ty = (tile / tpa) % tpa, tz = tile / (tpa * tpa);
   const uint32_t G = S >> 1;  // groups per tile edge (>= 1 when nout >= 1)
   const uint32_t gsh = G > 1 ? (uint32_t)__builtin_ctz(G) : 0u;
@@ -514,11 +521,61 @@ template <int PREC>
 static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, hipStream_t s) {
   const dim3 b(mips_threads(PREC, from_acc, from_acc && B.shadow));
   if (!from_acc)
-    hipLaunchKernelGGL((k_mips<PREC, false, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+    hipLaunchKernelGGL((k_mips<PREC, false, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
+                       (uint32_t)B.ordered);
   else if (B.shadow)
-    hipLaunchKernelGGL((k_mips<PREC, true, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+    hipLaunchKernelGGL((k_mips<PREC, true, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
+                       (uint32_t)B.ordered);
   else
-    hipLaunchKernelGGL((k_mips<PREC, true, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout);
+    hipLaunchKernelGGL((k_mips<PREC, true, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
+                       (uint32_t)B.ordered);
+}
+
+// The injecting launch's tiles for one scene, densest first: a tile is listed when its bins
+// hold pairs (it may hold voxels) or its stored texels are not all zero (it must be
+// cleared); tiles whose bins hold at least kDenseTilePairs pairs go to the front of the
+// list, the others to the back, with one atomic per wavefront and class on the two counts
+// after the list (cleared before the launch). Unlisted tiles hold no voxel and only zeros,
+// and are left as they are. A dense tile's inject sets the injecting launch's length (on
+// the room, about 3200 occupied voxels against a median of 430), so those start first,
+// and empty tiles no longer take a workgroup slot each. A tile's bins' pair count follows
+// its occupied voxel count closely (correlation 0.96 on the room); 384 pairs marks about
+// the densest third of the room's occupied tiles.
+constexpr uint32_t kDenseTilePairs = 384;
+__global__ __launch_bounds__(256) void k_tiles(const MipScene* __restrict__ scenes) {
+  const MipScene& P = scenes[0];
+  const uint32_t ns = P.nk[0], S = ns < 16u ? ns : 16u, tpa = ns / S, ntl = tpa * tpa * tpa;
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint32_t dense = 0, other = 0;
+  if (t < ntl) {
+    const uint32_t tx = t % tpa, ty = (t / tpa) % tpa, tz = t / (tpa * tpa);
+    const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
+    unsigned long long pairs = 0;
+    for (uint32_t z = (tz * S) >> bs; z <= (tz * S + S - 1u) >> bs; z++)
+      for (uint32_t y = (ty * S) >> bs; y <= (ty * S + S - 1u) >> bs; y++)
+        for (uint32_t x = (tx * S) >> bs; x <= (tx * S + S - 1u) >> bs; x++)
+          pairs += (uint32_t)gld(P.bin_packed + x + nb * (y + nb * z));
+    const bool listed = pairs != 0 || gld(P.live + t) != 0u;
+    dense = listed && pairs >= kDenseTilePairs ? 1u : 0u;
+    other = listed && !dense ? 1u : 0u;
+  }
+  const uint32_t lane = __lane_id();
+  const uint64_t dm = __ballot(dense), om = __ballot(other);
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t dbase = 0, obase = 0;
+  if (lane == 0) {
+    if (dm) dbase = gatomic_add(P.order + ntl, (uint32_t)__popcll(dm));
+    if (om) obase = gatomic_add(P.order + ntl + 1, (uint32_t)__popcll(om));
+  }
+  dbase = __shfl(dbase, 0);
+  obase = __shfl(obase, 0);
+  if (dense) gst(P.order + dbase + (uint32_t)__popcll(dm & below), t);
+  if (other) gst(P.order + ntl - 1u - (obase + (uint32_t)__popcll(om & below)), t);
+}
+
+hipError_t launch_tiles(const MipBatch& B, uint32_t ntiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, s, B.scenes);
+  return hipGetLastError();
 }
 
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s) {

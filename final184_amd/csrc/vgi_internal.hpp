@@ -169,6 +169,8 @@ struct MipScene {
   uint32_t* live;  // per 16^3 tile of level 0: 1 while the tile's stored texels are not all zero;
                    // after the tiles' words, 8 words per tile: bit (ly + 16 lz) while that
                    // x-row's stored level-0 texels are not all zero
+  uint32_t* order; // the injecting launch's tiles, dense ones first (one word per tile), then
+                   // the dense and other tiles' counts (k_tiles)
   void* level[VGI_MAX_LEVELS];
   uint32_t nk[VGI_MAX_LEVELS];
   uint32_t n;
@@ -193,6 +195,7 @@ struct MipBatch {
   const float* lut;        // 256 entries: (i/255)^2.2
   uint32_t src, nout;
   bool shadow;             // some light of the batch casts shadows (the injecting launch walks)
+  bool ordered;            // one scene, injecting launch: block b takes the listed tile b
 };
 
 // One scene of a trace batch (device memory).
@@ -392,6 +395,7 @@ hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, u
 hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_blocks, hipStream_t s);
 hipError_t launch_voxelize_fallback(const VoxBatch& B, uint32_t grid_blocks, hipStream_t s);
 hipError_t launch_mips(const MipBatch& B, uint32_t prec, bool from_acc, uint32_t blocks, hipStream_t s);
+hipError_t launch_tiles(const MipBatch& B, uint32_t ntiles, hipStream_t s);
 hipError_t launch_trace(const TraceBatch& B, bool buffer_path, hipStream_t s);
 // Morton keys of every pixel of a batch (entries seg[0]..seg[nscenes], padding included).
 hipError_t launch_morton_keys(const TraceBatch& B, unsigned long long* keys, uint32_t* pix, uint32_t entries,
