@@ -233,11 +233,14 @@ __device__ __forceinline__ uint32_t half_weight(float w) {
 constexpr uint32_t kHalfOne = 0x00003C00u;  // w = 1: (1, 0)
 
 // ---- wave-uniform level: buffer loads with SGPR descriptor and pitches ----
-// FP32: when every active lane's footprint is the same 2x2x2 texel block (usual at coarse
+// When every active lane's footprint is the same 2x2x2 texel block (usual at coarse
 // levels, where a texel spans many pixels' cone positions), the block is read once with
-// scalar loads and broadcast, so the texture-address path does no work for it. That path
-// bounds the FP32 form (eight 16-byte loads per gather); the FP16 form, with half the
-// bytes, is bound by VALU instead, and the vote would only add instructions there. The
+// scalar loads and broadcast, so the texture-address path does no work for it. Vector
+// loads of 16 bytes a lane (eight a gather in FP32, four in the 8-byte and x-pair forms)
+// keep that path near its limit even where VALU issue bounds the kernel: the vote took
+// the FP16 x-pair trace from 0.86 to 0.71 ms and plain FP16 from 0.92 to 0.87 ms. The
+// 8-byte and x-pair forms vote only for a sample's first (coarser) gather, whose
+// footprint is uniform more often; voting for the second as well was slower. The
 // weighted sum is the same either way.
 typedef uint32_t u32x8_t __attribute__((ext_vector_type(8)));
 #define VGI_CONSTANT __attribute__((address_space(4)))
@@ -286,8 +289,8 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
   } else {
     const Foot f = foot_weights(c, wl);
     const float wxy[4][2] = {{f.w00, f.w10}, {f.w01, f.w11}, {f.w00, f.w10}, {f.w01, f.w11}};
+    const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
     if constexpr (PREC == VGI_PREC_FP32) {
-      const uint32_t off0 = __builtin_amdgcn_readfirstlane(off);
       if (__all(off == off0)) {
         const char* base = (const char*)L.ptr + off0;
   #pragma unroll
@@ -302,6 +305,18 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
         }
         return;
       }
+    } else if (INIT && __all(off == off0)) {
+      const char* base = (const char*)L.ptr + off0;
+  #pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float wz = q < 2 ? f.wz0 : f.wz1;
+        const vgi_u32x4 e = *(const VGI_CONSTANT vgi_u32x4*)(base + soff[q]);
+        const uint32_t a0 = e[0], a1 = e[1], a2 = e[2], a3 = e[3];
+        Acc<PREC>::add(acc, wxy[q][0] * wz, a0, a1);
+        Acc<PREC>::add(acc, wxy[q][1] * wz, a2, a3);
+        abits |= (a1 | a3) & 0xFFFF0000u;  // opacity is the high half
+      }
+      return;
     }
     const __amdgpu_buffer_rsrc_t r =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(L.ptr), 0, (int)L.bytes32, 0x00020000);  // unsigned field
