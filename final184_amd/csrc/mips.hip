@@ -550,12 +550,26 @@ __global__ __launch_bounds__(256) void k_tiles(const MipScene* __restrict__ scen
   if (t < ntl) {
     const uint32_t tx = t % tpa, ty = (t / tpa) % tpa, tz = t / (tpa * tpa);
     const uint32_t bs = P.bin_shift, nb = P.nbins_axis;
+    const uint32_t x0 = (tx * S) >> bs, y0 = (ty * S) >> bs, z0 = (tz * S) >> bs;
+    const uint32_t bpa = (S >> bs) > 1u ? S >> bs : 1u;  // bins per tile edge
+    const uint32_t live = gld(P.live + t);
     unsigned long long pairs = 0;
-    for (uint32_t z = (tz * S) >> bs; z <= (tz * S + S - 1u) >> bs; z++)
-      for (uint32_t y = (ty * S) >> bs; y <= (ty * S + S - 1u) >> bs; y++)
-        for (uint32_t x = (tx * S) >> bs; x <= (tx * S + S - 1u) >> bs; x++)
-          pairs += (uint32_t)gld(P.bin_packed + x + nb * (y + nb * z));
-    const bool listed = pairs != 0 || gld(P.live + t) != 0u;
+    if (bpa <= 2u) {  // bins of >= 8 voxels (or one bin for grids under 8): all loads in flight together
+      uint32_t v[8];
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; i++) {
+        const uint32_t dx = i & 1u, dy = (i >> 1) & 1u, dz = i >> 2;
+        v[i] = dx < bpa && dy < bpa && dz < bpa ? (uint32_t)gld(P.bin_packed + (x0 + dx) + nb * ((y0 + dy) + nb * (z0 + dz)))
+                                                : 0u;
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < 8u; i++) pairs += v[i];
+    } else {
+      for (uint32_t z = z0; z < z0 + bpa; z++)
+        for (uint32_t y = y0; y < y0 + bpa; y++)
+          for (uint32_t x = x0; x < x0 + bpa; x++) pairs += (uint32_t)gld(P.bin_packed + x + nb * (y + nb * z));
+    }
+    const bool listed = pairs != 0 || live != 0u;
     dense = listed && pairs >= kDenseTilePairs ? 1u : 0u;
     other = listed && !dense ? 1u : 0u;
   }

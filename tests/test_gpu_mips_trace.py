@@ -83,6 +83,25 @@ def test_mips_bit_exact(ctx, soup, prec):
     f.release()
 
 
+@pytest.mark.parametrize("bin_size", [16, 32])
+@pytest.mark.parametrize("prec", ["fp32", "fp16x2"])
+def test_mips_bit_exact_wide_bins(ctx, soup, prec, bin_size):
+    # bins of 16 and 32 voxels: each 16^3 mip tile lies inside one bin, so the injecting
+    # launch's tile list counts one bin's pairs per tile (two bins per tile edge at bin 8)
+    w = soup["w"]
+    print(f"case {w.name} mips {prec} bin {bin_size}", flush=True)
+    try:
+        ctx.set_tuning(bin_size=bin_size)
+        f = Frame(ctx, w, prec)
+    finally:
+        ctx.set_tuning()
+    ref = orc.inject_mips(soup["acc"], w.grid, w.lights, ORC_PREC[prec])
+    for k in range(w.grid["levels"]):
+        got = f.scene.read_level(k)
+        assert np.array_equal(got, ref[k]), f"level {k}: {(got != ref[k]).sum()} values differ"
+    f.release()
+
+
 def test_mips_room_levels(ctx):
     w = sg.room(ntri=60_000, n=128, levels=7, width=32, height=18)
     print(f"case {w.name} mips", flush=True)
