@@ -270,7 +270,7 @@ __device__ __forceinline__ void gather_uniform(const TraceParams& P, uint32_t k,
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const vgi_u32x4 e = *(const VGI_CONSTANT vgi_u32x4*)(base + soff[q]);
-        if (q == 0) AccPair::first<true>(acc, pw.r[q], e[0], e[1], e[2], e[3]);
+        if (q == 0 && INIT) AccPair::first<true>(acc, pw.r[q], e[0], e[1], e[2], e[3]);
         else AccPair::add(acc, pw.r[q], e[0], e[1], e[2], e[3]);
         abits |= e[3];
       }
