@@ -18,6 +18,7 @@
 // host, uploads them with one copy from a pinned staging slot, and launches the
 // batch-native kernels.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -170,12 +171,11 @@ struct vgi_workspace {
   size_t sort_pix_bytes = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  // pinned: the last call's batch pairs, work units, large triangles and split bins,
-  // readable once the event `settled` (recorded behind the call's k_settle) has completed
+  // pinned: the last call's batch pairs, work units, large triangles and split bins, then
+  // its call number, which k_settle stores after the four totals (release order)
   unsigned long long* hist = nullptr;
-  hipEvent_t settled = nullptr;
-  bool settled_pending = false;
-  uint32_t seq = 0;  // last voxelize call number on this stream (never 0)
+  uint32_t seq = 0;        // number of the last voxelize call on this stream (never 0)
+  uint32_t hist_read = 0;  // the call whose totals were taken into the maxima below
   uint64_t hist_pairs = 0, hist_units = 0, hist_big = 0, hist_split = 0;  // largest seen
 
   size_t device_bytes() const {
@@ -200,6 +200,7 @@ struct vgi_context {
   bool ev_vox = false, ev_mips = false, ev_trace = false;
   hipEvent_t tstart = nullptr, tstop = nullptr;
   hipEvent_t dep = nullptr;  // vgi_stream_wait (a wait captures the record it follows)
+  std::atomic<uint32_t> vox_calls{0};  // voxelize call numbers, unique across the context's streams
   vgi_timings timings{};
   std::mutex mu;  // tuning, the workspace map
   std::map<hipStream_t, std::unique_ptr<vgi_workspace>> ws;
@@ -226,14 +227,21 @@ struct vgi_scene {
   bool has_voxels = false, has_mips = false;
   uint64_t tris_in = 0;
   // the last voxelize call's statistics: written by the device into the pinned record after
-  // the bin scan (`result`, call number vox_seq); vox_done is recorded on the call's stream
-  // after that, and vgi_scene_get_stats waits for it before reading
+  // the bin scan (`result`), its call number (vox_seq) stored last, so the record is
+  // complete once the number matches; no event is recorded between the call's kernels
   SceneVoxResult* result = nullptr;
-  hipEvent_t vox_done = nullptr;
   uint32_t vox_seq = 0;
 };
 
 namespace {
+
+// A voxelize call's number (never 0), unique across the context's streams: a scene's
+// record matches its last call's number only once that call's k_settle has stored it.
+uint32_t next_vox_call(vgi_context* ctx) {
+  uint32_t v = ctx->vox_calls.fetch_add(1u, std::memory_order_relaxed) + 1u;
+  if (v == 0u) v = ctx->vox_calls.fetch_add(1u, std::memory_order_relaxed) + 1u;
+  return v;
+}
 
 size_t scene_device_bytes(const vgi_scene* s) {
   size_t b = s->acc_bytes + s->mask_bytes + s->bin_packed_bytes + s->live_bytes;
@@ -252,7 +260,6 @@ void free_workspace(vgi_workspace* w) {
     if (s.done) (void)hipEventDestroy(s.done);
   }
   if (w->hist) (void)hipHostFree(w->hist);
-  if (w->settled) (void)hipEventDestroy(w->settled);
 }
 
 hipStream_t pick(vgi_context* ctx, void* stream) { return stream ? (hipStream_t)stream : ctx->stream; }
@@ -504,7 +511,7 @@ struct VoxPlan {
 int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, size_t zeroed_bytes, void* scan_tmp,
                             size_t scan_tmp_bytes, size_t nblk_t, size_t nblk_b, uint32_t ts, uint32_t grid_blocks,
                             uint32_t big_blocks, uint32_t zero_blocks, vgi_scene* const* scenes, uint32_t count,
-                            hipEvent_t settled, hipStream_t st, bool prof) {
+                            hipStream_t st, bool prof) {
   const uint32_t cus = (uint32_t)ctx->num_cus;
   const uint32_t persistent = cus * 4;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[0], st));
@@ -517,11 +524,7 @@ int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, s
   size_t tmp_bytes = scan_tmp_bytes;
   HIP_TRY(scan_bins(scan_tmp, &tmp_bytes, B.packed_all, B.scan_all, B.total_bins + 1, st));
   HIP_TRY(launch_settle(B, st));
-  for (uint32_t i = 0; i < count; i++) {
-    scenes[i]->vox_seq = B.seq;
-    HIP_TRY(hipEventRecord(scenes[i]->vox_done, st));
-  }
-  if (settled) HIP_TRY(hipEventRecord(settled, st));
+  for (uint32_t i = 0; i < count; i++) scenes[i]->vox_seq = B.seq;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
   HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
@@ -588,21 +591,15 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
     TRY(scene_reserve(p.s, p.n, p.nbins));
   }
 
-  // ---- history: the largest batch totals of earlier calls on this stream that have
-  // completed (read without waiting; a call still in flight is read by a later one) ----
-  if (w->settled_pending) {
-    const hipError_t q = hipEventQuery(w->settled);
-    if (q == hipSuccess) {
-      w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
-      w->hist_units = std::max<uint64_t>(w->hist_units, w->hist[1]);
-      w->hist_big = std::max<uint64_t>(w->hist_big, w->hist[2]);
-      w->hist_split = std::max<uint64_t>(w->hist_split, w->hist[3]);
-      w->settled_pending = false;
-    } else if (q != hipErrorNotReady) {
-      HIP_TRY(q);
-    } else {
-      (void)hipGetLastError();
-    }
+  // ---- history: the largest batch totals of earlier calls on this stream (read without
+  // waiting, once the last call's k_settle has stored them; a call still in flight is read
+  // by a later one). No later call is enqueued yet, so nothing rewrites them meanwhile. ----
+  if (w->hist && w->hist_read != w->seq && __atomic_load_n(&w->hist[4], __ATOMIC_ACQUIRE) == w->seq) {
+    w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
+    w->hist_units = std::max<uint64_t>(w->hist_units, w->hist[1]);
+    w->hist_big = std::max<uint64_t>(w->hist_big, w->hist[2]);
+    w->hist_split = std::max<uint64_t>(w->hist_split, w->hist[3]);
+    w->hist_read = w->seq;
   }
 
   // ---- scratch sized for this batch (grown; queued work on this stream is drained
@@ -643,13 +640,11 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
-  if (!w->hist) {
-    HIP_TRY(hipHostMalloc((void**)&w->hist, 4 * sizeof(unsigned long long), hipHostMallocCoherent));
-    std::memset(w->hist, 0, 4 * sizeof(unsigned long long));
-    // no system-scope fence: k_settle writes the history with system-scope stores
-    HIP_TRY(hipEventCreateWithFlags(&w->settled, hipEventDisableTiming | hipEventDisableSystemFence));
+  if (!w->hist) {  // k_settle writes it with system-scope stores
+    HIP_TRY(hipHostMalloc((void**)&w->hist, kHostHistWords * sizeof(unsigned long long), hipHostMallocCoherent));
+    std::memset(w->hist, 0, kHostHistWords * sizeof(unsigned long long));
   }
-  if (++w->seq == 0u) w->seq = 1u;
+  w->seq = next_vox_call(ctx);
 
   // ---- descriptor tables: scenes | meshes | triangle blocks | bin blocks ----
   const bool multi = count > 1;
@@ -740,9 +735,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const uint32_t zero_blocks = (uint32_t)std::min<uint64_t>(
       std::min<uint64_t>(cus * 4, total_bins), w->hist_split ? w->hist_split + w->hist_split / 4 + 1 : cus / 4);
   TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_need, w->scan_tmp, w->scan_tmp_bytes, nblk_t, nblk_b, ts,
-                              grid_blocks, big_blocks, zero_blocks, scs.data(), count, w->settled, st,
-                              ctx->profiling != 0));
-  w->settled_pending = true;
+                              grid_blocks, big_blocks, zero_blocks, scs.data(), count, st, ctx->profiling != 0));
   for (uint32_t i = 0; i < count; i++) {
     vgi_scene* s = plan[i].s;
     s->grid = items[i].grid;
@@ -1411,10 +1404,8 @@ int vgi_scene_create(vgi_context* ctx, vgi_scene** out) {
   HIP_TRY(hipSetDevice(ctx->device));
   vgi_scene* s = new vgi_scene();
   s->ctx = ctx;
-  if (hipHostMalloc((void**)&s->result, sizeof(SceneVoxResult), hipHostMallocCoherent) != hipSuccess ||
-      hipEventCreateWithFlags(&s->vox_done, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
+  if (hipHostMalloc((void**)&s->result, sizeof(SceneVoxResult), hipHostMallocCoherent) != hipSuccess) {
     (void)hipGetLastError();
-    if (s->result) (void)hipHostFree(s->result);
     delete s;
     return fail(VGI_E_NOMEM, "scene record allocation failed");
   }
@@ -1434,7 +1425,6 @@ int vgi_scene_release(vgi_scene* s) {
     if (s->level[k]) (void)hipFree(s->level[k]);
   if (s->live) (void)hipFree(s->live);
   if (s->result) (void)hipHostFree(s->result);
-  if (s->vox_done) (void)hipEventDestroy(s->vox_done);
   delete s;
   return VGI_OK;
 }
@@ -1458,10 +1448,14 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   std::memset(out, 0, sizeof(*out));
   out->triangles_in = s->tris_in;
   if (s->vox_seq != 0) {  // the voxelize call's own statistics, once the device stored them
-    HIP_TRY(hipSetDevice(s->ctx->device));
-    HIP_TRY(hipEventSynchronize(s->vox_done));
-    const SceneVoxResult* r = s->result;  // written by k_settle before the event
-    if (r->seq != s->vox_seq) return fail(VGI_E_HIP, "voxelize statistics of the scene's last call are missing");
+    const SceneVoxResult* r = s->result;  // k_settle stores its call number after the rest
+    if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq) {
+      // that call's k_settle has not run yet: wait for the device's queued work
+      HIP_TRY(hipSetDevice(s->ctx->device));
+      HIP_TRY(hipDeviceSynchronize());
+      if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq)
+        return fail(VGI_E_HIP, "voxelize statistics of the scene's last call are missing");
+    }
     out->triangles_rejected = r->st.rejected;
     out->triangles_degenerate = r->st.degenerate;
     out->bin_pairs = r->st.pairs;
@@ -1517,7 +1511,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(nbins + 1), pick(ctx, stream)));
   Carve cv;
   const size_t o_desc = cv.take(sizeof(NoAllocDesc));
-  const size_t o_hist = cv.take(4 * sizeof(unsigned long long));
+  const size_t o_hist = cv.take(kHostHistWords * sizeof(unsigned long long));
   const size_t o_zero = cv.take(zeroed_bytes);
   bool textured = false;
   for (uint32_t k = 0; k < num_meshes; k++)
@@ -1593,7 +1587,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.work_cap = (uint32_t)work_cap64;
   B.overflow = (uint32_t*)(base + o_zero + ovf_off);
   B.host_hist = (unsigned long long*)(base + o_hist);  // device memory: nothing reads it back
-  B.seq = sc->vox_seq + 1u ? sc->vox_seq + 1u : 1u;
+  B.seq = next_vox_call(ctx);
   B.nscenes = 1;
   B.total_bins = (uint32_t)nbins;
   B.textured = textured ? 1u : 0u;
@@ -1601,7 +1595,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   const uint32_t cus = (uint32_t)ctx->num_cus;
   TRY(launch_voxelize_kernels(ctx, B, zb, zeroed_bytes, base + o_stmp, scan_need, ceil_div(ntri, kBlockTris),
                               ceil_div(nbins, kBlockBins), ts, grid_blocks, cus,
-                              (uint32_t)std::min<uint64_t>(cus / 4, nbins), &sc, 1, nullptr, st, false));
+                              (uint32_t)std::min<uint64_t>(cus / 4, nbins), &sc, 1, st, false));
   sc->grid = *grid;
   sc->bin_shift = bin_shift;
   sc->nbins_axis = nba;

@@ -74,8 +74,12 @@ struct DevStats {
 };
 static_assert(sizeof(DevStats) == 32, "DevStats layout");
 
+// Words of a call's pinned batch totals (VoxBatch::host_hist).
+constexpr uint32_t kHostHistWords = 5;
+
 // A scene's result of one voxelize call, stored by k_settle after the bin scan into the
-// scene's pinned host record (read by vgi_scene_get_stats once the call has completed).
+// scene's pinned host record, seq once the other words are complete: the record is whole
+// once seq equals the call's number (read by vgi_scene_get_stats).
 struct SceneVoxResult {
   DevStats st;
   unsigned long long totals;  // work units << 32 | pairs of the scene's bins
@@ -138,7 +142,8 @@ struct VoxBatch {
   uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps;
                                     // the two words after it (cleared with it) are the unit
                                     // order's front and back cursors
-  unsigned long long* host_hist;    // pinned: batch pairs, units, large triangles, split bins
+  unsigned long long* host_hist;    // pinned (kHostHistWords): batch pairs, units, large
+                                    // triangles, split bins, then seq (stored after them)
   uint32_t seq;                     // this call's number (never 0)
   uint32_t nscenes;
   uint32_t total_bins;

@@ -269,8 +269,9 @@ int vgi_sync(vgi_context* ctx, void* stream);
  * the accumulator and the mip levels; memory is sized on the first voxelize. */
 int vgi_scene_create(vgi_context* ctx, vgi_scene** out);
 int vgi_scene_release(vgi_scene* scene);
-/* Waits until the scene's last voxelize call has stored its statistics on the device
- * (the bin scan of that call has run), then reads them. */
+/* Reads the statistics of the scene's last voxelize call. When that call's bin scan has
+ * not yet run on the device, it first waits for the device's queued work
+ * (hipDeviceSynchronize), so it may wait for other streams too. */
 int vgi_scene_get_stats(const vgi_scene* scene, vgi_scene_stats* out);
 /* Device bytes a scene of this grid and precision holds while resident: accumulator,
  * occupancy mask, bin table and mip levels (admission control). The voxelize scratch
