@@ -172,7 +172,7 @@ struct vgi_workspace {
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   // pinned: the last call's batch pairs, work units, large triangles and split bins, then
-  // its call number, which k_settle stores after the four totals (release order)
+  // its call number, which the settle step stores once the four totals are complete
   unsigned long long* hist = nullptr;
   uint32_t seq = 0;        // number of the last voxelize call on this stream (never 0)
   uint32_t hist_read = 0;  // the call whose totals were taken into the maxima below
@@ -236,7 +236,7 @@ struct vgi_scene {
 namespace {
 
 // A voxelize call's number (never 0), unique across the context's streams: a scene's
-// record matches its last call's number only once that call's k_settle has stored it.
+// record matches its last call's number only once that call's settle step has stored it.
 uint32_t next_vox_call(vgi_context* ctx) {
   uint32_t v = ctx->vox_calls.fetch_add(1u, std::memory_order_relaxed) + 1u;
   if (v == 0u) v = ctx->vox_calls.fetch_add(1u, std::memory_order_relaxed) + 1u;
@@ -503,9 +503,10 @@ struct VoxPlan {
 
 // The kernels of one voxelize call once its descriptors and scratch are in place: B holds
 // their device addresses, the zeroed_bytes at zb (bin counters, per-scene statistics, the
-// overflow word) are cleared first, and each scene's statistics event is recorded after
-// k_settle. Everything after the scan reads its sizes on the device: k_scatter's role
-// blocks are sized from the host's counts (triangle and bin blocks) and persistent for the
+// overflow word) are cleared first; the first wavefront of k_scatter settles the call
+// (statistics, overflow word). Everything after the scan reads its sizes on the device:
+// k_scatter's role blocks are sized from the host's counts (triangle and bin blocks) and
+// persistent for the
 // large triangles and the split bins, whose counts only the device knows; k_voxelize takes
 // one work unit per workgroup up to grid_blocks and loops over the rest.
 int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, size_t zeroed_bytes, void* scan_tmp,
@@ -523,8 +524,7 @@ int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, s
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
   size_t tmp_bytes = scan_tmp_bytes;
   HIP_TRY(scan_bins(scan_tmp, &tmp_bytes, B.packed_all, B.scan_all, B.total_bins + 1, st));
-  HIP_TRY(launch_settle(B, st));
-  for (uint32_t i = 0; i < count; i++) scenes[i]->vox_seq = B.seq;
+  for (uint32_t i = 0; i < count; i++) scenes[i]->vox_seq = B.seq;  // k_scatter stores it
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
   HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[3], st));
@@ -592,7 +592,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   }
 
   // ---- history: the largest batch totals of earlier calls on this stream (read without
-  // waiting, once the last call's k_settle has stored them; a call still in flight is read
+  // waiting, once the last call's settle step has stored them; a call still in flight is read
   // by a later one). No later call is enqueued yet, so nothing rewrites them meanwhile. ----
   if (w->hist && w->hist_read != w->seq && __atomic_load_n(&w->hist[4], __ATOMIC_ACQUIRE) == w->seq) {
     w->hist_pairs = std::max<uint64_t>(w->hist_pairs, w->hist[0]);
@@ -612,7 +612,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const size_t zeroed_need = ovf_off + 256;
   // The pair list holds 4 pairs per triangle plus 64 Ki (a triangle in at most 8 bins is
   // the common case; large triangles add more), or 5/4 of the most an earlier call on this
-  // stream needed. A call that needs more is still exact: k_settle sees it and the
+  // stream needed. A call that needs more is still exact: k_scatter sees it and the
   // per-triangle path runs, and the next call is sized from it.
   const uint64_t pair_want = std::max<uint64_t>(4ull * total_tri + 65536, w->hist_pairs + w->hist_pairs / 4 + 4096);
   const uint32_t pair_cap = (uint32_t)std::min<uint64_t>(pair_want, 0xFFFFFFF0ull);
@@ -640,7 +640,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
-  if (!w->hist) {  // k_settle writes it with system-scope stores
+  if (!w->hist) {  // the settle step writes it with system-scope stores
     HIP_TRY(hipHostMalloc((void**)&w->hist, kHostHistWords * sizeof(unsigned long long), hipHostMallocCoherent));
     std::memset(w->hist, 0, kHostHistWords * sizeof(unsigned long long));
   }
@@ -1448,9 +1448,9 @@ int vgi_scene_get_stats(const vgi_scene* s, vgi_scene_stats* out) {
   std::memset(out, 0, sizeof(*out));
   out->triangles_in = s->tris_in;
   if (s->vox_seq != 0) {  // the voxelize call's own statistics, once the device stored them
-    const SceneVoxResult* r = s->result;  // k_settle stores its call number after the rest
+    const SceneVoxResult* r = s->result;  // the settle step stores its call number after the rest
     if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq) {
-      // that call's k_settle has not run yet: wait for the device's queued work
+      // that call's settle step has not run yet: wait for the device's queued work
       HIP_TRY(hipSetDevice(s->ctx->device));
       HIP_TRY(hipDeviceSynchronize());
       if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != s->vox_seq)

@@ -77,9 +77,10 @@ static_assert(sizeof(DevStats) == 32, "DevStats layout");
 // Words of a call's pinned batch totals (VoxBatch::host_hist).
 constexpr uint32_t kHostHistWords = 5;
 
-// A scene's result of one voxelize call, stored by k_settle after the bin scan into the
-// scene's pinned host record, seq once the other words are complete: the record is whole
-// once seq equals the call's number (read by vgi_scene_get_stats).
+// A scene's result of one voxelize call, stored by the settle step (the first wavefront of
+// k_scatter, after the bin scan) into the scene's pinned host record, seq once the other
+// words are complete: the record is whole once seq equals the call's number (read by
+// vgi_scene_get_stats).
 struct SceneVoxResult {
   DevStats st;
   unsigned long long totals;  // work units << 32 | pairs of the scene's bins
@@ -393,7 +394,6 @@ hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
                      uint32_t count, hipStream_t s);
 hipError_t launch_probe(uint32_t kind, void* buf, uint32_t blocks, uint32_t iters, hipStream_t s);
-hipError_t launch_settle(const VoxBatch& B, hipStream_t s);
 hipError_t launch_stage_desc(const NoAllocDesc& d, NoAllocDesc* dst, hipStream_t s);
 hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,
                           uint32_t zero_blocks, hipStream_t s);

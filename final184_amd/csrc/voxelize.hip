@@ -10,9 +10,9 @@
 // scenes' bins are concatenated so that one scan and one persistent voxelize launch
 // cover the whole batch. Pipeline (one stream, enqueued without waiting: the host sizes
 // the pair list and the work map before the call from the triangle count and what earlier
-// calls on the stream needed; after step 4 k_settle checks that the pairs and work units
-// fit, and when they do not, steps 5 and 6 give way to a per-triangle path (7) that needs
-// neither):
+// calls on the stream needed; after step 4 every k_scatter wavefront checks that the pairs
+// and work units fit, and when they do not, steps 5 and 6 give way to a per-triangle path
+// (7) that needs neither):
 //   1 k_setup       one thread per triangle: fetch, model transform, snap to 1/256 voxel,
 //                   exact integer normal, dominant axis (VoxelGS rule), quantized normal,
 //                   voxel bounding box. A triangle whose bounding box spans <= 8 bins
@@ -24,9 +24,10 @@
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
 //                   bins with > 1 unit are listed.
 //   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
-//                   offset and its first work unit at once; k_settle stores the per-scene
-//                   totals and statistics and sets the overflow word.
-//   5 k_scatter     one launch: work unit -> (scene, bin) and the units' order (those of
+//                   offset and its first work unit at once.
+//   5 k_scatter     one launch; its first wavefront settles the call (per-scene totals
+//                   and statistics to the host records, the overflow word for steps 6
+//                   and 7); work unit -> (scene, bin) and the units' order (those of
 //                   two or more chunks first); small triangles to their kept
 //                   slots (no atomics); large triangles after the small ones of each bin;
 //                   clear the accumulator and occupancy mask of bins with several units.
@@ -541,11 +542,14 @@ hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in,
   return rocprim::exclusive_scan(tmp, *tmp_bytes, in, out, 0ull, (size_t)count, rocprim::plus<unsigned long long>(), s);
 }
 
-// After the scan (one wavefront): each scene's totals (work units << 32 | pairs of its
-// bins) and statistics go to its pinned host record; the batch's exact pair count (a
-// 64-bit sum of the per-scene counts, which the packed 32-bit field cannot hold beyond
-// 2^32) and unit count are checked against the pair list and work map the host sized, and
-// the overflow word tells steps 5-7 which path to take. The batch totals (pairs, units,
+// The settle step, after the scan (the first wavefront of k_scatter; every wavefront of
+// k_scatter takes the overflow decision itself, call_overflow): each scene's totals (work
+// units << 32 | pairs of its bins) and statistics go to its pinned host record; the
+// batch's exact pair count (a 64-bit sum of the per-scene counts, which the packed 32-bit
+// field cannot hold beyond 2^32) and unit count are checked against the pair list and
+// work map the host sized, and the overflow word tells steps 6-7 which path to take (it
+// was a kernel of its own, k_settle: folding it into k_scatter saved a launch, about
+// 3 us a frame). The batch totals (pairs, units,
 // large triangles, split bins) go to the stream's pinned history, from which the host
 // sizes later calls. Both are coherent host memory, written with system-scope stores; each
 // record's call number (and the history's) is stored once the other words are complete,
@@ -565,9 +569,24 @@ __device__ __forceinline__ void sys_store_last(unsigned long long* p, unsigned l
   sys_store(p, v);
 }
 
-__global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __restrict__ scenes) {
+// The call's overflow decision, taken by a whole wavefront (every lane returns it): the
+// batch's exact pair count and its unit count against the pair list and the work map.
+__device__ __forceinline__ uint32_t call_overflow(const VoxBatch& B, const VoxScene* __restrict__ scenes) {
+  const uint32_t lane = __lane_id();
+  unsigned long long pairs = 0;
+  for (uint32_t sc = lane; sc < B.nscenes; sc += 64) pairs += gld(&scenes[sc].stats->pairs);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+  const uint32_t units = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);
+  // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is exact
+  return (pairs > B.pair_cap || units > B.work_cap) ? 1u : 0u;
+}
+
+// The settle step, by one wavefront (the first of k_scatter's first workgroup).
+__device__ void settle_wave(const VoxBatch& B, const VoxScene* __restrict__ scenes, uint32_t ovf) {
+  const uint32_t lane = __lane_id();
   unsigned long long pairs = 0, big = 0, split = 0;
-  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
+  for (uint32_t sc = lane; sc < B.nscenes; sc += 64) {
     pairs += gld(&scenes[sc].stats->pairs);
     big += gld(&scenes[sc].stats->big_count);
     split += gld(&scenes[sc].stats->zero_count);
@@ -579,9 +598,7 @@ __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __res
     split += __shfl_xor(split, o);
   }
   const uint32_t units = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);
-  // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is exact
-  const uint32_t ovf = (pairs > B.pair_cap || units > B.work_cap) ? 1u : 0u;
-  for (uint32_t sc = threadIdx.x; sc < B.nscenes; sc += 64) {
+  for (uint32_t sc = lane; sc < B.nscenes; sc += 64) {
     const VoxScene& P = scenes[sc];
     const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
     SceneVoxResult r;
@@ -604,7 +621,7 @@ __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __res
       if (q != kSeqWord) sys_store(dst + q, src[q]);
     sys_store_last(dst + kSeqWord, src[kSeqWord]);
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     gst(B.overflow, ovf);
     sys_store(B.host_hist + 0, pairs);
     sys_store(B.host_hist + 1, (unsigned long long)units);
@@ -615,10 +632,6 @@ __global__ __launch_bounds__(64) void k_settle(VoxBatch B, const VoxScene* __res
   }
 }
 
-hipError_t launch_settle(const VoxBatch& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_settle, dim3(1), dim3(64), 0, s, B, B.scenes);
-  return hipGetLastError();
-}
 
 // work unit -> (scene, bin), from the scanned unit offsets; workgroup blk of the bins. Also
 // the order in which k_voxelize takes the units: those of two or more chunks first, from
@@ -749,7 +762,9 @@ __device__ __forceinline__ void clear_bins_block(const VoxBatch& Bt, const VoxSc
 __global__ __launch_bounds__(256) void k_scatter(VoxBatch B, const VoxScene* __restrict__ scenes, uint32_t nblk_b,
                                                  uint32_t nblk_t, uint32_t nblk_big) {
   const uint32_t b = blockIdx.x;
-  if (gld(B.overflow) != 0u) {
+  const uint32_t ovf = call_overflow(B, scenes);  // the same in every wavefront
+  if (b == 0 && threadIdx.x < 64) settle_wave(B, scenes, ovf);
+  if (ovf != 0u) {
     if (b < nblk_b) clear_bins_block(B, scenes, b);
     return;
   }
@@ -1244,8 +1259,7 @@ hipError_t launch_voxelize(const VoxBatch& B, uint32_t tile, uint32_t grid_block
 // 7 overflow path: per-triangle voxelize with global atomics
 // ---------------------------------------------------------------------------
 
-// Runs only when k_settle set the overflow word (k_scatter then cleared every bin with
-// pairs). A kernel of its own: inlined into k_voxelize it would spill that kernel's
+// Runs only when k_scatter set the overflow word (and cleared every bin with pairs). A kernel of its own: inlined into k_voxelize it would spill that kernel's
 // registers, called from it the call would cut its occupancy. Wavefront w of the launch
 // takes triangles w, w + nwaves, ... of every scene; its
 // lanes walk the triangle's voxel box, keep the voxels the exact closed-box test accepts
