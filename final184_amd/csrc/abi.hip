@@ -224,6 +224,10 @@ struct vgi_scene {
   uint32_t* live = nullptr;  // per 16^3 tile of level 0: a word, then 8 row words (the mip kernel's skip state)
   size_t live_bytes = 0;
   uint32_t alloc_levels = 0, alloc_n = 0, alloc_prec = 0;
+  // the injecting launch's tile-list counts are zero once the work queued so far has run
+  // (the tile state was just cleared, or the last ordered build's later launch zeroed
+  // them); otherwise the next ordered build clears them first
+  bool tile_counts_zero = false;
   bool has_voxels = false, has_mips = false;
   uint64_t tris_in = 0;
   // the last voxelize call's statistics: written by the device into the pinned record after
@@ -782,6 +786,7 @@ int scene_levels(vgi_scene* s, uint32_t precision, hipStream_t st) {
     }
     s->live_bytes = bytes;
     HIP_TRY(launch_fill_u32(s->live, 0u, bytes / 4, st));
+    s->tile_counts_zero = true;
   }
   for (uint32_t k = 0; k < g.levels; k++) {
     const size_t p = (size_t)(g.n >> k) + 2;
@@ -908,6 +913,7 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
   const bool prof = ctx->profiling != 0;
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[5], st));
   size_t off = 0;
+  uint32_t* list_counts = nullptr;  // an ordered build's tile-list counts
   for (size_t j = 0; j < stages.size(); j++) {
     MipBatch B{};
     B.scenes = (const MipScene*)(sl->d + o_scenes);
@@ -921,12 +927,17 @@ int mips_batch(vgi_context* ctx, const vgi_mips_item* items, uint32_t count, uin
     // follow the pair count: the room's shadowed builds were 3-5% slower in list order)
     B.ordered = j == 0 && !table && stages[0][0].nout > 0 && !shadow;
     if (B.ordered) {  // the injecting launch's tile list, dense tiles first
-      const vgi_grid& g = items[0].scene->grid;
+      vgi_scene* sc = items[0].scene;
+      const vgi_grid& g = sc->grid;
       const size_t tpa = g.n / std::min<uint32_t>(g.n, 16u), ntl = tpa * tpa * tpa;
-      HIP_TRY(launch_fill_u32(items[0].scene->live + 10 * ntl, 0u, 2, st));
+      list_counts = sc->live + 10 * ntl;
+      if (!sc->tile_counts_zero) HIP_TRY(launch_fill_u32(list_counts, 0u, 2, st));
+      sc->tile_counts_zero = false;
       HIP_TRY(launch_tiles(B, (uint32_t)ntl, st));
     }
+    B.clear_counts = j == 1 ? list_counts : nullptr;  // the next build then needs no clearing launch
     HIP_TRY(launch_mips(B, precision, j == 0, (uint32_t)stages[j].size(), st));
+    if (B.clear_counts) items[0].scene->tile_counts_zero = true;
     off += stages[j].size();
   }
   if (prof) {

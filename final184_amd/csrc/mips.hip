@@ -330,13 +330,16 @@ template <int PREC, bool SRC_ACC, bool SHADOW>
 __global__ __launch_bounds__(1024) void k_mips(const MipScene* __restrict__ scenes,
                                                const MipBlock* __restrict__ blocks,
                                                const float* __restrict__ lut, uint32_t src1, uint32_t nout1,
-                                               uint32_t ordered) {
+                                               uint32_t ordered, uint32_t* __restrict__ clear_counts) {
   using TX = Texel<PREC>;
   constexpr uint32_t kThreads = mips_threads(PREC, SRC_ACC, SHADOW);
   __shared__ float4 s_a[512];
   __shared__ float4 s_b[64];
   const MipBlock blk = blocks ? blocks[blockIdx.x] : MipBlock{0u, blockIdx.x, src1, nout1};
   const MipScene& P = scenes[blk.scene];
+  // a later launch of an ordered build: the injecting launch, the list's only reader, has
+  // completed, so the counts are zeroed here for the next build's k_tiles
+  if (!SRC_ACC && clear_counts && blockIdx.x == 0 && threadIdx.x < 2) gst(clear_counts + threadIdx.x, 0u);
   const uint32_t src = blk.src, nout = blk.nout;
   const uint32_t ns = P.nk[src];
   const uint32_t S = ns < 16u ? ns : 16u;  // source tile edge (a power of two)
@@ -344,7 +347,8 @@ __global__ __launch_bounds__(1024) void k_mips(const MipScene* __restrict__ scen
   uint32_t tile = blk.tile;
   if (SRC_ACC && ordered) {  // block b takes the b-th tile of k_tiles' list; unlisted tiles keep their zeros
     const uint32_t ntl = tpa * tpa * tpa, b = blockIdx.x;
-    const uint32_t nd = gld(P.order + ntl), nl = gld(P.order + ntl + 1);
+    // (clamped: a list never holds more than ntl tiles)
+    const uint32_t nd = min(gld(P.order + ntl), ntl), nl = min(gld(P.order + ntl + 1), ntl - nd);
     if (b >= nd + nl) return;  // uniform
     tile = gld(P.order + (b < nd ? b : ntl - 1u - (b - nd)));
   }
@@ -522,13 +526,13 @@ static void launch_mips_prec(const MipBatch& B, bool from_acc, dim3 g, hipStream
   const dim3 b(mips_threads(PREC, from_acc, from_acc && B.shadow));
   if (!from_acc)
     hipLaunchKernelGGL((k_mips<PREC, false, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
-                       (uint32_t)B.ordered);
+                       (uint32_t)B.ordered, B.clear_counts);
   else if (B.shadow)
     hipLaunchKernelGGL((k_mips<PREC, true, true>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
-                       (uint32_t)B.ordered);
+                       (uint32_t)B.ordered, (uint32_t*)nullptr);
   else
     hipLaunchKernelGGL((k_mips<PREC, true, false>), g, b, 0, s, B.scenes, B.blocks, B.lut, B.src, B.nout,
-                       (uint32_t)B.ordered);
+                       (uint32_t)B.ordered, (uint32_t*)nullptr);
 }
 
 // The injecting launch's tiles for one scene, densest first: a tile is listed when its bins

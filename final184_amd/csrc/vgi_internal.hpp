@@ -202,6 +202,8 @@ struct MipBatch {
   uint32_t src, nout;
   bool shadow;             // some light of the batch casts shadows (the injecting launch walks)
   bool ordered;            // one scene, injecting launch: block b takes the listed tile b
+  uint32_t* clear_counts;  // a later launch of an ordered build: the list's two counts, which
+                           // its first workgroup zeroes for the next build (else null)
 };
 
 // One scene of a trace batch (device memory).
