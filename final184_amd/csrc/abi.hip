@@ -526,8 +526,6 @@ int launch_voxelize_kernels(vgi_context* ctx, const VoxBatch& B, uint32_t* zb, s
   HIP_TRY(launch_big_count(B, persistent, st));
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[1], st));
   HIP_TRY(launch_bins(B, (uint32_t)nblk_b, st));
-  size_t tmp_bytes = scan_tmp_bytes;
-  HIP_TRY(scan_bins(scan_tmp, &tmp_bytes, B.packed_all, B.scan_all, B.total_bins + 1, st));
   for (uint32_t i = 0; i < count; i++) scenes[i]->vox_seq = B.seq;  // k_scatter stores it
   if (prof) HIP_TRY(hipEventRecord(ctx->ev[2], st));
   HIP_TRY(launch_scatter(B, (uint32_t)nblk_b, (uint32_t)nblk_t, big_blocks, zero_blocks, st));
@@ -613,7 +611,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const size_t stats_off = align_up(3 * total_bins * 4, 256);
   const size_t stats_bytes = (size_t)count * sizeof(DevStats);
   const size_t ovf_off = align_up(stats_off + stats_bytes, 256);
-  const size_t zeroed_need = ovf_off + 256;
+  const size_t flags_off = ovf_off + 256;  // the bin scan's per-workgroup flags
+  const size_t zeroed_need = flags_off + align_up(std::max<size_t>(nblk_b, 1) * 4, 256);
   // The pair list holds 4 pairs per triangle plus 64 Ki (a triangle in at most 8 bins is
   // the common case; large triangles add more), or 5/4 of the most an earlier call on this
   // stream needed. A call that needs more is still exact: k_scatter sees it and the
@@ -625,8 +624,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const uint64_t work_cap64 = total_bins + pair_cap / std::max(1u, tn.chunk_tris) + 1;
   if (work_cap64 >= (1ull << 32)) return fail(VGI_E_RANGE, "batch too large (%zu bins)", total_bins);
   const uint32_t work_cap = (uint32_t)work_cap64;
-  size_t scan_need = 0;
-  HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(total_bins + 1), st));
+  const size_t scan_need = std::max<size_t>(nblk_b, 1) * 2 * sizeof(unsigned long long);  // totals, prefixes
   const size_t tneed = std::max<size_t>(total_tri, 1);
   const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
                     (textured && w->tritex_bytes < tneed * sizeof(TriTex)) ||
@@ -714,6 +712,8 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.bin_blocks = multi ? (const uint2*)(sl->d + o_bblk) : nullptr;
   B.packed_all = (unsigned long long*)w->packed_all;
   B.scan_all = (unsigned long long*)w->scan_all;
+  B.scan_flags = (uint32_t*)((uint8_t*)w->zeroed + flags_off);
+  B.scan_vals = (unsigned long long*)w->scan_tmp;
   B.work_map = (uint2*)w->work_map;
   B.unit_order = (uint32_t*)(B.work_map + work_cap);
   B.pairs_all = (uint32_t*)w->pairs;
@@ -1517,9 +1517,10 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   const uint64_t work_cap64 = nbins + pair_cap64 / std::max(1u, tn.chunk_tris) + 1;
   const size_t stats_off = align_up(3 * nbins * 4, 256);
   const size_t ovf_off = align_up(stats_off + sizeof(DevStats), 256);
-  const size_t zeroed_bytes = ovf_off + 256;
-  size_t scan_need = 0;
-  HIP_TRY(scan_bins(nullptr, &scan_need, nullptr, nullptr, (uint32_t)(nbins + 1), pick(ctx, stream)));
+  const size_t nblk_b = ceil_div(nbins, kBlockBins);
+  const size_t flags_off = ovf_off + 256;  // the bin scan's per-workgroup flags
+  const size_t zeroed_bytes = flags_off + align_up(nblk_b * 4, 256);
+  const size_t scan_need = nblk_b * 2 * sizeof(unsigned long long);  // totals, prefixes
   Carve cv;
   const size_t o_desc = cv.take(sizeof(NoAllocDesc));
   const size_t o_hist = cv.take(kHostHistWords * sizeof(unsigned long long));
@@ -1591,6 +1592,8 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.bin_blocks = nullptr;
   B.packed_all = (unsigned long long*)(base + o_packed);
   B.scan_all = (unsigned long long*)(base + o_scan);
+  B.scan_flags = (uint32_t*)(base + o_zero + flags_off);
+  B.scan_vals = (unsigned long long*)(base + o_stmp);
   B.work_map = (uint2*)(base + o_work);
   B.unit_order = (uint32_t*)(B.work_map + work_cap64);
   B.pairs_all = (uint32_t*)(base + o_pairs);

@@ -134,7 +134,10 @@ struct VoxBatch {
   const uint2* tri_blocks;          // (scene, first triangle) per setup/emit workgroup
   const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
   unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
-  unsigned long long* scan_all;     // exclusive scan of packed_all
+  unsigned long long* scan_all;     // exclusive scan of the packed bin entries (k_bins)
+  uint32_t* scan_flags;             // per k_bins workgroup: 0, 1 total published, 2 prefix
+                                    // published (cleared with the overflow word)
+  unsigned long long* scan_vals;    // per k_bins workgroup: its total, then its inclusive prefix
   uint2* work_map;                  // (scene, bin) of every work unit (work_cap entries)
   uint32_t* unit_order;             // the order k_voxelize takes the units in (work_cap entries)
   uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn
@@ -142,7 +145,8 @@ struct VoxBatch {
   uint32_t work_cap;                // entries of work_map
   uint32_t* overflow;               // device word: 1 when the pairs or units exceed the caps;
                                     // the two words after it (cleared with it) are the unit
-                                    // order's front and back cursors
+                                    // order's front and back cursors, the third is set when
+                                    // the bin scan's look-back gave up
   unsigned long long* host_hist;    // pinned (kHostHistWords): batch pairs, units, large
                                     // triangles, split bins, then seq (stored after them)
   uint32_t seq;                     // this call's number (never 0)
@@ -393,8 +397,6 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_setup(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s);
-hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
-                     uint32_t count, hipStream_t s);
 hipError_t launch_probe(uint32_t kind, void* buf, uint32_t blocks, uint32_t iters, hipStream_t s);
 hipError_t launch_stage_desc(const NoAllocDesc& d, NoAllocDesc* dst, hipStream_t s);
 hipError_t launch_scatter(const VoxBatch& B, uint32_t nblk_b, uint32_t nblk_t, uint32_t big_blocks,

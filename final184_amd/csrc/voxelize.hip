@@ -22,9 +22,9 @@
 //   2 k_big_count   one wavefront per listed triangle: its bins, tested cooperatively.
 //   3 k_bins        per bin: pairs = small + large, work items = ceil(pairs / chunk),
 //                   work units = ceil(items / unit_items), packed as (units << 32 | pairs);
-//                   bins with > 1 unit are listed.
-//   4 scan          one 64-bit exclusive scan over the batch gives every bin its pair
-//                   offset and its first work unit at once.
+//                   bins with > 1 unit are listed. In the same pass:
+//   4 scan          one 64-bit exclusive scan over the batch (a look-back between the
+//                   workgroups) gives every bin its pair offset and its first work unit.
 //   5 k_scatter     one launch; its first wavefront settles the call (per-scene totals
 //                   and statistics to the host records, the overflow word for steps 6
 //                   and 7); work unit -> (scene, bin) and the units' order (those of
@@ -51,7 +51,6 @@
 // voxel set and the integer sums do not depend on order, rounding or thread mapping.
 #include <cstddef>
 
-#include <rocprim/device/device_scan.hpp>
 
 #include "vgi_internal.hpp"
 
@@ -509,37 +508,135 @@ hipError_t launch_big_count(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
 // 3 per-bin totals, 4 scan over every bin of the batch
 // ---------------------------------------------------------------------------
 
+// Device-coherent (agent scope) word accesses for the lookback between workgroups, which
+// may run on different XCDs: they bypass the XCD's own L2.
+__device__ __forceinline__ void agent_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void agent_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long agent_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t agent_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls of a predecessor's flag before a wait gives up (then the call takes the
+// per-triangle path, which needs no scan). Workgroups are dispatched in index order and a
+// workgroup waits only for lower ones, so a wait ends long before this.
+constexpr uint32_t kScanSpin = 1u << 22;
+
+// Per bin: pairs = small + large, units, the packed (units << 32 | pairs) entry, the list
+// of bins with several units, the scene's exact pair count; and the batch-wide exclusive
+// scan of the packed entries in one pass (replacing a separate two-kernel scan): each
+// workgroup scans its 256 bins, publishes its total (flag 1), finds its prefix by looking
+// back over its predecessors 64 at a time (wavefront 0: a predecessor's inclusive prefix,
+// flag 2, ends the look-back, totals of the ones after it are added), publishes its
+// inclusive prefix (flag 2) and stores its bins' scanned entries. A value is stored and
+// complete before its flag. Workgroups follow the batch's bin order (scenes in turn), so
+// workgroup j's bins precede workgroup j + 1's.
 __global__ __launch_bounds__(256) void k_bins(VoxBatch B, const VoxScene* __restrict__ scenes) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) gst(B.packed_all + B.total_bins, 0ull);
+  __shared__ unsigned long long s_wsum[4], s_prefix;
+  const uint32_t j = blockIdx.x, nblk = gridDim.x;
   const uint2 blk = bin_block(B);
   const VoxScene& P = scenes[blk.x];
   const uint32_t nbins = P.nbins_axis * P.nbins_axis * P.nbins_axis;
   const uint32_t b = blk.y + threadIdx.x;
-  unsigned long long total = 0;
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  unsigned long long total = 0, packed = 0;
   if (b < nbins) {
     const uint32_t pairs = gld(P.bin_small + ctr(P, b)) + gld(P.bin_big + ctr(P, b));
     const uint32_t items = (pairs + P.chunk - 1) / P.chunk;
     const uint32_t units = (items + P.unit_items - 1) / P.unit_items;
-    const unsigned long long packed = ((unsigned long long)units << 32) | pairs;
+    packed = ((unsigned long long)units << 32) | pairs;
     gst(P.bin_packed + b, packed);
-    gst(B.packed_all + P.bin_base + b, packed);
     if (units > 1) gst(P.zero_list + gatomic_add(&P.stats->zero_count, 1u), b);
     total = pairs;
   }
   // exact 64-bit pair count per scene (the packed scan keeps only 32 bits of it)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
-  if (__lane_id() == 0 && total) gatomic_add(&P.stats->pairs, total);
+  if (lane == 0 && total) gatomic_add(&P.stats->pairs, total);
+  // the workgroup's inclusive scan
+  unsigned long long incl = packed;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const unsigned long long t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63u) s_wsum[wave] = incl;
+  __syncthreads();
+  unsigned long long woff = 0, agg = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; w++) {
+    if (w < wave) woff += s_wsum[w];
+    agg += s_wsum[w];
+  }
+  // the prefix of the bins before this workgroup's
+  unsigned long long* agg_v = B.scan_vals;          // per workgroup: its total
+  unsigned long long* inc_v = B.scan_vals + nblk;   // per workgroup: its inclusive prefix
+  if (wave == 0) {
+    unsigned long long prefix = 0;
+    if (j == 0) {
+      if (lane == 0) {
+        agent_store(inc_v, agg);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        agent_store(B.scan_flags, 2u);
+      }
+    } else {
+      if (lane == 0) {
+        agent_store(agg_v + j, agg);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        agent_store(B.scan_flags + j, 1u);
+      }
+      bool failed = false;
+      for (int top = (int)j - 1;; top -= 64) {  // lane l looks at workgroup top - l
+        const int k = top - (int)lane;
+        uint32_t f = 2u;  // before workgroup 0: an empty prefix
+        unsigned long long v = 0;
+        if (k >= 0) {
+          f = 0u;
+          for (uint32_t spin = 0; spin < kScanSpin; spin++) {
+            f = agent_load(B.scan_flags + k);
+            if (f != 0u) break;
+            __builtin_amdgcn_s_sleep(1);
+          }
+          if (f == 0u) {
+            failed = true;
+            f = 2u;
+          } else {
+            v = agent_load(f == 2u ? inc_v + k : agg_v + k);
+          }
+        }
+        const uint64_t pm = __ballot(f == 2u);
+        const int m = pm ? __ffsll((unsigned long long)pm) - 1 : 63;  // nearest with its prefix
+        unsigned long long sv = (int)lane <= m ? v : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o);
+        prefix += sv;
+        if (pm) break;
+      }
+      failed = __any(failed);
+      if (lane == 0) {
+        agent_store(inc_v + j, prefix + agg);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        agent_store(B.scan_flags + j, 2u);
+        if (failed) agent_store(B.overflow + 3, 1u);  // the call takes the per-triangle path
+      }
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const unsigned long long prefix = s_prefix;
+  if (b < nbins) gst(B.scan_all + P.bin_base + b, prefix + woff + incl - packed);
+  if (j == nblk - 1u && threadIdx.x == 0) gst(B.scan_all + B.total_bins, prefix + agg);  // the batch's totals
 }
 
 hipError_t launch_bins(const VoxBatch& B, uint32_t blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_bins, dim3(blocks), dim3(kBlockBins), 0, s, B, B.scenes);
   return hipGetLastError();
-}
-
-hipError_t scan_bins(void* tmp, size_t* tmp_bytes, const unsigned long long* in, unsigned long long* out,
-                     uint32_t count, hipStream_t s) {
-  return rocprim::exclusive_scan(tmp, *tmp_bytes, in, out, 0ull, (size_t)count, rocprim::plus<unsigned long long>(), s);
 }
 
 // The settle step, after the scan (the first wavefront of k_scatter; every wavefront of
@@ -578,8 +675,9 @@ __device__ __forceinline__ uint32_t call_overflow(const VoxBatch& B, const VoxSc
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
   const uint32_t units = (uint32_t)(gld(B.scan_all + B.total_bins) >> 32);
-  // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is exact
-  return (pairs > B.pair_cap || units > B.work_cap) ? 1u : 0u;
+  // with pairs <= pair_cap < 2^32 no pair count carried into the unit field, so units is
+  // exact; the fourth word is set when the bin scan's look-back gave up (never seen)
+  return (pairs > B.pair_cap || units > B.work_cap || gld(B.overflow + 3) != 0u) ? 1u : 0u;
 }
 
 // The settle step, by one wavefront (the first of k_scatter's first workgroup).
