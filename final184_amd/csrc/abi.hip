@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -201,6 +202,7 @@ struct vgi_context {
   hipEvent_t tstart = nullptr, tstop = nullptr;
   hipEvent_t dep = nullptr;  // vgi_stream_wait (a wait captures the record it follows)
   std::atomic<uint32_t> vox_calls{0};  // voxelize call numbers, unique across the context's streams
+  uint32_t scan_spin = kScanSpin;      // the bin scan's look-back limit (VGI_SCAN_SPIN, for tests)
   vgi_timings timings{};
   std::mutex mu;  // tuning, the workspace map
   std::map<hipStream_t, std::unique_ptr<vgi_workspace>> ws;
@@ -714,6 +716,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.scan_all = (unsigned long long*)w->scan_all;
   B.scan_flags = (uint32_t*)((uint8_t*)w->zeroed + flags_off);
   B.scan_vals = (unsigned long long*)w->scan_tmp;
+  B.scan_spin = ctx->scan_spin;
   B.work_map = (uint2*)w->work_map;
   B.unit_order = (uint32_t*)(B.work_map + work_cap);
   B.pairs_all = (uint32_t*)w->pairs;
@@ -1298,6 +1301,7 @@ int vgi_create(vgi_context** out, int device) {
   vgi_context* ctx = new vgi_context();
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;
+  if (const char* e = std::getenv("VGI_SCAN_SPIN")) ctx->scan_spin = (uint32_t)std::strtoul(e, nullptr, 10);
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -1594,6 +1598,7 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.scan_all = (unsigned long long*)(base + o_scan);
   B.scan_flags = (uint32_t*)(base + o_zero + flags_off);
   B.scan_vals = (unsigned long long*)(base + o_stmp);
+  B.scan_spin = ctx->scan_spin;
   B.work_map = (uint2*)(base + o_work);
   B.unit_order = (uint32_t*)(B.work_map + work_cap64);
   B.pairs_all = (uint32_t*)(base + o_pairs);

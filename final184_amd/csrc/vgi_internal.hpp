@@ -77,6 +77,9 @@ static_assert(sizeof(DevStats) == 32, "DevStats layout");
 // Words of a call's pinned batch totals (VoxBatch::host_hist).
 constexpr uint32_t kHostHistWords = 5;
 
+// Default polls of a predecessor's flag before the bin scan's look-back gives up.
+constexpr uint32_t kScanSpin = 1u << 22;
+
 // A scene's result of one voxelize call, stored by the settle step (the first wavefront of
 // k_scatter, after the bin scan) into the scene's pinned host record, seq once the other
 // words are complete: the record is whole once seq equals the call's number (read by
@@ -138,6 +141,7 @@ struct VoxBatch {
   uint32_t* scan_flags;             // per k_bins workgroup: 0, 1 total published, 2 prefix
                                     // published (cleared with the overflow word)
   unsigned long long* scan_vals;    // per k_bins workgroup: its total, then its inclusive prefix
+  uint32_t scan_spin;               // polls of a predecessor's flag before the look-back gives up
   uint2* work_map;                  // (scene, bin) of every work unit (work_cap entries)
   uint32_t* unit_order;             // the order k_voxelize takes the units in (work_cap entries)
   uint32_t* pairs_all;              // triangle ids grouped by bin, every scene's bins in turn

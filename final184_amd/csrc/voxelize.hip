@@ -523,10 +523,10 @@ __device__ __forceinline__ uint32_t agent_load(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Polls of a predecessor's flag before a wait gives up (then the call takes the
-// per-triangle path, which needs no scan). Workgroups are dispatched in index order and a
-// workgroup waits only for lower ones, so a wait ends long before this.
-constexpr uint32_t kScanSpin = 1u << 22;
+// VoxBatch::scan_spin polls of a predecessor's flag before a wait gives up (then the call
+// takes the per-triangle path, which needs no scan): kScanSpin, or what VGI_SCAN_SPIN held
+// when the context was created (tests set 0 to take that path). Workgroups are dispatched
+// in index order and a workgroup waits only for lower ones, so a wait ends long before.
 
 // Per bin: pairs = small + large, units, the packed (units << 32 | pairs) entry, the list
 // of bins with several units, the scene's exact pair count; and the batch-wide exclusive
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void k_bins(VoxBatch B, const VoxScene* __rest
         unsigned long long v = 0;
         if (k >= 0) {
           f = 0u;
-          for (uint32_t spin = 0; spin < kScanSpin; spin++) {
+          for (uint32_t spin = 0; spin < B.scan_spin; spin++) {
             f = agent_load(B.scan_flags + k);
             if (f != 0u) break;
             __builtin_amdgcn_s_sleep(1);

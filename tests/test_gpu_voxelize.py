@@ -286,3 +286,22 @@ def test_voxelize_returns_without_waiting(ctx):
     finally:
         tracer.release()
         target.release()
+
+
+def test_scan_lookback_giving_up_takes_per_triangle_path(monkeypatch):
+    # With a look-back limit of 0 polls, every k_bins workgroup after the first gives up
+    # at once (no wait at all): the call must take the per-triangle path and still give
+    # the oracle's voxels. 64^3 with 8^3 bins is 512 bins, two k_bins workgroups.
+    vgi = _vgi()
+    monkeypatch.setenv("VGI_SCAN_SPIN", "0")
+    c = vgi.Context()
+    try:
+        w = sg.random_soup(ntri=6000, n=64)
+        print(f"case {w.name} scan look-back limit 0", flush=True)
+        mesh = vgi.MeshData(c, w.positions.reshape(-1, 3), albedo=w.albedo)
+        acc, st = _gpu_acc(c, [mesh], w.grid)
+        ref, _ = orc.voxelize(w.positions, w.grid, albedo=w.albedo)
+        assert st.voxelize_fallback == 1
+        assert np.array_equal(acc, ref), _diff_msg(acc, ref)
+    finally:
+        c.close()
