@@ -156,8 +156,6 @@ struct vgi_workspace {
   size_t zeroed_bytes = 0;
   void* zero_list = nullptr;
   size_t zero_list_bytes = 0;
-  void* packed_all = nullptr;
-  size_t packed_bytes = 0;
   void* scan_all = nullptr;
   size_t scan_bytes = 0;
   void* work_map = nullptr;
@@ -180,7 +178,7 @@ struct vgi_workspace {
   uint64_t hist_pairs = 0, hist_units = 0, hist_big = 0, hist_split = 0;  // largest seen
 
   size_t device_bytes() const {
-    size_t b = tris_bytes + tritex_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes + packed_bytes +
+    size_t b = tris_bytes + tritex_bytes + tri_slot_bytes + big_list_bytes + zeroed_bytes + zero_list_bytes +
                scan_bytes + work_bytes + pairs_bytes + scan_tmp_bytes + sort_keys_bytes + sort_pix_bytes +
                sort_tmp_bytes;
     for (const Slot& s : slot) b += s.cap;
@@ -256,8 +254,8 @@ size_t scene_device_bytes(const vgi_scene* s) {
 }
 
 void free_workspace(vgi_workspace* w) {
-  void* dev[] = {w->tris, w->tritex, w->tri_slot, w->big_list, w->zeroed, w->zero_list, w->packed_all,
-                 w->scan_all, w->work_map, w->pairs, w->scan_tmp, w->sort_keys, w->sort_pix, w->sort_tmp};
+  void* dev[] = {w->tris,     w->tritex,   w->tri_slot, w->big_list, w->zeroed,    w->zero_list, w->scan_all,
+                 w->work_map, w->pairs,    w->scan_tmp, w->sort_keys, w->sort_pix, w->sort_tmp};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   for (Slot& s : w->slot) {
@@ -630,7 +628,7 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   const size_t tneed = std::max<size_t>(total_tri, 1);
   const bool grow = w->tris_bytes < tneed * sizeof(TriRec) || w->zeroed_bytes < zeroed_need ||
                     (textured && w->tritex_bytes < tneed * sizeof(TriTex)) ||
-                    w->packed_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need ||
+                    w->scan_bytes < (total_bins + 1) * 8 || w->scan_tmp_bytes < scan_need ||
                     w->pairs_bytes < (size_t)pair_cap * 4 || w->work_bytes < (size_t)work_cap * kWorkEntryBytes;
   if (grow) HIP_TRY(hipStreamSynchronize(st));
   TRY(grow_bytes(&w->pairs, &w->pairs_bytes, (size_t)pair_cap * 4, "bin pairs"));
@@ -641,7 +639,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   TRY(grow_bytes(&w->big_list, &w->big_list_bytes, tneed * 4, "large-triangle list"));
   TRY(grow_bytes(&w->zeroed, &w->zeroed_bytes, zeroed_need, "bin counters"));
   TRY(grow_bytes(&w->zero_list, &w->zero_list_bytes, std::max<size_t>(total_bins, 1) * 4, "bin list"));
-  TRY(grow_bytes(&w->packed_all, &w->packed_bytes, (total_bins + 1) * 8, "bin table"));
   TRY(grow_bytes(&w->scan_all, &w->scan_bytes, (total_bins + 1) * 8, "bin scan"));
   TRY(grow_bytes(&w->scan_tmp, &w->scan_tmp_bytes, std::max<size_t>(scan_need, 1), "scan scratch"));
   if (!w->hist) {  // the settle step writes it with system-scope stores
@@ -712,7 +709,6 @@ int voxelize_batch(vgi_context* ctx, const vgi_voxelize_item* items, uint32_t co
   B.scenes = (const VoxScene*)(sl->d + o_scenes);
   B.tri_blocks = multi ? (const uint2*)(sl->d + o_tblk) : nullptr;
   B.bin_blocks = multi ? (const uint2*)(sl->d + o_bblk) : nullptr;
-  B.packed_all = (unsigned long long*)w->packed_all;
   B.scan_all = (unsigned long long*)w->scan_all;
   B.scan_flags = (uint32_t*)((uint8_t*)w->zeroed + flags_off);
   B.scan_vals = (unsigned long long*)w->scan_tmp;
@@ -1537,7 +1533,6 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   const size_t o_slot = cv.take(std::max<uint64_t>(ntri, 1) * kSmallBins * 4);
   const size_t o_big = cv.take(std::max<uint64_t>(ntri, 1) * 4);
   const size_t o_zlist = cv.take(nbins * 4);
-  const size_t o_packed = cv.take((nbins + 1) * 8);
   const size_t o_scan = cv.take((nbins + 1) * 8);
   const size_t o_stmp = cv.take(std::max<size_t>(scan_need, 1));
   const size_t o_pairs = cv.take(pair_cap64 * 4);
@@ -1594,7 +1589,6 @@ int vgi_voxelize_noalloc(vgi_context* ctx, vgi_scene* sc, const vgi_mesh* meshes
   B.scenes = &dd->scene;
   B.tri_blocks = nullptr;
   B.bin_blocks = nullptr;
-  B.packed_all = (unsigned long long*)(base + o_packed);
   B.scan_all = (unsigned long long*)(base + o_scan);
   B.scan_flags = (uint32_t*)(base + o_zero + flags_off);
   B.scan_vals = (unsigned long long*)(base + o_stmp);

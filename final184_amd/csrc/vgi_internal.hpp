@@ -136,7 +136,6 @@ struct VoxBatch {
   const VoxScene* scenes;
   const uint2* tri_blocks;          // (scene, first triangle) per setup/emit workgroup
   const uint2* bin_blocks;          // (scene, first bin) per bins/workmap workgroup
-  unsigned long long* packed_all;   // total_bins + 1, scene bins concatenated
   unsigned long long* scan_all;     // exclusive scan of the packed bin entries (k_bins)
   uint32_t* scan_flags;             // per k_bins workgroup: 0, 1 total published, 2 prefix
                                     // published (cleared with the overflow word)
