@@ -228,3 +228,32 @@ def test_full_batch_vs_oracle(ctx, order):
         print(f"worst scene rel L2 vs FP64 = {worst:.3e}", flush=True)
     finally:
         ctx.set_tuning()
+
+
+def test_bin_scan_across_more_workgroups_than_fit_at_once(ctx):
+    """Two 512^3 scenes (1024 k_bins workgroups each with 8^3 bins) ahead of a 64^3 soup in
+    one voxelize batch: the soup's workgroups come after 2048 others, more than the GPU
+    holds at once (8 of them per CU), so its bins' offsets come from a look-back over
+    predecessors that ran in earlier waves. The soup's voxels must equal the oracle's,
+    and no scene may have taken the per-triangle path."""
+    vgi = _vgi()
+    big = [sg.random_soup(seed=s, ntri=400, n=512, levels=1, width=8, height=8) for s in (11, 12)]
+    small = sg.random_soup(ntri=6000, n=64, width=8, height=8)
+    print("case voxelize batch 2 x 512^3 + 64^3 (2050 k_bins workgroups)", flush=True)
+    scenes, jobs = [], []
+    try:
+        ctx.set_tuning(bin_size=8)
+        for w in big + [small]:
+            s = ctx.scene()
+            scenes.append(s)
+            jobs.append((s, [vgi.MeshData(ctx, w.positions.reshape(-1, 3), albedo=w.albedo)], vgi.make_grid(**w.grid)))
+        ctx.voxelize_batch(jobs)
+        for s in scenes:
+            assert s.stats().voxelize_fallback == 0
+        ref, _ = orc.voxelize(small.positions, small.grid, albedo=small.albedo)
+        got = scenes[-1].read_voxels()
+        assert np.array_equal(got, ref), f"{(got != ref).any(axis=-1).sum()} voxels differ"
+    finally:
+        ctx.set_tuning()
+        for s in scenes:
+            s.release()
